@@ -1,0 +1,100 @@
+"""ctypes binding of ``libhfjk.so`` (the C ABI of ``include/hfjk.h``).
+
+The library is loaded from this package's directory only. If it is missing
+the import of any GPU entry point raises :class:`BackendError`: there is no
+CPU path behind this package.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+from hafnium_amd.errors import BackendError, InputError
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libhfjk.so")
+
+FP64 = 0
+FP32 = 1
+BF16X3 = 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+
+OK = 0
+ERR_INPUT = 1
+ERR_BACKEND = 2
+ERR_HIP = 3
+ERR_WORKSPACE = 4
+ERR_SOLVER = 5
+
+_c_int = ctypes.c_int
+_c_ll = ctypes.c_longlong
+_c_size = ctypes.c_size_t
+_c_u64 = ctypes.c_uint64
+_c_u32 = ctypes.c_uint32
+_vp = ctypes.c_void_p
+
+# (name, restype, argtypes)
+SIGNATURES = [
+    ("hfjk_version", _c_int, []),
+    ("hfjk_last_error", ctypes.c_char_p, []),
+    ("hfjk_create", _c_int, [ctypes.POINTER(_vp), _c_int]),
+    ("hfjk_destroy", _c_int, [_vp]),
+    ("hfjk_packed_elems", _c_ll, [_c_int]),
+    ("hfjk_pack", _c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp]),
+    ("hfjk_generate", _c_int, [_vp, _c_u64, _c_int, _c_int, _c_int, _vp, _vp]),
+    ("hfjk_normal_matrix", _c_int, [_vp, _c_u64, _c_int, _c_int, _c_u32, _vp, _vp]),
+    ("hfjk_split", _c_int, [_vp, _vp, _c_ll, _c_int, _vp, _vp]),
+    ("hfjk_bind", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp]),
+    ("hfjk_workspace_bytes", _c_int, [_vp, _c_int, _c_int, ctypes.POINTER(_c_size)]),
+    ("hfjk_coulomb", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_profile", _c_int, [_vp, _c_int]),
+    ("hfjk_profile_read", _c_int, [_vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), _c_int]),
+]
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load ``libhfjk.so`` once; raise :class:`BackendError` when it is absent."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise BackendError(
+                    f"{LIB_PATH} is not built: run `python -m hafnium_amd._build` "
+                    "(the DF-J/K path has no CPU fallback)"
+                )
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, restype, argtypes in SIGNATURES:
+                fn = getattr(lib, name)
+                fn.restype = restype
+                fn.argtypes = argtypes
+            _lib = lib
+    return _lib
+
+
+def check(status: int) -> None:
+    """Map a status of the C ABI to the package's exceptions."""
+    if status == OK:
+        return
+    message = load().hfjk_last_error().decode(errors="replace")
+    if status == ERR_INPUT:
+        raise InputError(message)
+    if status == ERR_BACKEND:
+        raise BackendError(message)
+    raise BackendError(f"hfjk error {status}: {message}")
+
+
+def precision_code(precision: str | int) -> int:
+    if isinstance(precision, int):
+        if precision not in PRECISIONS.values():
+            raise InputError(f"unknown precision {precision}")
+        return precision
+    try:
+        return PRECISIONS[precision]
+    except KeyError:
+        raise InputError(f"precision is one of {sorted(PRECISIONS)}, got {precision!r}") from None

@@ -1,0 +1,321 @@
+"""Density-fitted Coulomb and exchange contractions on an MI355X.
+
+:class:`MI355XCoulombExchange` satisfies the reference's ``CoulombExchange``
+protocol (hafnium/scf/coulomb_exchange.py:53-71): ``n_ao``, ``coulomb(D)``
+and ``exchange(D)`` on ``[ao, ao]`` arrays. It is the drop-in for
+``DensityFittedCoulombExchange`` (coulomb_exchange.py:124-190) with the
+fitted tensor held in HBM in tiled packed storage and every contraction
+running in the HIP kernels of ``libhfjk.so``. PyTorch owns the buffers and
+supplies the stream; no arithmetic of the path runs in PyTorch or on the
+host.
+
+Besides the protocol it offers ``exchange_occupied(C)``, the occupied-orbital
+factorisation ``K = sum_P (B_P C)(B_P C)^T`` (coulomb_exchange.py:129-133)
+that the package's own SCF driver calls with the occupied orbitals, and
+``precision`` ("fp64", "fp32" or "bf16x3") selecting the arithmetic of the
+contractions.
+
+The RHF assembly of the reference's ``coulomb_matrix`` / ``exchange_matrix``
+(coulomb_exchange.py:407-462: J of the spin-traced density ``2 D_a``, K of
+the stored block ``D_a``) is :func:`rhf_coulomb_exchange`.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Any
+
+import numpy as np
+
+from hafnium_amd import _lib
+from hafnium_amd.errors import BackendError, InputError
+
+__all__ = ["MI355XCoulombExchange", "rhf_coulomb_exchange", "SYMMETRY_TOLERANCE"]
+
+SYMMETRY_TOLERANCE = 1e-12
+"""Largest ``max|B - B^T| / max|B|`` accepted for a fitted tensor."""
+
+
+def _torch():
+    try:
+        import torch
+    except ImportError as error:  # pragma: no cover - torch is part of the image
+        raise BackendError("PyTorch-ROCm is required to own the device buffers") from error
+    return torch
+
+
+class _Context:
+    """One ``hfjk_ctx`` per device, destroyed with the Python object."""
+
+    def __init__(self, device_index: int) -> None:
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(lib.hfjk_create(ctypes.byref(handle), device_index))
+        self.handle = handle
+        self.lib = lib
+
+    def __del__(self) -> None:
+        handle = getattr(self, "handle", None)
+        if handle is not None and handle.value:
+            self.lib.hfjk_destroy(handle)
+            self.handle = None
+
+
+def _stream(torch, device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(tensor.data_ptr())
+
+
+class MI355XCoulombExchange:
+    """Contractions with a fitted tensor ``B[aux, ao, ao]`` resident in HBM.
+
+    Parameters
+    ----------
+    df_tensor
+        ``[aux, ao, ao]`` float64, symmetric in the two AO indices: a torch
+        tensor (on a GPU or the host) or a NumPy array. It is packed into the
+        tiled storage once; the caller's array is not kept.
+    precision
+        Arithmetic of the contractions: "fp64" (default), "fp32" or "bf16x3".
+    device
+        Torch device of the GPU; defaults to the tensor's GPU or ``cuda:0``.
+    """
+
+    def __init__(self, df_tensor: Any, *, precision: str = "fp64", device: Any = None) -> None:
+        torch = _torch()
+        shape = tuple(df_tensor.shape)
+        if len(shape) != 3 or shape[1] != shape[2]:
+            raise InputError("the fitted tensor has shape [aux, ao, ao]")
+        if shape[0] < 1 or shape[1] < 1:
+            raise InputError("the fitted tensor has shape [aux, ao, ao] with aux, ao > 0")
+        if isinstance(df_tensor, np.ndarray):
+            if np.iscomplexobj(df_tensor):
+                raise BackendError("complex classes are not implemented on MI355X")
+            df_tensor = torch.from_numpy(np.ascontiguousarray(df_tensor, dtype=np.float64))
+        if df_tensor.is_complex():
+            raise BackendError("complex classes are not implemented on MI355X")
+        self._device = torch.device(device) if device is not None else (
+            df_tensor.device if df_tensor.is_cuda else torch.device("cuda", 0)
+        )
+        if self._device.type != "cuda":
+            raise BackendError("the DF-J/K path runs on a GPU device only")
+        if not torch.cuda.is_available():
+            raise BackendError("no GPU is visible to PyTorch-ROCm")
+        self._init_common(shape[0], shape[1], precision)
+        full = df_tensor.to(device=self._device, dtype=torch.float64).contiguous()
+        stats = torch.zeros(2, dtype=torch.float64, device=self._device)
+        lib = self._ctx.lib
+        _lib.check(lib.hfjk_pack(self._ctx.handle, _ptr(full), self._naux, self._nao, _ptr(self._b64),
+                                 _ptr(stats), self._stream()))
+        asym, mag = (float(v) for v in stats.cpu())
+        del full
+        if asym > SYMMETRY_TOLERANCE * max(mag, 1e-300):
+            raise InputError(
+                f"the fitted tensor is not symmetric in its AO indices (max |B - B^T| = {asym:.3e})"
+            )
+        self._bind()
+
+    @classmethod
+    def synthetic(cls, seed: int, naux: int, nao: int, *, precision: str = "fp64", device: Any = None,
+                  q0: int = 0) -> "MI355XCoulombExchange":
+        """The seeded synthetic tensor of ``hafnium_amd.synthetic``, generated in HBM.
+
+        Aux functions ``q0 .. q0 + naux - 1`` of the seed's tensor.
+        """
+        torch = _torch()
+        self = cls.__new__(cls)
+        self._device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        if not torch.cuda.is_available():
+            raise BackendError("no GPU is visible to PyTorch-ROCm")
+        if naux < 1 or nao < 1:
+            raise InputError("naux and nao are positive")
+        self._init_common(naux, nao, precision)
+        _lib.check(self._ctx.lib.hfjk_generate(self._ctx.handle, seed, q0, naux, nao, _ptr(self._b64),
+                                               self._stream()))
+        self._bind()
+        return self
+
+    def _init_common(self, naux: int, nao: int, precision: str) -> None:
+        torch = _torch()
+        self._naux = int(naux)
+        self._nao = int(nao)
+        self._precision = _lib.precision_code(precision)
+        self._ctx = _Context(self._device.index if self._device.index is not None else 0)
+        self._packed = int(self._ctx.lib.hfjk_packed_elems(self._nao))
+        self._b64 = torch.empty(self._naux * self._packed, dtype=torch.float64, device=self._device)
+        self._b32 = None
+        self._b16 = None
+        self._workspace = None
+
+    def _stream(self) -> ctypes.c_void_p:
+        return _stream(_torch(), self._device)
+
+    def _bind(self) -> None:
+        lib = self._ctx.lib
+        _lib.check(lib.hfjk_bind(
+            self._ctx.handle, self._naux, self._nao, _ptr(self._b64),
+            _ptr(self._b32) if self._b32 is not None else None,
+            _ptr(self._b16) if self._b16 is not None else None,
+        ))
+
+    def _ensure_copy(self, precision: int) -> None:
+        """Derive the FP32 or BF16 hi/lo copy of B the first time a mode needs it."""
+        torch = _torch()
+        count = self._naux * self._packed
+        lib = self._ctx.lib
+        if precision in (_lib.FP32, _lib.BF16X3) and self._b32 is None:
+            # The exchange MFMA kernels of both mixed modes read the FP32 copy.
+            self._b32 = torch.empty(count, dtype=torch.float32, device=self._device)
+            _lib.check(lib.hfjk_split(self._ctx.handle, _ptr(self._b64), count, _lib.FP32, _ptr(self._b32),
+                                      self._stream()))
+            self._bind()
+        if precision == _lib.BF16X3 and self._b16 is None:
+            self._b16 = torch.empty(2 * count, dtype=torch.int16, device=self._device)
+            _lib.check(lib.hfjk_split(self._ctx.handle, _ptr(self._b64), count, _lib.BF16X3, _ptr(self._b16),
+                                      self._stream()))
+            self._bind()
+
+    def _workspace_for(self, nocc: int, precision: int):
+        torch = _torch()
+        size = ctypes.c_size_t()
+        _lib.check(self._ctx.lib.hfjk_workspace_bytes(self._ctx.handle, nocc, precision, ctypes.byref(size)))
+        need = max(int(size.value), 256)
+        if self._workspace is None or self._workspace.numel() < need:
+            self._workspace = None
+            self._workspace = torch.empty(need + 256, dtype=torch.uint8, device=self._device)
+        base = self._workspace.data_ptr()
+        aligned = (base + 255) & ~255
+        return ctypes.c_void_p(aligned), self._workspace.numel() - (aligned - base)
+
+    # -------------------------------------------------------------- protocol
+
+    @property
+    def n_ao(self) -> int:
+        """Number of atomic orbitals."""
+        return self._nao
+
+    @property
+    def n_aux(self) -> int:
+        """Number of auxiliary functions."""
+        return self._naux
+
+    @property
+    def precision(self) -> str:
+        return {v: k for k, v in _lib.PRECISIONS.items()}[self._precision]
+
+    @precision.setter
+    def precision(self, value: str) -> None:
+        self._precision = _lib.precision_code(value)
+
+    @property
+    def device(self):
+        return self._device
+
+    def _matrix_in(self, array: Any, name: str):
+        torch = _torch()
+        n = self._nao
+        if isinstance(array, np.ndarray):
+            if np.iscomplexobj(array):
+                raise BackendError("complex classes are not implemented on MI355X")
+            array = torch.from_numpy(np.ascontiguousarray(array, dtype=np.float64))
+        if array.is_complex():
+            raise BackendError("complex classes are not implemented on MI355X")
+        if tuple(array.shape) != (n, n):
+            raise InputError(f"the {name} has shape {tuple(array.shape)}, the integrals act on {n} atomic orbitals")
+        return array.to(device=self._device, dtype=torch.float64).contiguous()
+
+    @staticmethod
+    def _out_like(value, template):
+        if isinstance(template, np.ndarray):
+            return value.cpu().numpy()
+        return value
+
+    def coulomb(self, density: Any, *, precision: str | None = None):
+        """``J = sum_P B[P] <B[P], D>`` (coulomb_exchange.py:176-182)."""
+        torch = _torch()
+        prec = self._precision if precision is None else _lib.precision_code(precision)
+        d = self._matrix_in(density, "density")
+        self._ensure_copy(prec)
+        out = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
+        ws, nbytes = self._workspace_for(1, prec)
+        _lib.check(self._ctx.lib.hfjk_coulomb(self._ctx.handle, _ptr(d), _ptr(out), prec, ws, nbytes,
+                                              self._stream()))
+        return self._out_like(out, density)
+
+    def exchange(self, density: Any, *, precision: str | None = None):
+        """``K = sum_P B[P] D B[P]`` (coulomb_exchange.py:184-190) for a symmetric density.
+
+        The density is factorised on the GPU (``D = C_+ C_+^T - C_- C_-^T``)
+        and the occupied-form kernels run on the factors, so indefinite and
+        fractionally occupied densities give the density-form result.
+        """
+        torch = _torch()
+        prec = self._precision if precision is None else _lib.precision_code(precision)
+        d = self._matrix_in(density, "density")
+        self._ensure_copy(prec)
+        out = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
+        ws, nbytes = self._workspace_for(self._nao, prec)
+        _lib.check(self._ctx.lib.hfjk_exchange_density(self._ctx.handle, _ptr(d), _ptr(out), prec, ws, nbytes,
+                                                       self._stream()))
+        return self._out_like(out, density)
+
+    def exchange_occupied(self, orbitals: Any, signs: Any = None, *, precision: str | None = None):
+        """``K = sum_P (B[P] C) diag(signs) (B[P] C)^T`` for ``orbitals`` ``[ao, occ]``."""
+        torch = _torch()
+        prec = self._precision if precision is None else _lib.precision_code(precision)
+        template = orbitals
+        if isinstance(orbitals, np.ndarray):
+            orbitals = torch.from_numpy(np.ascontiguousarray(orbitals, dtype=np.float64))
+        if orbitals.is_complex():
+            raise BackendError("complex classes are not implemented on MI355X")
+        if orbitals.ndim != 2 or orbitals.shape[0] != self._nao:
+            raise InputError(f"the orbitals have shape {tuple(orbitals.shape)}, expected [{self._nao}, occ]")
+        nocc = int(orbitals.shape[1])
+        c = orbitals.to(device=self._device, dtype=torch.float64).contiguous()
+        s = None
+        if signs is not None:
+            s = torch.as_tensor(np.asarray(signs, dtype=np.float64) if isinstance(signs, (list, np.ndarray))
+                                else signs).to(device=self._device, dtype=torch.float64).contiguous()
+            if s.shape != (nocc,):
+                raise InputError("one sign per orbital column")
+        self._ensure_copy(prec)
+        out = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
+        ws, nbytes = self._workspace_for(max(nocc, 1), prec)
+        _lib.check(self._ctx.lib.hfjk_exchange_occ(
+            self._ctx.handle, _ptr(c), nocc, _ptr(s) if s is not None else None, _ptr(out), prec, ws, nbytes,
+            self._stream()))
+        return self._out_like(out, template)
+
+    # --------------------------------------------------------------- timing
+
+    def profile(self, enable: bool = True) -> None:
+        """Record HIP events around every kernel of later calls."""
+        _lib.check(self._ctx.lib.hfjk_profile(self._ctx.handle, 1 if enable else 0))
+
+    def stage_times(self) -> list[tuple[str, float]]:
+        """``(stage, milliseconds)`` of the last call, read from its HIP events."""
+        names = (ctypes.c_char_p * 64)()
+        ms = (ctypes.c_float * 64)()
+        count = self._ctx.lib.hfjk_profile_read(self._ctx.handle, names, ms, 64)
+        if count < 0:
+            _lib.check(-count)
+        return [(names[i].decode(), float(ms[i])) for i in range(min(count, 64))]
+
+    def packed_tensor(self, precision: str = "fp64"):
+        """The resident tiled packed tensor copy (for tests and diagnostics)."""
+        prec = _lib.precision_code(precision)
+        self._ensure_copy(prec)
+        return {_lib.FP64: self._b64, _lib.FP32: self._b32, _lib.BF16X3: self._b16}[prec]
+
+
+def rhf_coulomb_exchange(two_body, density_alpha):
+    """``(J[2 D_a], K[D_a])`` of a real restricted density block.
+
+    The molecular RHF branch of the reference's ``coulomb_matrix`` (J of the
+    spin trace ``2 D_a``, coulomb_exchange.py:407-427) and ``exchange_matrix``
+    (K of the stored block, one spin sub-block: :444-462, :504-509).
+    """
+    return two_body.coulomb(2.0 * density_alpha), two_body.exchange(density_alpha)

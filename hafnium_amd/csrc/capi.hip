@@ -1,0 +1,538 @@
+// C ABI of the DF-J/K library: argument checks, workspace carving and the
+// launch sequences of J and K. No exception crosses this boundary; every
+// entry point returns a status and leaves a thread-local message.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "hfjk_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Failure {
+  int code;
+  std::string message;
+};
+
+[[noreturn]] void fail(int code, const std::string& message) { throw Failure{code, message}; }
+
+void check_hip(hipError_t err, const char* what) {
+  if (err != hipSuccess) fail(HFJK_ERR_HIP, std::string(what) + ": " + hipGetErrorString(err));
+}
+
+template <typename F>
+int guarded(F&& body) {
+  try {
+    body();
+    g_last_error.clear();
+    return HFJK_OK;
+  } catch (const Failure& f) {
+    g_last_error = f.message;
+    return f.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return HFJK_ERR_HIP;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return HFJK_ERR_HIP;
+  } catch (...) {
+    g_last_error = "unknown failure";
+    return HFJK_ERR_HIP;
+  }
+}
+
+const char* precision_name(int precision) {
+  switch (precision) {
+    case HFJK_FP64: return "fp64";
+    case HFJK_FP32: return "fp32";
+    case HFJK_BF16X3: return "bf16x3";
+    default: return "unknown";
+  }
+}
+
+void check_precision(int precision) {
+  if (precision != HFJK_FP64 && precision != HFJK_FP32 && precision != HFJK_BF16X3)
+    fail(HFJK_ERR_INPUT, "precision must be HFJK_FP64, HFJK_FP32 or HFJK_BF16X3");
+}
+
+void check_ctx(const hfjk_ctx* ctx) {
+  if (ctx == nullptr) fail(HFJK_ERR_INPUT, "null context");
+}
+
+void check_bound(const hfjk_ctx* ctx, int precision) {
+  if (ctx->naux <= 0 || ctx->nao <= 0) fail(HFJK_ERR_INPUT, "no fitted tensor is bound (call hfjk_bind)");
+  const void* b = precision == HFJK_FP64 ? (const void*)ctx->b64
+                  : precision == HFJK_FP32 ? (const void*)ctx->b32
+                                           : (const void*)ctx->b16;
+  if (b == nullptr)
+    fail(HFJK_ERR_BACKEND, std::string("no ") + precision_name(precision) + " copy of the fitted tensor is bound");
+}
+
+const void* bound_tensor(const hfjk_ctx* ctx, int precision) {
+  return precision == HFJK_FP64 ? (const void*)ctx->b64
+         : precision == HFJK_FP32 ? (const void*)ctx->b32
+                                  : (const void*)ctx->b16;
+}
+
+size_t elem_size(int precision) { return precision == HFJK_FP64 ? 8 : 4; }
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Split counts (deterministic functions of the shapes).
+int coulomb_splits(long long ntiles, int naux) {
+  long long s = (2048 + ntiles - 1) / ntiles;
+  long long cap = std::max(1, naux / 4);
+  return (int)std::max(1LL, std::min(s, cap));
+}
+
+int syrk_splits(long long npairs, int naux) {
+  long long s = (8192 + npairs - 1) / npairs;
+  return (int)std::max(1LL, std::min(s, (long long)naux));
+}
+
+// Exchange operands in BF16X3 are not implemented by a dedicated kernel yet;
+// the exchange in that precision runs on the FP32 tensor copy through the
+// FP32 MFMA path only when that copy is bound (see exchange_precision).
+
+struct Layout {
+  // density factorisation region
+  size_t dens_a, dens_w, dens_e, dens_info, dens_stats, dens_c, dens_signs, dens_end;
+  // coulomb region (shares the base of the exchange region)
+  size_t cw, cgp, cg, cjp, c_end;
+  // exchange region
+  size_t xc, xs, xx, xp, x_end;
+  size_t total;
+  int nsplit_j, nsplit_k, ldo, xrows, npairs;
+  long long p, ntiles;
+};
+
+Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
+  Layout L{};
+  const int n = ctx->nao, naux = ctx->naux;
+  const size_t es = elem_size(precision);
+  L.p = hfjk::packed_elems(n);
+  L.ntiles = hfjk::n_tile_pairs(n);
+  L.ldo = hfjk::occ_ld(std::max(nocc, 1));
+  L.xrows = hfjk::x_rows(n);
+  const int nb = L.xrows / hfjk::kBlock;
+  L.npairs = nb * (nb + 1) / 2;
+  L.nsplit_j = coulomb_splits(L.ntiles, naux);
+  L.nsplit_k = syrk_splits(L.npairs, naux);
+  size_t off = 0;
+  const size_t nn = (size_t)n * n;
+  L.dens_a = off; off = align_up(off + nn * 8);
+  L.dens_w = off; off = align_up(off + (size_t)n * 8);
+  L.dens_e = off; off = align_up(off + (size_t)n * 8);
+  L.dens_info = off; off = align_up(off + 8);
+  L.dens_stats = off; off = align_up(off + 2 * 8);
+  L.dens_c = off; off = align_up(off + nn * 8);
+  L.dens_signs = off; off = align_up(off + (size_t)n * 8);
+  L.dens_end = off;
+  const size_t base = off;
+  // coulomb
+  off = base;
+  L.cw = off; off = align_up(off + (size_t)L.p * 8);
+  L.cgp = off; off = align_up(off + (size_t)L.ntiles * naux * 8);
+  L.cg = off; off = align_up(off + (size_t)naux * 8);
+  L.cjp = off; off = align_up(off + (size_t)L.nsplit_j * L.p * 8);
+  L.c_end = off;
+  // exchange
+  off = base;
+  const size_t npad = (size_t)hfjk::n_tiles(n) * hfjk::kTile;
+  L.xc = off; off = align_up(off + npad * L.ldo * es);
+  L.xs = off; off = align_up(off + (size_t)L.ldo * 8);
+  L.xx = off; off = align_up(off + (size_t)naux * L.xrows * L.ldo * es);
+  L.xp = off; off = align_up(off + (size_t)L.nsplit_k * L.npairs * hfjk::kBlock * hfjk::kBlock * es);
+  L.x_end = off;
+  L.total = std::max(L.c_end, L.x_end);
+  return L;
+}
+
+void check_workspace(const Layout& L, void* ws, size_t bytes) {
+  if (ws == nullptr && L.total > 0) fail(HFJK_ERR_WORKSPACE, "null workspace");
+  if (bytes < L.total)
+    fail(HFJK_ERR_WORKSPACE, "workspace of " + std::to_string(bytes) + " bytes is smaller than the " +
+                                 std::to_string(L.total) + " bytes hfjk_workspace_bytes reports");
+  if (((uintptr_t)ws & 255) != 0) fail(HFJK_ERR_WORKSPACE, "workspace must be 256-byte aligned");
+}
+
+template <typename T>
+T* at(void* ws, size_t off) {
+  return (T*)((char*)ws + off);
+}
+
+// RAII stage timer: records an event pair around the launches in its scope.
+struct Stage {
+  hfjk_ctx* ctx;
+  hipStream_t s;
+  hfjk::StageTimer* timer = nullptr;
+  Stage(hfjk_ctx* c, hipStream_t stream, const char* name) : ctx(c), s(stream) {
+    if (!ctx->profile) return;
+    if (ctx->n_stages == ctx->stages.size()) {
+      hfjk::StageTimer t;
+      check_hip(hipEventCreate(&t.start), "hipEventCreate");
+      check_hip(hipEventCreate(&t.stop), "hipEventCreate");
+      ctx->stages.push_back(t);
+    }
+    timer = &ctx->stages[ctx->n_stages++];
+    timer->name = name;
+    check_hip(hipEventRecord(timer->start, s), "hipEventRecord");
+  }
+  ~Stage() {
+    // A failed record leaves the stage unreadable; hfjk_profile_read reports it.
+    if (timer != nullptr) (void)hipEventRecord(timer->stop, s);
+  }
+};
+
+void begin_call(hfjk_ctx* ctx) {
+  ctx->n_stages = 0;
+  check_hip(hipSetDevice(ctx->device), "hipSetDevice");
+}
+
+void run_coulomb(hfjk_ctx* ctx, const double* density, double* out, int precision, void* ws, const Layout& L,
+                 hipStream_t s) {
+  const void* b = bound_tensor(ctx, precision);
+  double* w = at<double>(ws, L.cw);
+  double* gp = at<double>(ws, L.cgp);
+  double* g = at<double>(ws, L.cg);
+  double* jp = at<double>(ws, L.cjp);
+  {
+    Stage st(ctx, s, "coulomb_weights");
+    check_hip(hfjk::launch_weights(density, ctx->nao, w, s), "weights launch");
+  }
+  {
+    Stage st(ctx, s, "coulomb_gamma");
+    check_hip(hfjk::launch_gamma(b, precision, L.p, ctx->naux, (int)L.ntiles, w, gp, s), "gamma launch");
+    check_hip(hfjk::launch_gamma_reduce(gp, (int)L.ntiles, ctx->naux, g, s), "gamma reduce launch");
+  }
+  {
+    Stage st(ctx, s, "coulomb_accumulate");
+    check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, g, jp, s),
+              "coulomb accumulate launch");
+  }
+  {
+    Stage st(ctx, s, "coulomb_unpack");
+    check_hip(hfjk::launch_coulomb_unpack(jp, L.nsplit_j, L.p, ctx->nao, out, s), "coulomb unpack launch");
+  }
+}
+
+void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double* signs, double* out, int precision,
+                  void* ws, const Layout& L, hipStream_t s) {
+  // The exchange kernels run in FP64 or FP32 MFMA; BF16X3 operands use the
+  // FP32 path on the FP32 copy when no split kernel exists for them.
+  const int kprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;
+  const void* b = bound_tensor(ctx, kprec);
+  if (b == nullptr)
+    fail(HFJK_ERR_BACKEND, std::string("the ") + precision_name(precision) +
+                               " exchange needs the fp32 copy of the fitted tensor");
+  void* cp = at<void>(ws, L.xc);
+  double* sg = at<double>(ws, L.xs);
+  void* x = at<void>(ws, L.xx);
+  void* part = at<void>(ws, L.xp);
+  {
+    Stage st(ctx, s, "exchange_prepare");
+    check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, kprec, cp, s), "orbitals launch");
+    check_hip(hfjk::launch_occ_signs(signs, nocc, L.ldo, sg, s), "signs launch");
+  }
+  {
+    Stage st(ctx, s, "exchange_half_transform");
+    check_hip(hfjk::launch_half_transform(b, kprec, L.p, ctx->naux, ctx->nao, cp, L.ldo, x, s),
+              "half transform launch");
+  }
+  {
+    Stage st(ctx, s, "exchange_syrk");
+    check_hip(hfjk::launch_syrk(x, kprec, ctx->naux, ctx->nao, L.ldo, sg, L.nsplit_k, part, s), "syrk launch");
+  }
+  {
+    Stage st(ctx, s, "exchange_reduce");
+    check_hip(hfjk::launch_syrk_reduce(part, kprec, L.nsplit_k, ctx->nao, out, s), "syrk reduce launch");
+  }
+}
+
+rocblas_handle blas_handle(hfjk_ctx* ctx) {
+  if (ctx->blas == nullptr) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) fail(HFJK_ERR_SOLVER, "rocblas_create_handle failed");
+    ctx->blas = h;
+  }
+  return (rocblas_handle)ctx->blas;
+}
+
+__global__ void density_stats_kernel(const double* __restrict__ d, int n, unsigned long long* __restrict__ stats) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  double asym = 0.0, mag = 0.0;
+  if (idx < (long long)n * n) {
+    const int r = (int)(idx / n), c = (int)(idx % n);
+    asym = fabs(d[idx] - d[(long long)c * n + r]);
+    mag = fabs(d[idx]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double a = __shfl_xor(asym, off);
+    const double m = __shfl_xor(mag, off);
+    asym = a > asym ? a : asym;
+    mag = m > mag ? m : mag;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (asym > 0.0) atomicMax(&stats[0], (unsigned long long)__double_as_longlong(asym));
+    if (mag > 0.0) atomicMax(&stats[1], (unsigned long long)__double_as_longlong(mag));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfjk_version(void) { return 100; }
+
+const char* hfjk_last_error(void) { return g_last_error.c_str(); }
+
+int hfjk_create(hfjk_ctx** out, int device) {
+  return guarded([&] {
+    if (out == nullptr) fail(HFJK_ERR_INPUT, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    check_hip(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count)
+      fail(HFJK_ERR_INPUT, "device " + std::to_string(device) + " out of range (" + std::to_string(count) + ")");
+    hipDeviceProp_t prop;
+    check_hip(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      fail(HFJK_ERR_BACKEND, std::string("the DF-J/K kernels are built for gfx950 (MI355X), not ") + prop.gcnArchName);
+    hfjk_ctx* ctx = new hfjk_ctx();
+    ctx->device = device;
+    *out = ctx;
+  });
+}
+
+int hfjk_destroy(hfjk_ctx* ctx) {
+  return guarded([&] {
+    if (ctx == nullptr) return;
+    (void)hipSetDevice(ctx->device);
+    for (auto& t : ctx->stages) {
+      if (t.start) (void)hipEventDestroy(t.start);
+      if (t.stop) (void)hipEventDestroy(t.stop);
+    }
+    if (ctx->blas != nullptr) rocblas_destroy_handle((rocblas_handle)ctx->blas);
+    delete ctx;
+  });
+}
+
+long long hfjk_packed_elems(int nao) { return nao > 0 ? hfjk::packed_elems(nao) : 0; }
+
+int hfjk_pack(hfjk_ctx* ctx, const double* full, int naux, int nao, double* packed, double* stats, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (naux <= 0 || nao <= 0) fail(HFJK_ERR_INPUT, "the fitted tensor has shape [aux, ao, ao] with aux, ao > 0");
+    if (full == nullptr || packed == nullptr) fail(HFJK_ERR_INPUT, "null tensor pointer");
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    Stage st(ctx, s, "pack");
+    if (stats != nullptr) check_hip(hipMemsetAsync(stats, 0, 2 * sizeof(double), s), "hipMemsetAsync");
+    check_hip(hfjk::launch_pack(full, naux, nao, packed, (unsigned long long*)stats, s), "pack launch");
+  });
+}
+
+int hfjk_generate(hfjk_ctx* ctx, uint64_t seed, int q0, int naux, int nao, double* packed, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (naux <= 0 || nao <= 0 || q0 < 0) fail(HFJK_ERR_INPUT, "naux, nao > 0 and q0 >= 0 are required");
+    if (packed == nullptr) fail(HFJK_ERR_INPUT, "null output pointer");
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    Stage st(ctx, s, "generate");
+    check_hip(hfjk::launch_generate(seed, q0, naux, nao, packed, s), "generate launch");
+  });
+}
+
+int hfjk_normal_matrix(hfjk_ctx* ctx, uint64_t seed, int rows, int cols, uint32_t tag, double* out, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (rows < 0 || cols < 0) fail(HFJK_ERR_INPUT, "negative shape");
+    if (out == nullptr && rows * (long long)cols > 0) fail(HFJK_ERR_INPUT, "null output pointer");
+    begin_call(ctx);
+    check_hip(hfjk::launch_normal_matrix(seed, rows, cols, tag, out, (hipStream_t)stream), "normal launch");
+  });
+}
+
+int hfjk_split(hfjk_ctx* ctx, const double* packed, long long count, int precision, void* out, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (precision != HFJK_FP32 && precision != HFJK_BF16X3)
+      fail(HFJK_ERR_INPUT, "hfjk_split makes HFJK_FP32 or HFJK_BF16X3 copies");
+    if (count < 0) fail(HFJK_ERR_INPUT, "negative count");
+    if (count > 0 && (packed == nullptr || out == nullptr)) fail(HFJK_ERR_INPUT, "null pointer");
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    Stage st(ctx, s, "split");
+    check_hip(hfjk::launch_split(packed, count, precision, out, s), "split launch");
+  });
+}
+
+int hfjk_bind(hfjk_ctx* ctx, int naux, int nao, const double* b64, const float* b32, const uint16_t* b16) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (naux <= 0 || nao <= 0) fail(HFJK_ERR_INPUT, "the fitted tensor has shape [aux, ao, ao] with aux, ao > 0");
+    if (b64 == nullptr && b32 == nullptr && b16 == nullptr) fail(HFJK_ERR_INPUT, "no tensor copy given");
+    if (nao > 46000) fail(HFJK_ERR_INPUT, "nao beyond the supported 46000");
+    ctx->naux = naux;
+    ctx->nao = nao;
+    ctx->b64 = b64;
+    ctx->b32 = b32;
+    ctx->b16 = b16;
+  });
+}
+
+int hfjk_workspace_bytes(hfjk_ctx* ctx, int nocc, int precision, size_t* bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_precision(precision);
+    if (bytes == nullptr) fail(HFJK_ERR_INPUT, "null output pointer");
+    if (ctx->naux <= 0) fail(HFJK_ERR_INPUT, "no fitted tensor is bound (call hfjk_bind)");
+    if (nocc < 0) fail(HFJK_ERR_INPUT, "negative nocc");
+    const int kprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;
+    *bytes = make_layout(ctx, std::max(nocc, 1), kprec).total;
+  });
+}
+
+int hfjk_coulomb(hfjk_ctx* ctx, const double* density, double* coulomb, int precision, void* workspace,
+                 size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_precision(precision);
+    check_bound(ctx, precision);
+    if (density == nullptr || coulomb == nullptr) fail(HFJK_ERR_INPUT, "null matrix pointer");
+    const Layout L = make_layout(ctx, 1, precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32);
+    if (workspace_bytes < L.c_end || workspace == nullptr)
+      fail(HFJK_ERR_WORKSPACE, "workspace smaller than the coulomb needs (" + std::to_string(L.c_end) + " bytes)");
+    if (((uintptr_t)workspace & 255) != 0) fail(HFJK_ERR_WORKSPACE, "workspace must be 256-byte aligned");
+    begin_call(ctx);
+    run_coulomb(ctx, density, coulomb, precision, workspace, L, (hipStream_t)stream);
+  });
+}
+
+int hfjk_exchange_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, const double* signs, double* exchange,
+                      int precision, void* workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_precision(precision);
+    if (ctx->naux <= 0) fail(HFJK_ERR_INPUT, "no fitted tensor is bound (call hfjk_bind)");
+    if (nocc < 0) fail(HFJK_ERR_INPUT, "negative nocc");
+    if (exchange == nullptr) fail(HFJK_ERR_INPUT, "null output pointer");
+    if (nocc > 0 && orbitals == nullptr) fail(HFJK_ERR_INPUT, "null orbitals pointer");
+    const int kprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;
+    const Layout L = make_layout(ctx, std::max(nocc, 1), kprec);
+    if (workspace_bytes < L.x_end || workspace == nullptr)
+      fail(HFJK_ERR_WORKSPACE, "workspace smaller than the exchange needs (" + std::to_string(L.x_end) + " bytes)");
+    if (((uintptr_t)workspace & 255) != 0) fail(HFJK_ERR_WORKSPACE, "workspace must be 256-byte aligned");
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    if (nocc == 0) {
+      check_hip(hipMemsetAsync(exchange, 0, sizeof(double) * ctx->nao * ctx->nao, s), "hipMemsetAsync");
+      return;
+    }
+    run_exchange(ctx, orbitals, nocc, signs, exchange, precision, workspace, L, s);
+  });
+}
+
+int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange, int precision, void* workspace,
+                          size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_precision(precision);
+    if (ctx->naux <= 0) fail(HFJK_ERR_INPUT, "no fitted tensor is bound (call hfjk_bind)");
+    if (density == nullptr || exchange == nullptr) fail(HFJK_ERR_INPUT, "null matrix pointer");
+    const int n = ctx->nao;
+    const int kprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;
+    const Layout L = make_layout(ctx, n, kprec);
+    check_workspace(L, workspace, workspace_bytes);
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    double* a = at<double>(workspace, L.dens_a);
+    double* w = at<double>(workspace, L.dens_w);
+    double* e = at<double>(workspace, L.dens_e);
+    rocblas_int* info = at<rocblas_int>(workspace, L.dens_info);
+    unsigned long long* stats = at<unsigned long long>(workspace, L.dens_stats);
+    double* cf = at<double>(workspace, L.dens_c);
+    double* sg = at<double>(workspace, L.dens_signs);
+    {
+      Stage st(ctx, s, "density_eigh");
+      check_hip(hipMemsetAsync(stats, 0, 2 * sizeof(unsigned long long), s), "hipMemsetAsync");
+      const long long nn = (long long)n * n;
+      hipLaunchKernelGGL(density_stats_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, density, n, stats);
+      check_hip(hipGetLastError(), "density stats launch");
+      check_hip(hipMemcpyAsync(a, density, sizeof(double) * nn, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+      rocblas_handle h = blas_handle(ctx);
+      if (rocblas_set_stream(h, s) != rocblas_status_success) fail(HFJK_ERR_SOLVER, "rocblas_set_stream failed");
+      // The density is symmetric, so its row-major storage is its
+      // column-major storage; eigenvectors come back column-major.
+      if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, e, info) !=
+          rocblas_status_success)
+        fail(HFJK_ERR_SOLVER, "rocsolver_dsyevd failed to launch");
+    }
+    std::vector<double> vals(n);
+    unsigned long long host_stats[2];
+    rocblas_int host_info = 0;
+    check_hip(hipMemcpyAsync(vals.data(), w, sizeof(double) * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    check_hip(hipMemcpyAsync(host_stats, stats, sizeof(host_stats), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    check_hip(hipMemcpyAsync(&host_info, info, sizeof(host_info), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    double asym, mag;
+    std::memcpy(&asym, &host_stats[0], sizeof(double));
+    std::memcpy(&mag, &host_stats[1], sizeof(double));
+    if (!(asym <= 1e-12 * (mag > 0 ? mag : 1.0)))
+      fail(HFJK_ERR_INPUT, "the density is not symmetric (max |D - D^T| = " + std::to_string(asym) + ")");
+    if (host_info != 0) fail(HFJK_ERR_SOLVER, "rocsolver_dsyevd did not converge (info " + std::to_string(host_info) + ")");
+    double top = 0.0;
+    for (double v : vals) top = std::max(top, std::fabs(v));
+    const double cut = 1e-12 * top;
+    int nneg = 0, npos = 0;
+    for (int j = 0; j < n && vals[j] < -cut; ++j) ++nneg;
+    for (int j = n - 1; j >= nneg && vals[j] > cut; --j) ++npos;
+    const int r = nneg + npos;
+    if (r == 0) {
+      check_hip(hipMemsetAsync(exchange, 0, sizeof(double) * n * n, s), "hipMemsetAsync");
+      return;
+    }
+    {
+      Stage st(ctx, s, "density_factor");
+      check_hip(hfjk::launch_split_factor(a, w, n, nneg, npos, cf, sg, s), "split factor launch");
+    }
+    run_exchange(ctx, cf, r, sg, exchange, precision, workspace, make_layout(ctx, r, kprec), s);
+  });
+}
+
+int hfjk_profile(hfjk_ctx* ctx, int enable) {
+  return guarded([&] {
+    check_ctx(ctx);
+    ctx->profile = enable != 0;
+  });
+}
+
+int hfjk_profile_read(hfjk_ctx* ctx, const char** names, float* ms, int max) {
+  int count = 0;
+  const int status = guarded([&] {
+    check_ctx(ctx);
+    for (size_t i = 0; i < ctx->n_stages; ++i) {
+      auto& t = ctx->stages[i];
+      check_hip(hipEventSynchronize(t.stop), "hipEventSynchronize");
+      if ((int)i < max) {
+        float v = 0.f;
+        check_hip(hipEventElapsedTime(&v, t.start, t.stop), "hipEventElapsedTime");
+        if (names) names[i] = t.name.c_str();
+        if (ms) ms[i] = v;
+      }
+    }
+    count = (int)ctx->n_stages;
+  });
+  return status == HFJK_OK ? count : -status;
+}
+
+}  // extern "C"

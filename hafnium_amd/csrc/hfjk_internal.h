@@ -1,0 +1,102 @@
+// Internal declarations of the MI355X DF-J/K library (gfx950, CDNA4).
+//
+// Storage of the fitted tensor B[aux, ao, ao] (symmetric in the two AO
+// indices): each B[P] is cut into 32 x 32 tiles, zero-padded past nao; only
+// the tiles (r, c) with r >= c are kept, at tile index r (r + 1) / 2 + c,
+// each row-major and complete (diagonal tiles hold both triangles).
+// One aux function therefore spans packed_elems(nao) = nt (nt + 1) / 2 * 1024
+// elements, nt = ceil(nao / 32). The FP64 copy is the master; the FP32 copy
+// and the BF16 hi/lo planes (hi = bf16(fp32(b)), lo = bf16(fp32(b) - hi))
+// are derived from it for the mixed-precision modes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hfjk.h"
+
+namespace hfjk {
+
+constexpr int kTile = 32;        // layout tile edge
+constexpr int kTileElems = kTile * kTile;
+constexpr int kBlock = 64;       // row block of the exchange kernels
+constexpr int kThreads = 256;    // threads per workgroup of every kernel
+
+inline int n_tiles(int nao) { return (nao + kTile - 1) / kTile; }
+inline long long n_tile_pairs(int nao) {
+  long long nt = n_tiles(nao);
+  return nt * (nt + 1) / 2;
+}
+inline long long packed_elems(int nao) { return n_tile_pairs(nao) * kTileElems; }
+// Rows of the half-transformed X per aux function: a multiple of kBlock.
+inline int x_rows(int nao) { return ((nao + kBlock - 1) / kBlock) * kBlock; }
+// Leading dimension of C and X: nocc rounded up to 32.
+inline int occ_ld(int nocc) { return ((nocc + 31) / 32) * 32; }
+
+struct Error {
+  int code;
+  std::string message;
+};
+
+struct StageTimer {
+  std::string name;
+  hipEvent_t start = nullptr;
+  hipEvent_t stop = nullptr;
+};
+
+}  // namespace hfjk
+
+struct hfjk_ctx {
+  int device = 0;
+  // Bound tensor (borrowed device pointers).
+  int naux = 0;
+  int nao = 0;
+  const double* b64 = nullptr;
+  const float* b32 = nullptr;
+  const uint16_t* b16 = nullptr;  // hi plane [naux][P], then lo plane [naux][P]
+  // Stage timing of the last call when profiling is on.
+  bool profile = false;
+  std::vector<hfjk::StageTimer> stages;
+  size_t n_stages = 0;
+  // rocSOLVER / rocBLAS handle for the density factorisation (lazy).
+  void* blas = nullptr;
+};
+
+namespace hfjk {
+
+// Kernel launchers (gen.hip). Each returns a hipError_t.
+hipError_t launch_generate(uint64_t seed, int q0, int naux, int nao, double* out, hipStream_t s);
+hipError_t launch_pack(const double* full, int naux, int nao, double* out,
+                       unsigned long long* asym_bits, hipStream_t s);
+hipError_t launch_split(const double* in, long long count, int precision, void* out, hipStream_t s);
+hipError_t launch_normal_matrix(uint64_t seed, int rows, int cols, uint32_t tag, double* out, hipStream_t s);
+
+// Coulomb (jk.hip).
+hipError_t launch_weights(const double* density, int nao, double* w, hipStream_t s);
+hipError_t launch_gamma(const void* b, int precision, long long p, int naux, int ntiles_total,
+                        const double* w, double* partial, hipStream_t s);
+hipError_t launch_gamma_reduce(const double* partial, int nchunks, int naux, double* gamma, hipStream_t s);
+hipError_t launch_coulomb_accumulate(const void* b, int precision, long long p, int naux, int nchunks,
+                                     int nsplit, const double* gamma, double* partial, hipStream_t s);
+hipError_t launch_coulomb_unpack(const double* partial, int nsplit, long long p, int nao,
+                                 double* out, hipStream_t s);
+
+// Exchange (jk.hip).
+hipError_t launch_orbitals(const double* c, int nao, int nocc, int precision, void* out, hipStream_t s);
+hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao,
+                                 const void* c, int ldo, void* x, hipStream_t s);
+hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
+                       int nsplit, void* partial, hipStream_t s);
+hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, int nao, double* out,
+                              hipStream_t s);
+hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* signs_out, hipStream_t s);
+
+// Density factorisation helpers (jk.hip).
+hipError_t launch_split_factor(const double* vecs, const double* vals, int nao, int first, int count,
+                               double* c, double* signs, hipStream_t s);
+
+}  // namespace hfjk

@@ -1,0 +1,149 @@
+/*
+ * hfjk.h — C ABI of the MI355X (gfx950) density-fitted Coulomb/exchange build.
+ *
+ * Replaces the J/K contractions of the reference's CoulombExchange protocol
+ * (hafnium/scf/coulomb_exchange.py:53-71) as implemented by
+ * DensityFittedCoulombExchange (hafnium/scf/coulomb_exchange.py:124-190):
+ *   coulomb(D)  J = sum_P B[P] <B[P], D>            (:176-182)
+ *   exchange(D) K = sum_P B[P] D B[P]               (:184-190)
+ * and the occupied-orbital factorisation the reference lists as a later
+ * optimisation behind the same interface (:129-133):
+ *   K = sum_P (B[P] C)(B[P] C)^T                     for D = C C^T.
+ *
+ * Conventions
+ *  - Every function returns an int status: HFJK_OK (0) or an HFJK_ERR_*
+ *    code; nothing throws across the ABI (the reference's rule for its own
+ *    native layer: crates/hafnium-core/src/lib.rs:22). hfjk_last_error()
+ *    returns the message of the calling thread's last failure.
+ *  - All array pointers are DEVICE pointers, borrowed for the duration of the
+ *    call (or, for hfjk_bind, until the next bind / destroy); the caller owns
+ *    every buffer. Matrices are row-major float64 unless stated otherwise.
+ *  - `stream` is a hipStream_t (NULL = the default stream). Calls enqueue
+ *    work and return without synchronising unless stated otherwise.
+ *  - Inputs are never modified; outputs are fully overwritten.
+ *
+ * Storage of B ("tiled packed"): each B[P] is cut into 32 x 32 tiles,
+ * zero-padded past nao; the tiles (r, c) with r >= c are stored at tile index
+ * r (r + 1) / 2 + c, each row-major and complete. hfjk_packed_elems(nao)
+ * elements per aux function. B must be symmetric in its two AO indices, as
+ * every fitted tensor is ((P|uv) = (P|vu), CONVENTIONS.md:208-212 of the
+ * reference); hfjk_pack measures the asymmetry of a full tensor.
+ */
+#ifndef HFJK_H
+#define HFJK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define HFJK_API __attribute__((visibility("default")))
+#else
+#define HFJK_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFJK_OK 0
+#define HFJK_ERR_INPUT 1     /* shape/argument error (the reference's InputError) */
+#define HFJK_ERR_BACKEND 2   /* unsupported on this device or not bound (BackendError) */
+#define HFJK_ERR_HIP 3       /* HIP runtime or launch failure */
+#define HFJK_ERR_WORKSPACE 4 /* workspace smaller than hfjk_workspace_bytes */
+#define HFJK_ERR_SOLVER 5    /* rocSOLVER failure in the density factorisation */
+
+/* Arithmetic of the contractions. FP64 everywhere; FP32 products on MFMA
+ * with FP32 accumulation inside a tile and FP64 across tiles; BF16X3 = each
+ * FP32 operand split into bf16 hi + lo, hi*hi + hi*lo + lo*hi on MFMA with
+ * FP32 accumulation, FP64 across tiles. */
+#define HFJK_FP64 0
+#define HFJK_FP32 1
+#define HFJK_BF16X3 2
+
+typedef struct hfjk_ctx hfjk_ctx;
+
+/* Library version (major*10000 + minor*100 + patch). */
+HFJK_API int hfjk_version(void);
+
+/* Message of the calling thread's last failing call ("" if none). */
+HFJK_API const char *hfjk_last_error(void);
+
+/* One context per device. No counterpart in the reference (its builder is a
+ * frozen dataclass holding B: coulomb_exchange.py:124-145). */
+HFJK_API int hfjk_create(hfjk_ctx **ctx, int device);
+HFJK_API int hfjk_destroy(hfjk_ctx *ctx);
+
+/* Elements per aux function in the tiled packed storage. */
+HFJK_API long long hfjk_packed_elems(int nao);
+
+/* Full [naux][nao][nao] float64 -> tiled packed [naux][packed_elems].
+ * stats (device, two float64, may be NULL) receives max |B[P,u,v] -
+ * B[P,v,u]| and max |B[P,u,v]| over the tensor. Stands in for the construction-time checks of
+ * DensityFittedCoulombExchange.__post_init__ (coulomb_exchange.py:142-145). */
+HFJK_API int hfjk_pack(hfjk_ctx *ctx, const double *full, int naux, int nao, double *packed,
+              double *stats, void *stream);
+
+/* Seeded synthetic fitted tensor: aux functions q0 .. q0+naux-1 of
+ * B[q, m, n] = z(m, n, q) exp(-|m - n| / 20), z ~ N(0, 1) from Philox4x32-10
+ * keyed by `seed` (the generator of hafnium_amd/synthetic.py, bit for bit),
+ * written in tiled packed storage. No counterpart in the reference. */
+HFJK_API int hfjk_generate(hfjk_ctx *ctx, uint64_t seed, int q0, int naux, int nao, double *packed,
+                  void *stream);
+
+/* Seeded N(0, 1) matrix [rows][cols] with Philox counter (r, c, 0xFFFFFFFF,
+ * tag); the Gaussian matrix of the synthetic occupied orbitals. */
+HFJK_API int hfjk_normal_matrix(hfjk_ctx *ctx, uint64_t seed, int rows, int cols, uint32_t tag,
+                       double *out, void *stream);
+
+/* Derived copies of a packed FP64 tensor of `count` elements:
+ * precision HFJK_FP32 -> float[count]; HFJK_BF16X3 -> uint16 bf16 hi
+ * plane [count] followed by lo plane [count]. */
+HFJK_API int hfjk_split(hfjk_ctx *ctx, const double *packed, long long count, int precision, void *out,
+               void *stream);
+
+/* Borrow the tiled packed tensor(s) for later calls. Any of b64, b32, b16
+ * may be NULL; a later call in a precision whose copy is NULL fails with
+ * HFJK_ERR_BACKEND. Mirrors DensityFittedCoulombExchange(df_tensor)
+ * (coulomb_exchange.py:124-145). */
+HFJK_API int hfjk_bind(hfjk_ctx *ctx, int naux, int nao, const double *b64, const float *b32,
+              const uint16_t *b16);
+
+/* Workspace bytes that hfjk_coulomb / hfjk_exchange_occ /
+ * hfjk_exchange_density need for the bound tensor, `nocc` occupied columns
+ * (use nao for hfjk_exchange_density) and `precision`. */
+HFJK_API int hfjk_workspace_bytes(hfjk_ctx *ctx, int nocc, int precision, size_t *bytes);
+
+/* J = sum_P B[P] <B[P], D>; density [nao][nao], coulomb [nao][nao].
+ * Replaces DensityFittedCoulombExchange.coulomb (coulomb_exchange.py:176-182). */
+HFJK_API int hfjk_coulomb(hfjk_ctx *ctx, const double *density, double *coulomb, int precision,
+                 void *workspace, size_t workspace_bytes, void *stream);
+
+/* K = sum_P (B[P] C) S (B[P] C)^T with C = orbitals [nao][nocc] and
+ * S = diag(signs) (signs: device float64 [nocc] of +1/-1, or NULL for all
+ * +1). The occupied-orbital form of DensityFittedCoulombExchange.exchange
+ * (coulomb_exchange.py:129-133, :184-190). */
+HFJK_API int hfjk_exchange_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, const double *signs,
+                      double *exchange, int precision, void *workspace, size_t workspace_bytes,
+                      void *stream);
+
+/* K = sum_P B[P] D B[P] for a symmetric (possibly indefinite) density:
+ * D = V diag(l) V^T by rocSOLVER dsyevd, eigenpairs with |l| <= 1e-12 max|l|
+ * dropped, C = V |l|^1/2, S = sign(l), then hfjk_exchange_occ. Drop-in for
+ * DensityFittedCoulombExchange.exchange (coulomb_exchange.py:184-190).
+ * Synchronises `stream` once (the retained rank sizes the launch). The
+ * density must be symmetric to 1e-12 relative (checked: HFJK_ERR_INPUT). */
+HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double *exchange, int precision,
+                          void *workspace, size_t workspace_bytes, void *stream);
+
+/* Stage timing: when enabled, every later call records a HIP event pair
+ * around each kernel it launches. hfjk_profile_read synchronises on the
+ * events of the last call and returns up to `max` (name, milliseconds)
+ * pairs; the return value is the number of stages (or a negative error). */
+HFJK_API int hfjk_profile(hfjk_ctx *ctx, int enable);
+HFJK_API int hfjk_profile_read(hfjk_ctx *ctx, const char **names, float *ms, int max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFJK_H */

@@ -1,0 +1,85 @@
+"""Write the golden fixtures of the DF-J/K tests.
+
+Run from the repository root in a container where the reference tree is
+mounted at /root/reference:  python tests/golden/make_golden.py
+
+* water_sto3g.npz: arrays copied from the reference's stored PySCF 2.14
+  fixtures (tests/reference/data/{one_electron,density_fitting,four_center,
+  scf,mp2}_water_sto-3g.json): S, T, V [7,7], metric [113,113],
+  three_center [113,7,7], eri [7,7,7,7], positions in Bohr, the exact RHF
+  energy and orbital energies, and the DF-RHF energy (mp2 fixture's
+  reference_energy). Data only; no reference code is run.
+* synthetic_c0.npz: the seeded configuration nao=100, naux=300, nocc=20,
+  seed 0 of the project's generator: a sample of B entries at fixed
+  indices (pins the generator bit for bit), C_occ, D, and the oracle's J, K
+  and charges (density form).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+REF = "/root/reference/tests/reference/data"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def load(name):
+    with open(os.path.join(REF, name)) as fh:
+        return json.load(fh)
+
+
+def water():
+    one = load("one_electron_water_sto-3g.json")
+    df = load("density_fitting_water_sto-3g.json")
+    four = load("four_center_water_sto-3g.json")
+    scf = load("scf_water_sto-3g.json")
+    mp2 = load("mp2_water_sto-3g.json")
+    np.savez_compressed(
+        os.path.join(OUT, "water_sto3g.npz"),
+        overlap=np.array(one["overlap"]),
+        kinetic=np.array(one["kinetic"]),
+        nuclear_attraction=np.array(one["nuclear_attraction"]),
+        positions_bohr=np.array(one["provenance"]["positions_bohr"]),
+        charges=np.array([8.0, 1.0, 1.0]),
+        metric=np.array(df["metric"]),
+        three_center=np.array(df["three_center"]),
+        eri=np.array(four["eri"]),
+        rhf_energy=np.array(scf["energy"]),
+        rhf_mo_energies=np.array(scf["mo_energies"]),
+        rhf_mo_occupations=np.array(scf["mo_occupations"]),
+        df_rhf_energy=np.array(mp2["reference_energy"]),
+    )
+
+
+def synthetic_c0():
+    from hafnium_amd import synthetic
+    from oracle import jk_ref
+
+    seed, naux, nao, nocc = 0, 300, 100, 20
+    b, c, d = synthetic.jk_inputs(seed, naux, nao, nocc)
+    rng = np.random.default_rng(12345)
+    q = rng.integers(0, naux, 4096)
+    m = rng.integers(0, nao, 4096)
+    n = rng.integers(0, nao, 4096)
+    np.savez_compressed(
+        os.path.join(OUT, "synthetic_c0.npz"),
+        seed=seed, naux=naux, nao=nao, nocc=nocc,
+        sample_q=q, sample_m=m, sample_n=n, sample_b=b[q, m, n],
+        orbitals=c, density=d,
+        coulomb=jk_ref.coulomb(b, d),
+        exchange=jk_ref.exchange(b, d),
+        charges=jk_ref.charges(b, d),
+    )
+
+
+if __name__ == "__main__":
+    water()
+    synthetic_c0()
+    print("written", sorted(os.listdir(OUT)))

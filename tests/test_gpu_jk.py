@@ -1,0 +1,162 @@
+"""Parity of the HIP DF-J/K path with the oracle, through the C ABI, on an MI355X."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from hafnium_amd import synthetic
+from hafnium_amd.errors import BackendError, InputError
+from oracle import jk_ref, layout_ref, scf_ref
+from tests.tolerances import BF16X3_REL_TOL, ENERGY_TOL, FP32_REL_TOL, FP64_REL_TOL
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": FP64_REL_TOL, "fp32": FP32_REL_TOL, "bf16x3": BF16X3_REL_TOL}
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+
+    assert torch.cuda.is_available(), "the gpu tests need an MI355X"
+    return torch
+
+
+def builder(b, **kw):
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    return MI355XCoulombExchange(b, **kw)
+
+
+def rel(a, b):
+    return jk_ref.max_relative_error(np.asarray(a), np.asarray(b))
+
+
+def test_generator_matches_numpy_bit_for_bit(torch):
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    for seed, naux, nao, q0 in ((0, 3, 70, 0), (9, 2, 33, 5)):
+        jk = MI355XCoulombExchange.synthetic(seed, naux, nao, q0=q0)
+        packed = jk.packed_tensor("fp64").view(naux, -1).cpu().numpy()
+        expect = layout_ref.pack(synthetic.tensor_slice_full(seed, nao, q0, q0 + naux))
+        np.testing.assert_array_equal(packed, expect)
+
+
+def test_normal_matrix_matches_numpy(torch):
+    import ctypes
+
+    from hafnium_amd import _lib
+    from hafnium_amd.coulomb_exchange import _Context
+
+    ctx = _Context(0)
+    out = torch.empty((37, 5), dtype=torch.float64, device="cuda")
+    _lib.check(ctx.lib.hfjk_normal_matrix(ctx.handle, 3, 37, 5, synthetic.TAG_ORBITALS,
+                                          ctypes.c_void_p(out.data_ptr()), None))
+    k, i = np.meshgrid(np.arange(37), np.arange(5), indexing="ij")
+    expect = synthetic.normal_from_counters(k, i, 0xFFFFFFFF, synthetic.TAG_ORBITALS, 3)
+    np.testing.assert_array_equal(out.cpu().numpy(), expect)
+
+
+def test_pack_matches_the_layout(torch):
+    b = synthetic.tensor_full(2, 4, 45)
+    jk = builder(b)
+    np.testing.assert_array_equal(jk.packed_tensor().view(4, -1).cpu().numpy(), layout_ref.pack(b))
+
+
+def test_fp64_matches_golden_c0(torch, synthetic_c0):
+    g = synthetic_c0
+    jk = builder(synthetic.tensor_full(int(g["seed"]), int(g["naux"]), int(g["nao"])))
+    d = g["density"]
+    j = jk.coulomb(d)
+    k_dens = jk.exchange(d)
+    k_occ = jk.exchange_occupied(g["orbitals"])
+    assert rel(j, g["coulomb"]) < FP64_REL_TOL
+    assert rel(k_dens, g["exchange"]) < FP64_REL_TOL
+    assert rel(k_occ, g["exchange"]) < FP64_REL_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("naux,nao,nocc", [(1, 1, 1), (7, 33, 1), (5, 64, 33), (11, 65, 40), (3, 130, 130), (17, 97, 70)])
+def test_ragged_shapes_match_the_oracle(torch, precision, naux, nao, nocc):
+    seed = 100 + nao
+    b, c, d = synthetic.jk_inputs(seed, naux, nao, nocc) if nocc <= nao else (None, None, None)
+    jk = builder(b, precision=precision)
+    tol = TOL[precision]
+    assert rel(jk.coulomb(d), jk_ref.coulomb(b, d)) < tol
+    k_ref = jk_ref.exchange(b, d)
+    assert rel(jk.exchange_occupied(c), k_ref) < tol
+    assert rel(jk.exchange(d), k_ref) < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_indefinite_and_nonsymmetric_densities(torch, precision):
+    rng = np.random.default_rng(5)
+    b = synthetic.tensor_full(6, 20, 50)
+    a = rng.standard_normal((50, 50))
+    sym = a + a.T  # indefinite, full rank
+    jk = builder(b, precision=precision)
+    assert rel(jk.exchange(sym), jk_ref.exchange(b, sym)) < TOL[precision]
+    # J depends on the symmetric part only, as the oracle's full contraction.
+    assert rel(jk.coulomb(a), jk_ref.coulomb(b, a)) < TOL[precision]
+    with pytest.raises(InputError, match="not symmetric"):
+        jk.exchange(a)
+
+
+def test_signed_orbitals_give_the_difference_of_exchanges(torch):
+    b = synthetic.tensor_full(8, 9, 40)
+    c = synthetic.occupied_orbitals(8, 40, 10)
+    signs = np.array([1.0] * 6 + [-1.0] * 4)
+    jk = builder(b)
+    expect = jk_ref.exchange_occupied(b, c[:, :6]) - jk_ref.exchange_occupied(b, c[:, 6:])
+    assert rel(jk.exchange_occupied(c, signs), expect) < FP64_REL_TOL
+
+
+def test_fp64_is_deterministic(torch):
+    b, c, d = synthetic.jk_inputs(12, 40, 90, 20)
+    jk = builder(b)
+    j1, k1 = jk.coulomb(d), jk.exchange_occupied(c)
+    j2, k2 = jk.coulomb(d), jk.exchange_occupied(c)
+    np.testing.assert_array_equal(j1, j2)
+    np.testing.assert_array_equal(k1, k2)
+
+
+def test_full_width_slice_of_the_large_configuration(torch):
+    """nao=1500, nocc=250 (the large configuration's shapes) on a slice of 3 aux functions."""
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    nao, nocc, q0, naux = 1500, 250, 4321, 3
+    b = synthetic.tensor_slice_full(1, nao, q0, q0 + naux)
+    c = synthetic.occupied_orbitals(1, nao, nocc)
+    d = c @ c.T
+    for precision in ("fp64", "fp32", "bf16x3"):
+        jk = MI355XCoulombExchange.synthetic(1, naux, nao, q0=q0, precision=precision)
+        assert rel(jk.coulomb(d), jk_ref.coulomb(b, d)) < TOL[precision]
+        assert rel(jk.exchange_occupied(c), jk_ref.exchange_occupied(b, c)) < TOL[precision]
+
+
+def test_water_df_rhf_energy_through_the_gpu_path(torch, water):
+    overlap = water["overlap"]
+    core = water["kinetic"] + water["nuclear_attraction"]
+    e_nuc = scf_ref.nuclear_repulsion(water["charges"], water["positions_bohr"])
+    b = scf_ref.df_tensor(water["metric"], water["three_center"], "cholesky")
+    jk = builder(b)
+    result = scf_ref.rhf(overlap, core, jk.coulomb, jk.exchange, 5, e_nuc)
+    assert result.energy == pytest.approx(float(water["df_rhf_energy"]), abs=ENERGY_TOL)
+
+
+def test_malformed_inputs_are_rejected(torch):
+    with pytest.raises(InputError, match="aux, ao, ao"):
+        builder(np.zeros((3, 2, 4)))
+    with pytest.raises(BackendError, match="complex"):
+        builder(np.zeros((2, 3, 3), dtype=complex))
+    asym = np.random.default_rng(0).standard_normal((2, 5, 5))
+    with pytest.raises(InputError, match="not symmetric"):
+        builder(asym)
+    jk = builder(synthetic.tensor_full(1, 2, 6))
+    with pytest.raises(InputError, match="atomic orbitals"):
+        jk.coulomb(np.eye(7))
+    with pytest.raises(BackendError, match="complex"):
+        jk.exchange(np.eye(6, dtype=complex))
+    with pytest.raises(InputError):
+        jk.exchange_occupied(np.ones((5, 2)))
