@@ -166,8 +166,7 @@ class MI355XCoulombExchange:
         torch = _torch()
         count = self._naux * self._packed
         lib = self._ctx.lib
-        if precision in (_lib.FP32, _lib.BF16X3) and self._b32 is None:
-            # The exchange MFMA kernels of both mixed modes read the FP32 copy.
+        if precision == _lib.FP32 and self._b32 is None:
             self._b32 = torch.empty(count, dtype=torch.float32, device=self._device)
             _lib.check(lib.hfjk_split(self._ctx.handle, _ptr(self._b64), count, _lib.FP32, _ptr(self._b32),
                                       self._stream()))
@@ -296,13 +295,18 @@ class MI355XCoulombExchange:
         _lib.check(self._ctx.lib.hfjk_profile(self._ctx.handle, 1 if enable else 0))
 
     def stage_times(self) -> list[tuple[str, float]]:
-        """``(stage, milliseconds)`` of the last call, read from its HIP events."""
-        names = (ctypes.c_char_p * 64)()
-        ms = (ctypes.c_float * 64)()
-        count = self._ctx.lib.hfjk_profile_read(self._ctx.handle, names, ms, 64)
+        """``(stage, milliseconds)`` of every kernel stage since the last read.
+
+        Read from the HIP events the library records on the call's stream
+        while profiling is on; reading synchronises on them and clears them.
+        """
+        cap = 4096
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_float * cap)()
+        count = self._ctx.lib.hfjk_profile_read(self._ctx.handle, names, ms, cap)
         if count < 0:
             _lib.check(-count)
-        return [(names[i].decode(), float(ms[i])) for i in range(min(count, 64))]
+        return [(names[i].decode(), float(ms[i])) for i in range(min(count, cap))]
 
     def packed_tensor(self, precision: str = "fp64"):
         """The resident tiled packed tensor copy (for tests and diagnostics)."""

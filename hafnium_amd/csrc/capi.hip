@@ -169,13 +169,15 @@ T* at(void* ws, size_t off) {
   return (T*)((char*)ws + off);
 }
 
+constexpr size_t kMaxStages = 4096;
+
 // RAII stage timer: records an event pair around the launches in its scope.
 struct Stage {
   hfjk_ctx* ctx;
   hipStream_t s;
   hfjk::StageTimer* timer = nullptr;
   Stage(hfjk_ctx* c, hipStream_t stream, const char* name) : ctx(c), s(stream) {
-    if (!ctx->profile) return;
+    if (!ctx->profile || ctx->n_stages >= kMaxStages) return;
     if (ctx->n_stages == ctx->stages.size()) {
       hfjk::StageTimer t;
       check_hip(hipEventCreate(&t.start), "hipEventCreate");
@@ -193,7 +195,6 @@ struct Stage {
 };
 
 void begin_call(hfjk_ctx* ctx) {
-  ctx->n_stages = 0;
   check_hip(hipSetDevice(ctx->device), "hipSetDevice");
 }
 
@@ -226,34 +227,44 @@ void run_coulomb(hfjk_ctx* ctx, const double* density, double* out, int precisio
 
 void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double* signs, double* out, int precision,
                   void* ws, const Layout& L, hipStream_t s) {
-  // The exchange kernels run in FP64 or FP32 MFMA; BF16X3 operands use the
-  // FP32 path on the FP32 copy when no split kernel exists for them.
-  const int kprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;
-  const void* b = bound_tensor(ctx, kprec);
-  if (b == nullptr)
-    fail(HFJK_ERR_BACKEND, std::string("the ") + precision_name(precision) +
-                               " exchange needs the fp32 copy of the fitted tensor");
+  // FP64 and FP32 run the same MFMA kernels on their tensor copy; BF16X3
+  // runs the split-product kernels on the bf16 hi/lo planes. The FP32 and
+  // BF16X3 buffers have the same sizes (4 bytes per element).
+  check_bound(ctx, precision);
+  const void* b = bound_tensor(ctx, precision);
   void* cp = at<void>(ws, L.xc);
   double* sg = at<double>(ws, L.xs);
   void* x = at<void>(ws, L.xx);
   void* part = at<void>(ws, L.xp);
+  const bool split = precision == HFJK_BF16X3;
   {
     Stage st(ctx, s, "exchange_prepare");
-    check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, kprec, cp, s), "orbitals launch");
+    if (split)
+      check_hip(hfjk::launch_orbitals_bf16x3(orbitals, ctx->nao, nocc, cp, s), "orbitals launch");
+    else
+      check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, precision, cp, s), "orbitals launch");
     check_hip(hfjk::launch_occ_signs(signs, nocc, L.ldo, sg, s), "signs launch");
   }
   {
     Stage st(ctx, s, "exchange_half_transform");
-    check_hip(hfjk::launch_half_transform(b, kprec, L.p, ctx->naux, ctx->nao, cp, L.ldo, x, s),
-              "half transform launch");
+    if (split)
+      check_hip(hfjk::launch_half_transform_bf16x3((const uint16_t*)b, L.p, ctx->naux, ctx->nao, cp, L.ldo, x, s),
+                "half transform launch");
+    else
+      check_hip(hfjk::launch_half_transform(b, precision, L.p, ctx->naux, ctx->nao, cp, L.ldo, x, s),
+                "half transform launch");
   }
   {
     Stage st(ctx, s, "exchange_syrk");
-    check_hip(hfjk::launch_syrk(x, kprec, ctx->naux, ctx->nao, L.ldo, sg, L.nsplit_k, part, s), "syrk launch");
+    if (split)
+      check_hip(hfjk::launch_syrk_bf16x3(x, ctx->naux, ctx->nao, L.ldo, sg, L.nsplit_k, part, s), "syrk launch");
+    else
+      check_hip(hfjk::launch_syrk(x, precision, ctx->naux, ctx->nao, L.ldo, sg, L.nsplit_k, part, s), "syrk launch");
   }
   {
     Stage st(ctx, s, "exchange_reduce");
-    check_hip(hfjk::launch_syrk_reduce(part, kprec, L.nsplit_k, ctx->nao, out, s), "syrk reduce launch");
+    const int rprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;  // BF16X3 partials are FP32
+    check_hip(hfjk::launch_syrk_reduce(part, rprec, L.nsplit_k, ctx->nao, out, s), "syrk reduce launch");
   }
 }
 
@@ -267,12 +278,14 @@ rocblas_handle blas_handle(hfjk_ctx* ctx) {
 }
 
 __global__ void density_stats_kernel(const double* __restrict__ d, int n, unsigned long long* __restrict__ stats) {
-  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
   double asym = 0.0, mag = 0.0;
-  if (idx < (long long)n * n) {
+  const long long total = (long long)n * n;
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long long)gridDim.x * 256) {
     const int r = (int)(idx / n), c = (int)(idx % n);
-    asym = fabs(d[idx] - d[(long long)c * n + r]);
-    mag = fabs(d[idx]);
+    const double a = fabs(d[idx] - d[(long long)c * n + r]);
+    const double m = fabs(d[idx]);
+    asym = a > asym ? a : asym;
+    mag = m > mag ? m : mag;
   }
   for (int off = 32; off > 0; off >>= 1) {
     const double a = __shfl_xor(asym, off);
@@ -466,7 +479,7 @@ int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange
       Stage st(ctx, s, "density_eigh");
       check_hip(hipMemsetAsync(stats, 0, 2 * sizeof(unsigned long long), s), "hipMemsetAsync");
       const long long nn = (long long)n * n;
-      hipLaunchKernelGGL(density_stats_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, density, n, stats);
+      hipLaunchKernelGGL(density_stats_kernel, dim3(hfjk::grid_for(nn)), dim3(256), 0, s, density, n, stats);
       check_hip(hipGetLastError(), "density stats launch");
       check_hip(hipMemcpyAsync(a, density, sizeof(double) * nn, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
       rocblas_handle h = blas_handle(ctx);
@@ -513,6 +526,7 @@ int hfjk_profile(hfjk_ctx* ctx, int enable) {
   return guarded([&] {
     check_ctx(ctx);
     ctx->profile = enable != 0;
+    ctx->n_stages = 0;
   });
 }
 
@@ -531,6 +545,7 @@ int hfjk_profile_read(hfjk_ctx* ctx, const char** names, float* ms, int max) {
       }
     }
     count = (int)ctx->n_stages;
+    ctx->n_stages = 0;
   });
   return status == HFJK_OK ? count : -status;
 }

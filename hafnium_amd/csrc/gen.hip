@@ -158,11 +158,12 @@ __global__ __launch_bounds__(kThreads) void generate_kernel(uint64_t seed, int q
 
 __global__ __launch_bounds__(kThreads) void normal_matrix_kernel(uint64_t seed, int rows, int cols,
                                                                   uint32_t tag, double* __restrict__ out) {
-  const long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (e >= (long long)rows * cols) return;
-  const int r = (int)(e / cols);
-  const int c = (int)(e % cols);
-  out[e] = det_normal((uint32_t)r, (uint32_t)c, 0xFFFFFFFFu, tag, seed);
+  const long long total = (long long)rows * cols;
+  for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x; e < total; e += (long long)gridDim.x * kThreads) {
+    const int r = (int)(e / cols);
+    const int c = (int)(e % cols);
+    out[e] = det_normal((uint32_t)r, (uint32_t)c, 0xFFFFFFFFu, tag, seed);
+  }
 }
 
 // Pack: one block per (tile, q); 256 threads x 4 elements.
@@ -221,14 +222,13 @@ __device__ inline float bf16_to_float(uint16_t h) { return __uint_as_float(((uin
 
 __global__ __launch_bounds__(kThreads) void split32_kernel(const double* __restrict__ in, long long count,
                                                             float* __restrict__ out) {
-  const long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (e < count) out[e] = (float)in[e];
+  for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x; e < count; e += (long long)gridDim.x * kThreads)
+    out[e] = (float)in[e];
 }
 
 __global__ __launch_bounds__(kThreads) void split16_kernel(const double* __restrict__ in, long long count,
                                                             uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
-  const long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (e < count) {
+  for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x; e < count; e += (long long)gridDim.x * kThreads) {
     const float x = (float)in[e];
     const uint16_t h = bf16_rne(x);
     hi[e] = h;
@@ -256,7 +256,7 @@ hipError_t launch_generate(uint64_t seed, int q0, int naux, int nao, double* out
 hipError_t launch_normal_matrix(uint64_t seed, int rows, int cols, uint32_t tag, double* out, hipStream_t s) {
   const long long n = (long long)rows * cols;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(normal_matrix_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+  hipLaunchKernelGGL(normal_matrix_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s,
                      seed, rows, cols, tag, out);
   return hipGetLastError();
 }
@@ -277,7 +277,7 @@ hipError_t launch_pack(const double* full, int naux, int nao, double* out, unsig
 
 hipError_t launch_split(const double* in, long long count, int precision, void* out, hipStream_t s) {
   if (count == 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((count + kThreads - 1) / kThreads);
+  const unsigned blocks = grid_for(count);
   if (precision == HFJK_FP32) {
     hipLaunchKernelGGL(split32_kernel, dim3(blocks), dim3(kThreads), 0, s, in, count, (float*)out);
   } else {

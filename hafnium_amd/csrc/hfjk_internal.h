@@ -37,6 +37,16 @@ inline int x_rows(int nao) { return ((nao + kBlock - 1) / kBlock) * kBlock; }
 // Leading dimension of C and X: nocc rounded up to 32.
 inline int occ_ld(int nocc) { return ((nocc + 31) / 32) * 32; }
 
+// Blocks of a grid-stride launch over `count` elements. A dispatch holds at
+// most 2^32 work-items per dimension, so flat launches are capped and the
+// kernels loop.
+inline unsigned grid_for(long long count) {
+  long long blocks = (count + kThreads - 1) / kThreads;
+  if (blocks < 1) blocks = 1;
+  if (blocks > (1LL << 20)) blocks = 1LL << 20;
+  return (unsigned)blocks;
+}
+
 struct Error {
   int code;
   std::string message;
@@ -94,6 +104,13 @@ hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo,
 hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, int nao, double* out,
                               hipStream_t s);
 hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* signs_out, hipStream_t s);
+
+// BF16x3 exchange (jk_bf16.hip): operands as bf16 hi/lo planes.
+hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s);
+hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
+                                        void* x, hipStream_t s);
+hipError_t launch_syrk_bf16x3(const void* x, int naux, int nao, int ldo, const double* signs, int nsplit,
+                              void* partial, hipStream_t s);
 
 // Density factorisation helpers (jk.hip).
 hipError_t launch_split_factor(const double* vecs, const double* vals, int nao, int first, int count,

@@ -181,20 +181,22 @@ __global__ __launch_bounds__(kThreads) void coulomb_accumulate_kernel(const void
 __global__ __launch_bounds__(kThreads) void coulomb_unpack_kernel(const double* __restrict__ partial, int nsplit,
                                                                    long long p, int nao,
                                                                    double* __restrict__ out) {
-  const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (idx >= (long long)nao * nao) return;
-  const int row = (int)(idx / nao);
-  const int col = (int)(idx % nao);
-  const int r = row / kTile, c = col / kTile;
-  long long src;
-  if (r >= c) {
-    src = tile_index(r, c) * kTileElems + (row % kTile) * kTile + col % kTile;
-  } else {
-    src = tile_index(c, r) * kTileElems + (col % kTile) * kTile + row % kTile;
+  const long long total = (long long)nao * nao;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int row = (int)(idx / nao);
+    const int col = (int)(idx % nao);
+    const int r = row / kTile, c = col / kTile;
+    long long src;
+    if (r >= c) {
+      src = tile_index(r, c) * kTileElems + (row % kTile) * kTile + col % kTile;
+    } else {
+      src = tile_index(c, r) * kTileElems + (col % kTile) * kTile + row % kTile;
+    }
+    double s = 0.0;
+    for (int k = 0; k < nsplit; ++k) s += partial[(long long)k * p + src];
+    out[idx] = s;
   }
-  double s = 0.0;
-  for (int k = 0; k < nsplit; ++k) s += partial[(long long)k * p + src];
-  out[idx] = s;
 }
 
 // ---------------------------------------------------------------- Exchange
@@ -203,12 +205,14 @@ __global__ __launch_bounds__(kThreads) void coulomb_unpack_kernel(const double* 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void orbitals_kernel(const double* __restrict__ c, int nao, int nocc,
                                                              int npad, int ldo, T* __restrict__ out) {
-  const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (idx >= (long long)npad * ldo) return;
-  const int k = (int)(idx / ldo);
-  const int i = (int)(idx % ldo);
-  const double v = (k < nao && i < nocc) ? c[(long long)k * nocc + i] : 0.0;
-  out[idx] = (T)v;
+  const long long total = (long long)npad * ldo;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int k = (int)(idx / ldo);
+    const int i = (int)(idx % ldo);
+    const double v = (k < nao && i < nocc) ? c[(long long)k * nocc + i] : 0.0;
+    out[idx] = (T)v;
+  }
 }
 
 // X[q][m][i] = sum_k B[q][m][k] C[k][i]. Workgroup tile 64 rows x BN
@@ -370,23 +374,26 @@ __global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x,
 template <typename T>
 __global__ __launch_bounds__(kThreads) void syrk_reduce_kernel(const T* __restrict__ partial, int nsplit,
                                                                 int npairs, int nao, double* __restrict__ out) {
-  const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (idx >= (long long)nao * nao) return;
-  const int row = (int)(idx / nao);
-  const int col = (int)(idx % nao);
-  const int bm = row / kBlock, bn = col / kBlock;
-  long long pair;
-  int local;
-  if (bm >= bn) {
-    pair = tile_index(bm, bn);
-    local = (row % kBlock) * kBlock + col % kBlock;
-  } else {
-    pair = tile_index(bn, bm);
-    local = (col % kBlock) * kBlock + row % kBlock;
+  const long long total = (long long)nao * nao;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int row = (int)(idx / nao);
+    const int col = (int)(idx % nao);
+    const int bm = row / kBlock, bn = col / kBlock;
+    long long pair;
+    int local;
+    if (bm >= bn) {
+      pair = tile_index(bm, bn);
+      local = (row % kBlock) * kBlock + col % kBlock;
+    } else {
+      pair = tile_index(bn, bm);
+      local = (col % kBlock) * kBlock + row % kBlock;
+    }
+    double s = 0.0;
+    for (int k = 0; k < nsplit; ++k)
+      s += (double)partial[((long long)k * npairs + pair) * (kBlock * kBlock) + local];
+    out[idx] = s;
   }
-  double s = 0.0;
-  for (int k = 0; k < nsplit; ++k) s += (double)partial[((long long)k * npairs + pair) * (kBlock * kBlock) + local];
-  out[idx] = s;
 }
 
 __global__ __launch_bounds__(kThreads) void occ_signs_kernel(const double* __restrict__ in, int nocc, int ldo,
@@ -404,14 +411,16 @@ __global__ __launch_bounds__(kThreads) void split_factor_kernel(const double* __
                                                                  int nneg, int npos, double* __restrict__ c,
                                                                  double* __restrict__ signs) {
   const int r = nneg + npos;
-  const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (idx >= (long long)nao * r) return;
-  const int k = (int)(idx / r);
-  const int jj = (int)(idx % r);
-  const int j = jj < nneg ? jj : (nao - npos) + (jj - nneg);
-  const double l = vals[j];
-  c[idx] = vecs[(long long)j * nao + k] * sqrt(fabs(l));
-  if (k == 0) signs[jj] = l < 0.0 ? -1.0 : 1.0;
+  const long long total = (long long)nao * r;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int k = (int)(idx / r);
+    const int jj = (int)(idx % r);
+    const int j = jj < nneg ? jj : (nao - npos) + (jj - nneg);
+    const double l = vals[j];
+    c[idx] = vecs[(long long)j * nao + k] * sqrt(fabs(l));
+    if (k == 0) signs[jj] = l < 0.0 ? -1.0 : 1.0;
+  }
 }
 
 }  // namespace
@@ -459,7 +468,7 @@ hipError_t launch_coulomb_accumulate(const void* b, int precision, long long p, 
 hipError_t launch_coulomb_unpack(const double* partial, int nsplit, long long p, int nao, double* out,
                                  hipStream_t s) {
   const long long n = (long long)nao * nao;
-  hipLaunchKernelGGL(coulomb_unpack_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+  hipLaunchKernelGGL(coulomb_unpack_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s,
                      partial, nsplit, p, nao, out);
   return hipGetLastError();
 }
@@ -468,7 +477,7 @@ hipError_t launch_orbitals(const double* c, int nao, int nocc, int precision, vo
   const int npad = n_tiles(nao) * kTile;
   const int ldo = occ_ld(nocc);
   const long long n = (long long)npad * ldo;
-  const unsigned blocks = (unsigned)((n + kThreads - 1) / kThreads);
+  const unsigned blocks = grid_for(n);
   if (precision == HFJK_FP64)
     hipLaunchKernelGGL(orbitals_kernel<double>, dim3(blocks), dim3(kThreads), 0, s, c, nao, nocc, npad, ldo,
                        (double*)out);
@@ -527,7 +536,7 @@ hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, in
   const int nb = x_rows(nao) / kBlock;
   const int npairs = nb * (nb + 1) / 2;
   const long long n = (long long)nao * nao;
-  const unsigned blocks = (unsigned)((n + kThreads - 1) / kThreads);
+  const unsigned blocks = grid_for(n);
   if (precision == HFJK_FP64)
     hipLaunchKernelGGL(syrk_reduce_kernel<double>, dim3(blocks), dim3(kThreads), 0, s, (const double*)partial, nsplit,
                        npairs, nao, out);
@@ -547,7 +556,7 @@ hipError_t launch_split_factor(const double* vecs, const double* vals, int nao, 
                                double* signs, hipStream_t s) {
   const long long n = (long long)nao * (nneg + npos);
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(split_factor_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, vecs,
+  hipLaunchKernelGGL(split_factor_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, vecs,
                      vals, nao, nneg, npos, c, signs);
   return hipGetLastError();
 }

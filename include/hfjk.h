@@ -135,10 +135,12 @@ HFJK_API int hfjk_exchange_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, 
 HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double *exchange, int precision,
                           void *workspace, size_t workspace_bytes, void *stream);
 
-/* Stage timing: when enabled, every later call records a HIP event pair
- * around each kernel it launches. hfjk_profile_read synchronises on the
- * events of the last call and returns up to `max` (name, milliseconds)
- * pairs; the return value is the number of stages (or a negative error). */
+/* Stage timing: when enabled, every later call records a HIP event pair on
+ * its stream around each kernel stage it launches (at most 4096 stages are
+ * kept). hfjk_profile_read synchronises on the recorded events, returns up
+ * to `max` (name, milliseconds) pairs in launch order and forgets them; the
+ * return value is the number of stages recorded (or a negative error). The
+ * names stay valid until the next call on the context. */
 HFJK_API int hfjk_profile(hfjk_ctx *ctx, int enable);
 HFJK_API int hfjk_profile_read(hfjk_ctx *ctx, const char **names, float *ms, int max);
 

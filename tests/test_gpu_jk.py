@@ -160,3 +160,25 @@ def test_malformed_inputs_are_rejected(torch):
         jk.exchange(np.eye(6, dtype=complex))
     with pytest.raises(InputError):
         jk.exchange_occupied(np.ones((5, 2)))
+
+
+def test_mixed_copies_of_a_tensor_beyond_two_to_the_32_elements(torch):
+    """The FP32 and BF16 copies cover every element when naux * packed > 2**32."""
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    nao, naux = 1500, 3800  # 3800 * 1155072 = 4.39e9 packed elements
+    jk = MI355XCoulombExchange.synthetic(2, naux, nao)
+    b64 = jk.packed_tensor("fp64")
+    assert b64.numel() > 2**32
+    idx = torch.tensor([0, 2**32 - 1, 2**32, 2**32 + 12345, b64.numel() - 1], device=b64.device)
+    b32 = jk.packed_tensor("fp32")
+    np.testing.assert_array_equal(b32[idx].cpu().numpy(), b64[idx].float().cpu().numpy())
+    del b32
+    jk._b32 = None
+    b16 = jk.packed_tensor("bf16x3")
+    count = b64.numel()
+    hi = b16[idx].cpu().numpy().astype(np.uint16).astype(np.uint32) << 16
+    lo = b16[idx + count].cpu().numpy().astype(np.uint16).astype(np.uint32) << 16
+    value = hi.view(np.float32).astype(np.float64) + lo.view(np.float32).astype(np.float64)
+    x = b64[idx].cpu().numpy()
+    assert np.all(np.abs(value - x) <= 2.0**-15 * np.abs(x) + 1e-300)
