@@ -51,6 +51,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel, n, q, o):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
+
+    The counters cannot be read inside this process; profiles/traffic_large.json
+    holds the scaled per-launch values of the last counter runs on this workload.
+    """
+    path = os.path.join(ROOT, "profiles", "traffic_large.json")
+    if (n, q, o) != (1500, 5000, 250) or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        data = json.load(fh)
+    return data.get("hbm_bytes_per_launch", {}).get(kernel)
+
+
 def cpu_baseline(jk, wl, slice_q):
     """Density-form J+K of the oracle on `slice_q` aux functions, scaled to naux."""
     import numpy as np
@@ -110,7 +124,20 @@ def main(argv=None):
     ap.add_argument("--workload", default="large", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-slice", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 2 GiB read+write elementwise kernel (counter calibration)")
     args = ap.parse_args(argv)
+
+    if args.copy_calibration:
+        import torch
+
+        x = torch.ones(2**28, dtype=torch.float64, device="cuda")  # 2 GiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.add(x, 0.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8}))
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the aux-index shard "
@@ -194,9 +221,13 @@ def main(argv=None):
     if x_ms:
         achieved = x_flops / (x_ms * 1e-3) / 1e12
         peak = PEAK_TFLOPS[head]
-        roofline = {"bound": "mfma", "kernel": "half_transform_kernel", "achieved": achieved, "peak": peak,
-                    "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                    "algorithmic_flops_per_launch": x_flops}
+        kernel = {"fp64": "half_transform_kernel<double, 128>", "fp32": "half_transform_kernel<float, 128>",
+                  "bf16x3": "half_transform_bf16x3_kernel<128>"}[head]
+        roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved, "peak": peak,
+                    "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(kernel, n, q, o),
+                    "algorithmic_flops_per_launch": x_flops,
+                    "algorithmic_bytes_per_launch": q * n * (n + 1) / 2 * (8 if head == "fp64" else 4)
+                    + n * q * o * (8 if head == "fp64" else 4)}
         if syrk_ms:
             roofline["syrk_achieved"] = (q * n * n * o) / (syrk_ms * 1e-3) / 1e12
 
