@@ -93,9 +93,12 @@ int coulomb_splits(long long ntiles, int naux) {
   return (int)std::max(1LL, std::min(s, cap));
 }
 
+// Aux-slice splits of the SYRK: a multiple of 8 (one XCD per split group),
+// about 2048 workgroups in total, never more slices than 8 * ceil(naux / 8).
 int syrk_splits(long long npairs, int naux) {
-  long long s = (8192 + npairs - 1) / npairs;
-  return (int)std::max(1LL, std::min(s, (long long)naux));
+  long long s = (2048 + npairs - 1) / npairs;
+  s = std::min(s, (long long)naux);
+  return (int)(((std::max(1LL, s) + 7) / 8) * 8);
 }
 
 // Exchange operands in BF16X3 are not implemented by a dedicated kernel yet;
@@ -123,7 +126,8 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   L.ldo = hfjk::occ_ld(std::max(nocc, 1));
   L.xrows = hfjk::x_rows(n);
   const int nb = L.xrows / hfjk::kBlock;
-  L.npairs = nb * (nb + 1) / 2;
+  (void)nb;
+  L.npairs = hfjk::syrk_pairs(n);
   L.nsplit_j = coulomb_splits(L.ntiles, naux);
   L.nsplit_k = syrk_splits(L.npairs, naux);
   size_t off = 0;
@@ -150,7 +154,8 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   L.xc = off; off = align_up(off + npad * L.ldo * es);
   L.xs = off; off = align_up(off + (size_t)L.ldo * 8);
   L.xx = off; off = align_up(off + (size_t)naux * L.xrows * L.ldo * es);
-  L.xp = off; off = align_up(off + (size_t)L.nsplit_k * L.npairs * hfjk::kBlock * hfjk::kBlock * es);
+  const size_t tile2 = (size_t)hfjk::syrk_tile() * hfjk::syrk_tile();
+  L.xp = off; off = align_up(off + (size_t)L.nsplit_k * L.npairs * tile2 * es);
   L.x_end = off;
   L.total = std::max(L.c_end, L.x_end);
   return L;
@@ -256,10 +261,7 @@ void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double*
   }
   {
     Stage st(ctx, s, "exchange_syrk");
-    if (split)
-      check_hip(hfjk::launch_syrk_bf16x3(x, ctx->naux, ctx->nao, L.ldo, sg, L.nsplit_k, part, s), "syrk launch");
-    else
-      check_hip(hfjk::launch_syrk(x, precision, ctx->naux, ctx->nao, L.ldo, sg, L.nsplit_k, part, s), "syrk launch");
+    check_hip(hfjk::launch_syrk(x, precision, ctx->naux, ctx->nao, L.ldo, sg, L.nsplit_k, part, s), "syrk launch");
   }
   {
     Stage st(ctx, s, "exchange_reduce");

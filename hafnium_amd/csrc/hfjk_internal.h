@@ -105,12 +105,14 @@ hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, in
                               hipStream_t s);
 hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* signs_out, hipStream_t s);
 
-// BF16x3 exchange (jk_bf16.hip): operands as bf16 hi/lo planes.
+// BF16x3 exchange (xk.hip): operands as bf16 hi/lo planes; launch_syrk
+// takes HFJK_BF16X3 for the split-product kernel.
 hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s);
 hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
                                         void* x, hipStream_t s);
-hipError_t launch_syrk_bf16x3(const void* x, int naux, int nao, int ldo, const double* signs, int nsplit,
-                              void* partial, hipStream_t s);
+// Lower-triangle 128 x 128 output tiles of the SYRK and the tile edge.
+int syrk_pairs(int nao);
+int syrk_tile();
 
 // Density factorisation helpers (jk.hip).
 hipError_t launch_split_factor(const double* vecs, const double* vals, int nao, int first, int count,

@@ -215,187 +215,6 @@ __global__ __launch_bounds__(kThreads) void orbitals_kernel(const double* __rest
   }
 }
 
-// X[q][m][i] = sum_k B[q][m][k] C[k][i]. Workgroup tile 64 rows x BN
-// columns; waves 2 x 2, each 32 x BN/2 as (2 x BN/32) MFMA 16x16 blocks.
-// The k loop walks the 32-wide tile columns of the packed tensor: the two
-// 32-row tiles of the block are read directly (r >= kt) or transposed
-// (r < kt, stored as tile (kt, r)) into the k-major LDS tile As[k][m].
-template <typename T, int BN>
-__global__ __launch_bounds__(kThreads) void half_transform_kernel(const T* __restrict__ b, long long p, int nt,
-                                                                   const T* __restrict__ c, int ldo,
-                                                                   T* __restrict__ x, int xrows, int qbase) {
-  constexpr int NI = BN / 32;
-  __shared__ T As[kTile][kBlock + 2];
-  __shared__ T Bs[kTile][BN + 2];
-  const int nb = blockIdx.x;
-  const int mb = blockIdx.y;
-  const int q = qbase + blockIdx.z;
-  const int n0 = nb * BN;
-  const T* bq = b + (long long)q * p;
-  const int t = threadIdx.x;
-  const int lane = t & 63;
-  const int wave = t >> 6;
-  const int wm = wave & 1;
-  const int wn = wave >> 1;
-  typename Mfma<T>::acc_t acc[2][NI];
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < NI; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
-
-  for (int kt = 0; kt < nt; ++kt) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r = 2 * mb + h;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int e = t * 4 + v;
-        const int a = e / kTile;
-        const int bc = e % kTile;
-        if (r >= nt) {
-          As[bc][h * kTile + a] = (T)0;
-        } else if (r >= kt) {
-          // element (a, bc) of tile (r, kt) is B[m = r*32 + a][k = kt*32 + bc]
-          As[bc][h * kTile + a] = bq[tile_index(r, kt) * kTileElems + e];
-        } else {
-          // element (a, bc) of tile (kt, r) is B[k = kt*32 + a][m = r*32 + bc]
-          As[a][h * kTile + bc] = bq[tile_index(kt, r) * kTileElems + e];
-        }
-      }
-    }
-    for (int idx = t; idx < kTile * BN; idx += kThreads) {
-      const int kk = idx / BN;
-      const int jj = idx % BN;
-      const int col = n0 + jj;
-      Bs[kk][jj] = (col < ldo) ? c[(long long)(kt * kTile + kk) * ldo + col] : (T)0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < kTile; kk += 4) {
-      T af[2], bf[NI];
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi) af[mi] = As[kk + (lane >> 4)][wm * 32 + mi * 16 + (lane & 15)];
-#pragma unroll
-      for (int ni = 0; ni < NI; ++ni) bf[ni] = Bs[kk + (lane >> 4)][wn * (BN / 2) + ni * 16 + (lane & 15)];
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = Mfma<T>::mma(af[mi], bf[ni], acc[mi][ni]);
-    }
-    __syncthreads();
-  }
-  T* xq = x + (long long)q * xrows * ldo;
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < NI; ++ni) {
-      const int col = n0 + wn * (BN / 2) + ni * 16 + (lane & 15);
-      if (col < ldo) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = mb * kBlock + wm * 32 + mi * 16 + Mfma<T>::row(lane, r);
-          xq[(long long)row * ldo + col] = acc[mi][ni][r];
-        }
-      }
-    }
-}
-
-// partial[s][pair][64][64] = sum over the s-th aux slice and all columns i of
-// X[q][mb*64 + .][i] sign[i] X[q][nb*64 + .][i], pair = mb (mb + 1) / 2 + nb.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x, int naux, int xrows, int ldo,
-                                                         const double* __restrict__ signs, int nsplit,
-                                                         int npairs, T* __restrict__ partial) {
-  __shared__ T As[kTile][kBlock + 2];
-  __shared__ T Bs[kTile][kBlock + 2];
-  const int pair = blockIdx.x;
-  const int s = blockIdx.y;
-  int mb, nb;
-  tile_rc(pair, mb, nb);
-  const int qb = (int)((long long)s * naux / nsplit);
-  const int qe = (int)((long long)(s + 1) * naux / nsplit);
-  const int t = threadIdx.x;
-  const int lane = t & 63;
-  const int wave = t >> 6;
-  const int wm = wave & 1;
-  const int wn = wave >> 1;
-  typename Mfma<T>::acc_t acc[2][2];
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
-
-  for (int q = qb; q < qe; ++q) {
-    const T* xm = x + ((long long)q * xrows + mb * kBlock) * ldo;
-    const T* xn = x + ((long long)q * xrows + nb * kBlock) * ldo;
-    for (int i0 = 0; i0 < ldo; i0 += kTile) {
-#pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        const int idx = t + kThreads * v;
-        const int row = idx / kTile;
-        const int col = idx % kTile;
-        As[col][row] = xm[(long long)row * ldo + i0 + col];
-        Bs[col][row] = xn[(long long)row * ldo + i0 + col] * (T)signs[i0 + col];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < kTile; kk += 4) {
-        T af[2], bf[2];
-#pragma unroll
-        for (int mi = 0; mi < 2; ++mi) af[mi] = As[kk + (lane >> 4)][wm * 32 + mi * 16 + (lane & 15)];
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni) bf[ni] = Bs[kk + (lane >> 4)][wn * 32 + ni * 16 + (lane & 15)];
-#pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-          for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = Mfma<T>::mma(af[mi], bf[ni], acc[mi][ni]);
-      }
-      __syncthreads();
-    }
-  }
-  T* dst = partial + ((long long)s * npairs + pair) * (kBlock * kBlock);
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 2; ++ni) {
-      const int col = wn * 32 + ni * 16 + (lane & 15);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = wm * 32 + mi * 16 + Mfma<T>::row(lane, r);
-        dst[row * kBlock + col] = acc[mi][ni][r];
-      }
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kThreads) void syrk_reduce_kernel(const T* __restrict__ partial, int nsplit,
-                                                                int npairs, int nao, double* __restrict__ out) {
-  const long long total = (long long)nao * nao;
-  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
-       idx += (long long)gridDim.x * kThreads) {
-    const int row = (int)(idx / nao);
-    const int col = (int)(idx % nao);
-    const int bm = row / kBlock, bn = col / kBlock;
-    long long pair;
-    int local;
-    if (bm >= bn) {
-      pair = tile_index(bm, bn);
-      local = (row % kBlock) * kBlock + col % kBlock;
-    } else {
-      pair = tile_index(bn, bm);
-      local = (col % kBlock) * kBlock + row % kBlock;
-    }
-    double s = 0.0;
-    for (int k = 0; k < nsplit; ++k)
-      s += (double)partial[((long long)k * npairs + pair) * (kBlock * kBlock) + local];
-    out[idx] = s;
-  }
-}
-
 __global__ __launch_bounds__(kThreads) void occ_signs_kernel(const double* __restrict__ in, int nocc, int ldo,
                                                               double* __restrict__ out) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
@@ -484,65 +303,6 @@ hipError_t launch_orbitals(const double* c, int nao, int nocc, int precision, vo
   else
     hipLaunchKernelGGL(orbitals_kernel<float>, dim3(blocks), dim3(kThreads), 0, s, c, nao, nocc, npad, ldo,
                        (float*)out);
-  return hipGetLastError();
-}
-
-template <typename T, int BN>
-static hipError_t launch_half_transform_t(const T* b, long long p, int naux, int nao, const T* c, int ldo, T* x,
-                                          hipStream_t s) {
-  const int nt = n_tiles(nao);
-  const int xrows = x_rows(nao);
-  for (int base = 0; base < naux; base += 65535) {
-    const int count = naux - base < 65535 ? naux - base : 65535;
-    const dim3 grid((unsigned)((ldo + BN - 1) / BN), (unsigned)(xrows / kBlock), (unsigned)count);
-    hipLaunchKernelGGL((half_transform_kernel<T, BN>), grid, dim3(kThreads), 0, s, b, p, nt, c, ldo, x, xrows, base);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-  }
-  return hipSuccess;
-}
-
-template <typename T>
-static hipError_t launch_half_transform_bn(const T* b, long long p, int naux, int nao, const T* c, int ldo, T* x,
-                                           hipStream_t s) {
-  if (ldo <= 32) return launch_half_transform_t<T, 32>(b, p, naux, nao, c, ldo, x, s);
-  if (ldo <= 64) return launch_half_transform_t<T, 64>(b, p, naux, nao, c, ldo, x, s);
-  return launch_half_transform_t<T, 128>(b, p, naux, nao, c, ldo, x, s);
-}
-
-hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao, const void* c, int ldo,
-                                 void* x, hipStream_t s) {
-  if (precision == HFJK_FP64)
-    return launch_half_transform_bn<double>((const double*)b, p, naux, nao, (const double*)c, ldo, (double*)x, s);
-  return launch_half_transform_bn<float>((const float*)b, p, naux, nao, (const float*)c, ldo, (float*)x, s);
-}
-
-hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs, int nsplit,
-                       void* partial, hipStream_t s) {
-  const int xrows = x_rows(nao);
-  const int nb = xrows / kBlock;
-  const int npairs = nb * (nb + 1) / 2;
-  const dim3 grid((unsigned)npairs, (unsigned)nsplit);
-  if (precision == HFJK_FP64)
-    hipLaunchKernelGGL(syrk_kernel<double>, grid, dim3(kThreads), 0, s, (const double*)x, naux, xrows, ldo, signs,
-                       nsplit, npairs, (double*)partial);
-  else
-    hipLaunchKernelGGL(syrk_kernel<float>, grid, dim3(kThreads), 0, s, (const float*)x, naux, xrows, ldo, signs,
-                       nsplit, npairs, (float*)partial);
-  return hipGetLastError();
-}
-
-hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, int nao, double* out, hipStream_t s) {
-  const int nb = x_rows(nao) / kBlock;
-  const int npairs = nb * (nb + 1) / 2;
-  const long long n = (long long)nao * nao;
-  const unsigned blocks = grid_for(n);
-  if (precision == HFJK_FP64)
-    hipLaunchKernelGGL(syrk_reduce_kernel<double>, dim3(blocks), dim3(kThreads), 0, s, (const double*)partial, nsplit,
-                       npairs, nao, out);
-  else
-    hipLaunchKernelGGL(syrk_reduce_kernel<float>, dim3(blocks), dim3(kThreads), 0, s, (const float*)partial, nsplit,
-                       npairs, nao, out);
   return hipGetLastError();
 }
 

@@ -1,0 +1,825 @@
+// Exchange kernels for gfx950 (CDNA4): the half-transform X[P] = B[P] C and
+// the symmetric rank-k update K = sum_P X[P] S X[P]^T, on MFMA, in three
+// arithmetics (FP64, FP32, BF16x3).
+//
+// Structure shared by all of them:
+//  * one 256-thread workgroup (4 waves of 64) per output tile;
+//  * global -> register prefetch of the next k step while the MFMAs of the
+//    current one run from LDS (one LDS buffer, two barriers per step);
+//  * the SYRK grid is XCD-aware: workgroups are dispatched round-robin over
+//    the 8 XCDs, so linear id b runs on XCD b % 8, and every tile of one
+//    aux-slice split s is mapped to the same XCD. The slice of X that those
+//    tiles stream is then fetched from HBM into one L2 instead of eight.
+//
+// MFMA lane layouts (measured on gfx950):
+//  * v_mfma_f64_16x16x4f64 : A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k,
+//    D register r of lane l = D[l / 16 + 4 r][l % 16];
+//  * v_mfma_f32_16x16x4f32 : same A/B, D register r of lane l = D[4 (l / 16) + r][l % 16];
+//  * v_mfma_f32_16x16x32_bf16 : A[i][8 (l / 16) + e] in element e of lane l with
+//    i = l % 16, B[8 (l / 16) + e][j] likewise with j = l % 16; D as f32 16x16x4.
+//
+// FP64/FP32 operands are single elements per lane, so the packed B tiles are
+// kept in LDS in their stored (row-major) order and a transposed tile is read
+// with swapped addressing. BF16x3 operands are 8 consecutive k per lane, so
+// tiles stored transposed are transposed in registers (4 x 4 blocks) on their
+// way into a [row][k] LDS tile.
+#include <algorithm>
+
+#include "hfjk_internal.h"
+
+namespace hfjk {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ long long tidx(int r, int c) { return (long long)r * (r + 1) / 2 + c; }
+
+__device__ __forceinline__ void tile_of(int tile, int& r, int& c) {
+  int rr = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
+  while ((rr + 1) * (rr + 2) / 2 <= tile) ++rr;
+  while (rr * (rr + 1) / 2 > tile) --rr;
+  r = rr;
+  c = tile - rr * (rr + 1) / 2;
+}
+
+__device__ __forceinline__ uint16_t bf16_of(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float f32_of(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+// Logical index of linear workgroup b of a W-workgroup 1-D grid: XCD x
+// (b % 8) receives the consecutive logical range [x W/8, (x+1) W/8), so
+// neighbouring tiles (which share operands) share one L2.
+__device__ __forceinline__ int xcd_remap(int b, int w) {
+  const int full = w & ~7;
+  if (b >= full) return b;
+  return (b & 7) * (full >> 3) + (b >> 3);
+}
+
+template <typename T>
+struct Op;
+
+template <>
+struct Op<double> {
+  typedef d4 acc_t;
+  typedef double2 vec2;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+template <>
+struct Op<float> {
+  typedef f4 acc_t;
+  typedef float2 vec2;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// Zero a vector unless `keep`, component by component (a select of whole
+// vectors is lowered through a private-memory copy).
+__device__ __forceinline__ uint2 mask2(bool keep, uint2 v) {
+  const uint32_t m = keep ? 0xFFFFFFFFu : 0u;
+  return make_uint2(v.x & m, v.y & m);
+}
+
+__device__ __forceinline__ uint4 mask4(bool keep, uint4 v) {
+  const uint32_t m = keep ? 0xFFFFFFFFu : 0u;
+  return make_uint4(v.x & m, v.y & m, v.z & m, v.w & m);
+}
+
+// 8 bf16 from LDS as one 16-byte load, reinterpreted as the MFMA operand.
+__device__ __forceinline__ bf8 lds_bf8(const uint16_t* p) {
+  const uint4 v = *(const uint4*)p;
+  return __builtin_bit_cast(bf8, v);
+}
+
+__device__ __forceinline__ f4 mma3(bf8 ah, bf8 al, bf8 bh, bf8 bl, f4 acc) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+}
+
+// Wave arrangement of a 64 x BN half-transform tile: WN waves across the
+// columns; each wave covers (64 / WM) rows and (BN / WN) columns.
+template <int BN>
+struct XShape {
+  static constexpr int WN = BN >= 256 ? 4 : 2;
+  static constexpr int WM = 4 / WN;
+  static constexpr int MI = 64 / WM / 16;   // 16-row blocks per wave
+  static constexpr int NI = BN / WN / 16;   // 16-column blocks per wave
+};
+
+// ------------------------------------------------------------------ FP64 / FP32 half-transform
+
+// X[q][m][i] = sum_k B[q][m][k] C[k][i]; workgroup tile 64 rows (two layout
+// tile rows r = 2 mb, 2 mb + 1) x BN columns; one layout tile column kt per
+// k step.
+template <typename T, int BN>
+__global__ __launch_bounds__(kThreads) void xform_kernel(const T* __restrict__ b, long long p, int nt,
+                                                          const T* __restrict__ c, int ldo, T* __restrict__ x,
+                                                          int xrows, int qbase, int nblocks_n, int nblocks_m,
+                                                          int nwg) {
+  typedef XShape<BN> S;
+  constexpr int CCH = kTile * BN / 2 / kThreads;  // 2-element chunks of the C tile per thread
+  __shared__ T At[2][kTile][kTile + 1];
+  __shared__ T Cs[kTile][BN + 1];
+  const int logical = xcd_remap(blockIdx.x, nwg);
+  const int nb = logical % nblocks_n;
+  const int mb = (logical / nblocks_n) % nblocks_m;
+  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  const int n0 = nb * BN;
+  const T* bq = b + (long long)q * p;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave % S::WM;
+  const int wn = wave / S::WM;
+
+  typename Op<T>::acc_t acc[S::MI][S::NI];
+#pragma unroll
+  for (int mi = 0; mi < S::MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < S::NI; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
+
+  // Prefetch registers: 4 consecutive elements of each of the two tiles,
+  // and CCH 2-element chunks of the C tile.
+  typename Op<T>::vec2 ra[2][2];
+  typename Op<T>::vec2 rc[CCH];
+
+  auto load = [&](int kt) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = 2 * mb + h;
+      if (r < nt) {
+        const long long off = (r >= kt ? tidx(r, kt) : tidx(kt, r)) * kTileElems + t * 4;
+        const typename Op<T>::vec2* src = (const typename Op<T>::vec2*)(bq + off);
+        ra[h][0] = src[0];
+        ra[h][1] = src[1];
+      } else {
+        ra[h][0].x = ra[h][0].y = ra[h][1].x = ra[h][1].y = (T)0;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;
+      const int kk = chunk / (BN / 2);
+      const int col = n0 + (chunk % (BN / 2)) * 2;
+      if (col < ldo) {
+        rc[v] = *(const typename Op<T>::vec2*)(c + (long long)(kt * kTile + kk) * ldo + col);
+      } else {
+        rc[v].x = rc[v].y = (T)0;
+      }
+    }
+  };
+  auto store = [&]() {
+    const int a = (t * 4) / kTile;
+    const int bc = (t * 4) % kTile;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      At[h][a][bc + 0] = ra[h][0].x;
+      At[h][a][bc + 1] = ra[h][0].y;
+      At[h][a][bc + 2] = ra[h][1].x;
+      At[h][a][bc + 3] = ra[h][1].y;
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;
+      const int kk = chunk / (BN / 2);
+      const int jj = (chunk % (BN / 2)) * 2;
+      Cs[kk][jj] = rc[v].x;
+      Cs[kk][jj + 1] = rc[v].y;
+    }
+  };
+
+  load(0);
+  store();
+  __syncthreads();
+  for (int kt = 0; kt < nt; ++kt) {
+    if (kt + 1 < nt) load(kt + 1);
+    // Tile h is stored as (r, kt) when r >= kt (read A[i][k] = At[h][i][k]) and
+    // as (kt, r) otherwise (read A[i][k] = At[h][k][i]).
+    bool direct[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) direct[h] = (2 * mb + h) >= kt;
+#pragma unroll
+    for (int kk = 0; kk < kTile; kk += 4) {
+      const int k = kk + (lane >> 4);
+      T af[S::MI], bf[S::NI];
+#pragma unroll
+      for (int mi = 0; mi < S::MI; ++mi) {
+        const int row = wm * (64 / S::WM) + mi * 16 + (lane & 15);
+        const int h = row >> 5;
+        const int i = row & 31;
+        af[mi] = direct[h] ? At[h][i][k] : At[h][k][i];
+      }
+#pragma unroll
+      for (int ni = 0; ni < S::NI; ++ni) bf[ni] = Cs[k][wn * (BN / S::WN) + ni * 16 + (lane & 15)];
+#pragma unroll
+      for (int mi = 0; mi < S::MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < S::NI; ++ni) acc[mi][ni] = Op<T>::mma(af[mi], bf[ni], acc[mi][ni]);
+    }
+    __syncthreads();
+    if (kt + 1 < nt) {
+      store();
+      __syncthreads();
+    }
+  }
+  T* xq = x + (long long)q * xrows * ldo;
+#pragma unroll
+  for (int mi = 0; mi < S::MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < S::NI; ++ni) {
+      const int col = n0 + wn * (BN / S::WN) + ni * 16 + (lane & 15);
+      if (col < ldo) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = mb * kBlock + wm * (64 / S::WM) + mi * 16 + Op<T>::row(lane, r);
+          xq[(long long)row * ldo + col] = acc[mi][ni][r];
+        }
+      }
+    }
+}
+
+// ------------------------------------------------------------------ BF16x3 half-transform
+
+constexpr int kRowK = 40;  // bf16 LDS row: 32 k + 8 pad (80 bytes, 16-byte aligned rows)
+
+// X in BF16x3: workgroup tile 128 rows (four layout tile rows) x BN columns,
+// waves 2 x 2 each (64 x BN/2). A is staged [m][k]; C^T (stored [i][k])
+// is staged [i][k].
+template <int BN>
+__global__ __launch_bounds__(kThreads) void xform_bf16x3_kernel(
+    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, long long p, int nt,
+    const uint16_t* __restrict__ cthi, const uint16_t* __restrict__ ctlo, int npad, int ldo,
+    uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo, int xrows, int qbase, int nblocks_n, int nblocks_m,
+    int nwg) {
+  constexpr int MI = 4;            // 64 rows per wave
+  constexpr int NI = BN / 2 / 16;  // BN/2 columns per wave
+  constexpr int CCH = BN * 4 / kThreads > 0 ? BN * 4 / kThreads : 1;  // 16-byte chunks of C^T per plane
+  __shared__ __attribute__((aligned(16))) uint16_t Ah[128][kRowK];
+  __shared__ __attribute__((aligned(16))) uint16_t Al[128][kRowK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bh[BN][kRowK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bl[BN][kRowK];
+  const int logical = xcd_remap(blockIdx.x, nwg);
+  const int nb = logical % nblocks_n;
+  const int mb = (logical / nblocks_n) % nblocks_m;  // 128-row block
+  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  const int n0 = nb * BN;
+  const uint16_t* hq = bhi + (long long)q * p;
+  const uint16_t* lq = blo + (long long)q * p;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+
+  f4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // Each thread owns one 4 x 4 block of one of the four layout tiles.
+  const int th = t >> 6;            // tile row within the block: r = 4 mb + th
+  const int a0 = ((t & 63) >> 3) * 4;
+  const int b0 = (t & 7) * 4;
+  const int r_own = 4 * mb + th;
+  uint2 rah[4], ral[4];
+  uint4 rch[CCH], rcl[CCH];
+
+  // Loads are unconditional (from a valid clamped address) and masked by a
+  // select, which keeps the prefetch arrays in registers.
+  auto load = [&](int kt) {
+    const bool own = r_own < nt;
+    const int rc = own ? r_own : 0;
+    const long long base = (rc >= kt ? tidx(rc, kt) : tidx(kt, rc)) * kTileElems + a0 * kTile + b0;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const uint2 h = *(const uint2*)(hq + base + rr * kTile);
+      const uint2 l = *(const uint2*)(lq + base + rr * kTile);
+      rah[rr] = mask2(own, h);
+      ral[rr] = mask2(own, l);
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;  // BN rows x 4 chunks of 8 bf16
+      const int jj = chunk >> 2;
+      const int k8 = (chunk & 3) * 8;
+      const int col = n0 + jj;
+      const bool ok = chunk < BN * 4 && col < ldo;
+      const long long off = ok ? (long long)col * npad + kt * kTile + k8 : 0;
+      const uint4 h = *(const uint4*)(cthi + off);
+      const uint4 l = *(const uint4*)(ctlo + off);
+      rch[v] = mask4(ok, h);
+      rcl[v] = mask4(ok, l);
+    }
+  };
+  auto store = [&](int kt) {
+    const bool direct = r_own >= kt;  // also true for r_own >= nt (zeros)
+    if (direct) {
+      // element (a, b) is B[m = r*32 + a][k = kt*32 + b]
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        *(uint2*)&Ah[th * kTile + a0 + rr][b0] = rah[rr];
+        *(uint2*)&Al[th * kTile + a0 + rr][b0] = ral[rr];
+      }
+    } else {
+      // element (a, b) is B[k = kt*32 + a][m = r*32 + b]: transpose the 4 x 4 block
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        uint2 h, l;
+        const int sh = (cc & 1) * 16;
+        auto pick = [&](const uint2& v) -> uint32_t { return ((cc < 2 ? v.x : v.y) >> sh) & 0xFFFFu; };
+        h.x = pick(rah[0]) | (pick(rah[1]) << 16);
+        h.y = pick(rah[2]) | (pick(rah[3]) << 16);
+        l.x = pick(ral[0]) | (pick(ral[1]) << 16);
+        l.y = pick(ral[2]) | (pick(ral[3]) << 16);
+        *(uint2*)&Ah[th * kTile + b0 + cc][a0] = h;
+        *(uint2*)&Al[th * kTile + b0 + cc][a0] = l;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;
+      if (chunk < BN * 4) {
+        const int jj = chunk >> 2;
+        const int k8 = (chunk & 3) * 8;
+        *(uint4*)&Bh[jj][k8] = rch[v];
+        *(uint4*)&Bl[jj][k8] = rcl[v];
+      }
+    }
+  };
+
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nt; ++kt) {
+    if (kt + 1 < nt) load(kt + 1);
+    const int kq = 8 * (lane >> 4);
+    bf8 ah[MI], al[MI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) {
+      ah[mi] = lds_bf8(&Ah[wm * 64 + mi * 16 + (lane & 15)][kq]);
+      al[mi] = lds_bf8(&Al[wm * 64 + mi * 16 + (lane & 15)][kq]);
+    }
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      const bf8 bh = lds_bf8(&Bh[wn * (BN / 2) + ni * 16 + (lane & 15)][kq]);
+      const bf8 bl = lds_bf8(&Bl[wn * (BN / 2) + ni * 16 + (lane & 15)][kq]);
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi) acc[mi][ni] = mma3(ah[mi], al[mi], bh, bl, acc[mi][ni]);
+    }
+    __syncthreads();
+    if (kt + 1 < nt) {
+      store(kt + 1);
+      __syncthreads();
+    }
+  }
+  const long long xbase = (long long)q * xrows * ldo;
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      const int col = n0 + wn * (BN / 2) + ni * 16 + (lane & 15);
+      if (col < ldo) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = mb * 128 + wm * 64 + mi * 16 + 4 * (lane >> 4) + r;
+          if (row < xrows) {
+            const float v = acc[mi][ni][r];
+            const uint16_t h = bf16_of(v);
+            const long long off = xbase + (long long)row * ldo + col;
+            xhi[off] = h;
+            xlo[off] = bf16_of(v - f32_of(h));
+          }
+        }
+      }
+    }
+}
+
+// Orbitals [nao][nocc] float64 -> transposed split planes Ct[ldo][npad]
+// (hi plane, then lo plane), zero-padded.
+__global__ __launch_bounds__(kThreads) void orbitals_bf16x3_kernel(const double* __restrict__ c, int nao, int nocc,
+                                                                    int npad, int ldo, uint16_t* __restrict__ hi,
+                                                                    uint16_t* __restrict__ lo) {
+  const long long total = (long long)npad * ldo;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int i = (int)(idx / npad);
+    const int k = (int)(idx % npad);
+    const float v = (k < nao && i < nocc) ? (float)c[(long long)k * nocc + i] : 0.0f;
+    const uint16_t h = bf16_of(v);
+    hi[idx] = h;
+    lo[idx] = bf16_of(v - f32_of(h));
+  }
+}
+
+// ------------------------------------------------------------------ SYRK
+
+constexpr int kSyrkTile = 128;
+
+// Split s and tile pair of linear workgroup b (XCD-aware, nsplit % 8 == 0).
+__device__ __forceinline__ void syrk_coords(int b, int npairs, int& s, int& pair) {
+  const int xcd = b & 7;
+  const int slot = b >> 3;
+  s = xcd + 8 * (slot / npairs);
+  pair = slot % npairs;
+}
+
+// FP64 / FP32: partial[s][pair][128][128] over the aux slice s, all i.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x, int naux, int xrows, int ldo,
+                                                         const double* __restrict__ signs, int nsplit, int npairs,
+                                                         T* __restrict__ partial) {
+  constexpr int MI = 4, NI = 4;  // wave tile 64 x 64
+  __shared__ T As[kSyrkTile][kTile + 1];
+  __shared__ T Bs[kSyrkTile][kTile + 1];
+  int s, pair;
+  syrk_coords(blockIdx.x, npairs, s, pair);
+  int mb, nb;
+  tile_of(pair, mb, nb);
+  const int qb = (int)((long long)s * naux / nsplit);
+  const int qe = (int)((long long)(s + 1) * naux / nsplit);
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+  typename Op<T>::acc_t acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
+
+  // 128 rows x 32 columns per operand = 2048 2-element chunks: 8 per thread.
+  typename Op<T>::vec2 ra[8], rb[8];
+  const int nchunk = ldo / kTile;
+  const int mrows = min(kSyrkTile, xrows - mb * kSyrkTile);
+  const int nrows = min(kSyrkTile, xrows - nb * kSyrkTile);
+  auto load = [&](int q, int i0) {
+    const T* xm = x + ((long long)q * xrows + mb * kSyrkTile) * ldo + i0;
+    const T* xn = x + ((long long)q * xrows + nb * kSyrkTile) * ldo + i0;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int chunk = t + kThreads * v;
+      const int row = chunk >> 4;
+      const int c2 = (chunk & 15) * 2;
+      const bool okm = row < mrows;
+      const bool okn = row < nrows;
+      const typename Op<T>::vec2 a = *(const typename Op<T>::vec2*)(xm + (long long)(okm ? row : 0) * ldo + c2);
+      const typename Op<T>::vec2 bb = *(const typename Op<T>::vec2*)(xn + (long long)(okn ? row : 0) * ldo + c2);
+      const T sm = okm ? (T)1 : (T)0;
+      const T sn0 = okn ? (T)signs[i0 + c2] : (T)0;
+      const T sn1 = okn ? (T)signs[i0 + c2 + 1] : (T)0;
+      ra[v].x = a.x * sm;
+      ra[v].y = a.y * sm;
+      rb[v].x = bb.x * sn0;
+      rb[v].y = bb.y * sn1;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int chunk = t + kThreads * v;
+      const int row = chunk >> 4;
+      const int c2 = (chunk & 15) * 2;
+      As[row][c2] = ra[v].x;
+      As[row][c2 + 1] = ra[v].y;
+      Bs[row][c2] = rb[v].x;
+      Bs[row][c2 + 1] = rb[v].y;
+    }
+  };
+
+  const int steps = (qe - qb) * nchunk;
+  if (steps > 0) {
+    load(qb, 0);
+    store();
+  }
+  __syncthreads();
+  for (int step = 0; step < steps; ++step) {
+    const int nxt = step + 1;
+    if (nxt < steps) load(qb + nxt / nchunk, (nxt % nchunk) * kTile);
+#pragma unroll
+    for (int kk = 0; kk < kTile; kk += 4) {
+      const int k = kk + (lane >> 4);
+      T af[MI], bf[NI];
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi) af[mi] = As[wm * 64 + mi * 16 + (lane & 15)][k];
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni) bf[ni] = Bs[wn * 64 + ni * 16 + (lane & 15)][k];
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = Op<T>::mma(af[mi], bf[ni], acc[mi][ni]);
+    }
+    __syncthreads();
+    if (nxt < steps) {
+      store();
+      __syncthreads();
+    }
+  }
+  T* dst = partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile);
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      const int col = wn * 64 + ni * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = wm * 64 + mi * 16 + Op<T>::row(lane, r);
+        dst[row * kSyrkTile + col] = acc[mi][ni][r];
+      }
+    }
+}
+
+// BF16x3: partial (FP32) [s][pair][128][128].
+__global__ __launch_bounds__(kThreads) void syrk_bf16x3_kernel(const uint16_t* __restrict__ xhi,
+                                                                const uint16_t* __restrict__ xlo, int naux,
+                                                                int xrows, int ldo, const double* __restrict__ signs,
+                                                                int nsplit, int npairs, float* __restrict__ partial) {
+  constexpr int MI = 4, NI = 4;
+  __shared__ __attribute__((aligned(16))) uint16_t Ah[kSyrkTile][kRowK];
+  __shared__ __attribute__((aligned(16))) uint16_t Al[kSyrkTile][kRowK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bh[kSyrkTile][kRowK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bl[kSyrkTile][kRowK];
+  int s, pair;
+  syrk_coords(blockIdx.x, npairs, s, pair);
+  int mb, nb;
+  tile_of(pair, mb, nb);
+  const int qb = (int)((long long)s * naux / nsplit);
+  const int qe = (int)((long long)(s + 1) * naux / nsplit);
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+  f4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // 128 rows x 32 bf16 per plane = 512 16-byte chunks: 2 per thread per plane.
+  uint4 rah[2], ral[2], rbh[2], rbl[2];
+  const int nchunk = ldo / kTile;
+  const int mrows = min(kSyrkTile, xrows - mb * kSyrkTile);
+  const int nrows = min(kSyrkTile, xrows - nb * kSyrkTile);
+  auto load = [&](int q, int i0) {
+    const long long om = ((long long)q * xrows + mb * kSyrkTile) * ldo + i0;
+    const long long on = ((long long)q * xrows + nb * kSyrkTile) * ldo + i0;
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int chunk = t + kThreads * v;
+      const int row = chunk >> 2;
+      const int c8 = (chunk & 3) * 8;
+      const bool okm = row < mrows;
+      const bool okn = row < nrows;
+      const long long am = om + (long long)(okm ? row : 0) * ldo + c8;
+      const long long an = on + (long long)(okn ? row : 0) * ldo + c8;
+      const uint4 mh = *(const uint4*)(xhi + am);
+      const uint4 ml = *(const uint4*)(xlo + am);
+      uint4 h = *(const uint4*)(xhi + an);
+      uint4 l = *(const uint4*)(xlo + an);
+      rah[v] = mask4(okm, mh);
+      ral[v] = mask4(okm, ml);
+      // A negative sign flips the sign bit of the element in both planes.
+      const double* sg = signs + i0 + c8;
+      const uint32_t f0 = (sg[0] < 0.0 ? 0x8000u : 0u) | (sg[1] < 0.0 ? 0x80000000u : 0u);
+      const uint32_t f1 = (sg[2] < 0.0 ? 0x8000u : 0u) | (sg[3] < 0.0 ? 0x80000000u : 0u);
+      const uint32_t f2 = (sg[4] < 0.0 ? 0x8000u : 0u) | (sg[5] < 0.0 ? 0x80000000u : 0u);
+      const uint32_t f3 = (sg[6] < 0.0 ? 0x8000u : 0u) | (sg[7] < 0.0 ? 0x80000000u : 0u);
+      h.x ^= f0;
+      h.y ^= f1;
+      h.z ^= f2;
+      h.w ^= f3;
+      l.x ^= f0;
+      l.y ^= f1;
+      l.z ^= f2;
+      l.w ^= f3;
+      rbh[v] = mask4(okn, h);
+      rbl[v] = mask4(okn, l);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int chunk = t + kThreads * v;
+      const int row = chunk >> 2;
+      const int c8 = (chunk & 3) * 8;
+      *(uint4*)&Ah[row][c8] = rah[v];
+      *(uint4*)&Al[row][c8] = ral[v];
+      *(uint4*)&Bh[row][c8] = rbh[v];
+      *(uint4*)&Bl[row][c8] = rbl[v];
+    }
+  };
+
+  const int steps = (qe - qb) * nchunk;
+  if (steps > 0) {
+    load(qb, 0);
+    store();
+  }
+  __syncthreads();
+  for (int step = 0; step < steps; ++step) {
+    const int nxt = step + 1;
+    if (nxt < steps) load(qb + nxt / nchunk, (nxt % nchunk) * kTile);
+    const int kq = 8 * (lane >> 4);
+    bf8 ah[MI], al[MI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) {
+      ah[mi] = lds_bf8(&Ah[wm * 64 + mi * 16 + (lane & 15)][kq]);
+      al[mi] = lds_bf8(&Al[wm * 64 + mi * 16 + (lane & 15)][kq]);
+    }
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      const bf8 bh = lds_bf8(&Bh[wn * 64 + ni * 16 + (lane & 15)][kq]);
+      const bf8 bl = lds_bf8(&Bl[wn * 64 + ni * 16 + (lane & 15)][kq]);
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi) acc[mi][ni] = mma3(ah[mi], al[mi], bh, bl, acc[mi][ni]);
+    }
+    __syncthreads();
+    if (nxt < steps) {
+      store();
+      __syncthreads();
+    }
+  }
+  float* dst = partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile);
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      const int col = wn * 64 + ni * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = wm * 64 + mi * 16 + 4 * (lane >> 4) + r;
+        dst[row * kSyrkTile + col] = acc[mi][ni][r];
+      }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void syrk_reduce_kernel(const T* __restrict__ partial, int nsplit,
+                                                                int npairs, int nao, double* __restrict__ out) {
+  const long long total = (long long)nao * nao;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int row = (int)(idx / nao);
+    const int col = (int)(idx % nao);
+    const int bm = row / kSyrkTile, bn = col / kSyrkTile;
+    long long pair;
+    int local;
+    if (bm >= bn) {
+      pair = tidx(bm, bn);
+      local = (row % kSyrkTile) * kSyrkTile + col % kSyrkTile;
+    } else {
+      pair = tidx(bn, bm);
+      local = (col % kSyrkTile) * kSyrkTile + row % kSyrkTile;
+    }
+    double sum = 0.0;
+    for (int k = 0; k < nsplit; ++k)
+      sum += (double)partial[((long long)k * npairs + pair) * (kSyrkTile * kSyrkTile) + local];
+    out[idx] = sum;
+  }
+}
+
+template <typename T, int BN>
+hipError_t launch_x_t(const T* b, long long p, int naux, int nao, const T* c, int ldo, T* x, hipStream_t s) {
+  const int nt = n_tiles(nao);
+  const int xrows = x_rows(nao);
+  const int nbn = (ldo + BN - 1) / BN;
+  const int nbm = xrows / kBlock;
+  const int per = std::max(1, (1 << 24) / (nbn * nbm));  // keep a launch below 2^32 work-items
+  for (int base = 0; base < naux; base += per) {
+    const int count = std::min(per, naux - base);
+    const int nwg = nbn * nbm * count;
+    hipLaunchKernelGGL((xform_kernel<T, BN>), dim3((unsigned)nwg), dim3(kThreads), 0, s, b, p, nt, c, ldo, x, xrows,
+                       base, nbn, nbm, nwg);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+template <int BN>
+hipError_t launch_x16_t(const uint16_t* bhi, const uint16_t* blo, long long p, int naux, int nao,
+                        const uint16_t* cthi, const uint16_t* ctlo, int ldo, uint16_t* xhi, uint16_t* xlo,
+                        hipStream_t s) {
+  const int nt = n_tiles(nao);
+  const int npad = nt * kTile;
+  const int xrows = x_rows(nao);
+  const int nbm = (xrows + 127) / 128;
+  const int nbn = (ldo + BN - 1) / BN;
+  const int per = std::max(1, (1 << 24) / (nbn * nbm));
+  for (int base = 0; base < naux; base += per) {
+    const int count = std::min(per, naux - base);
+    const int nwg = nbn * nbm * count;
+    hipLaunchKernelGGL((xform_bf16x3_kernel<BN>), dim3((unsigned)nwg), dim3(kThreads), 0, s, bhi, blo, p, nt, cthi,
+                       ctlo, npad, ldo, xhi, xlo, xrows, base, nbn, nbm, nwg);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s) {
+  const int npad = n_tiles(nao) * kTile;
+  const int ldo = occ_ld(nocc);
+  const long long n = (long long)npad * ldo;
+  uint16_t* hi = (uint16_t*)out;
+  hipLaunchKernelGGL(orbitals_bf16x3_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, c, nao, nocc, npad, ldo, hi,
+                     hi + n);
+  return hipGetLastError();
+}
+
+int syrk_pairs(int nao) {
+  const int nb = (x_rows(nao) + kSyrkTile - 1) / kSyrkTile;
+  return nb * (nb + 1) / 2;
+}
+
+int syrk_tile() { return kSyrkTile; }
+
+hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao, const void* c, int ldo,
+                                 void* x, hipStream_t s) {
+  if (precision == HFJK_FP64) {
+    const double* bb = (const double*)b;
+    const double* cc = (const double*)c;
+    double* xx = (double*)x;
+    if (ldo <= 32) return launch_x_t<double, 32>(bb, p, naux, nao, cc, ldo, xx, s);
+    if (ldo <= 64) return launch_x_t<double, 64>(bb, p, naux, nao, cc, ldo, xx, s);
+    if (ldo <= 128) return launch_x_t<double, 128>(bb, p, naux, nao, cc, ldo, xx, s);
+    return launch_x_t<double, 256>(bb, p, naux, nao, cc, ldo, xx, s);
+  }
+  const float* bb = (const float*)b;
+  const float* cc = (const float*)c;
+  float* xx = (float*)x;
+  if (ldo <= 32) return launch_x_t<float, 32>(bb, p, naux, nao, cc, ldo, xx, s);
+  if (ldo <= 64) return launch_x_t<float, 64>(bb, p, naux, nao, cc, ldo, xx, s);
+  if (ldo <= 128) return launch_x_t<float, 128>(bb, p, naux, nao, cc, ldo, xx, s);
+  return launch_x_t<float, 256>(bb, p, naux, nao, cc, ldo, xx, s);
+}
+
+hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
+                                        void* x, hipStream_t s) {
+  const uint16_t* bhi = b;
+  const uint16_t* blo = b + (long long)naux * p;
+  const long long nct = (long long)n_tiles(nao) * kTile * ldo;
+  const uint16_t* cthi = (const uint16_t*)ct;
+  const uint16_t* ctlo = cthi + nct;
+  const long long nx = (long long)naux * x_rows(nao) * ldo;
+  uint16_t* xhi = (uint16_t*)x;
+  uint16_t* xlo = xhi + nx;
+  if (ldo <= 32) return launch_x16_t<32>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
+  if (ldo <= 64) return launch_x16_t<64>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
+  if (ldo <= 128) return launch_x16_t<128>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
+  return launch_x16_t<256>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
+}
+
+hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs, int nsplit,
+                       void* partial, hipStream_t s) {
+  const int xrows = x_rows(nao);
+  const int npairs = syrk_pairs(nao);
+  const unsigned grid = (unsigned)(npairs * nsplit);
+  if (precision == HFJK_FP64)
+    hipLaunchKernelGGL(syrk_kernel<double>, dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux, xrows, ldo,
+                       signs, nsplit, npairs, (double*)partial);
+  else if (precision == HFJK_FP32)
+    hipLaunchKernelGGL(syrk_kernel<float>, dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux, xrows, ldo, signs,
+                       nsplit, npairs, (float*)partial);
+  else {
+    const long long nx = (long long)naux * xrows * ldo;
+    const uint16_t* xhi = (const uint16_t*)x;
+    hipLaunchKernelGGL(syrk_bf16x3_kernel, dim3(grid), dim3(kThreads), 0, s, xhi, xhi + nx, naux, xrows, ldo, signs,
+                       nsplit, npairs, (float*)partial);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, int nao, double* out, hipStream_t s) {
+  const int npairs = syrk_pairs(nao);
+  const long long n = (long long)nao * nao;
+  if (precision == HFJK_FP64)
+    hipLaunchKernelGGL(syrk_reduce_kernel<double>, dim3(grid_for(n)), dim3(kThreads), 0, s, (const double*)partial,
+                       nsplit, npairs, nao, out);
+  else
+    hipLaunchKernelGGL(syrk_reduce_kernel<float>, dim3(grid_for(n)), dim3(kThreads), 0, s, (const float*)partial,
+                       nsplit, npairs, nao, out);
+  return hipGetLastError();
+}
+
+}  // namespace hfjk

@@ -77,10 +77,13 @@ def test_fp64_matches_golden_c0(torch, synthetic_c0):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
-@pytest.mark.parametrize("naux,nao,nocc", [(1, 1, 1), (7, 33, 1), (5, 64, 33), (11, 65, 40), (3, 130, 130), (17, 97, 70)])
+@pytest.mark.parametrize("naux,nao,nocc", [(1, 1, 1), (7, 33, 1), (5, 64, 33), (11, 65, 40), (3, 130, 130),
+                                           (17, 97, 70), (4, 200, 100), (3, 300, 300), (9, 129, 17)])
 def test_ragged_shapes_match_the_oracle(torch, precision, naux, nao, nocc):
+    # The occupied widths cover every column-tile shape of the half-transform
+    # (32, 64, 128, 256 and two 256-wide tiles) and partial 128-row SYRK tiles.
     seed = 100 + nao
-    b, c, d = synthetic.jk_inputs(seed, naux, nao, nocc) if nocc <= nao else (None, None, None)
+    b, c, d = synthetic.jk_inputs(seed, naux, nao, nocc)
     jk = builder(b, precision=precision)
     tol = TOL[precision]
     assert rel(jk.coulomb(d), jk_ref.coulomb(b, d)) < tol
