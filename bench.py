@@ -124,6 +124,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="large", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-slice", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="use the library's default tiles")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB read+write elementwise kernel (counter calibration)")
     args = ap.parse_args(argv)
@@ -168,8 +169,16 @@ def main(argv=None):
     f_build = flops_per_build(n, q, o)
     results = {}
     reference = None
+    from hafnium_amd.autotune import autotune
+
     for mode in modes:
         jk.precision = mode
+        tuning = None
+        if not args.no_tune:
+            tuned = autotune(jk, c, mode, cache_path=None)
+            tuning = {"x_bn": tuned.x_bn, "syrk_splits": tuned.syrk_splits, "x_ms": tuned.x_ms,
+                      "syrk_ms": tuned.syrk_ms, "seconds": round(tuned.seconds, 2)}
+            log(f"{mode}: tuned {tuning}")
         for _ in range(args.warmup):
             jk.coulomb(d)
             jk.exchange_occupied(c)
@@ -194,6 +203,7 @@ def main(argv=None):
             "builds_per_s": args.steps / elapsed,
             "ms_per_build": 1000.0 * elapsed / args.steps,
             "stage_ms": {k_: round(v, 4) for k_, v in avg.items()},
+            "tuning": tuning,
         }
         if mode == "fp64":
             reference = (j_np, k_np)
@@ -221,8 +231,9 @@ def main(argv=None):
     if x_ms:
         achieved = x_flops / (x_ms * 1e-3) / 1e12
         peak = PEAK_TFLOPS[head]
-        kernel = {"fp64": "half_transform_kernel<double, 128>", "fp32": "half_transform_kernel<float, 128>",
-                  "bf16x3": "half_transform_bf16x3_kernel<128>"}[head]
+        tile = (main_entry.get("tuning") or {}).get("x_bn") or (256 if o > 128 else 128)
+        kernel = {"fp64": f"xform_kernel<double, {tile}>", "fp32": f"xform_kernel<float, {tile}>",
+                  "bf16x3": f"xform_bf16x3_kernel<{tile}>"}[head]
         roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved, "peak": peak,
                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(kernel, n, q, o),
                     "algorithmic_flops_per_launch": x_flops,

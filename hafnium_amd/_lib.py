@@ -20,6 +20,10 @@ FP32 = 1
 BF16X3 = 2
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
+TUNE_X_BN = 1
+TUNE_X16_BN = 2
+TUNE_SYRK_SPLITS = 3
+
 OK = 0
 ERR_INPUT = 1
 ERR_BACKEND = 2
@@ -50,6 +54,8 @@ SIGNATURES = [
     ("hfjk_coulomb", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_set_tuning", _c_int, [_vp, _c_int, _c_int]),
+    ("hfjk_get_tuning", _c_int, [_vp, _c_int, ctypes.POINTER(_c_int)]),
     ("hfjk_profile", _c_int, [_vp, _c_int]),
     ("hfjk_profile_read", _c_int, [_vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), _c_int]),
 ]

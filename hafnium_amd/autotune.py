@@ -1,0 +1,139 @@
+"""Autotuner of the exchange kernels' tile schedule.
+
+For a bound tensor, an occupied width and a precision it times every
+candidate column tile of the half-transform and every candidate SYRK split
+count with the library's HIP-event stage timers, keeps the fastest of each
+(the two kernels are independent: the split count only moves the SYRK, the
+tile only the half-transform) and leaves them set on the context. Results are
+cached in a JSON file keyed by device, precision and shape, so a later run
+(or another process) reuses them without timing.
+
+No counterpart in the reference (its J/K has no tunable kernels).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+from dataclasses import dataclass, field
+
+from hafnium_amd import _lib
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "hafnium_amd", "tuning.json")
+TILE_CANDIDATES = (32, 64, 128, 256)
+
+
+@dataclass
+class TuningResult:
+    key: str
+    x_bn: int
+    syrk_splits: int
+    x_ms: dict = field(default_factory=dict)
+    syrk_ms: dict = field(default_factory=dict)
+    seconds: float = 0.0
+    from_cache: bool = False
+
+
+def _device_name(jk) -> str:
+    import torch
+
+    return torch.cuda.get_device_properties(jk.device).name or "gpu"
+
+
+def tuning_key(jk, nocc: int, precision: str) -> str:
+    return f"{_device_name(jk)}|{precision}|nao={jk.n_ao}|naux={jk.n_aux}|nocc={nocc}"
+
+
+def _split_candidates(jk) -> list[int]:
+    nb = (-(-jk.n_ao // 64) * 64 + 127) // 128
+    pairs = nb * (nb + 1) // 2
+    out = set()
+    for target in (512, 1024, 2048, 4096):
+        s = max(1, min(-(-target // pairs), jk.n_aux))
+        out.add(-(-s // 8) * 8)
+    return sorted(out)
+
+
+def _stage_ms(stages, name):
+    vals = [ms for n, ms in stages if n == name]
+    return min(vals) if vals else float("inf")
+
+
+def autotune(jk, orbitals, precision: str = "fp64", *, repeats: int = 2, cache_path: str | None = DEFAULT_CACHE,
+             use_cache: bool = True) -> TuningResult:
+    """Choose and set the fastest half-transform tile and SYRK split count.
+
+    ``orbitals`` is a representative ``[nao, nocc]`` matrix (values do not
+    affect the timing). The chosen parameters stay set on ``jk``.
+    """
+    import torch
+
+    nocc = int(orbitals.shape[1])
+    key = tuning_key(jk, nocc, precision)
+    tile_param = _lib.TUNE_X16_BN if precision == "bf16x3" else _lib.TUNE_X_BN
+    if use_cache and cache_path and os.path.exists(cache_path):
+        with open(cache_path) as fh:
+            cached = json.load(fh).get(key)
+        if cached:
+            jk.set_tuning(tile_param, cached["x_bn"])
+            jk.set_tuning(_lib.TUNE_SYRK_SPLITS, cached["syrk_splits"])
+            return TuningResult(key, cached["x_bn"], cached["syrk_splits"], cached.get("x_ms", {}),
+                                cached.get("syrk_ms", {}), 0.0, True)
+
+    start = time.perf_counter()
+    ldo = -(-nocc // 32) * 32
+    tiles = [t for t in TILE_CANDIDATES if t // 2 < ldo or t == 32]
+    splits = _split_candidates(jk)
+    x_ms: dict[int, float] = {}
+    syrk_ms: dict[int, float] = {}
+    jk.profile(True)
+    try:
+        jk.stage_times()
+        # Half-transform tile, with the default split count.
+        jk.set_tuning(_lib.TUNE_SYRK_SPLITS, 0)
+        for tile in tiles:
+            jk.set_tuning(tile_param, tile)
+            jk.exchange_occupied(orbitals, precision=precision)  # warm-up
+            torch.cuda.synchronize(jk.device)
+            jk.stage_times()
+            for _ in range(repeats):
+                jk.exchange_occupied(orbitals, precision=precision)
+            x_ms[tile] = _stage_ms(jk.stage_times(), "exchange_half_transform")
+        best_tile = min(x_ms, key=x_ms.get)
+        jk.set_tuning(tile_param, best_tile)
+        for split in splits:
+            jk.set_tuning(_lib.TUNE_SYRK_SPLITS, split)
+            jk.exchange_occupied(orbitals, precision=precision)
+            torch.cuda.synchronize(jk.device)
+            jk.stage_times()
+            for _ in range(repeats):
+                jk.exchange_occupied(orbitals, precision=precision)
+            stages = jk.stage_times()
+            syrk_ms[split] = _stage_ms(stages, "exchange_syrk") + _stage_ms(stages, "exchange_reduce")
+        best_split = min(syrk_ms, key=syrk_ms.get)
+        jk.set_tuning(_lib.TUNE_SYRK_SPLITS, best_split)
+    finally:
+        jk.profile(False)
+    result = TuningResult(key, best_tile, best_split, {str(k): v for k, v in x_ms.items()},
+                          {str(k): v for k, v in syrk_ms.items()}, time.perf_counter() - start)
+    if cache_path:
+        _store(cache_path, key, result)
+    return result
+
+
+def _store(path: str, key: str, result: TuningResult) -> None:
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    data = {}
+    if os.path.exists(path):
+        try:
+            with open(path) as fh:
+                data = json.load(fh)
+        except (OSError, ValueError):
+            data = {}
+    data[key] = {"x_bn": result.x_bn, "syrk_splits": result.syrk_splits, "x_ms": result.x_ms,
+                 "syrk_ms": result.syrk_ms}
+    tmp = path + ".tmp"
+    with open(tmp, "w") as fh:
+        json.dump(data, fh, indent=1)
+    os.replace(tmp, path)

@@ -288,6 +288,17 @@ class MI355XCoulombExchange:
             self._stream()))
         return self._out_like(out, template)
 
+    # --------------------------------------------------------------- tuning
+
+    def set_tuning(self, param: int, value: int) -> None:
+        """Set a kernel tuning parameter (``_lib.TUNE_*``; 0 restores the default)."""
+        _lib.check(self._ctx.lib.hfjk_set_tuning(self._ctx.handle, int(param), int(value)))
+
+    def get_tuning(self, param: int) -> int:
+        out = ctypes.c_int()
+        _lib.check(self._ctx.lib.hfjk_get_tuning(self._ctx.handle, int(param), ctypes.byref(out)))
+        return int(out.value)
+
     # --------------------------------------------------------------- timing
 
     def profile(self, enable: bool = True) -> None:

@@ -129,7 +129,7 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   (void)nb;
   L.npairs = hfjk::syrk_pairs(n);
   L.nsplit_j = coulomb_splits(L.ntiles, naux);
-  L.nsplit_k = syrk_splits(L.npairs, naux);
+  L.nsplit_k = ctx->syrk_splits > 0 ? ((ctx->syrk_splits + 7) / 8) * 8 : syrk_splits(L.npairs, naux);
   size_t off = 0;
   const size_t nn = (size_t)n * n;
   L.dens_a = off; off = align_up(off + nn * 8);
@@ -253,10 +253,11 @@ void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double*
   {
     Stage st(ctx, s, "exchange_half_transform");
     if (split)
-      check_hip(hfjk::launch_half_transform_bf16x3((const uint16_t*)b, L.p, ctx->naux, ctx->nao, cp, L.ldo, x, s),
+      check_hip(hfjk::launch_half_transform_bf16x3((const uint16_t*)b, L.p, ctx->naux, ctx->nao, cp, L.ldo, x,
+                                                   ctx->x16_bn, s),
                 "half transform launch");
     else
-      check_hip(hfjk::launch_half_transform(b, precision, L.p, ctx->naux, ctx->nao, cp, L.ldo, x, s),
+      check_hip(hfjk::launch_half_transform(b, precision, L.p, ctx->naux, ctx->nao, cp, L.ldo, x, ctx->x_bn, s),
                 "half transform launch");
   }
   {
@@ -521,6 +522,40 @@ int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange
       check_hip(hfjk::launch_split_factor(a, w, n, nneg, npos, cf, sg, s), "split factor launch");
     }
     run_exchange(ctx, cf, r, sg, exchange, precision, workspace, make_layout(ctx, r, kprec), s);
+  });
+}
+
+int hfjk_set_tuning(hfjk_ctx* ctx, int param, int value) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (value < 0) fail(HFJK_ERR_INPUT, "tuning values are non-negative");
+    switch (param) {
+      case HFJK_TUNE_X_BN:
+      case HFJK_TUNE_X16_BN:
+        if (value != 0 && value != 32 && value != 64 && value != 128 && value != 256)
+          fail(HFJK_ERR_INPUT, "the half-transform column tile is 0, 32, 64, 128 or 256");
+        (param == HFJK_TUNE_X_BN ? ctx->x_bn : ctx->x16_bn) = value;
+        break;
+      case HFJK_TUNE_SYRK_SPLITS:
+        if (value > 65536) fail(HFJK_ERR_INPUT, "at most 65536 SYRK splits");
+        ctx->syrk_splits = value;
+        break;
+      default:
+        fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
+    }
+  });
+}
+
+int hfjk_get_tuning(hfjk_ctx* ctx, int param, int* value) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (value == nullptr) fail(HFJK_ERR_INPUT, "null output pointer");
+    switch (param) {
+      case HFJK_TUNE_X_BN: *value = ctx->x_bn; break;
+      case HFJK_TUNE_X16_BN: *value = ctx->x16_bn; break;
+      case HFJK_TUNE_SYRK_SPLITS: *value = ctx->syrk_splits; break;
+      default: fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
+    }
   });
 }
 

@@ -74,6 +74,10 @@ struct hfjk_ctx {
   size_t n_stages = 0;
   // rocSOLVER / rocBLAS handle for the density factorisation (lazy).
   void* blas = nullptr;
+  // Tuning (0 = default heuristics).
+  int x_bn = 0;
+  int x16_bn = 0;
+  int syrk_splits = 0;
 };
 
 namespace hfjk {
@@ -97,8 +101,9 @@ hipError_t launch_coulomb_unpack(const double* partial, int nsplit, long long p,
 
 // Exchange (jk.hip).
 hipError_t launch_orbitals(const double* c, int nao, int nocc, int precision, void* out, hipStream_t s);
+// bn = column tile (32, 64, 128, 256) or 0 for the default for ldo.
 hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao,
-                                 const void* c, int ldo, void* x, hipStream_t s);
+                                 const void* c, int ldo, void* x, int bn, hipStream_t s);
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
                        int nsplit, void* partial, hipStream_t s);
 hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, int nao, double* out,
@@ -109,7 +114,7 @@ hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* s
 // takes HFJK_BF16X3 for the split-product kernel.
 hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s);
 hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
-                                        void* x, hipStream_t s);
+                                        void* x, int bn, hipStream_t s);
 // Lower-triangle 128 x 128 output tiles of the SYRK and the tile edge.
 int syrk_pairs(int nao);
 int syrk_tile();

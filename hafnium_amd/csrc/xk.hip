@@ -754,28 +754,37 @@ int syrk_pairs(int nao) {
 
 int syrk_tile() { return kSyrkTile; }
 
+// Column tile: the tuned value when one is set, else the smallest of
+// 32/64/128/256 that covers ldo.
+static int pick_bn(int bn, int ldo) {
+  if (bn == 32 || bn == 64 || bn == 128 || bn == 256) return bn;
+  return ldo <= 32 ? 32 : ldo <= 64 ? 64 : ldo <= 128 ? 128 : 256;
+}
+
 hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao, const void* c, int ldo,
-                                 void* x, hipStream_t s) {
+                                 void* x, int bn, hipStream_t s) {
+  bn = pick_bn(bn, ldo);
   if (precision == HFJK_FP64) {
     const double* bb = (const double*)b;
     const double* cc = (const double*)c;
     double* xx = (double*)x;
-    if (ldo <= 32) return launch_x_t<double, 32>(bb, p, naux, nao, cc, ldo, xx, s);
-    if (ldo <= 64) return launch_x_t<double, 64>(bb, p, naux, nao, cc, ldo, xx, s);
-    if (ldo <= 128) return launch_x_t<double, 128>(bb, p, naux, nao, cc, ldo, xx, s);
+    if (bn == 32) return launch_x_t<double, 32>(bb, p, naux, nao, cc, ldo, xx, s);
+    if (bn == 64) return launch_x_t<double, 64>(bb, p, naux, nao, cc, ldo, xx, s);
+    if (bn == 128) return launch_x_t<double, 128>(bb, p, naux, nao, cc, ldo, xx, s);
     return launch_x_t<double, 256>(bb, p, naux, nao, cc, ldo, xx, s);
   }
   const float* bb = (const float*)b;
   const float* cc = (const float*)c;
   float* xx = (float*)x;
-  if (ldo <= 32) return launch_x_t<float, 32>(bb, p, naux, nao, cc, ldo, xx, s);
-  if (ldo <= 64) return launch_x_t<float, 64>(bb, p, naux, nao, cc, ldo, xx, s);
-  if (ldo <= 128) return launch_x_t<float, 128>(bb, p, naux, nao, cc, ldo, xx, s);
+  if (bn == 32) return launch_x_t<float, 32>(bb, p, naux, nao, cc, ldo, xx, s);
+  if (bn == 64) return launch_x_t<float, 64>(bb, p, naux, nao, cc, ldo, xx, s);
+  if (bn == 128) return launch_x_t<float, 128>(bb, p, naux, nao, cc, ldo, xx, s);
   return launch_x_t<float, 256>(bb, p, naux, nao, cc, ldo, xx, s);
 }
 
 hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
-                                        void* x, hipStream_t s) {
+                                        void* x, int bn, hipStream_t s) {
+  bn = pick_bn(bn, ldo);
   const uint16_t* bhi = b;
   const uint16_t* blo = b + (long long)naux * p;
   const long long nct = (long long)n_tiles(nao) * kTile * ldo;
@@ -784,9 +793,9 @@ hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux
   const long long nx = (long long)naux * x_rows(nao) * ldo;
   uint16_t* xhi = (uint16_t*)x;
   uint16_t* xlo = xhi + nx;
-  if (ldo <= 32) return launch_x16_t<32>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
-  if (ldo <= 64) return launch_x16_t<64>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
-  if (ldo <= 128) return launch_x16_t<128>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
+  if (bn == 32) return launch_x16_t<32>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
+  if (bn == 64) return launch_x16_t<64>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
+  if (bn == 128) return launch_x16_t<128>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
   return launch_x16_t<256>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
 }
 

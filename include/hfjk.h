@@ -135,6 +135,19 @@ HFJK_API int hfjk_exchange_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, 
 HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double *exchange, int precision,
                           void *workspace, size_t workspace_bytes, void *stream);
 
+/* Tuning parameters of the kernels (0 = the library's default choice).
+ * HFJK_TUNE_X_BN: column tile of the FP64/FP32 half-transform (32, 64, 128, 256).
+ * HFJK_TUNE_X16_BN: column tile of the BF16x3 half-transform (32, 64, 128, 256).
+ * HFJK_TUNE_SYRK_SPLITS: aux-slice splits of the SYRK (rounded up to a multiple of 8).
+ * Set by the project's autotuner (hafnium_amd/autotune.py); the workspace size
+ * depends on HFJK_TUNE_SYRK_SPLITS, so query it after tuning. No counterpart
+ * in the reference. */
+#define HFJK_TUNE_X_BN 1
+#define HFJK_TUNE_X16_BN 2
+#define HFJK_TUNE_SYRK_SPLITS 3
+HFJK_API int hfjk_set_tuning(hfjk_ctx *ctx, int param, int value);
+HFJK_API int hfjk_get_tuning(hfjk_ctx *ctx, int param, int *value);
+
 /* Stage timing: when enabled, every later call records a HIP event pair on
  * its stream around each kernel stage it launches (at most 4096 stages are
  * kept). hfjk_profile_read synchronises on the recorded events, returns up

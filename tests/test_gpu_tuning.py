@@ -1,0 +1,49 @@
+"""Every tile schedule the autotuner can choose gives the oracle's J/K."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from hafnium_amd import _lib, synthetic
+from oracle import jk_ref
+from tests.tolerances import BF16X3_REL_TOL, FP32_REL_TOL, FP64_REL_TOL
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": FP64_REL_TOL, "fp32": FP32_REL_TOL, "bf16x3": BF16X3_REL_TOL}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+def test_all_tiles_and_splits_agree_with_the_oracle(precision):
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    naux, nao, nocc = 13, 150, 90
+    b, c, _ = synthetic.jk_inputs(77, naux, nao, nocc)
+    expect = jk_ref.exchange_occupied(b, c)
+    jk = MI355XCoulombExchange(b, precision=precision)
+    param = _lib.TUNE_X16_BN if precision == "bf16x3" else _lib.TUNE_X_BN
+    for tile in (32, 64, 128, 256):
+        for splits in (8, 16, 24):
+            jk.set_tuning(param, tile)
+            jk.set_tuning(_lib.TUNE_SYRK_SPLITS, splits)
+            k = jk.exchange_occupied(c)
+            assert jk_ref.max_relative_error(k, expect) < TOL[precision], (tile, splits)
+    with pytest.raises(ValueError):
+        jk.set_tuning(param, 48)
+
+
+def test_autotune_sets_a_valid_schedule(tmp_path):
+    from hafnium_amd.autotune import autotune
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    b, c, _ = synthetic.jk_inputs(5, 40, 200, 60)
+    jk = MI355XCoulombExchange(b, precision="bf16x3")
+    cache = str(tmp_path / "tuning.json")
+    result = autotune(jk, c, "bf16x3", cache_path=cache)
+    assert result.x_bn in (32, 64, 128, 256) and result.syrk_splits % 8 == 0
+    assert jk.get_tuning(_lib.TUNE_X16_BN) == result.x_bn
+    again = autotune(MI355XCoulombExchange(b, precision="bf16x3"), c, "bf16x3", cache_path=cache)
+    assert again.from_cache and (again.x_bn, again.syrk_splits) == (result.x_bn, result.syrk_splits)
+    k = jk.exchange_occupied(c)
+    assert jk_ref.max_relative_error(k, jk_ref.exchange_occupied(b, c)) < BF16X3_REL_TOL
