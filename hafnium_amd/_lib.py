@@ -54,6 +54,16 @@ SIGNATURES = [
     ("hfjk_coulomb", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_dgemm", _c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_double, _vp, _c_int, _vp,
+                            _c_int, ctypes.c_double, _vp, _c_int, _vp]),
+    ("hfjk_dsyevd", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    ("hfjk_dpotrf", _c_int, [_vp, _c_int, _vp, _vp, _vp]),
+    ("hfjk_dtrsm_lower", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp]),
+    ("hfjk_combine", _c_int, [_vp, _c_ll, ctypes.c_double, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp,
+                              _vp]),
+    ("hfjk_antisymmetrize", _c_int, [_vp, _c_int, _vp, _vp, _vp]),
+    ("hfjk_dot_workspace", _c_ll, []),
+    ("hfjk_ddot", _c_int, [_vp, _c_ll, _vp, _vp, _vp, _vp, _vp]),
     ("hfjk_set_tuning", _c_int, [_vp, _c_int, _c_int]),
     ("hfjk_get_tuning", _c_int, [_vp, _c_int, ctypes.POINTER(_c_int)]),
     ("hfjk_profile", _c_int, [_vp, _c_int]),

@@ -304,6 +304,8 @@ __global__ void density_stats_kernel(const double* __restrict__ d, int n, unsign
 
 }  // namespace
 
+void hfjk_internal_set_error(const char* message) { g_last_error = message != nullptr ? message : ""; }
+
 extern "C" {
 
 int hfjk_version(void) { return 100; }

@@ -80,6 +80,9 @@ struct hfjk_ctx {
   int syrk_splits = 0;
 };
 
+// Sets the calling thread's hfjk_last_error message (capi.hip).
+void hfjk_internal_set_error(const char* message);
+
 namespace hfjk {
 
 // Kernel launchers (gen.hip). Each returns a hipError_t.

@@ -1,0 +1,52 @@
+"""The fitted tensor on the GPU: ``B = L^-1 (P|uv)`` with the metric ``(P|Q) = L L^T``.
+
+The reference forms ``B = M^-1/2 (P|uv)`` from an eigendecomposition of the
+metric with eigenvalues below 1e-10 of the largest dropped
+(hafnium/integrals/density_fitting.py:97-107, :181-197). J and K depend on B
+only through ``B^T B = (uv|P) M^-1 (Q|wx)``, which any factor ``B = F (P|uv)``
+with ``F^T F = M^-1`` reproduces, so the FP64 Cholesky factor (rocSOLVER
+dpotrf + rocBLAS dtrsm) gives the same J and K whenever the metric is
+positive definite (no eigenvalue would be dropped). A metric that is not
+raises ``BackendError``; the eigenvalue-dropping route is then the
+reference's.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from hafnium_amd.errors import InputError
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def df_tensor(dense, metric, three_center):
+    """``L^-1 (P|uv)`` ``[aux, ao, ao]`` as a device tensor."""
+    torch = _torch()
+    m = torch.as_tensor(np.asarray(metric) if isinstance(metric, np.ndarray) else metric,
+                        dtype=torch.float64).to(dense.device).contiguous()
+    t = torch.as_tensor(np.asarray(three_center) if isinstance(three_center, np.ndarray) else three_center,
+                        dtype=torch.float64).to(dense.device).contiguous()
+    naux = m.shape[0]
+    if tuple(m.shape) != (naux, naux):
+        raise InputError("the metric is square")
+    if t.ndim != 3 or t.shape[0] != naux or t.shape[1] != t.shape[2]:
+        raise InputError("three-centre integrals have shape [aux, ao, ao]")
+    n = t.shape[1]
+    return dense.cholesky_solve(m, t.reshape(naux, n * n)).reshape(naux, n, n)
+
+
+def fitted_coulomb_exchange(metric, three_center, *, precision: str = "fp64", device=None):
+    """An ``MI355XCoulombExchange`` of the fitted tensor built on the GPU."""
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange, _Context
+    from hafnium_amd.linalg import Dense
+
+    torch = _torch()
+    dev = torch.device(device) if device is not None else torch.device("cuda", 0)
+    ctx = _Context(dev.index or 0)
+    b = df_tensor(Dense(ctx, dev), metric, three_center)
+    return MI355XCoulombExchange(b, precision=precision, device=dev)

@@ -135,6 +135,37 @@ HFJK_API int hfjk_exchange_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, 
 HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double *exchange, int precision,
                           void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- FP64 dense steps of the SCF around the J/K build ------------------
+ * Replace the NumPy/array-API calls of the reference's SCF driver
+ * (hafnium/scf/driver.py:248-329, hamiltonian.py:331-370, diis.py:70-106,
+ * orthogonalization.py:16-42, density_fitting.py:97-107). All row-major
+ * FP64 device buffers; results are deterministic run to run. */
+
+/* C = alpha op(A) op(B) + beta C (rocBLAS dgemm); op = transpose if trans_*. */
+HFJK_API int hfjk_dgemm(hfjk_ctx *ctx, int trans_a, int trans_b, int m, int n, int k, double alpha,
+                        const double *a, int lda, const double *b, int ldb, double beta, double *c,
+                        int ldc, void *stream);
+/* Symmetric eigendecomposition (rocSOLVER dsyevd) of a [n][n]: eigenvalues
+ * ascending in w[n]; on exit row j of a is the j-th eigenvector. work[n] and
+ * info[1] are device scratch (info != 0: not converged). */
+HFJK_API int hfjk_dsyevd(hfjk_ctx *ctx, int n, double *a, double *w, double *work, int *info,
+                         void *stream);
+/* Cholesky a = L L^T (rocSOLVER dpotrf): the lower triangle of a receives L;
+ * the strict upper triangle keeps its input. info[1] device (> 0: not PD). */
+HFJK_API int hfjk_dpotrf(hfjk_ctx *ctx, int n, double *a, int *info, void *stream);
+/* b [m][n] <- L^-1 b with L the lower triangle of l [m][m] (rocBLAS dtrsm). */
+HFJK_API int hfjk_dtrsm_lower(hfjk_ctx *ctx, int m, int n, const double *l, double *b, void *stream);
+/* out = a x + b y + c z elementwise over n (y, z may be NULL). */
+HFJK_API int hfjk_combine(hfjk_ctx *ctx, long long n, double a, const double *x, double b,
+                          const double *y, double c, const double *z, double *out, void *stream);
+/* out = m - m^T for m [n][n] (out distinct from m). */
+HFJK_API int hfjk_antisymmetrize(hfjk_ctx *ctx, int n, const double *m, double *out, void *stream);
+/* Fixed-order dot product: *out (device) = sum x[i] y[i]; partial is device
+ * scratch of hfjk_dot_workspace() doubles. */
+HFJK_API long long hfjk_dot_workspace(void);
+HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double *y, double *partial,
+                       double *out, void *stream);
+
 /* Tuning parameters of the kernels (0 = the library's default choice).
  * HFJK_TUNE_X_BN: column tile of the FP64/FP32 half-transform (32, 64, 128, 256).
  * HFJK_TUNE_X16_BN: column tile of the BF16x3 half-transform (32, 64, 128, 256).

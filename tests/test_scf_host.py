@@ -1,0 +1,44 @@
+"""Host logic of the GPU SCF driver: DIIS solve, degenerate-group filling, options."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from hafnium_amd import scf
+from hafnium_amd.errors import InputError
+from oracle import scf_ref
+from tests.tolerances import STRUCTURE_TOL
+
+
+def test_diis_solve_matches_the_oracle_and_the_linear_model():
+    # Errors e_i = x_i - 2 on a one-dimensional linear model: c = (1/2, 1/2).
+    ov = 2.0 * np.array([[1.0, -1.0], [-1.0, 1.0]])
+    np.testing.assert_allclose(scf.solve_diis(ov), [0.5, 0.5], atol=STRUCTURE_TOL)
+    rng = np.random.default_rng(0)
+    e = rng.standard_normal((5, 30))
+    ov = e @ e.T
+    np.testing.assert_allclose(scf.solve_diis(ov), scf_ref.solve_diis(ov), atol=1e-12)
+    # Singular (repeated error) and all-zero systems stay finite.
+    dup = np.array([[1.0, 1.0, 1.0], [1.0, 1.0, 1.0], [1.0, 1.0, 1.0]])
+    assert np.all(np.isfinite(scf.solve_diis(dup)))
+    np.testing.assert_allclose(scf.solve_diis(np.zeros((2, 2))), [0.5, 0.5], atol=STRUCTURE_TOL)
+
+
+def test_degenerate_groups_share_electrons():
+    occ = scf.fill_degenerate_groups(np.array([-2.0, -1.0, -1.0, 0.5]), 2.0)
+    np.testing.assert_allclose(occ, [1.0, 0.5, 0.5, 0.0])
+    np.testing.assert_allclose(occ, scf_ref.fill_degenerate_groups(np.array([-2.0, -1.0, -1.0, 0.5]), 2.0))
+    with pytest.raises(InputError):
+        scf.fill_degenerate_groups(np.array([0.0, 1.0]), 3.0)
+
+
+def test_options_are_validated():
+    with pytest.raises(InputError):
+        scf.ScfOptions(max_iterations=0)
+    with pytest.raises(InputError):
+        scf.ScfOptions(precision="fp16")
+    with pytest.raises(InputError):
+        scf.ScfOptions(damping=1.0)
+    terms = scf.EnergyTerms(9.0, -120.0, 45.0, -9.0)
+    assert terms.total == pytest.approx(9.0 - 120.0 + 45.0 - 9.0)
