@@ -80,6 +80,36 @@ struct hfjk_ctx {
   int syrk_splits = 0;
 };
 
+namespace hfjk {
+// Persistent work fetch: the workgroup's next item of phase `ph` from its
+// XCD's share [total x / 8, total (x + 1) / 8), x = blockIdx.x % 8; returns
+// a value >= the share's end when the share is exhausted. All threads of
+// the workgroup must call it together.
+__device__ inline long long batch_next(const long long* prefix, unsigned long long* counters, int ninst, int ph,
+                                       long long& share_end) {
+  __shared__ long long s_item;
+  const long long total = prefix[(long long)ph * (ninst + 1) + ninst];
+  const int x = blockIdx.x & 7;
+  const long long lo = total * x / 8;
+  share_end = total * (x + 1) / 8;
+  if (threadIdx.x == 0) s_item = lo + (long long)atomicAdd(&counters[ph * 8 + x], 1ULL);
+  __syncthreads();
+  const long long item = s_item;
+  __syncthreads();
+  return item;
+}
+
+// Instance i with prefix[i] <= item < prefix[i + 1] (prefix non-decreasing).
+__device__ inline int batch_find(const long long* prefix, int ninst, long long item) {
+  int lo = 0, hi = ninst;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+}  // namespace hfjk
+
 // Sets the calling thread's hfjk_last_error message (capi.hip).
 void hfjk_internal_set_error(const char* message);
 
@@ -121,6 +151,64 @@ hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux
 // Lower-triangle 128 x 128 output tiles of the SYRK and the tile edge.
 int syrk_pairs(int nao);
 int syrk_tile();
+
+// ---------------------------------------------------------------- batched
+// One instance of a batched build: its shapes and the element offsets of
+// its buffers (input tensors, outputs and workspace regions).
+struct BatchInst {
+  int nao, naux, nocc, ldo;
+  int nt, xrows, npairs, nsplit;
+  int nbm, nbn;       // half-transform row / column blocks at the batch tile
+  long long p;        // packed elements per aux function
+  long long b_off;    // batch tensor (element units of the bound copy)
+  long long c_off;    // orbitals [nao][nocc] (float64)
+  long long d_off;    // density, J and K [nao][nao] (float64)
+  long long w_off;    // fp64 workspace: packed weights [p]
+  long long jp_off;   // fp64 workspace: packed J [p]
+  long long gp_off;   // fp64 workspace: charge partials [ntiles][naux]
+  long long g_off;    // fp64 workspace: charges [naux]
+  long long sg_off;   // fp64 workspace: signs [ldo]
+  long long cp_off;   // mode workspace: padded orbitals (elements of the mode's type; bf16: per plane)
+  long long x_off;    // mode workspace: X (bf16: per plane)
+  long long kp_off;   // mode workspace: SYRK partials (FP64 or FP32 elements)
+};
+
+enum BatchPhase {
+  kPhWeights = 0,
+  kPhGamma,
+  kPhGammaReduce,
+  kPhCoulomb,
+  kPhCoulombUnpack,
+  kPhOrbitals,
+  kPhXform,
+  kPhSyrk,
+  kPhSyrkReduce,
+  kPhCount
+};
+
+// Device view of a batch plan: instances, per-phase item prefix sums
+// ([kPhCount][ninst + 1]) and per-phase, per-XCD work counters ([kPhCount][8]).
+struct BatchView {
+  const BatchInst* insts;
+  const long long* prefix;
+  unsigned long long* counters;
+  int ninst;
+};
+
+// Launchers of the batched persistent kernels (batch.hip, xk.hip). `grid`
+// is a multiple of 8 workgroups; mode buffers are typed by `precision`.
+hipError_t launch_batch_reset(BatchView v, hipStream_t s);
+hipError_t launch_batch_coulomb(BatchView v, int precision, const void* b, long long b_plane, const double* d,
+                                double* ws64, double* j, int grid, hipStream_t s);
+hipError_t launch_batch_orbitals(BatchView v, int precision, const double* c, double* ws64, void* cbase,
+                                 long long c_plane, int grid, hipStream_t s);
+hipError_t launch_batch_xform(BatchView v, int precision, int bn, const void* b, long long b_plane, const void* cbase,
+                              long long c_plane, void* xbase, long long x_plane, int grid, hipStream_t s);
+hipError_t launch_batch_syrk(BatchView v, int precision, const void* xbase, long long x_plane, const double* ws64,
+                             void* kbase, int grid, hipStream_t s);
+hipError_t launch_batch_syrk_reduce(BatchView v, int precision, const void* kbase, double* k, int grid,
+                                    hipStream_t s);
+int batch_tile_rows(int precision);  // rows of a half-transform tile: 64 (FP64/FP32) or 128 (BF16x3)
 
 // Density factorisation helpers (jk.hip).
 hipError_t launch_split_factor(const double* vecs, const double* vals, int nao, int first, int count,

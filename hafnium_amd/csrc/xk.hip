@@ -125,20 +125,22 @@ struct XShape {
 // tile rows r = 2 mb, 2 mb + 1) x BN columns; one layout tile column kt per
 // k step.
 template <typename T, int BN>
-__global__ __launch_bounds__(kThreads) void xform_kernel(const T* __restrict__ b, long long p, int nt,
-                                                          const T* __restrict__ c, int ldo, T* __restrict__ x,
-                                                          int xrows, int qbase, int nblocks_n, int nblocks_m,
-                                                          int nwg) {
+struct XformSmem {
+  T At[2][kTile][kTile + 1];
+  T Cs[kTile][BN + 1];
+};
+
+// One 64 x BN tile (row block mb, column block nb) of X_q = B_q C for one
+// aux function: bq = its packed B, xq = its X rows. The whole workgroup
+// calls it; LDS is free again when it returns.
+template <typename T, int BN>
+__device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, const T* __restrict__ c, int ldo,
+                                           T* __restrict__ xq, int mb, int nb, XformSmem<T, BN>& sm) {
   typedef XShape<BN> S;
   constexpr int CCH = kTile * BN / 2 / kThreads;  // 2-element chunks of the C tile per thread
-  __shared__ T At[2][kTile][kTile + 1];
-  __shared__ T Cs[kTile][BN + 1];
-  const int logical = xcd_remap(blockIdx.x, nwg);
-  const int nb = logical % nblocks_n;
-  const int mb = (logical / nblocks_n) % nblocks_m;
-  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  auto& At = sm.At;
+  auto& Cs = sm.Cs;
   const int n0 = nb * BN;
-  const T* bq = b + (long long)q * p;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
@@ -237,7 +239,6 @@ __global__ __launch_bounds__(kThreads) void xform_kernel(const T* __restrict__ b
       __syncthreads();
     }
   }
-  T* xq = x + (long long)q * xrows * ldo;
 #pragma unroll
   for (int mi = 0; mi < S::MI; ++mi)
 #pragma unroll
@@ -253,6 +254,19 @@ __global__ __launch_bounds__(kThreads) void xform_kernel(const T* __restrict__ b
     }
 }
 
+template <typename T, int BN>
+__global__ __launch_bounds__(kThreads) void xform_kernel(const T* __restrict__ b, long long p, int nt,
+                                                          const T* __restrict__ c, int ldo, T* __restrict__ x,
+                                                          int xrows, int qbase, int nblocks_n, int nblocks_m,
+                                                          int nwg) {
+  __shared__ XformSmem<T, BN> sm;
+  const int logical = xcd_remap(blockIdx.x, nwg);
+  const int nb = logical % nblocks_n;
+  const int mb = (logical / nblocks_n) % nblocks_m;
+  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  xform_tile<T, BN>(b + (long long)q * p, nt, c, ldo, x + (long long)q * xrows * ldo, mb, nb, sm);
+}
+
 // ------------------------------------------------------------------ BF16x3 half-transform
 
 constexpr int kRowK = 40;  // bf16 LDS row: 32 k + 8 pad (80 bytes, 16-byte aligned rows)
@@ -261,25 +275,29 @@ constexpr int kRowK = 40;  // bf16 LDS row: 32 k + 8 pad (80 bytes, 16-byte alig
 // waves 2 x 2 each (64 x BN/2). A is staged [m][k]; C^T (stored [i][k])
 // is staged [i][k].
 template <int BN>
-__global__ __launch_bounds__(kThreads) void xform_bf16x3_kernel(
-    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, long long p, int nt,
-    const uint16_t* __restrict__ cthi, const uint16_t* __restrict__ ctlo, int npad, int ldo,
-    uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo, int xrows, int qbase, int nblocks_n, int nblocks_m,
-    int nwg) {
+struct Xform16Smem {
+  uint16_t Ah[128][kRowK];
+  uint16_t Al[128][kRowK];
+  uint16_t Bh[BN][kRowK];
+  uint16_t Bl[BN][kRowK];
+};
+
+// One 128 x BN tile of X_q in BF16x3: hq/lq = the packed hi/lo planes of
+// B_q, xhi/xlo = X_q's hi/lo rows (xrows rows of ldo).
+template <int BN>
+__device__ __forceinline__ void xform16_tile(const uint16_t* __restrict__ hq, const uint16_t* __restrict__ lq, int nt,
+                                             const uint16_t* __restrict__ cthi, const uint16_t* __restrict__ ctlo,
+                                             int npad, int ldo, uint16_t* __restrict__ xhi,
+                                             uint16_t* __restrict__ xlo, int xrows, int mb, int nb,
+                                             Xform16Smem<BN>& sm) {
   constexpr int MI = 4;            // 64 rows per wave
   constexpr int NI = BN / 2 / 16;  // BN/2 columns per wave
   constexpr int CCH = BN * 4 / kThreads > 0 ? BN * 4 / kThreads : 1;  // 16-byte chunks of C^T per plane
-  __shared__ __attribute__((aligned(16))) uint16_t Ah[128][kRowK];
-  __shared__ __attribute__((aligned(16))) uint16_t Al[128][kRowK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bh[BN][kRowK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bl[BN][kRowK];
-  const int logical = xcd_remap(blockIdx.x, nwg);
-  const int nb = logical % nblocks_n;
-  const int mb = (logical / nblocks_n) % nblocks_m;  // 128-row block
-  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  auto& Ah = sm.Ah;
+  auto& Al = sm.Al;
+  auto& Bh = sm.Bh;
+  auto& Bl = sm.Bl;
   const int n0 = nb * BN;
-  const uint16_t* hq = bhi + (long long)q * p;
-  const uint16_t* lq = blo + (long long)q * p;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
@@ -388,7 +406,6 @@ __global__ __launch_bounds__(kThreads) void xform_bf16x3_kernel(
       __syncthreads();
     }
   }
-  const long long xbase = (long long)q * xrows * ldo;
 #pragma unroll
   for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -401,13 +418,29 @@ __global__ __launch_bounds__(kThreads) void xform_bf16x3_kernel(
           if (row < xrows) {
             const float v = acc[mi][ni][r];
             const uint16_t h = bf16_of(v);
-            const long long off = xbase + (long long)row * ldo + col;
+            const long long off = (long long)row * ldo + col;
             xhi[off] = h;
             xlo[off] = bf16_of(v - f32_of(h));
           }
         }
       }
     }
+}
+
+template <int BN>
+__global__ __launch_bounds__(kThreads) void xform_bf16x3_kernel(
+    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, long long p, int nt,
+    const uint16_t* __restrict__ cthi, const uint16_t* __restrict__ ctlo, int npad, int ldo,
+    uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo, int xrows, int qbase, int nblocks_n, int nblocks_m,
+    int nwg) {
+  __shared__ __attribute__((aligned(16))) Xform16Smem<BN> sm;
+  const int logical = xcd_remap(blockIdx.x, nwg);
+  const int nb = logical % nblocks_n;
+  const int mb = (logical / nblocks_n) % nblocks_m;  // 128-row block
+  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  const long long xbase = (long long)q * xrows * ldo;
+  xform16_tile<BN>(bhi + (long long)q * p, blo + (long long)q * p, nt, cthi, ctlo, npad, ldo, xhi + xbase,
+                   xlo + xbase, xrows, mb, nb, sm);
 }
 
 // Orbitals [nao][nocc] float64 -> transposed split planes Ct[ldo][npad]
@@ -439,20 +472,21 @@ __device__ __forceinline__ void syrk_coords(int b, int npairs, int& s, int& pair
   pair = slot % npairs;
 }
 
-// FP64 / FP32: partial[s][pair][128][128] over the aux slice s, all i.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x, int naux, int xrows, int ldo,
-                                                         const double* __restrict__ signs, int nsplit, int npairs,
-                                                         T* __restrict__ partial) {
+struct SyrkSmem {
+  T As[kSyrkTile][kTile + 1];
+  T Bs[kSyrkTile][kTile + 1];
+};
+
+// FP64 / FP32: one 128 x 128 tile (row block mb, column block nb) of
+// sum_{q in [qb, qe)} X_q S X_q^T into dst[128][128]; x = the instance's X.
+template <typename T>
+__device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int qe, int xrows, int ldo,
+                                          const double* __restrict__ signs, int mb, int nb, T* __restrict__ dst,
+                                          SyrkSmem<T>& sm) {
   constexpr int MI = 4, NI = 4;  // wave tile 64 x 64
-  __shared__ T As[kSyrkTile][kTile + 1];
-  __shared__ T Bs[kSyrkTile][kTile + 1];
-  int s, pair;
-  syrk_coords(blockIdx.x, npairs, s, pair);
-  int mb, nb;
-  tile_of(pair, mb, nb);
-  const int qb = (int)((long long)s * naux / nsplit);
-  const int qe = (int)((long long)(s + 1) * naux / nsplit);
+  auto& As = sm.As;
+  auto& Bs = sm.Bs;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
@@ -533,7 +567,6 @@ __global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x,
       __syncthreads();
     }
   }
-  T* dst = partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile);
 #pragma unroll
   for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -547,22 +580,37 @@ __global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x,
     }
 }
 
-// BF16x3: partial (FP32) [s][pair][128][128].
-__global__ __launch_bounds__(kThreads) void syrk_bf16x3_kernel(const uint16_t* __restrict__ xhi,
-                                                                const uint16_t* __restrict__ xlo, int naux,
-                                                                int xrows, int ldo, const double* __restrict__ signs,
-                                                                int nsplit, int npairs, float* __restrict__ partial) {
-  constexpr int MI = 4, NI = 4;
-  __shared__ __attribute__((aligned(16))) uint16_t Ah[kSyrkTile][kRowK];
-  __shared__ __attribute__((aligned(16))) uint16_t Al[kSyrkTile][kRowK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bh[kSyrkTile][kRowK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bl[kSyrkTile][kRowK];
+template <typename T>
+__global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x, int naux, int xrows, int ldo,
+                                                         const double* __restrict__ signs, int nsplit, int npairs,
+                                                         T* __restrict__ partial) {
+  __shared__ SyrkSmem<T> sm;
   int s, pair;
   syrk_coords(blockIdx.x, npairs, s, pair);
   int mb, nb;
   tile_of(pair, mb, nb);
   const int qb = (int)((long long)s * naux / nsplit);
   const int qe = (int)((long long)(s + 1) * naux / nsplit);
+  syrk_tile<T>(x, qb, qe, xrows, ldo, signs, mb, nb,
+               partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile), sm);
+}
+
+struct Syrk16Smem {
+  uint16_t Ah[kSyrkTile][kRowK];
+  uint16_t Al[kSyrkTile][kRowK];
+  uint16_t Bh[kSyrkTile][kRowK];
+  uint16_t Bl[kSyrkTile][kRowK];
+};
+
+// BF16x3: one 128 x 128 FP32 tile of sum_{q in [qb, qe)} X_q S X_q^T.
+__device__ __forceinline__ void syrk16_tile(const uint16_t* __restrict__ xhi, const uint16_t* __restrict__ xlo,
+                                            int qb, int qe, int xrows, int ldo, const double* __restrict__ signs,
+                                            int mb, int nb, float* __restrict__ dst, Syrk16Smem& sm) {
+  constexpr int MI = 4, NI = 4;
+  auto& Ah = sm.Ah;
+  auto& Al = sm.Al;
+  auto& Bh = sm.Bh;
+  auto& Bl = sm.Bl;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
@@ -657,7 +705,6 @@ __global__ __launch_bounds__(kThreads) void syrk_bf16x3_kernel(const uint16_t* _
       __syncthreads();
     }
   }
-  float* dst = partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile);
 #pragma unroll
   for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -669,6 +716,21 @@ __global__ __launch_bounds__(kThreads) void syrk_bf16x3_kernel(const uint16_t* _
         dst[row * kSyrkTile + col] = acc[mi][ni][r];
       }
     }
+}
+
+__global__ __launch_bounds__(kThreads) void syrk_bf16x3_kernel(const uint16_t* __restrict__ xhi,
+                                                                const uint16_t* __restrict__ xlo, int naux,
+                                                                int xrows, int ldo, const double* __restrict__ signs,
+                                                                int nsplit, int npairs, float* __restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) Syrk16Smem sm;
+  int s, pair;
+  syrk_coords(blockIdx.x, npairs, s, pair);
+  int mb, nb;
+  tile_of(pair, mb, nb);
+  const int qb = (int)((long long)s * naux / nsplit);
+  const int qe = (int)((long long)(s + 1) * naux / nsplit);
+  syrk16_tile(xhi, xlo, qb, qe, xrows, ldo, signs, mb, nb,
+              partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile), sm);
 }
 
 template <typename T>
@@ -828,6 +890,197 @@ hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, in
   else
     hipLaunchKernelGGL(syrk_reduce_kernel<float>, dim3(grid_for(n)), dim3(kThreads), 0, s, (const float*)partial,
                        nsplit, npairs, nao, out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ batched (persistent)
+//
+// Every instance of a batch contributes items to each phase; a fixed grid of
+// workgroups pulls items from per-XCD counters (batch_next) until its
+// share is exhausted, so instances of very different sizes share one launch
+// and balance dynamically.
+
+namespace {
+
+template <typename T, int BN>
+__global__ __launch_bounds__(kThreads) void batch_xform_kernel(BatchView v, const T* __restrict__ b,
+                                                                const T* __restrict__ cbase, T* __restrict__ xbase) {
+  __shared__ XformSmem<T, BN> sm;
+  const long long* pre = v.prefix + (long long)kPhXform * (v.ninst + 1);
+  for (;;) {
+    long long end;
+    const long long item = batch_next(v.prefix, v.counters, v.ninst, kPhXform, end);
+    if (item >= end) break;
+    const int i = batch_find(pre, v.ninst, item);
+    const BatchInst in = v.insts[i];
+    const long long local = item - pre[i];
+    const int nb = (int)(local % in.nbn);
+    const int mb = (int)((local / in.nbn) % in.nbm);
+    const int q = (int)(local / ((long long)in.nbn * in.nbm));
+    xform_tile<T, BN>(b + in.b_off + (long long)q * in.p, in.nt, cbase + in.cp_off, in.ldo,
+                      xbase + in.x_off + (long long)q * in.xrows * in.ldo, mb, nb, sm);
+  }
+}
+
+template <int BN>
+__global__ __launch_bounds__(kThreads) void batch_xform16_kernel(BatchView v, const uint16_t* __restrict__ b,
+                                                                  long long b_plane, const uint16_t* __restrict__ cbase,
+                                                                  long long c_plane, uint16_t* __restrict__ xbase,
+                                                                  long long x_plane) {
+  __shared__ __attribute__((aligned(16))) Xform16Smem<BN> sm;
+  const long long* pre = v.prefix + (long long)kPhXform * (v.ninst + 1);
+  for (;;) {
+    long long end;
+    const long long item = batch_next(v.prefix, v.counters, v.ninst, kPhXform, end);
+    if (item >= end) break;
+    const int i = batch_find(pre, v.ninst, item);
+    const BatchInst in = v.insts[i];
+    const long long local = item - pre[i];
+    const int nb = (int)(local % in.nbn);
+    const int mb = (int)((local / in.nbn) % in.nbm);
+    const int q = (int)(local / ((long long)in.nbn * in.nbm));
+    const uint16_t* hq = b + in.b_off + (long long)q * in.p;
+    const uint16_t* ct = cbase + in.cp_off;
+    uint16_t* xh = xbase + in.x_off + (long long)q * in.xrows * in.ldo;
+    xform16_tile<BN>(hq, hq + b_plane, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh, xh + x_plane, in.xrows,
+                     mb, nb, sm);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void batch_syrk_kernel(BatchView v, const T* __restrict__ xbase,
+                                                               const double* __restrict__ ws64, T* __restrict__ kbase) {
+  __shared__ SyrkSmem<T> sm;
+  const long long* pre = v.prefix + (long long)kPhSyrk * (v.ninst + 1);
+  for (;;) {
+    long long end;
+    const long long item = batch_next(v.prefix, v.counters, v.ninst, kPhSyrk, end);
+    if (item >= end) break;
+    const int i = batch_find(pre, v.ninst, item);
+    const BatchInst in = v.insts[i];
+    const long long local = item - pre[i];
+    const int pair = (int)(local % in.npairs);
+    const int s = (int)(local / in.npairs);
+    int mb, nb;
+    tile_of(pair, mb, nb);
+    const int qb = (int)((long long)s * in.naux / in.nsplit);
+    const int qe = (int)((long long)(s + 1) * in.naux / in.nsplit);
+    syrk_tile<T>(xbase + in.x_off, qb, qe, in.xrows, in.ldo, ws64 + in.sg_off, mb, nb,
+                 kbase + in.kp_off + ((long long)s * in.npairs + pair) * (kSyrkTile * kSyrkTile), sm);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void batch_syrk16_kernel(BatchView v, const uint16_t* __restrict__ xbase,
+                                                                 long long x_plane, const double* __restrict__ ws64,
+                                                                 float* __restrict__ kbase) {
+  __shared__ __attribute__((aligned(16))) Syrk16Smem sm;
+  const long long* pre = v.prefix + (long long)kPhSyrk * (v.ninst + 1);
+  for (;;) {
+    long long end;
+    const long long item = batch_next(v.prefix, v.counters, v.ninst, kPhSyrk, end);
+    if (item >= end) break;
+    const int i = batch_find(pre, v.ninst, item);
+    const BatchInst in = v.insts[i];
+    const long long local = item - pre[i];
+    const int pair = (int)(local % in.npairs);
+    const int s = (int)(local / in.npairs);
+    int mb, nb;
+    tile_of(pair, mb, nb);
+    const int qb = (int)((long long)s * in.naux / in.nsplit);
+    const int qe = (int)((long long)(s + 1) * in.naux / in.nsplit);
+    const uint16_t* xh = xbase + in.x_off;
+    syrk16_tile(xh, xh + x_plane, qb, qe, in.xrows, in.ldo, ws64 + in.sg_off, mb, nb,
+                kbase + in.kp_off + ((long long)s * in.npairs + pair) * (kSyrkTile * kSyrkTile), sm);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void batch_syrk_reduce_kernel(BatchView v, const T* __restrict__ kbase,
+                                                                      double* __restrict__ kout) {
+  const long long* pre = v.prefix + (long long)kPhSyrkReduce * (v.ninst + 1);
+  for (;;) {
+    long long end;
+    const long long item = batch_next(v.prefix, v.counters, v.ninst, kPhSyrkReduce, end);
+    if (item >= end) break;
+    const int i = batch_find(pre, v.ninst, item);
+    const BatchInst in = v.insts[i];
+    const long long idx = (item - pre[i]) * kThreads + threadIdx.x;
+    if (idx < (long long)in.nao * in.nao) {
+      const int row = (int)(idx / in.nao);
+      const int col = (int)(idx % in.nao);
+      const int bm = row / kSyrkTile, bn = col / kSyrkTile;
+      long long pair;
+      int local;
+      if (bm >= bn) {
+        pair = tidx(bm, bn);
+        local = (row % kSyrkTile) * kSyrkTile + col % kSyrkTile;
+      } else {
+        pair = tidx(bn, bm);
+        local = (col % kSyrkTile) * kSyrkTile + row % kSyrkTile;
+      }
+      const T* part = kbase + in.kp_off;
+      double sum = 0.0;
+      for (int s = 0; s < in.nsplit; ++s)
+        sum += (double)part[((long long)s * in.npairs + pair) * (kSyrkTile * kSyrkTile) + local];
+      kout[in.d_off + idx] = sum;
+    }
+  }
+}
+
+}  // namespace
+
+int batch_tile_rows(int precision) { return precision == HFJK_BF16X3 ? 128 : kBlock; }
+
+hipError_t launch_batch_xform(BatchView v, int precision, int bn, const void* b, long long b_plane, const void* cbase,
+                              long long c_plane, void* xbase, long long x_plane, int grid, hipStream_t s) {
+  const dim3 g((unsigned)grid), blk(kThreads);
+  if (precision == HFJK_FP64) {
+    const double* bb = (const double*)b;
+    const double* cb = (const double*)cbase;
+    double* xb = (double*)xbase;
+    if (bn == 32) hipLaunchKernelGGL((batch_xform_kernel<double, 32>), g, blk, 0, s, v, bb, cb, xb);
+    else if (bn == 64) hipLaunchKernelGGL((batch_xform_kernel<double, 64>), g, blk, 0, s, v, bb, cb, xb);
+    else if (bn == 128) hipLaunchKernelGGL((batch_xform_kernel<double, 128>), g, blk, 0, s, v, bb, cb, xb);
+    else hipLaunchKernelGGL((batch_xform_kernel<double, 256>), g, blk, 0, s, v, bb, cb, xb);
+  } else if (precision == HFJK_FP32) {
+    const float* bb = (const float*)b;
+    const float* cb = (const float*)cbase;
+    float* xb = (float*)xbase;
+    if (bn == 32) hipLaunchKernelGGL((batch_xform_kernel<float, 32>), g, blk, 0, s, v, bb, cb, xb);
+    else if (bn == 64) hipLaunchKernelGGL((batch_xform_kernel<float, 64>), g, blk, 0, s, v, bb, cb, xb);
+    else if (bn == 128) hipLaunchKernelGGL((batch_xform_kernel<float, 128>), g, blk, 0, s, v, bb, cb, xb);
+    else hipLaunchKernelGGL((batch_xform_kernel<float, 256>), g, blk, 0, s, v, bb, cb, xb);
+  } else {
+    const uint16_t* bb = (const uint16_t*)b;
+    const uint16_t* cb = (const uint16_t*)cbase;
+    uint16_t* xb = (uint16_t*)xbase;
+    if (bn == 32) hipLaunchKernelGGL((batch_xform16_kernel<32>), g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
+    else if (bn == 64) hipLaunchKernelGGL((batch_xform16_kernel<64>), g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
+    else if (bn == 128) hipLaunchKernelGGL((batch_xform16_kernel<128>), g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
+    else hipLaunchKernelGGL((batch_xform16_kernel<256>), g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_syrk(BatchView v, int precision, const void* xbase, long long x_plane, const double* ws64,
+                             void* kbase, int grid, hipStream_t s) {
+  const dim3 g((unsigned)grid), blk(kThreads);
+  if (precision == HFJK_FP64)
+    hipLaunchKernelGGL(batch_syrk_kernel<double>, g, blk, 0, s, v, (const double*)xbase, ws64, (double*)kbase);
+  else if (precision == HFJK_FP32)
+    hipLaunchKernelGGL(batch_syrk_kernel<float>, g, blk, 0, s, v, (const float*)xbase, ws64, (float*)kbase);
+  else
+    hipLaunchKernelGGL(batch_syrk16_kernel, g, blk, 0, s, v, (const uint16_t*)xbase, x_plane, ws64, (float*)kbase);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_syrk_reduce(BatchView v, int precision, const void* kbase, double* k, int grid,
+                                    hipStream_t s) {
+  const dim3 g((unsigned)grid), blk(kThreads);
+  if (precision == HFJK_FP64)
+    hipLaunchKernelGGL(batch_syrk_reduce_kernel<double>, g, blk, 0, s, v, (const double*)kbase, k);
+  else
+    hipLaunchKernelGGL(batch_syrk_reduce_kernel<float>, g, blk, 0, s, v, (const float*)kbase, k);
   return hipGetLastError();
 }
 

@@ -135,6 +135,40 @@ HFJK_API int hfjk_exchange_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, 
 HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double *exchange, int precision,
                           void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- Batched J/K: many instances of uneven size, one persistent launch per phase ----
+ * No counterpart in the reference (it builds one molecule's J/K at a time:
+ * coulomb_exchange.py:176-190 per call). For instance i:
+ *   J_i = sum_P B_i[P] <B_i[P], D_i>,   K_i = sum_P (B_i[P] C_i)(B_i[P] C_i)^T.
+ * B_i is tiled packed at element offset b_offset (a multiple of 1024) of one
+ * batch tensor (FP64, FP32, or BF16 hi plane followed by the lo plane, each
+ * plane hfjk_batch_sizes()->tensor_elems long); C_i [nao][nocc] at c_offset of
+ * the orbital buffer; D_i, J_i, K_i [nao][nao] at d_offset of their buffers. */
+typedef struct {
+  int nao;
+  int naux;
+  int nocc;
+  long long b_offset;
+  long long c_offset;
+  long long d_offset;
+} hfjk_batch_instance;
+typedef struct hfjk_batch hfjk_batch;
+/* Plan a batch (host only): shapes, SYRK splits, offsets. tile = half-transform
+ * column tile (0 = default; 32, 64, 128, 256), chosen by the autotuner. */
+HFJK_API int hfjk_batch_create(hfjk_ctx *ctx, const hfjk_batch_instance *instances, int ninst,
+                               int precision, int tile, hfjk_batch **batch);
+HFJK_API int hfjk_batch_workspace_bytes(const hfjk_batch *batch, size_t *bytes);
+/* Elements of one tensor plane, of the orbital buffer and of each matrix buffer. */
+HFJK_API int hfjk_batch_sizes(const hfjk_batch *batch, long long *tensor_elems,
+                              long long *orbital_elems, long long *matrix_elems);
+/* Upload the plan into the workspace (synchronises `stream` once). */
+HFJK_API int hfjk_batch_bind(hfjk_batch *batch, void *workspace, size_t bytes, void *stream);
+/* Enqueue one batched J/K build; no host synchronisation and no allocation,
+ * so a run can be captured in a hipGraph. Runs on one bound workspace must not
+ * overlap. */
+HFJK_API int hfjk_batch_run(hfjk_batch *batch, const void *tensor, const double *orbitals,
+                            const double *density, double *coulomb, double *exchange, void *stream);
+HFJK_API int hfjk_batch_destroy(hfjk_batch *batch);
+
 /* ---- FP64 dense steps of the SCF around the J/K build ------------------
  * Replace the NumPy/array-API calls of the reference's SCF driver
  * (hafnium/scf/driver.py:248-329, hamiltonian.py:331-370, diis.py:70-106,
