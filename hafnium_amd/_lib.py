@@ -54,6 +54,10 @@ SIGNATURES = [
     ("hfjk_coulomb", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_basis_nfunc", _c_int, [_vp, ctypes.POINTER(_c_int)]),
+    ("hfjk_int1e", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    ("hfjk_int2c2e", _c_int, [_vp, _vp]),
+    ("hfjk_int3c2e", _c_int, [_vp, _vp, _vp]),
     ("hfjk_batch_create", _c_int, [_vp, _vp, _c_int, _c_int, _c_int, ctypes.POINTER(_vp)]),
     ("hfjk_batch_workspace_bytes", _c_int, [_vp, ctypes.POINTER(_c_size)]),
     ("hfjk_batch_sizes", _c_int, [_vp, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll)]),

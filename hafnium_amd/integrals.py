@@ -1,0 +1,56 @@
+"""One-electron and density-fitting integrals of a molecule (host, ``libhfjk.so``).
+
+The producer of the SCF inputs: S, T, V (the reference's
+``IntegralEngine.overlap/kinetic/nuclear_attraction``,
+hafnium/integrals/engine.py:108-377) and the metric ``(P|Q)`` and
+three-centre ``(P|uv)`` of ``DensityFitting.build``
+(hafnium/integrals/density_fitting.py:58-85), computed by the library's
+McMurchie-Davidson engine.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from hafnium_amd import _lib
+from hafnium_amd.molecule import BasisOnAtoms, Molecule
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def one_electron(basis: BasisOnAtoms, molecule: Molecule):
+    """``(S, T, V)`` ``[n, n]``."""
+    lib = _lib.load()
+    n = basis.n_functions
+    s, t, v = (np.empty((n, n)) for _ in range(3))
+    charges = np.asarray(molecule.charges, dtype=np.float64)
+    coords = np.ascontiguousarray(molecule.positions, dtype=np.float64)
+    b = basis.c_struct()
+    _lib.check(lib.hfjk_int1e(ctypes.byref(b), len(charges), _ptr(charges), _ptr(coords), _ptr(s), _ptr(t), _ptr(v)))
+    return s, t, v
+
+
+def metric(aux: BasisOnAtoms):
+    """``(P|Q)`` ``[naux, naux]``."""
+    lib = _lib.load()
+    m = np.empty((aux.n_functions, aux.n_functions))
+    b = aux.c_struct()
+    _lib.check(lib.hfjk_int2c2e(ctypes.byref(b), _ptr(m)))
+    return m
+
+
+def three_center(aux: BasisOnAtoms, orbital: BasisOnAtoms):
+    """``(P|uv)`` ``[naux, n, n]``."""
+    lib = _lib.load()
+    out = np.empty((aux.n_functions, orbital.n_functions, orbital.n_functions))
+    a, o = aux.c_struct(), orbital.c_struct()
+    _lib.check(lib.hfjk_int3c2e(ctypes.byref(a), ctypes.byref(o), _ptr(out)))
+    return out
+
+
+def water_from_fixture_positions(positions_bohr) -> Molecule:
+    return Molecule((8, 1, 1), np.asarray(positions_bohr, dtype=np.float64))

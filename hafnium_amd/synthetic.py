@@ -189,6 +189,33 @@ def occupied_orbitals(seed: int, nao: int, nocc: int) -> np.ndarray:
     return np.ascontiguousarray(q)
 
 
+TAG_BATCH = 0x4A4B0003
+"""Fourth Philox counter word of the batch shape draws."""
+
+
+def batch_shapes(count: int = 512, base_seed: int = 1000, nao_range=(40, 400), aux_ratio: float = 3.5,
+                 occ_range=(0.1, 0.3)):
+    """``[(seed, nao, naux, nocc)]`` of the batched workload.
+
+    Instance k has seed ``base_seed + k``; ``nao`` is uniform on the integers
+    of ``nao_range``, ``naux = round(aux_ratio * nao)`` and
+    ``nocc = round(U(occ_range) * nao)`` (at least 1), drawn from one Philox
+    block keyed by the instance seed.
+    """
+    out = []
+    lo, hi = nao_range
+    for k in range(count):
+        seed = base_seed + k
+        u1, u2 = uniform_pair(0, 0, 0, TAG_BATCH, seed)
+        ua = 1.0 - float(u1)  # in [0, 1)
+        nao = lo + min(int(ua * (hi - lo + 1)), hi - lo)
+        naux = int(round(aux_ratio * nao))
+        frac = occ_range[0] + (occ_range[1] - occ_range[0]) * float(u2)
+        nocc = max(1, min(nao, int(round(frac * nao))))
+        out.append((seed, nao, naux, nocc))
+    return out
+
+
 def jk_inputs(seed: int, naux: int, nao: int, nocc: int):
     """``(B, C, D)`` of one instance, B in full ``[naux, nao, nao]`` storage."""
     b = tensor_full(seed, naux, nao)

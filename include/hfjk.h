@@ -169,6 +169,32 @@ HFJK_API int hfjk_batch_run(hfjk_batch *batch, const void *tensor, const double 
                             const double *density, double *coulomb, double *exchange, void *stream);
 HFJK_API int hfjk_batch_destroy(hfjk_batch *batch);
 
+/* ---- Gaussian integrals (host) ---------------------------------------------
+ * The SCF inputs that the reference's Rust engine produces
+ * (crates/hafnium-core/src/integrals/one_electron.rs, two_electron.rs:74-310
+ * via hafnium/integrals/density_fitting.py:58-85): overlap, kinetic energy,
+ * nuclear attraction, (P|Q) and (P|uv), over contracted real solid-harmonic
+ * Gaussians (m = -l..l, p = x, y, z; functions contraction-major then m;
+ * coefficients of normalised primitives; unit-norm functions). Host arrays,
+ * row-major; l <= 5. */
+typedef struct {
+  int nshell;
+  const int *l;          /* [nshell] */
+  const int *nprim;      /* [nshell] */
+  const int *ncontr;     /* [nshell] */
+  const double *centers; /* [nshell][3], Bohr */
+  const double *exps;    /* primitives of all shells, concatenated */
+  const double *coefs;   /* per shell [ncontr][nprim], concatenated */
+} hfjk_basis;
+HFJK_API int hfjk_basis_nfunc(const hfjk_basis *basis, int *nfunc);
+/* S, T, V [n][n] (any output may be NULL); V over point charges at coords [natom][3]. */
+HFJK_API int hfjk_int1e(const hfjk_basis *basis, int natom, const double *charges, const double *coords,
+                        double *overlap, double *kinetic, double *nuclear);
+/* (P|Q) [naux][naux]. */
+HFJK_API int hfjk_int2c2e(const hfjk_basis *aux, double *metric);
+/* (P|uv) [naux][n][n]. */
+HFJK_API int hfjk_int3c2e(const hfjk_basis *aux, const hfjk_basis *orbital, double *out);
+
 /* ---- FP64 dense steps of the SCF around the J/K build ------------------
  * Replace the NumPy/array-API calls of the reference's SCF driver
  * (hafnium/scf/driver.py:248-329, hamiltonian.py:331-370, diis.py:70-106,

@@ -58,6 +58,17 @@ def water():
     )
 
 
+def water_ccpvdz():
+    one = load("one_electron_water_cc-pvdz.json")
+    np.savez_compressed(
+        os.path.join(OUT, "water_ccpvdz_one_electron.npz"),
+        overlap=np.array(one["overlap"]),
+        kinetic=np.array(one["kinetic"]),
+        nuclear_attraction=np.array(one["nuclear_attraction"]),
+        positions_bohr=np.array(one["provenance"]["positions_bohr"]),
+    )
+
+
 def synthetic_c0():
     from hafnium_amd import synthetic
     from oracle import jk_ref
@@ -81,5 +92,6 @@ def synthetic_c0():
 
 if __name__ == "__main__":
     water()
+    water_ccpvdz()
     synthetic_c0()
     print("written", sorted(os.listdir(OUT)))
