@@ -89,6 +89,14 @@ def load() -> ctypes.CDLL:
     global _lib
     with _lock:
         if _lib is None:
+            # One HIP runtime per process: PyTorch-ROCm ships its own
+            # libamdhip64.so.7 / librocblas.so.5 / librocsolver.so.0 and
+            # loads them by file name. Importing torch first registers those
+            # sonames, so the library's dependencies bind to the same runtime;
+            # loading the library first would bring in a second HIP runtime
+            # and torch would then see no GPU.
+            import torch  # noqa: F401
+
             if not os.path.exists(LIB_PATH):
                 raise BackendError(
                     f"{LIB_PATH} is not built: run `python -m hafnium_amd._build` "

@@ -187,7 +187,8 @@ class RestrictedHamiltonian:
         if n_alpha < 0 or n_alpha > n:
             raise InputError("0 <= n_alpha <= n_ao is required")
         self.two_body = two_body
-        self.n_alpha = int(n_alpha)
+        # Fractional counts are allowed for the averaged occupation (atoms).
+        self.n_alpha = int(n_alpha) if float(n_alpha).is_integer() else float(n_alpha)
         self.nuclear_repulsion = float(nuclear_repulsion)
         self.n_ao = n
         self.device = device
@@ -268,22 +269,37 @@ def _density(dense, c, occupations=None, n_alpha=None):
     return dense.gemm(scaled, c, tb=True), None
 
 
-def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOptions | None = None) -> ScfResult:
+def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOptions | None = None,
+              occupation: str = "aufbau") -> ScfResult:
     """Converge the closed-shell state (driver.py:154-213) with the precision policy.
 
     ``guess``: a density ``[n, n]`` (D_alpha) or ``None`` for the averaged
-    core-Hamiltonian guess (driver.py:216-246).
+    core-Hamiltonian guess (driver.py:216-246). ``occupation``: "aufbau"
+    (``AufbauOccupation``) or "averaged" (``AveragedOccupation``: degenerate
+    frontier groups share the electrons, for spherical atoms;
+    occupation.py:130-159).
     Raises ConvergenceError after ``max_iterations`` Fock builds; its
     ``last`` holds the result.
     """
     torch = _torch()
     options = ScfOptions() if options is None else options
+    if occupation not in ("aufbau", "averaged"):
+        raise InputError("occupation is 'aufbau' or 'averaged'")
     t0 = time.perf_counter()
     setup = _Setup(hamiltonian, options)
     dn = hamiltonian.dense
     n_alpha = hamiltonian.n_alpha
     if n_alpha > setup.kept:
         raise InputError(f"{n_alpha} occupied orbitals do not fit {setup.kept} retained functions")
+    if occupation == "aufbau" and not isinstance(n_alpha, int):
+        raise InputError(f"aufbau filling needs an integer count, got {n_alpha}")
+
+    def occupy(e, c):
+        if occupation == "averaged":
+            occ = fill_degenerate_groups(e.cpu().numpy(), n_alpha)
+            return _density(dn, c, occupations=occ)
+        return _density(dn, c, n_alpha=n_alpha)
+
     orbitals = None
     if guess is None:
         e, c = setup.solve(hamiltonian.core)
@@ -345,17 +361,20 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
             extrapolated = dn.combine(1.0, extrapolated, options.level_shift, virtual)
         previous = density
         e, c = setup.solve(extrapolated)
-        density, orbitals = _density(dn, c, n_alpha=n_alpha)
+        density, orbitals = occupy(e, c)
         if options.damping:
             density = dn.combine(1.0 - options.damping, density, options.damping, previous)
             orbitals = None
     final_fock = fock if converged else focks[-1]
     e, c = setup.solve(final_fock)
-    density, orbitals = _density(dn, c, n_alpha=n_alpha)
+    density, orbitals = occupy(e, c)
     fock, terms = hamiltonian.fock(density, orbitals, "fp64")
     energies = _energy_terms(hamiltonian, terms.cpu().numpy())
-    occupations = np.zeros(setup.kept)
-    occupations[:n_alpha] = 1.0
+    if occupation == "averaged":
+        occupations = fill_degenerate_groups(e.cpu().numpy(), n_alpha)
+    else:
+        occupations = np.zeros(setup.kept)
+        occupations[:n_alpha] = 1.0
     result = ScfResult(energies, c, e, occupations, density, fock, converged, history, time.perf_counter() - t0)
     if not converged:
         last = history[-1]
