@@ -78,6 +78,15 @@ SIGNATURES = [
     ("hfjk_get_tuning", _c_int, [_vp, _c_int, ctypes.POINTER(_c_int)]),
     ("hfjk_profile", _c_int, [_vp, _c_int]),
     ("hfjk_profile_read", _c_int, [_vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), _c_int]),
+    ("hfjk_mlp_gemm", _c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int,
+                               _c_int, _vp, _vp, _c_int, _c_int, _vp, _vp]),
+    ("hfjk_mlp_silu_backward", _c_int, [_vp, _c_ll, _vp, _vp, _vp, _vp]),
+    ("hfjk_mlp_column_sum", _c_int, [_vp, _c_int, _c_int, _vp, _c_int, _vp, _vp]),
+    ("hfjk_mlp_mse", _c_int, [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _vp, ctypes.c_float, _vp, _vp,
+                              _vp]),
+    ("hfjk_mlp_adam", _c_int, [_vp, _c_ll, _vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]),
+    ("hfjk_mlp_to_bf16", _c_int, [_vp, _c_ll, _vp, _vp, _vp]),
 ]
 
 _lock = threading.Lock()

@@ -40,11 +40,17 @@ class ClusterSystem:
     molecule: Molecule
     hamiltonian: RestrictedHamiltonian
     timings: dict = field(default_factory=dict)
+    metric: np.ndarray | None = None         # (P|Q) [naux, naux], host
+    three_center: np.ndarray | None = None   # (P|uv) [naux, n, n], host
+    aux_basis: str = AUX_BASIS
 
 
 def build_system(molecule: Molecule, *, basis: str = ORBITAL_BASIS, aux: str = AUX_BASIS,
-                 precision: str = "fp64", device=None) -> ClusterSystem:
-    """Integrals (host), fitted tensor (GPU Cholesky) and the Hamiltonian of a closed shell."""
+                 precision: str = "fp64", device=None, keep_integrals: bool = False) -> ClusterSystem:
+    """Integrals (host), fitted tensor (GPU Cholesky) and the Hamiltonian of a closed shell.
+
+    ``keep_integrals`` keeps the host metric and three-centre integrals on the
+    result (the learned guess needs them)."""
     from hafnium_amd.density_fitting import fitted_coulomb_exchange
 
     t0 = time.perf_counter()
@@ -63,7 +69,8 @@ def build_system(molecule: Molecule, *, basis: str = ORBITAL_BASIS, aux: str = A
         n_alpha = molecule.n_electrons // 2
     h = RestrictedHamiltonian(s, t + v, two_body, n_alpha, molecule.nuclear_repulsion())
     return ClusterSystem(molecule, h, {"integrals_s": t1 - t0, "fit_s": t2 - t1,
-                                       "nao": orb.n_functions, "naux": auxb.n_functions})
+                                       "nao": orb.n_functions, "naux": auxb.n_functions},
+                         m if keep_integrals else None, tc if keep_integrals else None, aux)
 
 
 def atomic_density_guess(molecule: Molecule, *, basis: str = ORBITAL_BASIS, aux: str = AUX_BASIS, device=None,

@@ -1,0 +1,551 @@
+"""Learned SCF starting guess: fitted-density coefficients predicted per atom.
+
+The model predicts, for every atom, the coefficients of the Coulomb-fitted
+electron density ``rho(r) = sum_P d_P chi_P(r)`` on that atom's auxiliary
+functions, ``d = M^-1 (P|uv) (2 D_alpha)`` (M the Coulomb metric), from
+descriptors of the atom's environment. Waters are rigid, so each atom uses
+the frame of its own water (origin O, z along the H-O-H bisector, x along
+H1 - H2): descriptors are computed in that frame, and the predicted
+coefficients of l > 0 functions are rotated back to the laboratory frame
+with the real-solid-harmonic rotation matrices of ``rotation_matrices``.
+
+Descriptors: for each neighbour element (H, O) within 6 Å, 16 radial
+Gaussians with centres from 0.5 to 6 Å, weighted by a cosine cutoff and by
+the real solid harmonics (l <= 2) of the neighbour direction in the local
+frame: 2 x 16 x 9 = 288 numbers per atom.
+
+Network: one MLP per element, 288 -> 256 -> 256 -> n_coef (SiLU), BF16
+weights with FP32 accumulation, run and trained by the library's own
+kernels (``hafnium_amd.mlp``). Targets are whitened with the element's
+one-centre Coulomb metric G = U^T U, y = U (c_local - mean) / scale, so the
+squared error is the Coulomb self-energy of each atom's density error.
+
+Guess: ``F0 = h + J[d] - K[D_sad]`` (Coulomb from the predicted density,
+exchange from the superposition of atomic densities); the aufbau density
+of F0 starts the SCF. When its energy is above the energy of the atomic
+guess itself, the SCF starts from the atomic guess instead (the fallback).
+
+Training data: converged FP64 SCF densities of seeded water clusters
+produced by this library's own GPU SCF (``generate_dataset``).
+
+No counterpart in the reference (its guesses are the core Hamiltonian and
+the superposition of atomic densities, hafnium/scf/driver.py:216-246 and
+hafnium/scf/atoms.py:76-119).
+"""
+
+from __future__ import annotations
+
+import argparse
+import math
+import os
+import sys
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from hafnium_amd.errors import InputError
+from hafnium_amd.molecule import ANGSTROM_IN_BOHR, BasisOnAtoms, Molecule, load_basis, water_cluster
+
+N_RADIAL = 16
+R_MIN, R_MAX = 0.5, 6.0  # Å
+L_DESCRIPTOR = 2
+ELEMENTS = (1, 8)
+N_FEATURES = len(ELEMENTS) * N_RADIAL * (L_DESCRIPTOR + 1) ** 2
+HIDDEN = (256, 256)
+MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "learned_guess_water.npz")
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+# ---------------------------------------------------------------- solid harmonics
+
+def _cart_list(l):
+    out = []
+    for i in range(l, -1, -1):
+        for j in range(l - i, -1, -1):
+            out.append((i, j, l - i - j))
+    return out
+
+
+def _sph_order(l):
+    return [1, -1, 0] if l == 1 else list(range(-l, l + 1))
+
+
+def _binom(n, k):
+    return math.comb(n, k) if 0 <= k <= n else 0
+
+
+def cart_to_sph(l: int) -> np.ndarray:
+    """Real solid harmonics as Cartesian monomial sums, in the integral engine's order
+    (p: x, y, z; l != 1: m = -l..l), from the Helgaker-Jorgensen-Olsen expansion."""
+    carts = _cart_list(l)
+    index = {c: k for k, c in enumerate(carts)}
+    out = np.zeros((2 * l + 1, len(carts)))
+    for row, m in enumerate(_sph_order(l)):
+        am = abs(m)
+        wm = 1 if m < 0 else 0
+        norm = 1.0 / (2 ** am * math.factorial(l)) * math.sqrt(
+            2.0 * math.factorial(l + am) * math.factorial(l - am) / (2.0 if m == 0 else 1.0))
+        for t in range((l - am) // 2 + 1):
+            for u in range(t + 1):
+                for w in range(wm, am + 1, 2):
+                    sign = -1.0 if (t + (w - wm) // 2) % 2 else 1.0
+                    c = sign * 0.25 ** t * _binom(l, t) * _binom(l - t, am + t) * _binom(t, u) * _binom(am, w)
+                    out[row, index[(2 * t + am - 2 * u - w, 2 * u + w, l - 2 * t - am)]] += norm * c
+    return out
+
+
+_SPH_CACHE: dict = {}
+
+
+def _eval_cart(l, v):
+    v = np.atleast_2d(v)
+    return np.stack([v[:, 0] ** i * v[:, 1] ** j * v[:, 2] ** k for i, j, k in _cart_list(l)], axis=-1)
+
+
+def solid_harmonics(l: int, vectors: np.ndarray) -> np.ndarray:
+    """``S_lm(v)`` ``[npoints, 2l+1]``, every m scaled to unit mean square on the unit sphere."""
+    if l not in _SPH_CACHE:
+        t = cart_to_sph(l)
+        # Gauss-Legendre in cos(theta) x uniform phi: exact for the degree-2l products.
+        x, wx = np.polynomial.legendre.leggauss(l + 2)
+        phi = np.linspace(0.0, 2 * np.pi, 2 * l + 3, endpoint=False)
+        ct, ph = np.meshgrid(x, phi, indexing="ij")
+        st = np.sqrt(1.0 - ct ** 2)
+        pts = np.stack([st * np.cos(ph), st * np.sin(ph), ct], axis=-1).reshape(-1, 3)
+        w = np.repeat(wx, len(phi)) * (2 * np.pi / len(phi))
+        vals = _eval_cart(l, pts) @ t.T
+        norms = np.sqrt((w[:, None] * vals ** 2).sum(0) / (4 * np.pi))
+        _SPH_CACHE[l] = t / norms[:, None]
+    return _eval_cart(l, vectors) @ _SPH_CACHE[l].T
+
+
+_ROT_POINTS = None
+
+
+def rotation_matrices(rot: np.ndarray, lmax: int) -> list[np.ndarray]:
+    """``D^l`` (l = 0..lmax) with ``S_l(rot^T v) = S_l(v) D^l`` for every v.
+
+    A density ``sum_m c_m S_m(r)`` equals ``sum_m c'_m S_m(rot^T r)`` when
+    ``c = D c'``. Solved by least squares on 64 fixed directions (exact: the
+    harmonics of one l span a rotation-invariant space).
+    """
+    global _ROT_POINTS
+    if _ROT_POINTS is None:
+        pts = np.random.default_rng(12345).standard_normal((64, 3))
+        _ROT_POINTS = pts / np.linalg.norm(pts, axis=1, keepdims=True)
+    out = []
+    for l in range(lmax + 1):
+        y = solid_harmonics(l, _ROT_POINTS)
+        y_rot = solid_harmonics(l, _ROT_POINTS @ rot)
+        d, *_ = np.linalg.lstsq(y, y_rot, rcond=None)
+        out.append(d)
+    return out
+
+
+# ---------------------------------------------------------------- frames and descriptors
+
+def water_frames(molecule: Molecule) -> np.ndarray:
+    """Per atom, the rotation whose columns are its water's local axes in the laboratory frame.
+
+    Atoms come as (O, H, H) per water, as ``water_cluster`` places them.
+    """
+    pos = molecule.positions
+    nat = len(molecule.charges)
+    if nat % 3 or any(tuple(molecule.charges[3 * k:3 * k + 3]) != (8, 1, 1) for k in range(nat // 3)):
+        raise InputError("the learned guess is defined for water clusters in (O, H, H) order")
+    rots = np.empty((nat, 3, 3))
+    for k in range(nat // 3):
+        o, h1, h2 = pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]
+        z = (h1 - o) + (h2 - o)
+        z /= np.linalg.norm(z)
+        x = h1 - h2
+        x -= np.dot(x, z) * z
+        x /= np.linalg.norm(x)
+        rots[3 * k:3 * k + 3] = np.stack([x, np.cross(z, x), z], axis=1)
+    return rots
+
+
+def descriptors(molecule: Molecule) -> np.ndarray:
+    """``[natom, N_FEATURES]`` environment descriptors in each atom's water frame."""
+    pos = molecule.positions / ANGSTROM_IN_BOHR  # Å
+    charges = np.asarray(molecule.charges)
+    rots = water_frames(molecule)
+    centres = np.linspace(R_MIN, R_MAX, N_RADIAL)
+    width = centres[1] - centres[0]
+    nat = len(charges)
+    feats = np.zeros((nat, len(ELEMENTS), N_RADIAL, (L_DESCRIPTOR + 1) ** 2))
+    for a in range(nat):
+        d = pos - pos[a]
+        r = np.linalg.norm(d, axis=1)
+        mask = (r > 1e-8) & (r < R_MAX)
+        if not np.any(mask):
+            continue
+        local = (d[mask] @ rots[a]) / r[mask, None]
+        cut = 0.5 * (np.cos(np.pi * r[mask] / R_MAX) + 1.0)
+        radial = np.exp(-0.5 * ((r[mask, None] - centres[None, :]) / width) ** 2) * cut[:, None]
+        ang = np.concatenate([solid_harmonics(l, local) for l in range(L_DESCRIPTOR + 1)], axis=1)
+        for e_i, z in enumerate(ELEMENTS):
+            sel = charges[mask] == z
+            if np.any(sel):
+                feats[a, e_i] = radial[sel].T @ ang[sel]
+    return feats.reshape(nat, -1)
+
+
+# ---------------------------------------------------------------- coefficient layout
+
+def shell_blocks(aux_name: str, z: int):
+    """``([(start, l), ...], n)``: the (2l+1)-blocks of element z's auxiliary functions."""
+    blocks = []
+    at = 0
+    for sh in load_basis(aux_name)[str(z)]:
+        for _ in sh["coefficients"]:
+            blocks.append((at, sh["l"]))
+            at += 2 * sh["l"] + 1
+    return blocks, at
+
+
+def _lmax(aux_name: str) -> int:
+    return max(sh["l"] for z in ELEMENTS for sh in load_basis(aux_name)[str(z)])
+
+
+def _rotate(coeffs, blocks, dmats, inverse=False):
+    out = coeffs.copy()
+    for start, l in blocks:
+        d = dmats[l].T if inverse else dmats[l]
+        out[start:start + 2 * l + 1] = d @ coeffs[start:start + 2 * l + 1]
+    return out
+
+
+def atom_offsets(molecule: Molecule, aux_name: str) -> list[int]:
+    offs, at = [], 0
+    for z in molecule.charges:
+        offs.append(at)
+        at += shell_blocks(aux_name, z)[1]
+    return offs
+
+
+def to_local(molecule: Molecule, coeffs: np.ndarray, aux_name: str) -> dict:
+    """Laboratory-frame coefficients -> ``{z: [atoms of z, n_coef_z]}`` in each atom's water frame.
+
+    The local coefficients c' of an atom satisfy c = D(R) c' (``rotation_matrices``),
+    so c' = D^T c (D is orthogonal).
+    """
+    rots = water_frames(molecule)
+    offs = atom_offsets(molecule, aux_name)
+    lmax = _lmax(aux_name)
+    out: dict = {}
+    for a, z in enumerate(molecule.charges):
+        blocks, n = shell_blocks(aux_name, z)
+        dm = rotation_matrices(rots[a], lmax)
+        out.setdefault(z, []).append(_rotate(coeffs[offs[a]:offs[a] + n], blocks, dm, inverse=True))
+    return {z: np.asarray(v) for z, v in out.items()}
+
+
+def to_lab(molecule: Molecule, local: dict, aux_name: str) -> np.ndarray:
+    """Inverse of :func:`to_local`."""
+    rots = water_frames(molecule)
+    offs = atom_offsets(molecule, aux_name)
+    lmax = _lmax(aux_name)
+    total = offs[-1] + shell_blocks(aux_name, molecule.charges[-1])[1]
+    out = np.zeros(total)
+    seen: dict = {}
+    for a, z in enumerate(molecule.charges):
+        blocks, n = shell_blocks(aux_name, z)
+        i = seen.get(z, 0)
+        out[offs[a]:offs[a] + n] = _rotate(np.asarray(local[z][i], dtype=np.float64), blocks,
+                                           rotation_matrices(rots[a], lmax))
+        seen[z] = i + 1
+    return out
+
+
+def fitted_coefficients(metric: np.ndarray, three_center: np.ndarray, density_alpha) -> np.ndarray:
+    """``d = M^-1 (P|uv) (2 D_alpha)``: the Coulomb-fitted coefficients of the total density."""
+    naux = metric.shape[0]
+    gamma = three_center.reshape(naux, -1) @ (2.0 * np.asarray(density_alpha)).reshape(-1)
+    return np.linalg.solve(metric, gamma)
+
+
+def coulomb_of_coefficients(three_center: np.ndarray, coeffs: np.ndarray) -> np.ndarray:
+    """``J_uv = sum_P (P|uv) d_P``: the Coulomb matrix of a fitted density (= J[2 D_alpha] when d fits D)."""
+    naux, n, _ = three_center.shape
+    return (coeffs @ three_center.reshape(naux, n * n)).reshape(n, n)
+
+
+def one_centre_metric(aux_name: str, z: int) -> np.ndarray:
+    from hafnium_amd import integrals
+
+    atom = Molecule((z,), np.zeros((1, 3)))
+    return integrals.metric(BasisOnAtoms(aux_name, atom))
+
+
+# ---------------------------------------------------------------- the model
+
+@dataclass
+class ElementModel:
+    z: int
+    feature_mean: np.ndarray
+    feature_scale: np.ndarray
+    target_mean: np.ndarray       # local coefficients
+    whitening: np.ndarray         # U with G = U^T U (one-centre metric)
+    target_scale: float
+    params: np.ndarray            # flat MLP parameters (FP32 master)
+
+    @property
+    def n_coef(self) -> int:
+        return len(self.target_mean)
+
+    def whiten(self, local: np.ndarray) -> np.ndarray:
+        return ((local - self.target_mean) @ self.whitening.T) / self.target_scale
+
+    def unwhiten(self, y: np.ndarray) -> np.ndarray:
+        return np.linalg.solve(self.whitening, (y * self.target_scale).T).T + self.target_mean
+
+    def normalise(self, feats: np.ndarray) -> np.ndarray:
+        return ((feats - self.feature_mean) / self.feature_scale).astype(np.float32)
+
+
+class LearnedGuessModel:
+    """Per-element networks and their normalisation; ``predict`` gives laboratory-frame coefficients."""
+
+    def __init__(self, elements: dict, aux_name: str, *, device=None) -> None:
+        self.elements = elements
+        self.aux_name = aux_name
+        self.device = device
+        self._nets: dict = {}
+
+    def net(self, z: int):
+        from hafnium_amd.mlp import Mlp
+
+        if z not in self._nets:
+            em = self.elements[z]
+            net = Mlp([N_FEATURES, *HIDDEN, em.n_coef], device=self.device)
+            net.set_params(em.params)
+            self._nets[z] = net
+        return self._nets[z]
+
+    def predict_local(self, molecule: Molecule) -> dict:
+        torch = _torch()
+        feats = descriptors(molecule)
+        charges = np.asarray(molecule.charges)
+        out = {}
+        for z in ELEMENTS:
+            sel = charges == z
+            if not np.any(sel):
+                continue
+            em = self.elements[z]
+            net = self.net(z)
+            x = torch.from_numpy(em.normalise(feats[sel])).to(net.device)
+            y = net.forward(x).cpu().numpy().astype(np.float64)
+            out[z] = em.unwhiten(y)
+        return out
+
+    def predict(self, molecule: Molecule) -> np.ndarray:
+        return to_lab(molecule, self.predict_local(molecule), self.aux_name)
+
+    def save(self, path: str) -> None:
+        arrays = {"aux_name": np.array(self.aux_name), "hidden": np.array(HIDDEN)}
+        for z, em in self.elements.items():
+            for key in ("feature_mean", "feature_scale", "target_mean", "whitening", "params"):
+                arrays[f"{z}_{key}"] = getattr(em, key)
+            arrays[f"{z}_target_scale"] = np.array(em.target_scale)
+        np.savez_compressed(path, **arrays)
+
+    @classmethod
+    def load(cls, path: str = MODEL_PATH, *, device=None) -> "LearnedGuessModel":
+        if not os.path.exists(path):
+            raise InputError(f"no trained learned-guess model at {path}")
+        with np.load(path, allow_pickle=False) as f:
+            if tuple(int(h) for h in f["hidden"]) != HIDDEN:
+                raise InputError("the stored model has another hidden width")
+            elements = {}
+            for z in ELEMENTS:
+                elements[z] = ElementModel(z, f[f"{z}_feature_mean"], f[f"{z}_feature_scale"],
+                                           f[f"{z}_target_mean"], f[f"{z}_whitening"],
+                                           float(f[f"{z}_target_scale"]), f[f"{z}_params"])
+            return cls(elements, str(f["aux_name"]), device=device)
+
+
+# ---------------------------------------------------------------- data
+
+def cluster_record(n_waters: int, seed: int, sad_cache: dict, *, device=None) -> dict:
+    """Converge (H2O)_n in FP64 from the atomic guess; descriptors and local fitted coefficients per atom."""
+    from hafnium_amd.clusters import atomic_density_guess, build_system, run_cluster
+
+    system = build_system(water_cluster(n_waters, seed), device=device, keep_integrals=True)
+    guess = atomic_density_guess(system.molecule, device=device, cache=sad_cache)
+    result, _ = run_cluster(n_waters, seed, system=system, guess=guess)
+    d = fitted_coefficients(system.metric, system.three_center, result.density.cpu().numpy())
+    local = to_local(system.molecule, d, system.aux_basis)
+    return {"features": descriptors(system.molecule), "charges": np.asarray(system.molecule.charges),
+            "local": local, "energy": result.energy, "iterations": result.n_iterations}
+
+
+def generate_dataset(seeds, sizes, path: str, *, device=None, log=print) -> dict:
+    """Converged FP64 densities of ``water_cluster(sizes[i], seeds[i])`` as per-element arrays in ``path``."""
+    sad_cache: dict = {}
+    feats = {z: [] for z in ELEMENTS}
+    targets = {z: [] for z in ELEMENTS}
+    cluster_of = {z: [] for z in ELEMENTS}
+    meta = []
+    t0 = time.perf_counter()
+    for index, (seed, n) in enumerate(zip(seeds, sizes)):
+        rec = cluster_record(int(n), int(seed), sad_cache, device=device)
+        for z in ELEMENTS:
+            sel = rec["charges"] == z
+            feats[z].append(rec["features"][sel].astype(np.float32))
+            targets[z].append(rec["local"][z])
+            cluster_of[z].extend([index] * int(sel.sum()))
+        meta.append((int(seed), int(n), rec["energy"], rec["iterations"]))
+        log(f"[data] {index + 1}/{len(seeds)} n={n} seed={seed} E={rec['energy']:.10f} it={rec['iterations']} "
+            f"t={time.perf_counter() - t0:.0f}s", flush=True)
+    arrays = {"meta": np.asarray(meta, dtype=np.float64)}
+    for z in ELEMENTS:
+        arrays[f"x_{z}"] = np.concatenate(feats[z])
+        arrays[f"y_{z}"] = np.concatenate(targets[z])
+        arrays[f"cluster_{z}"] = np.asarray(cluster_of[z], dtype=np.int32)
+    np.savez_compressed(path, **arrays)
+    return arrays
+
+
+# ---------------------------------------------------------------- training
+
+def train(data: dict, *, aux_name: str = "def2-universal-jkfit", epochs: int = 300, batch: int = 256,
+          lr: float = 1e-3, weight_decay: float = 1e-5, validation_fraction: float = 0.1, seed: int = 0,
+          device=None, log=print) -> tuple["LearnedGuessModel", dict]:
+    """Train one network per element on the GPU; returns the model with the best validation loss."""
+    from hafnium_amd.mlp import Mlp
+
+    torch = _torch()
+    dev = torch.device(device if device is not None else "cuda:0")
+    rng = np.random.default_rng(seed)
+    n_clusters = int(data["meta"].shape[0])
+    val_clusters = set(rng.choice(n_clusters, max(1, int(round(validation_fraction * n_clusters))),
+                                  replace=False).tolist())
+    elements = {}
+    report = {}
+    for z in ELEMENTS:
+        x = np.asarray(data[f"x_{z}"], dtype=np.float64)
+        local = np.asarray(data[f"y_{z}"], dtype=np.float64)
+        is_val = np.isin(np.asarray(data[f"cluster_{z}"]), list(val_clusters))
+        xt, lt = x[~is_val], local[~is_val]
+        g = one_centre_metric(aux_name, z)
+        u = np.linalg.cholesky(g).T
+        fmean = xt.mean(0)
+        fscale = np.maximum(xt.std(0), 1e-6)
+        tmean = lt.mean(0)
+        resid = (lt - tmean) @ u.T
+        tscale = float(np.sqrt(np.mean(resid ** 2))) or 1.0
+        em = ElementModel(z, fmean, fscale, tmean, u, tscale, np.zeros(0))
+        net = Mlp([N_FEATURES, *HIDDEN, em.n_coef], seed=seed + z, device=dev)
+        xs = torch.from_numpy(em.normalise(x)).to(dev)
+        ys = torch.from_numpy(em.whiten(local).astype(np.float32)).to(dev)
+        train_idx = np.nonzero(~is_val)[0].astype(np.int32)
+        val_idx = torch.from_numpy(np.nonzero(is_val)[0].astype(np.int32)).to(dev)
+        n_train = len(train_idx)
+        steps_per_epoch = max(1, n_train // batch)
+        total = epochs * steps_per_epoch
+        best = (math.inf, None, -1)
+        step = 0
+        t0 = time.perf_counter()
+        for epoch in range(epochs):
+            perm = torch.from_numpy(rng.permutation(train_idx)).to(dev)
+            for s in range(steps_per_epoch):
+                # Cosine decay to 5 % of the initial rate.
+                rate = lr * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * step / total)))
+                net.train_step(xs, ys, (perm, s * batch, (s + 1) * batch), lr=rate, weight_decay=weight_decay)
+                step += 1
+            if epoch % 10 == 9 or epoch == epochs - 1:
+                nv = int(val_idx.numel())
+                if nv:
+                    yv = net.forward(xs, (val_idx, 0, nv))
+                    val = float(net.loss(yv, ys, (val_idx, 0, nv), scale=1.0 / nv).item())
+                else:
+                    val = math.nan
+                if not val > best[0]:
+                    best = (val, net.params.cpu().numpy().copy(), epoch)
+                log(f"[train] z={z} epoch {epoch + 1}/{epochs} val={val:.4e} (per-atom whitened MSE x dim) "
+                    f"t={time.perf_counter() - t0:.1f}s", flush=True)
+        em.params = best[1]
+        elements[z] = em
+        # Baseline: the mean local density (what the network adds to).
+        nv = int(is_val.sum())
+        base = float(np.mean(np.sum(em.whiten(local[is_val]) ** 2, axis=1))) if nv else math.nan
+        report[z] = {"train_atoms": int(n_train), "val_atoms": nv, "best_val": best[0], "best_epoch": best[2],
+                     "mean_density_val": base, "seconds": time.perf_counter() - t0}
+    return LearnedGuessModel(elements, aux_name, device=dev), report
+
+
+# ---------------------------------------------------------------- the guess
+
+def learned_guess(system, model: LearnedGuessModel, sad_density: np.ndarray, *, precision: str = "fp64"):
+    """Starting density (D_alpha, device) from the predicted fitted density, with the atomic-guess fallback.
+
+    Returns ``(density, info)``; ``info["used"]`` is "learned" or "sad".
+    """
+    from hafnium_amd.scf import _density, _energy_terms, _Setup, ScfOptions
+
+    torch = _torch()
+    if system.three_center is None:
+        raise InputError("build the system with keep_integrals=True for the learned guess")
+    h = system.hamiltonian
+    t0 = time.perf_counter()
+    coeffs = model.predict(system.molecule)
+    j_pred = torch.from_numpy(coulomb_of_coefficients(system.three_center, coeffs)).to(h.device)
+    sad = torch.from_numpy(np.ascontiguousarray(sad_density)).to(h.device)
+    k_sad = h.two_body.exchange(sad, precision=precision)
+    dn = h.dense
+    f0 = dn.combine(1.0, h.core, 1.0, j_pred, -1.0, k_sad)
+    setup = _Setup(h, ScfOptions())
+    _, c = setup.solve(f0)
+    density, orbitals = _density(dn, c, n_alpha=h.n_alpha)
+    _, terms_l = h.fock(density, orbitals, precision)
+    _, terms_s = h.fock(sad, None, precision)
+    e_learned = _energy_terms(h, terms_l.cpu().numpy()).total
+    e_sad = _energy_terms(h, terms_s.cpu().numpy()).total
+    use = e_learned <= e_sad
+    info = {"used": "learned" if use else "sad", "energy_learned": e_learned, "energy_sad": e_sad,
+            "seconds": time.perf_counter() - t0}
+    return (density if use else sad), info
+
+
+# ---------------------------------------------------------------- command line
+
+def _main(argv=None) -> int:
+    p = argparse.ArgumentParser(prog="python -m hafnium_amd.learned_guess")
+    sub = p.add_subparsers(dest="cmd", required=True)
+    g = sub.add_parser("generate", help="converged FP64 cluster densities -> dataset .npz")
+    g.add_argument("--count", type=int, default=400)
+    g.add_argument("--sizes", default="2-8", help="water counts drawn uniformly from this range")
+    g.add_argument("--seed0", type=int, default=100000)
+    g.add_argument("--out", required=True)
+    t = sub.add_parser("train", help="dataset .npz -> model .npz")
+    t.add_argument("--data", required=True)
+    t.add_argument("--out", required=True)
+    t.add_argument("--epochs", type=int, default=300)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--batch", type=int, default=256)
+    args = p.parse_args(argv)
+    if args.cmd == "generate":
+        lo, hi = (int(v) for v in args.sizes.split("-"))
+        rng = np.random.default_rng(args.seed0)
+        sizes = rng.integers(lo, hi + 1, args.count)
+        seeds = args.seed0 + np.arange(args.count)
+        generate_dataset(seeds, sizes, args.out)
+    else:
+        with np.load(args.data, allow_pickle=False) as f:
+            data = {k: f[k] for k in f.files}
+        model, report = train(data, epochs=args.epochs, lr=args.lr, batch=args.batch)
+        model.save(args.out)
+        for z, r in report.items():
+            print(f"[train] element {z}: {r}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(_main())

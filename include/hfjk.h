@@ -248,6 +248,43 @@ HFJK_API int hfjk_get_tuning(hfjk_ctx *ctx, int param, int *value);
 HFJK_API int hfjk_profile(hfjk_ctx *ctx, int enable);
 HFJK_API int hfjk_profile_read(hfjk_ctx *ctx, const char **names, float *ms, int max);
 
+/* ---- Learned-guess MLP kernels (no counterpart in the reference: its
+ * guesses are the core Hamiltonian and the superposition of atomic
+ * densities, hafnium/scf/driver.py:216-246 and hafnium/scf/atoms.py:76-119).
+ * Arithmetic: GEMM operands are rounded to bf16 (nearest even) on load,
+ * products accumulate in FP32; everything else is FP32. Device pointers,
+ * row-major storage. */
+
+/* c[m][n] = op(a) op(b) (+ c if accumulate) (+ bias[n]); h_silu (optional,
+ * same leading dimension as c) receives silu(c). op(a) is m x k: a is stored
+ * m x k (trans_a = 0) or k x m (trans_a = 1) with leading dimension lda, as
+ * fp32 or (a_bf16 = 1) bf16; storage row r of a is read from row a_rows[r]
+ * when a_rows is given (every index must address a stored row). op(b) is
+ * k x n: b stored k x n (trans_b = 0) or n x k (trans_b = 1). */
+HFJK_API int hfjk_mlp_gemm(hfjk_ctx *ctx, int m, int n, int k, const void *a, int a_bf16, int lda, int trans_a,
+                           const int *a_rows, const void *b, int b_bf16, int ldb, int trans_b, const float *bias,
+                           float *c, int ldc, int accumulate, float *h_silu, void *stream);
+/* grad_z = grad_h * silu'(z), elementwise over count. */
+HFJK_API int hfjk_mlp_silu_backward(hfjk_ctx *ctx, long long count, const float *grad_h, const float *z,
+                                    float *grad_z, void *stream);
+/* out[j] = sum_i x[i][j] over an m x n matrix (fixed summation order). */
+HFJK_API int hfjk_mlp_column_sum(hfjk_ctx *ctx, int m, int n, const float *x, int ldx, float *out, void *stream);
+/* loss[0] = scale sum_ij w_j (y_ij - t_ij)^2 and (grad optional, leading
+ * dimension ldy) grad_ij = 2 scale w_j (y_ij - t_ij); target row i is
+ * target_rows[i] when given, col_weights NULL means w = 1. */
+HFJK_API int hfjk_mlp_mse(hfjk_ctx *ctx, int m, int n, const float *y, int ldy, const float *target, int ldt,
+                          const int *target_rows, const float *col_weights, float scale, float *grad, float *loss,
+                          void *stream);
+/* Adam with decoupled weight decay over count parameters:
+ * m1 = b1 m1 + (1 - b1) g, m2 = b2 m2 + (1 - b2) g^2,
+ * p -= lr_t m1 / (sqrt(m2) + eps) + lr weight_decay p, and params_bf16
+ * (optional) = bf16(p). lr_t carries the bias corrections of the step. */
+HFJK_API int hfjk_mlp_adam(hfjk_ctx *ctx, long long count, float *params, const float *grads, float *moment1,
+                           float *moment2, uint16_t *params_bf16, float lr, float lr_t, float beta1,
+                           float beta2, float eps, float weight_decay, void *stream);
+/* dst = bf16(src) (nearest even) over count elements. */
+HFJK_API int hfjk_mlp_to_bf16(hfjk_ctx *ctx, long long count, const float *src, uint16_t *dst, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
