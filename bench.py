@@ -187,48 +187,71 @@ def bench_batched(args, dev):
 def bench_clusters(args, dev):
     """SCF time-to-solution on seeded water clusters, plain FP64 against the mixed policy.
 
-    (H2O)_n for the sizes in ``--cluster-sizes`` (seed = 100 + n), cc-pVDZ
-    with def2-universal-jkfit, converged to |dE| < 1e-8 Eh and a gradient
-    below 1e-6, both from the superposition-of-atomic-densities guess. The
-    SCF wall time excludes the integrals and the fit, which are the same for
-    both; the energy difference of the mixed run from the FP64 run is
-    reported per cluster.
+    (H2O)_n for the sizes in ``--cluster-sizes`` (seed = 100 + n, seeds the
+    learned guess was not trained on), cc-pVDZ with def2-universal-jkfit,
+    converged to |dE| < 1e-8 Eh and a gradient below 1e-6. Runs per cluster:
+    plain FP64 from the superposition-of-atomic-densities guess (the
+    baseline), the mixed policy from the same guess, and, when a trained
+    model is stored, FP64 and mixed from the learned guess. The SCF wall time
+    excludes the integrals and the fit, which are the same for every run; the
+    learned runs include the guess itself (descriptors, network, J of the
+    predicted density, K of the atomic guess, one diagonalisation and the two
+    energies of the fallback test). Energy differences are against the FP64
+    baseline.
     """
+    import os
+
     import torch
 
+    from hafnium_amd import learned_guess as lg
     from hafnium_amd.clusters import atomic_density_guess, build_system, run_cluster
     from hafnium_amd.molecule import water_cluster
 
+    model = lg.LearnedGuessModel.load(device=dev) if os.path.exists(lg.MODEL_PATH) and not args.no_learned else None
     cache = {}
     rows = []
     for n in [int(x) for x in args.cluster_sizes.split(",") if x]:
         mol = water_cluster(n, 100 + n)
-        system = build_system(mol, device=dev)
+        system = build_system(mol, device=dev, keep_integrals=model is not None)
         guess = atomic_density_guess(mol, device=dev, cache=cache)
         # Warm-up: kernels, plans and mixed tensor copies are prepared once per system.
         run_cluster(n, 100 + n, system=system, guess=guess, precision=args.cluster_mixed)
+        if model is not None:
+            lg.learned_guess(system, model, guess)
         entry = {"n_waters": n, "nao": system.timings["nao"], "naux": system.timings["naux"],
                  "integrals_s": round(system.timings["integrals_s"], 3), "fit_s": round(system.timings["fit_s"], 3)}
-        for label, precision in (("fp64", "fp64"), ("mixed", args.cluster_mixed)):
+        runs = [("fp64", "fp64", False), ("mixed", args.cluster_mixed, False)]
+        if model is not None:
+            runs += [("learned_fp64", "fp64", True), ("learned_mixed", args.cluster_mixed, True)]
+        for label, precision, learned in runs:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            result, _ = run_cluster(n, 100 + n, system=system, guess=guess, precision=precision)
+            start, info = guess, None
+            if learned:
+                start, info = lg.learned_guess(system, model, guess, precision=precision)
+            result, _ = run_cluster(n, 100 + n, system=system, guess=start, precision=precision)
             torch.cuda.synchronize()
             entry[label] = {"seconds": time.perf_counter() - t0, "iterations": result.n_iterations,
                             "energy": result.energy,
                             "mixed_iterations": sum(1 for r in result.history if r.precision != "fp64")}
-        entry["energy_diff"] = abs(entry["mixed"]["energy"] - entry["fp64"]["energy"])
-        entry["speedup"] = entry["fp64"]["seconds"] / entry["mixed"]["seconds"]
+            if info is not None:
+                entry[label]["guess"] = info["used"]
+                entry[label]["guess_seconds"] = info["seconds"]
+            entry[label]["energy_diff"] = abs(result.energy - entry["fp64"]["energy"])
+            entry[label]["speedup"] = entry["fp64"]["seconds"] / entry[label]["seconds"]
         rows.append(entry)
         log(f"cluster {entry}")
         del system
         torch.cuda.empty_cache()
-    total64 = sum(r["fp64"]["seconds"] for r in rows)
-    totalm = sum(r["mixed"]["seconds"] for r in rows)
-    return {"basis": "cc-pvdz", "aux": "def2-universal-jkfit", "guess": "atomic densities (DF)",
-            "energy_tolerance": 1e-8, "mixed_precision": args.cluster_mixed, "clusters": rows,
-            "total_seconds": {"fp64": total64, "mixed": totalm}, "speedup_total": total64 / totalm if totalm else None,
-            "max_energy_diff": max(r["energy_diff"] for r in rows) if rows else None}
+    labels = [r[0] for r in runs] if rows else []
+    totals = {label: sum(r[label]["seconds"] for r in rows) for label in labels}
+    out = {"basis": "cc-pvdz", "aux": "def2-universal-jkfit", "guess": "atomic densities (DF)",
+           "energy_tolerance": 1e-8, "mixed_precision": args.cluster_mixed, "clusters": rows,
+           "total_seconds": totals,
+           "speedup_total": {label: totals["fp64"] / totals[label] for label in labels if totals[label]},
+           "mean_iterations": {label: sum(r[label]["iterations"] for r in rows) / len(rows) for label in labels},
+           "max_energy_diff": max(r[label]["energy_diff"] for r in rows for label in labels) if rows else None}
+    return out
 
 
 def main(argv=None):
@@ -248,6 +271,7 @@ def main(argv=None):
     ap.add_argument("--no-clusters", action="store_true", help="skip the water-cluster SCF timings")
     ap.add_argument("--cluster-sizes", default="2,4,8,12")
     ap.add_argument("--cluster-mixed", default="bf16x3")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess cluster runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB read+write elementwise kernel (counter calibration)")
     args = ap.parse_args(argv)

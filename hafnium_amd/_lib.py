@@ -84,8 +84,8 @@ SIGNATURES = [
     ("hfjk_mlp_column_sum", _c_int, [_vp, _c_int, _c_int, _vp, _c_int, _vp, _vp]),
     ("hfjk_mlp_mse", _c_int, [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _vp, ctypes.c_float, _vp, _vp,
                               _vp]),
-    ("hfjk_mlp_adam", _c_int, [_vp, _c_ll, _vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                               ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]),
+    ("hfjk_mlp_adam", _c_int, [_vp, _c_ll, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp]),
     ("hfjk_mlp_to_bf16", _c_int, [_vp, _c_ll, _vp, _vp, _vp]),
 ]
 

@@ -197,18 +197,20 @@ __global__ __launch_bounds__(kT) void mse_kernel(int m, int n, const float* __re
 
 // Adam (decoupled weight decay): m, v moments; p -= lr_t m / (sqrt(v) + eps) + lr wd p;
 // the bf16 copy of p is refreshed in the same pass.
+// The host forms 1 - beta in double (1 - 0.999f in float loses 4 digits).
 __global__ __launch_bounds__(kT) void adam_kernel(long long count, float* __restrict__ p, const float* __restrict__ g,
                                                   float* __restrict__ m1, float* __restrict__ m2,
-                                                  uint16_t* __restrict__ p16, float lr, float lr_t, float beta1,
-                                                  float beta2, float eps, float weight_decay) {
+                                                  uint16_t* __restrict__ p16, float lr_t, float beta1,
+                                                  float one_minus_beta1, float beta2, float one_minus_beta2,
+                                                  float eps, float decay) {
   for (long long e = (long long)blockIdx.x * kT + threadIdx.x; e < count; e += (long long)gridDim.x * kT) {
     const float gi = g[e];
-    const float a = beta1 * m1[e] + (1.0f - beta1) * gi;
-    const float b = beta2 * m2[e] + (1.0f - beta2) * gi * gi;
+    const float a = beta1 * m1[e] + one_minus_beta1 * gi;
+    const float b = beta2 * m2[e] + one_minus_beta2 * gi * gi;
     m1[e] = a;
     m2[e] = b;
     float x = p[e];
-    x -= lr_t * a / (sqrtf(b) + eps) + lr * weight_decay * x;
+    x -= lr_t * a / (sqrtf(b) + eps) + decay * x;
     p[e] = x;
     if (p16) p16[e] = to_bf16(x);
   }
@@ -315,15 +317,18 @@ int hfjk_mlp_mse(hfjk_ctx* ctx, int m, int n, const float* y, int ldy, const flo
 }
 
 int hfjk_mlp_adam(hfjk_ctx* ctx, long long count, float* params, const float* grads, float* moment1, float* moment2,
-                  uint16_t* params_bf16, float lr, float lr_t, float beta1, float beta2, float eps,
-                  float weight_decay, void* stream) {
+                  uint16_t* params_bf16, double lr, double lr_t, double beta1, double beta2, double eps,
+                  double weight_decay, void* stream) {
   return mguard(ctx, [&] {
     if (count < 0 || (count > 0 && (params == nullptr || grads == nullptr || moment1 == nullptr ||
                                     moment2 == nullptr)))
       mfail(HFJK_ERR_INPUT, "hfjk_mlp_adam: null buffer");
+    if (!(beta1 >= 0.0 && beta1 < 1.0 && beta2 >= 0.0 && beta2 < 1.0 && eps >= 0.0))
+      mfail(HFJK_ERR_INPUT, "hfjk_mlp_adam: betas in [0, 1) and eps >= 0 are required");
     if (count == 0) return;
     hipLaunchKernelGGL(adam_kernel, dim3(hfjk::grid_for(count)), dim3(kT), 0, (hipStream_t)stream, count, params,
-                       grads, moment1, moment2, params_bf16, lr, lr_t, beta1, beta2, eps, weight_decay);
+                       grads, moment1, moment2, params_bf16, (float)lr_t, (float)beta1, (float)(1.0 - beta1),
+                       (float)beta2, (float)(1.0 - beta2), (float)eps, (float)(lr * weight_decay));
     mcheck(hipGetLastError(), "adam launch");
   });
 }

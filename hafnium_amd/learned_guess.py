@@ -448,7 +448,9 @@ def train(data: dict, *, aux_name: str = "def2-universal-jkfit", epochs: int = 3
         train_idx = np.nonzero(~is_val)[0].astype(np.int32)
         val_idx = torch.from_numpy(np.nonzero(is_val)[0].astype(np.int32)).to(dev)
         n_train = len(train_idx)
-        steps_per_epoch = max(1, n_train // batch)
+        if n_train == 0:
+            raise InputError(f"no training atoms of element {z}")
+        steps_per_epoch = (n_train + batch - 1) // batch  # the last minibatch may be short
         total = epochs * steps_per_epoch
         best = (math.inf, None, -1)
         step = 0
@@ -458,7 +460,8 @@ def train(data: dict, *, aux_name: str = "def2-universal-jkfit", epochs: int = 3
             for s in range(steps_per_epoch):
                 # Cosine decay to 5 % of the initial rate.
                 rate = lr * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * step / total)))
-                net.train_step(xs, ys, (perm, s * batch, (s + 1) * batch), lr=rate, weight_decay=weight_decay)
+                net.train_step(xs, ys, (perm, s * batch, min((s + 1) * batch, n_train)), lr=rate,
+                               weight_decay=weight_decay)
                 step += 1
             if epoch % 10 == 9 or epoch == epochs - 1:
                 nv = int(val_idx.numel())

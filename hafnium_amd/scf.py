@@ -12,8 +12,9 @@ the last result as ``last`` (hafnium/errors.py:33-43).
 
 Added for MI355X: a precision policy. J and K run in ``options.precision``
 ("bf16x3", "fp32" or "fp64") while the orbital-gradient norm (the DIIS
-error) is at or above ``options.switch_threshold``, then in FP64 for every
-remaining iteration; convergence is only declared on FP64 iterations, and
+error) is at or above ``options.switch_threshold`` and still falling
+(``mixed_phase_stalled``), then in FP64 for every remaining iteration;
+convergence is only declared on FP64 iterations, and
 the final rebuild is FP64, so the converged energy is that of the FP64
 operator. The Fock diagonalisation, DIIS, the gradient and all energy sums
 stay FP64. The K of the loop uses the occupied orbitals of the previous
@@ -57,6 +58,8 @@ class ScfOptions:
     precision: str = "fp64"
     switch_threshold: float = 1e-5
     reset_diis_on_switch: bool = True
+    stall_window: int = 2
+    stall_ratio: float = 0.7
 
     def __post_init__(self) -> None:
         if self.max_iterations < 1:
@@ -73,6 +76,23 @@ class ScfOptions:
             raise InputError("the overlap threshold is positive")
         if self.precision not in ("fp64", "fp32", "bf16x3"):
             raise InputError("precision is fp64, fp32 or bf16x3")
+        if self.stall_window < 1 or not 0.0 < self.stall_ratio <= 1.0:
+            raise InputError("stall_window >= 1 and 0 < stall_ratio <= 1 are required")
+
+
+def mixed_phase_stalled(gradient_norms, window: int, ratio: float) -> bool:
+    """True when the last ``window`` reduced-precision iterations did not bring the
+    orbital gradient below ``ratio`` times the best norm before them.
+
+    The J/K error of a reduced arithmetic sets a floor under the gradient
+    (about 1e-4 for BF16x3 on small water clusters, above the 1e-5 switch
+    threshold); once DIIS stops making progress against that floor, further
+    reduced-precision iterations are wasted.
+    """
+    g = list(gradient_norms)
+    if len(g) < window + 2:
+        return False
+    return min(g[-window:]) > ratio * min(g[:-window])
 
 
 @dataclass(frozen=True)
@@ -340,9 +360,10 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
         if precision == "fp64" and abs(change) < options.energy_tolerance and gnorm < options.gradient_tolerance:
             converged = True
             break
-        mixed_iterations = sum(1 for row in history if row.precision != "fp64")
+        mixed_norms = [row.gradient_norm for row in history if row.precision != "fp64"]
         if precision != "fp64" and (gnorm < options.switch_threshold
-                                    or mixed_iterations >= options.max_iterations // 2):
+                                    or mixed_phase_stalled(mixed_norms, options.stall_window, options.stall_ratio)
+                                    or len(mixed_norms) >= options.max_iterations // 2):
             precision = "fp64"
             if options.reset_diis_on_switch:
                 focks, errors = [fock], [grad]

@@ -278,10 +278,12 @@ HFJK_API int hfjk_mlp_mse(hfjk_ctx *ctx, int m, int n, const float *y, int ldy, 
 /* Adam with decoupled weight decay over count parameters:
  * m1 = b1 m1 + (1 - b1) g, m2 = b2 m2 + (1 - b2) g^2,
  * p -= lr_t m1 / (sqrt(m2) + eps) + lr weight_decay p, and params_bf16
- * (optional) = bf16(p). lr_t carries the bias corrections of the step. */
+ * (optional) = bf16(p). lr_t carries the bias corrections of the step. The
+ * hyperparameters are double so that 1 - beta is formed exactly; the update
+ * itself runs in FP32. */
 HFJK_API int hfjk_mlp_adam(hfjk_ctx *ctx, long long count, float *params, const float *grads, float *moment1,
-                           float *moment2, uint16_t *params_bf16, float lr, float lr_t, float beta1,
-                           float beta2, float eps, float weight_decay, void *stream);
+                           float *moment2, uint16_t *params_bf16, double lr, double lr_t, double beta1,
+                           double beta2, double eps, double weight_decay, void *stream);
 /* dst = bf16(src) (nearest even) over count elements. */
 HFJK_API int hfjk_mlp_to_bf16(hfjk_ctx *ctx, long long count, const float *src, uint16_t *dst, void *stream);
 

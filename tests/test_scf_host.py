@@ -42,3 +42,16 @@ def test_options_are_validated():
         scf.ScfOptions(damping=1.0)
     terms = scf.EnergyTerms(9.0, -120.0, 45.0, -9.0)
     assert terms.total == pytest.approx(9.0 - 120.0 + 45.0 - 9.0)
+
+
+def test_mixed_phase_stall_detection():
+    # Geometric convergence never stalls.
+    assert not scf.mixed_phase_stalled([1e-1, 3e-2, 1e-2, 3e-3, 1e-3], 2, 0.7)
+    # A floor near 1e-4 (the reduced-precision J/K noise) does.
+    assert scf.mixed_phase_stalled([1e-1, 1e-2, 1e-3, 1.2e-4, 1.1e-4, 1.3e-4], 2, 0.7)
+    # Too few iterations to judge.
+    assert not scf.mixed_phase_stalled([1e-1, 2e-1, 3e-1], 2, 0.7)
+    # One bad iteration followed by progress is not a stall.
+    assert not scf.mixed_phase_stalled([1e-1, 1e-2, 2e-2, 5e-3], 2, 0.7)
+    with pytest.raises(InputError):
+        scf.ScfOptions(stall_ratio=0.0)
