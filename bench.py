@@ -254,6 +254,95 @@ def bench_clusters(args, dev):
     return out
 
 
+def bench_scf_batch(args, dev):
+    """Time to converge a set of water clusters to 1e-8 Eh: one molecule at a time (plain FP64 from
+    the atomic-density guess, the baseline) against the batched driver (FP64; the mixed policy; the
+    mixed policy from the learned guess when a trained model is stored).
+
+    The set cycles through ``--batch-scf-sizes`` with seeds 300000 + i (not used for training).
+    Integrals and the fit are excluded from every timing (they are the same work for all runs); the
+    atomic densities are computed once per element beforehand and shared. Timings include the
+    starts: the atomic start's Fock build, and for the learned runs the descriptors, the network,
+    J of the predicted densities, one diagonalisation and the fallback test.
+    """
+    import os
+
+    import numpy as np
+    import torch
+
+    from hafnium_amd import learned_guess as lg
+    from hafnium_amd.batch_scf import BatchedRhf, atomic_start, learned_start, molecule_inputs
+    from hafnium_amd.clusters import atomic_density_guess, build_system, run_cluster
+    from hafnium_amd.molecule import water_cluster
+    from hafnium_amd.scf import ScfOptions
+
+    sizes = [int(x) for x in args.batch_scf_sizes.split(",") if x]
+    specs = [(sizes[i % len(sizes)], 300000 + i) for i in range(args.batch_scf_count)]
+    t0 = time.perf_counter()
+    inputs = [molecule_inputs(water_cluster(n, seed)) for n, seed in specs]
+    t_integrals = time.perf_counter() - t0
+    cache = {}
+    for x in inputs:
+        atomic_density_guess(x.molecule, device=dev, cache=cache)
+    serial, t_serial = [], 0.0
+    for x, (n, seed) in zip(inputs, specs):
+        system = build_system(x.molecule, device=dev, precomputed=(x.overlap, x.core, x.metric, x.three_center))
+        guess = atomic_density_guess(x.molecule, cache=cache)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        result, _ = run_cluster(n, seed, system=system, guess=guess)
+        torch.cuda.synchronize()
+        t_serial += time.perf_counter() - t
+        serial.append(result)
+        del system
+    torch.cuda.empty_cache()
+    log(f"serial fp64: {t_serial:.3f} s for {len(specs)} clusters")
+    t0 = time.perf_counter()
+    batch = BatchedRhf(inputs, device=dev)
+    t_fit = time.perf_counter() - t0
+    mixed = args.cluster_mixed
+    model = lg.LearnedGuessModel.load(device=dev) if os.path.exists(lg.MODEL_PATH) and not args.no_learned else None
+    runs = [("batched_fp64", "fp64", False), ("batched_mixed", mixed, False)]
+    if model is not None:
+        runs += [("batched_learned_fp64", "fp64", True), ("batched_learned_mixed", mixed, True)]
+
+    def solve(precision, learned):
+        opts = ScfOptions(precision=precision, energy_tolerance=1e-8, gradient_tolerance=1e-6)
+        start = atomic_start(batch, cache, precision)
+        info = None
+        if learned:
+            start = learned_start(batch, model, start, precision)
+            info = start.info
+        return batch.run(start, opts), info
+
+    for _, precision, learned in runs:  # warm-up: plans, tensor copies, network buffers
+        solve(precision, learned)
+    ref = np.array([r.energy for r in serial])
+    out = {"clusters": [{"n_waters": n, "seed": seed} for n, seed in specs], "count": len(specs),
+           "integrals_s": t_integrals, "batched_fit_s": t_fit,
+           "serial_fp64": {"seconds": t_serial, "mean_iterations": float(np.mean([r.n_iterations for r in serial]))}}
+    for label, precision, learned in runs:
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        results, info = solve(precision, learned)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        energies = np.array([r.energy for r in results])
+        entry = {"seconds": dt, "speedup_vs_serial_fp64": t_serial / dt,
+                 "mean_iterations": float(np.mean([r.iterations for r in results])),
+                 "max_energy_diff_vs_serial_fp64": float(np.max(np.abs(energies - ref))),
+                 "all_converged": all(r.converged for r in results)}
+        if info is not None:
+            entry["learned_starts"] = info["learned"]
+            entry["fallbacks"] = info["fallback"]
+            entry["guess_seconds"] = info["seconds"]
+        out[label] = entry
+        log(f"{label}: {entry}")
+    del batch
+    torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +361,9 @@ def main(argv=None):
     ap.add_argument("--cluster-sizes", default="2,4,8,12")
     ap.add_argument("--cluster-mixed", default="bf16x3")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess cluster runs")
+    ap.add_argument("--no-batch-scf", action="store_true", help="skip the batched-SCF time-to-solution runs")
+    ap.add_argument("--batch-scf-count", type=int, default=32)
+    ap.add_argument("--batch-scf-sizes", default="2,4,6,8,10,12")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB read+write elementwise kernel (counter calibration)")
     args = ap.parse_args(argv)
@@ -404,6 +496,11 @@ def main(argv=None):
         torch.cuda.empty_cache()
         clusters = bench_clusters(args, dev)
 
+    scf_batched = None
+    if not args.no_batch_scf and args.workload == "large":
+        torch.cuda.empty_cache()
+        scf_batched = bench_scf_batch(args, dev)
+
     line = {
         "metric": "DF-J/K builds/s + SCF time-to-1e-8 Eh, mixed/learned vs plain FP64, 1 GPU",
         "value": main_entry["builds_per_s"],
@@ -425,6 +522,7 @@ def main(argv=None):
         "modes": results,
         "batched": batched,
         "scf_clusters": clusters,
+        "scf_batched": scf_batched,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -63,6 +63,7 @@ SIGNATURES = [
     ("hfjk_batch_sizes", _c_int, [_vp, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll)]),
     ("hfjk_batch_bind", _c_int, [_vp, _vp, _c_size, _vp]),
     ("hfjk_batch_run", _c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("hfjk_batch_coulomb_charges", _c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("hfjk_batch_destroy", _c_int, [_vp]),
     ("hfjk_dgemm", _c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_double, _vp, _c_int, _vp,
                             _c_int, ctypes.c_double, _vp, _c_int, _vp]),
@@ -78,6 +79,17 @@ SIGNATURES = [
     ("hfjk_get_tuning", _c_int, [_vp, _c_int, ctypes.POINTER(_c_int)]),
     ("hfjk_profile", _c_int, [_vp, _c_int]),
     ("hfjk_profile_read", _c_int, [_vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), _c_int]),
+    ("hfjk_dgemm_batched", _c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_double, _vp, _c_int, _c_ll,
+                                    _vp, _c_int, _c_ll, ctypes.c_double, _vp, _c_int, _c_ll, _c_int, _vp]),
+    ("hfjk_dsyev_batched_work", _c_ll, [_c_int, _c_int]),
+    ("hfjk_dsyev_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp]),
+    ("hfjk_ddot_batched", _c_int, [_vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_ll, _vp, _vp]),
+    ("hfjk_antisymmetrize_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp]),
+    ("hfjk_orthogonalizer_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
+    ("hfjk_shift_discarded_batched", _c_int, [_vp, _c_int, _c_int, _vp, ctypes.c_double, _vp, _vp]),
+    ("hfjk_mask_columns_batched", _c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    ("hfjk_select_batched", _c_int, [_vp, _c_int, _c_ll, _vp, _vp, _vp, _vp, _vp]),
+    ("hfjk_copy_batched", _c_int, [_vp, _c_int, _vp, _vp, _vp, _c_ll, _vp, _c_int, _c_ll, _vp, _c_int, _vp]),
     ("hfjk_mlp_gemm", _c_int, [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int,
                                _c_int, _vp, _vp, _c_int, _c_int, _vp, _vp]),
     ("hfjk_mlp_silu_backward", _c_int, [_vp, _c_ll, _vp, _vp, _vp, _vp]),

@@ -73,6 +73,25 @@ class BatchedCoulombExchange:
         self._b64 = torch.empty(b_off, dtype=torch.float64, device=self.device)
         self._copies = {}
         self._plans = {}
+        self._layouts = {None: (self._inst, self.c_offsets, self.orbital_elems)}
+
+    def layout(self, noccs=None):
+        """``(instances, c_offsets, orbital_elems)`` for other occupied counts per instance
+        (``None``: the counts the batch was made with). The tensor is shared."""
+        key = None if noccs is None else tuple(int(x) for x in noccs)
+        if key not in self._layouts:
+            if len(key) != len(self.shapes):
+                raise InputError("one occupied count per instance is required")
+            inst = (_Instance * len(self.shapes))()
+            offs, c_off = [], 0
+            for i, (sh, no) in enumerate(zip(self.shapes, key)):
+                if not 1 <= no <= sh.nao:
+                    raise InputError(f"instance {i}: 1 <= nocc <= nao is required")
+                inst[i] = _Instance(sh.nao, sh.naux, no, self.b_offsets[i], c_off, self.d_offsets[i])
+                offs.append(c_off)
+                c_off += sh.nao * no
+            self._layouts[key] = (inst, offs, c_off)
+        return self._layouts[key]
 
     # ------------------------------------------------------------- inputs
 
@@ -89,16 +108,21 @@ class BatchedCoulombExchange:
 
     def set_tensors(self, tensors) -> None:
         """Pack full symmetric ``[naux, nao, nao]`` arrays into the batch tensor."""
+        for i, b in enumerate(tensors):
+            self.set_tensor(i, b)
+
+    def set_tensor(self, i: int, b) -> None:
+        """Pack instance ``i``'s full symmetric ``[naux, nao, nao]`` tensor."""
         torch = _torch()
         lib = self._ctx.lib
-        for sh, off, b in zip(self.shapes, self.b_offsets, tensors):
-            full = torch.as_tensor(np.asarray(b) if isinstance(b, np.ndarray) else b,
-                                   dtype=torch.float64).to(self.device).contiguous()
-            if tuple(full.shape) != (sh.naux, sh.nao, sh.nao):
-                raise InputError("a tensor does not match its instance shape")
-            ptr = ctypes.c_void_p(self._b64.data_ptr() + 8 * off)
-            _lib.check(lib.hfjk_pack(self._ctx.handle, ctypes.c_void_p(full.data_ptr()), sh.naux, sh.nao, ptr, None,
-                                     self._stream()))
+        sh, off = self.shapes[i], self.b_offsets[i]
+        full = torch.as_tensor(np.asarray(b) if isinstance(b, np.ndarray) else b,
+                               dtype=torch.float64).to(self.device).contiguous()
+        if tuple(full.shape) != (sh.naux, sh.nao, sh.nao):
+            raise InputError("a tensor does not match its instance shape")
+        ptr = ctypes.c_void_p(self._b64.data_ptr() + 8 * off)
+        _lib.check(lib.hfjk_pack(self._ctx.handle, ctypes.c_void_p(full.data_ptr()), sh.naux, sh.nao, ptr, None,
+                                 self._stream()))
         self._copies.clear()
 
     def pack_orbitals(self, orbitals):
@@ -144,24 +168,27 @@ class BatchedCoulombExchange:
         return self._copies[precision]
 
     def release(self, precision: str) -> None:
-        """Free the workspace and derived tensor copy of ``precision``."""
+        """Free the workspaces (every orbital layout) and derived tensor copy of ``precision``."""
         code = _lib.precision_code(precision)
-        plan = self._plans.pop(code, None)
-        if plan is not None:
+        for key in [k for k in self._plans if k[0] == code]:
+            plan = self._plans.pop(key)
             self._ctx.lib.hfjk_batch_destroy(plan[0])
         self._copies.pop(code, None)
 
-    def plan(self, precision: str = "fp64", tile: int = 0):
-        """Create (or reuse) the plan and workspace of ``precision`` with column tile ``tile``."""
+    def plan(self, precision: str = "fp64", tile: int = 0, noccs=None):
+        """Create (or reuse) the plan and workspace of ``precision`` with column tile ``tile``
+        for the orbital layout of ``noccs`` (see :meth:`layout`)."""
         torch = _torch()
         code = _lib.precision_code(precision)
-        key = code
+        inst, _, _ = self.layout(noccs)
+        key = (code, None if noccs is None else tuple(int(x) for x in noccs))
         if key in self._plans and self._plans[key][2] == tile:
             return self._plans[key]
-        self.release(precision) if key in self._plans else None
+        if key in self._plans:
+            self._ctx.lib.hfjk_batch_destroy(self._plans.pop(key)[0])
         lib = self._ctx.lib
         handle = ctypes.c_void_p()
-        _lib.check(lib.hfjk_batch_create(self._ctx.handle, self._inst, len(self.shapes), code, int(tile),
+        _lib.check(lib.hfjk_batch_create(self._ctx.handle, inst, len(self.shapes), code, int(tile),
                                          ctypes.byref(handle)))
         size = ctypes.c_size_t()
         _lib.check(lib.hfjk_batch_workspace_bytes(handle, ctypes.byref(size)))
@@ -174,11 +201,12 @@ class BatchedCoulombExchange:
 
     # ------------------------------------------------------------- run
 
-    def run(self, orbitals, densities, *, precision: str = "fp64", tile: int = 0, out=None):
-        """One batched J/K build; returns flat ``(J, K)`` buffers (see :meth:`matrix`)."""
+    def run(self, orbitals, densities, *, precision: str = "fp64", tile: int = 0, out=None, noccs=None):
+        """One batched J/K build; returns flat ``(J, K)`` buffers (see :meth:`matrix`).
+        ``orbitals`` follow the layout of ``noccs`` (default: the batch's own counts)."""
         torch = _torch()
         code = _lib.precision_code(precision)
-        handle, _, _ = self.plan(precision, tile)
+        handle, _, _ = self.plan(precision, tile, noccs)
         tensor = self._tensor(code)
         if out is None:
             out = (torch.empty(self.matrix_elems, dtype=torch.float64, device=self.device),
@@ -189,6 +217,22 @@ class BatchedCoulombExchange:
             ctypes.c_void_p(densities.data_ptr()), ctypes.c_void_p(j.data_ptr()), ctypes.c_void_p(k.data_ptr()),
             self._stream()))
         return j, k
+
+    def coulomb_from_charges(self, charges, *, precision: str = "fp64", out=None):
+        """``J_i = sum_P charges_i[P] B_i[P]`` (charges flat, instance after instance); flat J buffer."""
+        torch = _torch()
+        total = sum(sh.naux for sh in self.shapes)
+        if charges.dtype != torch.float64 or charges.numel() != total or charges.device != self.device:
+            raise InputError(f"charges are a float64 device vector of {total} values")
+        code = _lib.precision_code(precision)
+        handle, _, _ = self.plan(precision)
+        tensor = self._tensor(code)
+        if out is None:
+            out = torch.empty(self.matrix_elems, dtype=torch.float64, device=self.device)
+        _lib.check(self._ctx.lib.hfjk_batch_coulomb_charges(
+            handle, ctypes.c_void_p(tensor.data_ptr()), ctypes.c_void_p(charges.contiguous().data_ptr()),
+            ctypes.c_void_p(out.data_ptr()), self._stream()))
+        return out
 
     def capture(self, orbitals, densities, *, precision: str = "fp64", tile: int = 0):
         """A hipGraph (``torch.cuda.CUDAGraph``) of one batched build and its output buffers.

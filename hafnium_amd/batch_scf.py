@@ -1,0 +1,451 @@
+"""Batched restricted Hartree-Fock: many molecules converged together on one GPU.
+
+Every iteration advances all molecules at once: one batched DF-J/K build
+(``BatchedCoulombExchange``: one persistent launch per phase for every
+molecule), strided-batched GEMMs and a batched eigensolver on matrices
+zero-padded to the largest basis, and one host transfer of every
+molecule's energy terms and DIIS overlaps. Per molecule, the iteration is
+the one of ``solve_scf`` (hafnium/scf/driver.py:154-213 of the reference):
+F = h + J[2D] - K[D], the orbital gradient X^T (F D S - S D F) X, DIIS over
+the last ``diis_size`` Fock matrices (hafnium/scf/diis.py:27-106), the
+canonical orthogonaliser (orthogonalization.py:16-42) and aufbau filling.
+Padded and discarded directions of the orthogonal basis get a large
+diagonal shift, so they sort above every occupied level and never mix in.
+
+Precision policy: all molecules share the arithmetic of the J/K build; it
+switches to FP64 (and resets DIIS) once every unconverged molecule has its
+gradient below ``switch_threshold`` or has stalled (``mixed_phase_stalled``).
+Convergence of a molecule (|dE| < energy_tolerance and gradient below
+gradient_tolerance) is only declared on FP64 iterations; its energy and
+iteration count are recorded at that iteration while the batch runs on.
+
+Starts: from a density with a factor D = C C^T (the superposition of
+atomic densities: blocks of atomic natural orbitals), or from a guess Fock
+matrix (the learned guess), each evaluated with one batched build; the
+learned start falls back to the atomic one per molecule when its energy is
+higher.
+
+No counterpart in the reference (its driver solves one molecule at a time).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from hafnium_amd import _lib
+from hafnium_amd.errors import ConvergenceError, InputError
+from hafnium_amd.molecule import BasisOnAtoms, Molecule
+from hafnium_amd.scf import ScfOptions, mixed_phase_stalled, solve_diis
+
+DISCARDED_SHIFT = 1.0e3  # Eh added to padded / discarded directions of the orthogonal basis
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _p(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+@dataclass
+class MoleculeInputs:
+    """Host integrals of one molecule (closed shell)."""
+
+    molecule: Molecule
+    overlap: np.ndarray
+    core: np.ndarray
+    metric: np.ndarray
+    three_center: np.ndarray
+    n_alpha: int
+    nuclear_repulsion: float
+    aux_basis: str
+
+    @property
+    def nao(self) -> int:
+        return self.overlap.shape[0]
+
+    @property
+    def naux(self) -> int:
+        return self.metric.shape[0]
+
+
+def molecule_inputs(molecule: Molecule, *, basis: str = "cc-pvdz", aux: str = "def2-universal-jkfit"):
+    from hafnium_amd import integrals
+
+    if molecule.n_electrons % 2:
+        raise InputError("the batched driver is restricted closed-shell")
+    orb = BasisOnAtoms(basis, molecule)
+    auxb = BasisOnAtoms(aux, molecule)
+    s, t, v = integrals.one_electron(orb, molecule)
+    return MoleculeInputs(molecule, s, t + v, integrals.metric(auxb), integrals.three_center(auxb, orb),
+                          molecule.n_electrons // 2, molecule.nuclear_repulsion(), aux)
+
+
+@dataclass
+class BatchResult:
+    energy: float
+    converged: bool
+    iterations: int
+    seconds: float
+    history: list = field(default_factory=list)  # (energy, gradient norm, precision) per iteration
+
+
+@dataclass
+class _Start:
+    density: object      # padded [N, nm, nm]
+    fock: object         # padded [N, nm, nm]
+    terms: np.ndarray    # [3, N]: tr(D h), tr(D J[D]), tr(D K[D])
+    info: dict = field(default_factory=dict)
+
+
+class BatchedRhf:
+    """Host integrals of N molecules -> fitted tensors in one batch, padded FP64 state on the GPU."""
+
+    def __init__(self, inputs, *, device=None, eigensolver: str = "jacobi") -> None:
+        from hafnium_amd.batch import BatchedCoulombExchange
+        from hafnium_amd.coulomb_exchange import _Context
+        from hafnium_amd.linalg import Dense
+
+        torch = _torch()
+        self.inputs = list(inputs)
+        if not self.inputs:
+            raise InputError("a batch holds at least one molecule")
+        if eigensolver not in ("jacobi", "divide"):
+            raise InputError("eigensolver is 'jacobi' or 'divide'")
+        self.method = 1 if eigensolver == "jacobi" else 0
+        self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        self.count = len(self.inputs)
+        self.nm = max(x.nao for x in self.inputs)
+        self.om = max(x.n_alpha for x in self.inputs)
+        self.jk = BatchedCoulombExchange([(x.nao, x.naux, x.n_alpha) for x in self.inputs], device=self.device)
+        self._ctx = self.jk._ctx
+        self.lib = self._ctx.lib
+        dense = Dense(self._ctx, self.device)
+        t0 = time.perf_counter()
+        # Fit on the GPU, one molecule at a time: M = L L^T, B = L^-1 (P|uv); L is kept
+        # (the learned guess needs L^-1 (P|Q) d).
+        self.factors = []
+        info = torch.zeros(len(self.inputs), dtype=torch.int32, device=self.device)
+        for i, x in enumerate(self.inputs):
+            m = torch.from_numpy(np.ascontiguousarray(x.metric)).to(self.device)
+            _lib.check(self.lib.hfjk_dpotrf(self._ctx.handle, x.naux, _p(m), _p(info[i:i + 1]), self._stream()))
+            b = torch.from_numpy(np.ascontiguousarray(x.three_center).reshape(x.naux, -1)).to(self.device)
+            _lib.check(self.lib.hfjk_dtrsm_lower(self._ctx.handle, x.naux, x.nao * x.nao, _p(m), _p(b),
+                                                 self._stream()))
+            self.jk.set_tensor(i, b.view(x.naux, x.nao, x.nao))
+            del b
+            self.factors.append(m)
+        torch.cuda.synchronize(self.device)
+        bad = np.nonzero(info.cpu().numpy())[0]
+        if len(bad):
+            raise InputError(f"the metric of molecule(s) {bad.tolist()} is not positive definite")
+        self.fit_seconds = time.perf_counter() - t0
+        n, nm, cnt = self.nm, self.nm, self.count
+        self._nn = nm * nm
+        zeros = lambda *shape: torch.zeros(shape, dtype=torch.float64, device=self.device)  # noqa: E731
+        s_pad, h_pad = zeros(cnt, nm, nm), zeros(cnt, nm, nm)
+        for i, x in enumerate(self.inputs):
+            s_pad[i, :x.nao, :x.nao] = torch.from_numpy(x.overlap)
+            h_pad[i, :x.nao, :x.nao] = torch.from_numpy(x.core)
+        self.s, self.h = s_pad, h_pad
+        as_int = lambda v: torch.tensor(v, dtype=torch.int32, device=self.device)  # noqa: E731
+        as_ll = lambda v: torch.tensor(v, dtype=torch.int64, device=self.device)  # noqa: E731
+        self._nao = as_int([x.nao for x in self.inputs])
+        self._nocc = as_int([x.n_alpha for x in self.inputs])
+        self._moffs = as_ll(self.jk.d_offsets)
+        self._coffs = as_ll(self.jk.c_offsets)
+        self._max_mat = max(x.nao * x.nao for x in self.inputs)
+        self._max_orb = max(x.nao * x.n_alpha for x in self.inputs)
+        self.e_nuc = np.array([x.nuclear_repulsion for x in self.inputs])
+        # Orthogonaliser from the batched eigendecomposition of the padded overlaps.
+        vecs = s_pad.clone()
+        vals = zeros(cnt, nm)
+        self._eig(vecs, vals)
+        self.x = zeros(cnt, nm, nm)
+        self.kept = torch.zeros((cnt, nm), dtype=torch.int32, device=self.device)
+        _lib.check(self.lib.hfjk_orthogonalizer_batched(self._ctx.handle, nm, cnt, _p(vecs), _p(vals),
+                                                        ScfOptions().overlap_threshold, _p(self.x), _p(self.kept),
+                                                        self._stream()))
+        kept = self.kept.sum(dim=1).cpu().numpy()
+        for i, x in enumerate(self.inputs):
+            if kept[i] < x.n_alpha:
+                raise InputError(f"molecule {i}: {x.n_alpha} occupied orbitals do not fit {kept[i]} functions")
+        # Work buffers.
+        self._dc = torch.empty(self.jk.matrix_elems, dtype=torch.float64, device=self.device)
+        self._jc = torch.empty_like(self._dc)
+        self._kc = torch.empty_like(self._dc)
+        self._cc = torch.empty(self.jk.orbital_elems, dtype=torch.float64, device=self.device)
+        self._jpad, self._kpad = zeros(cnt, nm, nm), zeros(cnt, nm, nm)
+        self._t1, self._t2 = zeros(cnt, nm, nm), zeros(cnt, nm, nm)
+
+    # ------------------------------------------------------------ primitives
+    def _stream(self):
+        return ctypes.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)
+
+    def _eig(self, a, w):
+        torch = _torch()
+        cnt, nm = self.count, self.nm
+        work = torch.empty(int(self.lib.hfjk_dsyev_batched_work(nm, cnt)), dtype=torch.float64, device=self.device)
+        info = torch.zeros(cnt, dtype=torch.int32, device=self.device)
+        _lib.check(self.lib.hfjk_dsyev_batched(self._ctx.handle, nm, cnt, _p(a), _p(w), _p(work), _p(info),
+                                               self.method, self._stream()))
+        self._eig_info = info
+
+    def _gemm(self, ta, tb, m, n, k, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha=1.0, beta=0.0):
+        _lib.check(self.lib.hfjk_dgemm_batched(self._ctx.handle, int(ta), int(tb), m, n, k, alpha, _p(a), lda, sa,
+                                               _p(b), ldb, sb, beta, _p(c), ldc, sc, self.count, self._stream()))
+
+    def _mm(self, a, b, c, ta=False, tb=False):
+        nm, nn = self.nm, self._nn
+        self._gemm(ta, tb, nm, nm, nm, a, nm, nn, b, nm, nn, c, nm, nn)
+
+    def _copy(self, rows, cols, offs, max_elems, padded, ld, stride, compact, to_compact):
+        _lib.check(self.lib.hfjk_copy_batched(self._ctx.handle, self.count, _p(rows), _p(cols), _p(offs), max_elems,
+                                              _p(padded), ld, stride, _p(compact), int(to_compact), self._stream()))
+
+    def _dots(self, x, y, out):
+        _lib.check(self.lib.hfjk_ddot_batched(self._ctx.handle, self.count, self._nn, _p(x), self._nn, _p(y),
+                                              self._nn, _p(out), self._stream()))
+
+    def _combine(self, a, x, b, y, c, z, out):
+        _lib.check(self.lib.hfjk_combine(self._ctx.handle, out.numel(), a, _p(x), b, _p(y) if y is not None else None,
+                                         c, _p(z) if z is not None else None, _p(out), self._stream()))
+
+    # ------------------------------------------------------------ SCF steps
+    def fock(self, density, orbitals_compact, precision, out, terms, noccs=None):
+        """F = h + 2 J[D] - K[D] into ``out``; terms[0:3] (device rows) = tr(D h), tr(D J), tr(D K)."""
+        self._copy(self._nao, self._nao, self._moffs, self._max_mat, density, self.nm, self._nn, self._dc, True)
+        self.jk.run(orbitals_compact, self._dc, precision=precision, out=(self._jc, self._kc), noccs=noccs)
+        self._copy(self._nao, self._nao, self._moffs, self._max_mat, self._jpad, self.nm, self._nn, self._jc, False)
+        self._copy(self._nao, self._nao, self._moffs, self._max_mat, self._kpad, self.nm, self._nn, self._kc, False)
+        self._combine(1.0, self.h, 2.0, self._jpad, -1.0, self._kpad, out)
+        self._dots(density, self.h, terms[0])
+        self._dots(density, self._jpad, terms[1])
+        self._dots(density, self._kpad, terms[2])
+
+    def gradient(self, fock, density, out):
+        """X^T (F D S - S D F) X into ``out``."""
+        self._mm(fock, density, self._t1)
+        self._mm(self._t1, self.s, self._t2)
+        _lib.check(self.lib.hfjk_antisymmetrize_batched(self._ctx.handle, self.nm, self.count, _p(self._t2),
+                                                        _p(self._t1), self._stream()))
+        self._mm(self.x, self._t1, self._t2, ta=True)
+        self._mm(self._t2, self.x, out)
+
+    def solve(self, fock):
+        """Aufbau density and occupied orbitals of ``fock``: (D padded, C padded [N, nm, om], C compact)."""
+        torch = _torch()
+        nm, om, nn = self.nm, self.om, self._nn
+        self._mm(self.x, fock, self._t1, ta=True)
+        fp = torch.empty_like(self._t1)
+        self._mm(self._t1, self.x, fp)
+        _lib.check(self.lib.hfjk_shift_discarded_batched(self._ctx.handle, nm, self.count, _p(self.kept),
+                                                         DISCARDED_SHIFT, _p(fp), self._stream()))
+        w = torch.empty((self.count, nm), dtype=torch.float64, device=self.device)
+        self._eig(fp, w)
+        c_occ = torch.empty((self.count, nm, om), dtype=torch.float64, device=self.device)
+        # C_occ = X V_occ^T: rows 0..om-1 of each eigenvector matrix are the lowest levels.
+        self._gemm(False, True, nm, om, nm, self.x, nm, nn, fp, nm, nn, c_occ, om, nm * om)
+        _lib.check(self.lib.hfjk_mask_columns_batched(self._ctx.handle, nm, om, self.count, _p(self._nocc),
+                                                      _p(c_occ), self._stream()))
+        density = torch.empty((self.count, nm, nm), dtype=torch.float64, device=self.device)
+        self._gemm(False, True, nm, nm, om, c_occ, om, nm * om, c_occ, om, nm * om, density, nm, nn)
+        compact = torch.empty(self.jk.orbital_elems, dtype=torch.float64, device=self.device)
+        self._copy(self._nao, self._nocc, self._coffs, self._max_orb, c_occ, om, nm * om, compact, True)
+        return density, c_occ, compact
+
+    # ------------------------------------------------------------ starts
+    def start_from_factors(self, densities, factors, precision: str = "fp64") -> _Start:
+        """Start from host densities with factors D_i = C_i C_i^T ([nao_i, r_i])."""
+        torch = _torch()
+        cnt, nm = self.count, self.nm
+        d_pad = torch.zeros((cnt, nm, nm), dtype=torch.float64, device=self.device)
+        ranks = []
+        for i, (d, c) in enumerate(zip(densities, factors)):
+            n = self.inputs[i].nao
+            if d.shape != (n, n) or c.shape[0] != n or c.shape[1] < 1:
+                raise InputError(f"molecule {i}: density or factor has the wrong shape")
+            d_pad[i, :n, :n] = torch.from_numpy(np.ascontiguousarray(d))
+            ranks.append(int(c.shape[1]))
+        _, offs, total = self.jk.layout(ranks)
+        c_flat = np.empty(total)
+        for i, c in enumerate(factors):
+            c_flat[offs[i]:offs[i] + c.size] = np.ascontiguousarray(c).reshape(-1)
+        c_dev = torch.from_numpy(c_flat).to(self.device)
+        fock = torch.empty_like(d_pad)
+        terms = torch.empty((3, cnt), dtype=torch.float64, device=self.device)
+        self.fock(d_pad, c_dev, precision, fock, terms, noccs=ranks)
+        return _Start(d_pad, fock, terms.cpu().numpy(), {"coulomb": self._jpad.clone()})
+
+    def start_from_fock(self, guess_fock, precision: str = "fp64") -> _Start:
+        """Start from the aufbau density of a guess Fock matrix (padded [N, nm, nm], device)."""
+        torch = _torch()
+        density, _, compact = self.solve(guess_fock)
+        fock = torch.empty_like(density)
+        terms = torch.empty((3, self.count), dtype=torch.float64, device=self.device)
+        self.fock(density, compact, precision, fock, terms)
+        return _Start(density, fock, terms.cpu().numpy())
+
+    def energies(self, terms: np.ndarray) -> np.ndarray:
+        return self.e_nuc + 2.0 * terms[0] + 2.0 * terms[1] - terms[2]
+
+    def select(self, flags, first: _Start, second: _Start) -> _Start:
+        """Per molecule, ``first`` where flags[i] else ``second``."""
+        torch = _torch()
+        f = torch.tensor(np.asarray(flags, dtype=np.int32), device=self.device)
+        out_d = torch.empty_like(first.density)
+        out_f = torch.empty_like(first.fock)
+        for a, b, o in ((first.density, second.density, out_d), (first.fock, second.fock, out_f)):
+            _lib.check(self.lib.hfjk_select_batched(self._ctx.handle, self.count, self._nn, _p(f), _p(a), _p(b),
+                                                    _p(o), self._stream()))
+        terms = np.where(np.asarray(flags, dtype=bool)[None, :], first.terms, second.terms)
+        return _Start(out_d, out_f, terms)
+
+    # ------------------------------------------------------------ the loop
+    def run(self, start: _Start, options: ScfOptions | None = None, *, raise_on_failure: bool = True):
+        """Converge every molecule from ``start``; returns one ``BatchResult`` per molecule."""
+        torch = _torch()
+        options = ScfOptions() if options is None else options
+        if options.level_shift or options.damping:
+            raise InputError("the batched driver has no level shift or damping")
+        t0 = time.perf_counter()
+        cnt, nm, nn, m = self.count, self.nm, self._nn, options.diis_size
+        fh = torch.zeros((m, cnt, nm, nm), dtype=torch.float64, device=self.device)
+        gh = torch.zeros_like(fh)
+        scal = torch.empty(3 * cnt + cnt * m * m, dtype=torch.float64, device=self.device)
+        terms_dev = scal[:3 * cnt].view(3, cnt)
+        ov_dev = scal[3 * cnt:].view(cnt, m, m)
+        precision = options.precision
+        density = start.density
+        orbitals = None
+        results: list = [None] * cnt
+        history = [[] for _ in range(cnt)]
+        previous = np.full(cnt, math.inf)
+        valid: list[int] = []
+        slot = 0
+        for index in range(options.max_iterations):
+            fock = fh[slot]
+            if index == 0:
+                fock.copy_(start.fock)
+            else:
+                self.fock(density, orbitals, precision, fock, terms_dev)
+            self.gradient(fock, density, gh[slot])
+            # Every molecule's DIIS overlaps <G_a, G_b> over the stored slots, one batched GEMM.
+            self._gemm(False, True, m, m, nn, gh, cnt * nn, nn, gh, cnt * nn, nn, ov_dev, m, m * m)
+            host = scal.cpu().numpy()
+            terms = start.terms if index == 0 else host[:3 * cnt].reshape(3, cnt)
+            ov = 2.0 * host[3 * cnt:].reshape(cnt, m, m)
+            energy = self.energies(terms)
+            gnorm = np.sqrt(np.maximum(ov[:, slot, slot], 0.0))
+            change = energy - previous
+            previous = energy
+            now = time.perf_counter() - t0
+            for i in range(cnt):
+                history[i].append((float(energy[i]), float(gnorm[i]), precision))
+                if results[i] is None and precision == "fp64" and abs(change[i]) < options.energy_tolerance \
+                        and gnorm[i] < options.gradient_tolerance:
+                    results[i] = BatchResult(float(energy[i]), True, index + 1, now, history[i])
+            valid = (valid + [slot])[-m:]
+            if all(r is not None for r in results):
+                break
+            if precision != "fp64":
+                pending = [i for i in range(cnt) if results[i] is None]
+                mixed = [[g for (_, g, p) in history[i] if p != "fp64"] for i in pending]
+                if all(g[-1] < options.switch_threshold
+                       or mixed_phase_stalled(g, options.stall_window, options.stall_ratio) for g in mixed) \
+                        or len(mixed[0]) >= options.max_iterations // 2:
+                    precision = "fp64"
+                    if options.reset_diis_on_switch:
+                        valid = [slot]
+            # DIIS coefficients per molecule over the valid slots, then one batched combination.
+            coef = np.zeros((cnt, 1, m))
+            for i in range(cnt):
+                if len(valid) == 1:
+                    coef[i, 0, valid[0]] = 1.0
+                else:
+                    sub = ov[i][np.ix_(valid, valid)]
+                    coef[i, 0, valid] = solve_diis(sub)
+            coef_dev = torch.from_numpy(coef).to(self.device)
+            extrapolated = torch.empty((cnt, nm, nm), dtype=torch.float64, device=self.device)
+            self._gemm(False, False, 1, nn, m, coef_dev, m, m, fh, cnt * nn, nn, extrapolated, nn, nn)
+            density, _, orbitals = self.solve(extrapolated)
+            slot = (slot + 1) % m
+        total = time.perf_counter() - t0
+        if int((self._eig_info != 0).sum().item()):
+            raise ConvergenceError("the batched eigensolver did not converge for every molecule", None)
+        out = []
+        for i in range(cnt):
+            if results[i] is None:
+                last = history[i][-1]
+                results[i] = BatchResult(last[0], False, len(history[i]), total, history[i])
+            out.append(results[i])
+        if raise_on_failure and not all(r.converged for r in out):
+            bad = [i for i, r in enumerate(out) if not r.converged]
+            raise ConvergenceError(f"molecules {bad} did not converge in {options.max_iterations} iterations", out)
+        return out
+
+
+# ---------------------------------------------------------------- guesses
+
+def atomic_factors(molecule: Molecule, atom_densities: dict, tolerance: float = 1e-12):
+    """Block-diagonal superposition of atomic densities and its factor (atomic natural orbitals
+    scaled by the square roots of their occupations)."""
+    blocks_d, blocks_c = [], []
+    for z in molecule.charges:
+        d = np.asarray(atom_densities[z])
+        w, v = np.linalg.eigh(d)
+        keep = w > tolerance * max(w.max(), 1e-300)
+        blocks_d.append(d)
+        blocks_c.append(v[:, keep] * np.sqrt(w[keep]))
+    n = sum(b.shape[0] for b in blocks_d)
+    r = sum(c.shape[1] for c in blocks_c)
+    dens, fac = np.zeros((n, n)), np.zeros((n, r))
+    at = col = 0
+    for d, c in zip(blocks_d, blocks_c):
+        dens[at:at + d.shape[0], at:at + d.shape[0]] = d
+        fac[at:at + c.shape[0], col:col + c.shape[1]] = c
+        at += d.shape[0]
+        col += c.shape[1]
+    return dens, fac
+
+
+def atomic_start(batch: BatchedRhf, atom_densities: dict, precision: str = "fp64") -> _Start:
+    pairs = [atomic_factors(x.molecule, atom_densities) for x in batch.inputs]
+    return batch.start_from_factors([p[0] for p in pairs], [p[1] for p in pairs], precision)
+
+
+def learned_start(batch: BatchedRhf, model, sad: _Start, precision: str = "fp64") -> _Start:
+    """F0 = h + J[d] - K[D_atoms] from predicted fitted coefficients d; the start is the aufbau density
+    of F0 where its energy is not above the atomic start's, the atomic start elsewhere."""
+    torch = _torch()
+    t0 = time.perf_counter()
+    coeffs = model.predict_many([x.molecule for x in batch.inputs])
+    # Charges against B: L^-1 (P|Q) d, with the metric's Cholesky factor kept from the fit.
+    charges = []
+    for x, f, d in zip(batch.inputs, batch.factors, coeffs):
+        v = torch.from_numpy(x.metric @ d).to(batch.device).view(-1, 1).contiguous()
+        _lib.check(batch.lib.hfjk_dtrsm_lower(batch._ctx.handle, x.naux, 1, _p(f), _p(v), batch._stream()))
+        charges.append(v.view(-1))
+    j_c = batch.jk.coulomb_from_charges(torch.cat(charges), precision=precision)
+    j_pad = torch.zeros_like(batch.h)
+    batch._copy(batch._nao, batch._nao, batch._moffs, batch._max_mat, j_pad, batch.nm, batch._nn, j_c, False)
+    # F_sad = h + 2 J_sad - K_sad, so F0 = h + J[d] - K_sad = F_sad - 2 J_sad + J[d]: no second exchange build.
+    guess = torch.empty_like(batch.h)
+    batch._combine(1.0, sad.fock, -2.0, sad.info["coulomb"], 1.0, j_pad, guess)
+    learned = batch.start_from_fock(guess, precision)
+    e_learned = batch.energies(learned.terms)
+    e_sad = batch.energies(sad.terms)
+    flags = e_learned <= e_sad
+    chosen = batch.select(flags, learned, sad)
+    chosen.info = {"learned": int(flags.sum()), "fallback": int((~flags).sum()),
+                   "energy_learned": e_learned.tolist(), "energy_sad": e_sad.tolist(),
+                   "seconds": time.perf_counter() - t0}
+    return chosen

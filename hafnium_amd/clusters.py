@@ -46,19 +46,25 @@ class ClusterSystem:
 
 
 def build_system(molecule: Molecule, *, basis: str = ORBITAL_BASIS, aux: str = AUX_BASIS,
-                 precision: str = "fp64", device=None, keep_integrals: bool = False) -> ClusterSystem:
+                 precision: str = "fp64", device=None, keep_integrals: bool = False,
+                 precomputed=None) -> ClusterSystem:
     """Integrals (host), fitted tensor (GPU Cholesky) and the Hamiltonian of a closed shell.
 
     ``keep_integrals`` keeps the host metric and three-centre integrals on the
-    result (the learned guess needs them)."""
+    result (the learned guess needs them). ``precomputed`` = (S, T + V, metric,
+    three-centre) skips the integral step."""
     from hafnium_amd.density_fitting import fitted_coulomb_exchange
 
     t0 = time.perf_counter()
-    orb = BasisOnAtoms(basis, molecule)
-    auxb = BasisOnAtoms(aux, molecule)
-    s, t, v = integrals.one_electron(orb, molecule)
-    m = integrals.metric(auxb)
-    tc = integrals.three_center(auxb, orb)
+    if precomputed is None:
+        orb = BasisOnAtoms(basis, molecule)
+        auxb = BasisOnAtoms(aux, molecule)
+        s, t, v = integrals.one_electron(orb, molecule)
+        core = t + v
+        m = integrals.metric(auxb)
+        tc = integrals.three_center(auxb, orb)
+    else:
+        s, core, m, tc = precomputed
     t1 = time.perf_counter()
     two_body = fitted_coulomb_exchange(m, tc, precision=precision, device=device)
     _torch().cuda.synchronize()
@@ -67,9 +73,9 @@ def build_system(molecule: Molecule, *, basis: str = ORBITAL_BASIS, aux: str = A
         n_alpha = molecule.n_electrons / 2.0
     else:
         n_alpha = molecule.n_electrons // 2
-    h = RestrictedHamiltonian(s, t + v, two_body, n_alpha, molecule.nuclear_repulsion())
+    h = RestrictedHamiltonian(s, core, two_body, n_alpha, molecule.nuclear_repulsion())
     return ClusterSystem(molecule, h, {"integrals_s": t1 - t0, "fit_s": t2 - t1,
-                                       "nao": orb.n_functions, "naux": auxb.n_functions},
+                                       "nao": s.shape[0], "naux": m.shape[0]},
                          m if keep_integrals else None, tc if keep_integrals else None, aux)
 
 

@@ -159,6 +159,15 @@ __global__ __launch_bounds__(kThreads) void b_gamma_reduce_kernel(BatchView v, d
   BATCH_END
 }
 
+// gamma[q] from a caller's flat charge vector (instance i at q_off), 256 per item.
+__global__ __launch_bounds__(kThreads) void b_charges_load_kernel(BatchView v, const double* __restrict__ charges,
+                                                                  double* __restrict__ ws64) {
+  BATCH_LOOP(kPhGammaReduce)
+  const int q = (int)local * kThreads + threadIdx.x;
+  if (q < in.naux) ws64[in.g_off + q] = charges[in.q_off + q];
+  BATCH_END
+}
+
 // Packed J of one tile: sum_q gamma[q] B[q][tile], one tile per item.
 template <int PREC>
 __global__ __launch_bounds__(kThreads) void b_coulomb_kernel(BatchView v, const void* __restrict__ b, long long plane,
@@ -255,6 +264,20 @@ hipError_t launch_batch_coulomb(BatchView v, int precision, const void* b, long 
   return hipGetLastError();
 }
 
+hipError_t launch_batch_coulomb_charges(BatchView v, int precision, const void* b, long long b_plane,
+                                        const double* charges, double* ws64, double* j, int grid, hipStream_t s) {
+  const dim3 g((unsigned)grid), blk(kThreads);
+  hipLaunchKernelGGL(b_charges_load_kernel, g, blk, 0, s, v, charges, ws64);
+  if (precision == HFJK_FP64)
+    hipLaunchKernelGGL(b_coulomb_kernel<HFJK_FP64>, g, blk, 0, s, v, b, b_plane, ws64);
+  else if (precision == HFJK_FP32)
+    hipLaunchKernelGGL(b_coulomb_kernel<HFJK_FP32>, g, blk, 0, s, v, b, b_plane, ws64);
+  else
+    hipLaunchKernelGGL(b_coulomb_kernel<HFJK_BF16X3>, g, blk, 0, s, v, b, b_plane, ws64);
+  hipLaunchKernelGGL(b_coulomb_unpack_kernel, g, blk, 0, s, v, ws64, j);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_orbitals(BatchView v, int precision, const double* c, double* ws64, void* cbase,
                                  long long c_plane, int grid, hipStream_t s) {
   const dim3 g((unsigned)grid), blk(kThreads);
@@ -309,7 +332,7 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
   const int rows = hfjk::batch_tile_rows(precision);
   const int es = precision == HFJK_FP64 ? 8 : precision == HFJK_FP32 ? 4 : 2;  // bf16: per plane
   const long long tile2 = (long long)hfjk::syrk_tile() * hfjk::syrk_tile();
-  long long w64 = 0, cp = 0, xx = 0, kp = 0;
+  long long w64 = 0, cp = 0, xx = 0, kp = 0, qq = 0;
   for (int i = 0; i < ninst; ++i) {
     const hfjk_batch_instance& src = instances[i];
     if (src.nao < 1 || src.naux < 1 || src.nocc < 1 || src.nocc > src.nao || src.nao > 46000 ||
@@ -335,6 +358,8 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
     in.b_off = src.b_offset;
     in.c_off = src.c_offset;
     in.d_off = src.d_offset;
+    in.q_off = qq;
+    qq += src.naux;
     bt->b_total = std::max(bt->b_total, src.b_offset + (long long)src.naux * in.p);
     bt->c_total = std::max(bt->c_total, src.c_offset + (long long)src.nao * src.nocc);
     bt->d_total = std::max(bt->d_total, src.d_offset + (long long)src.nao * src.nao);
@@ -487,6 +512,36 @@ int hfjk_batch_run(hfjk_batch* batch, const void* tensor, const double* orbitals
   if (e == hipSuccess)
     e = hfjk::launch_batch_syrk(v, prec, xbase, batch->x_total, ws64, kbase, batch->grid_mfma, s);
   if (e == hipSuccess) e = hfjk::launch_batch_syrk_reduce(v, prec, kbase, exchange, batch->grid_elem, s);
+  if (e != hipSuccess) {
+    hfjk_internal_set_error((std::string("batched launch: ") + hipGetErrorString(e)).c_str());
+    return HFJK_ERR_HIP;
+  }
+  return HFJK_OK;
+}
+
+int hfjk_batch_coulomb_charges(hfjk_batch* batch, const void* tensor, const double* charges, double* coulomb,
+                               void* stream) {
+  if (batch == nullptr || batch->ws == nullptr) {
+    hfjk_internal_set_error("the batch is not bound to a workspace (hfjk_batch_bind)");
+    return HFJK_ERR_INPUT;
+  }
+  if (tensor == nullptr || charges == nullptr || coulomb == nullptr) {
+    hfjk_internal_set_error("hfjk_batch_coulomb_charges needs the tensor, the charges and the output");
+    return HFJK_ERR_INPUT;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  char* ws = (char*)batch->ws;
+  hfjk::BatchView v;
+  v.insts = (const hfjk::BatchInst*)(ws + batch->off_insts);
+  v.prefix = (const long long*)(ws + batch->off_prefix);
+  v.counters = (unsigned long long*)(ws + batch->off_counters);
+  v.ninst = batch->ninst;
+  double* ws64 = (double*)(ws + batch->off_ws64);
+  hipError_t e = hipSetDevice(batch->ctx->device);
+  if (e == hipSuccess) e = hfjk::launch_batch_reset(v, s);
+  if (e == hipSuccess)
+    e = hfjk::launch_batch_coulomb_charges(v, batch->precision, tensor, batch->b_total, charges, ws64, coulomb,
+                                           batch->grid_elem, s);
   if (e != hipSuccess) {
     hfjk_internal_set_error((std::string("batched launch: ") + hipGetErrorString(e)).c_str());
     return HFJK_ERR_HIP;

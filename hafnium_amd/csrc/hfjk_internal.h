@@ -171,6 +171,7 @@ struct BatchInst {
   long long cp_off;   // mode workspace: padded orbitals (elements of the mode's type; bf16: per plane)
   long long x_off;    // mode workspace: X (bf16: per plane)
   long long kp_off;   // mode workspace: SYRK partials (FP64 or FP32 elements)
+  long long q_off;    // first charge of the instance in a caller's flat charge vector (sum of earlier naux)
 };
 
 enum BatchPhase {
@@ -198,6 +199,8 @@ struct BatchView {
 // Launchers of the batched persistent kernels (batch.hip, xk.hip). `grid`
 // is a multiple of 8 workgroups; mode buffers are typed by `precision`.
 hipError_t launch_batch_reset(BatchView v, hipStream_t s);
+hipError_t launch_batch_coulomb_charges(BatchView v, int precision, const void* b, long long b_plane,
+                                        const double* charges, double* ws64, double* j, int grid, hipStream_t s);
 hipError_t launch_batch_coulomb(BatchView v, int precision, const void* b, long long b_plane, const double* d,
                                 double* ws64, double* j, int grid, hipStream_t s);
 hipError_t launch_batch_orbitals(BatchView v, int precision, const double* c, double* ws64, void* cbase,

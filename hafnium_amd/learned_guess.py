@@ -348,6 +348,27 @@ class LearnedGuessModel:
     def predict(self, molecule: Molecule) -> np.ndarray:
         return to_lab(molecule, self.predict_local(molecule), self.aux_name)
 
+    def predict_many(self, molecules) -> list[np.ndarray]:
+        """Laboratory-frame coefficients of many molecules: one network pass per element for all atoms."""
+        torch = _torch()
+        feats = [descriptors(m) for m in molecules]
+        charges = [np.asarray(m.charges) for m in molecules]
+        local = [dict() for _ in molecules]
+        for z in ELEMENTS:
+            rows = [f[c == z] for f, c in zip(feats, charges)]
+            total = sum(len(r) for r in rows)
+            if total == 0:
+                continue
+            em = self.elements[z]
+            net = self.net(z)
+            x = torch.from_numpy(em.normalise(np.concatenate(rows))).to(net.device)
+            y = em.unwhiten(net.forward(x).cpu().numpy().astype(np.float64))
+            at = 0
+            for k, r in enumerate(rows):
+                local[k][z] = y[at:at + len(r)]
+                at += len(r)
+        return [to_lab(m, loc, self.aux_name) for m, loc in zip(molecules, local)]
+
     def save(self, path: str) -> None:
         arrays = {"aux_name": np.array(self.aux_name), "hidden": np.array(HIDDEN)}
         for z, em in self.elements.items():

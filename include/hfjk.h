@@ -167,6 +167,12 @@ HFJK_API int hfjk_batch_bind(hfjk_batch *batch, void *workspace, size_t bytes, v
  * overlap. */
 HFJK_API int hfjk_batch_run(hfjk_batch *batch, const void *tensor, const double *orbitals,
                             const double *density, double *coulomb, double *exchange, void *stream);
+/* J_i = sum_P charges_i[P] B_i[P] for given charges (device float64; instance
+ * i's naux_i values start at the sum of the earlier instances' naux): the
+ * Coulomb matrix of a density known by its charges against B, e.g. a
+ * predicted fitted density (charges = L^-1 (P|Q) d with (P|Q) = L L^T). */
+HFJK_API int hfjk_batch_coulomb_charges(hfjk_batch *batch, const void *tensor, const double *charges,
+                                        double *coulomb, void *stream);
 HFJK_API int hfjk_batch_destroy(hfjk_batch *batch);
 
 /* ---- Gaussian integrals (host) ---------------------------------------------
@@ -247,6 +253,55 @@ HFJK_API int hfjk_get_tuning(hfjk_ctx *ctx, int param, int *value);
  * names stay valid until the next call on the context. */
 HFJK_API int hfjk_profile(hfjk_ctx *ctx, int enable);
 HFJK_API int hfjk_profile_read(hfjk_ctx *ctx, const char **names, float *ms, int max);
+
+/* ---- Batched FP64 dense steps (a batched SCF over many molecules) --------
+ * No counterpart in the reference (its SCF driver solves one molecule at a
+ * time: hafnium/scf/driver.py:154-213). Instances share one padded size n
+ * (zero-padded), laid out at a fixed stride; all pointers are device
+ * pointers, row-major. */
+
+/* For i < count: C_i = alpha op(A_i) op(B_i) + beta C_i, X_i = x + i * stride_x
+ * (rocBLAS dgemm_strided_batched). */
+HFJK_API int hfjk_dgemm_batched(hfjk_ctx *ctx, int trans_a, int trans_b, int m, int n, int k, double alpha,
+                                const double *a, int lda, long long stride_a, const double *b, int ldb,
+                                long long stride_b, double beta, double *c, int ldc, long long stride_c, int count,
+                                void *stream);
+/* Doubles of device scratch hfjk_dsyev_batched needs. */
+HFJK_API long long hfjk_dsyev_batched_work(int n, int count);
+/* Eigendecomposition of count symmetric [n][n] matrices at stride n*n:
+ * eigenvalues ascending in w[count][n]; row j of each matrix becomes its j-th
+ * eigenvector. method 0: divide and conquer (rocSOLVER dsyevd), 1: Jacobi
+ * (dsyevj). info[count] device (!= 0: that instance did not converge). */
+HFJK_API int hfjk_dsyev_batched(hfjk_ctx *ctx, int n, int count, double *a, double *w, double *work, int *info,
+                                int method, void *stream);
+/* out[i] = <x_i, y_i> over len elements, fixed summation order. */
+HFJK_API int hfjk_ddot_batched(hfjk_ctx *ctx, int count, long long len, const double *x, long long stride_x,
+                               const double *y, long long stride_y, double *out, void *stream);
+/* out_i = m_i - m_i^T for count [n][n] matrices at stride n*n. */
+HFJK_API int hfjk_antisymmetrize_batched(hfjk_ctx *ctx, int n, int count, const double *m, double *out,
+                                         void *stream);
+/* Canonical orthogonaliser from eigenpairs of the overlaps (rows of vecs):
+ * x_i[u][j] = vecs_i[j][u] / sqrt(vals_i[j]) where vals_i[j] > threshold, else 0;
+ * kept[i][j] = 1 / 0 accordingly (orthogonalization.py:16-42 of the reference). */
+HFJK_API int hfjk_orthogonalizer_batched(hfjk_ctx *ctx, int n, int count, const double *vecs, const double *vals,
+                                         double threshold, double *x, int *kept, void *stream);
+/* a_i[j][j] += value for every j with kept[i][j] == 0 (moves the discarded
+ * and padded directions above every occupied level). */
+HFJK_API int hfjk_shift_discarded_batched(hfjk_ctx *ctx, int n, int count, const int *kept, double value, double *a,
+                                          void *stream);
+/* a_i[r][c] = 0 for c >= keep[i] (count [rows][cols] matrices at stride rows*cols). */
+HFJK_API int hfjk_mask_columns_batched(hfjk_ctx *ctx, int rows, int cols, int count, const int *keep, double *a,
+                                       void *stream);
+/* out_i = flags[i] ? a_i : b_i for count blocks of len doubles (flags device int). */
+HFJK_API int hfjk_select_batched(hfjk_ctx *ctx, int count, long long len, const int *flags, const double *a,
+                                 const double *b, double *out, void *stream);
+/* Copies between padded (r, c) at i * stride + r * ld + c and compact
+ * offsets[i] + r * cols[i] + c, for r < rows[i], c < cols[i] (device int /
+ * long long arrays; cols[i] <= ld, rows[i] * ld <= stride, rows[i] * cols[i]
+ * <= max_elems). to_compact = 1: padded -> compact, 0: compact -> padded. */
+HFJK_API int hfjk_copy_batched(hfjk_ctx *ctx, int count, const int *rows, const int *cols, const long long *offsets,
+                               long long max_elems, double *padded, int ld, long long stride, double *compact,
+                               int to_compact, void *stream);
 
 /* ---- Learned-guess MLP kernels (no counterpart in the reference: its
  * guesses are the core Hamiltonian and the superposition of atomic
