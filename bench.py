@@ -353,6 +353,9 @@ def main(argv=None):
     ap.add_argument("--cpu-slice", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="use the library's default tiles")
+    ap.add_argument("--tune-cache", default=None,
+                    help="JSON file of tuned tiles: read when it holds this workload, written after tuning "
+                         "(a profiled run then launches exactly the timed run's kernels)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-instance batched workload")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -414,7 +417,7 @@ def main(argv=None):
         jk.precision = mode
         tuning = None
         if not args.no_tune:
-            tuned = autotune(jk, c, mode, cache_path=None)
+            tuned = autotune(jk, c, mode, cache_path=args.tune_cache)
             tuning = {"x_bn": tuned.x_bn, "syrk_splits": tuned.syrk_splits, "x_ms": tuned.x_ms,
                       "syrk_ms": tuned.syrk_ms, "seconds": round(tuned.seconds, 2)}
             log(f"{mode}: tuned {tuning}")

@@ -240,9 +240,10 @@ class BatchedRhf:
         self._mm(self.x, self._t1, self._t2, ta=True)
         self._mm(self._t2, self.x, out)
 
-    def solve(self, fock):
+    def solve(self, fock, mark=None):
         """Aufbau density and occupied orbitals of ``fock``: (D padded, C padded [N, nm, om], C compact)."""
         torch = _torch()
+        mark = mark or (lambda stage: None)
         nm, om, nn = self.nm, self.om, self._nn
         self._mm(self.x, fock, self._t1, ta=True)
         fp = torch.empty_like(self._t1)
@@ -250,7 +251,9 @@ class BatchedRhf:
         _lib.check(self.lib.hfjk_shift_discarded_batched(self._ctx.handle, nm, self.count, _p(self.kept),
                                                          DISCARDED_SHIFT, _p(fp), self._stream()))
         w = torch.empty((self.count, nm), dtype=torch.float64, device=self.device)
+        mark("solve_transform")
         self._eig(fp, w)
+        mark("solve_eigensolver")
         c_occ = torch.empty((self.count, nm, om), dtype=torch.float64, device=self.device)
         # C_occ = X V_occ^T: rows 0..om-1 of each eigenvector matrix are the lowest levels.
         self._gemm(False, True, nm, om, nm, self.x, nm, nn, fp, nm, nn, c_occ, om, nm * om)
@@ -260,6 +263,7 @@ class BatchedRhf:
         self._gemm(False, True, nm, nm, om, c_occ, om, nm * om, c_occ, om, nm * om, density, nm, nn)
         compact = torch.empty(self.jk.orbital_elems, dtype=torch.float64, device=self.device)
         self._copy(self._nao, self._nocc, self._coffs, self._max_orb, c_occ, om, nm * om, compact, True)
+        mark("solve_density")
         return density, c_occ, compact
 
     # ------------------------------------------------------------ starts
@@ -310,10 +314,22 @@ class BatchedRhf:
         return _Start(out_d, out_f, terms)
 
     # ------------------------------------------------------------ the loop
-    def run(self, start: _Start, options: ScfOptions | None = None, *, raise_on_failure: bool = True):
-        """Converge every molecule from ``start``; returns one ``BatchResult`` per molecule."""
+    def run(self, start: _Start, options: ScfOptions | None = None, *, raise_on_failure: bool = True,
+            profile: dict | None = None):
+        """Converge every molecule from ``start``; returns one ``BatchResult`` per molecule.
+
+        ``profile`` (a dict) receives wall seconds per stage, measured with a device
+        synchronisation after each stage (which slows the run slightly)."""
         torch = _torch()
         options = ScfOptions() if options is None else options
+        clock = [time.perf_counter()]
+
+        def mark(stage):
+            if profile is not None:
+                torch.cuda.synchronize(self.device)
+                now = time.perf_counter()
+                profile[stage] = profile.get(stage, 0.0) + now - clock[0]
+                clock[0] = now
         if options.level_shift or options.damping:
             raise InputError("the batched driver has no level shift or damping")
         t0 = time.perf_counter()
@@ -333,14 +349,18 @@ class BatchedRhf:
         slot = 0
         for index in range(options.max_iterations):
             fock = fh[slot]
+            mark("other")
             if index == 0:
                 fock.copy_(start.fock)
             else:
                 self.fock(density, orbitals, precision, fock, terms_dev)
+            mark("fock")
             self.gradient(fock, density, gh[slot])
+            mark("gradient")
             # Every molecule's DIIS overlaps <G_a, G_b> over the stored slots, one batched GEMM.
             self._gemm(False, True, m, m, nn, gh, cnt * nn, nn, gh, cnt * nn, nn, ov_dev, m, m * m)
             host = scal.cpu().numpy()
+            mark("overlaps_transfer")
             terms = start.terms if index == 0 else host[:3 * cnt].reshape(3, cnt)
             ov = 2.0 * host[3 * cnt:].reshape(cnt, m, m)
             energy = self.energies(terms)
@@ -373,10 +393,12 @@ class BatchedRhf:
                 else:
                     sub = ov[i][np.ix_(valid, valid)]
                     coef[i, 0, valid] = solve_diis(sub)
+            mark("host")
             coef_dev = torch.from_numpy(coef).to(self.device)
             extrapolated = torch.empty((cnt, nm, nm), dtype=torch.float64, device=self.device)
             self._gemm(False, False, 1, nn, m, coef_dev, m, m, fh, cnt * nn, nn, extrapolated, nn, nn)
-            density, _, orbitals = self.solve(extrapolated)
+            mark("extrapolate")
+            density, _, orbitals = self.solve(extrapolated, mark)
             slot = (slot + 1) % m
         total = time.perf_counter() - t0
         if int((self._eig_info != 0).sum().item()):

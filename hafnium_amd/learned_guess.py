@@ -538,6 +538,43 @@ def learned_guess(system, model: LearnedGuessModel, sad_density: np.ndarray, *, 
     return (density if use else sad), info
 
 
+def evaluate(model: "LearnedGuessModel", seeds, sizes, *, precision: str = "fp64", log=print) -> dict:
+    """Held-out comparison, one cluster at a time: SCF from the atomic guess against SCF from the
+    learned guess (same Hamiltonian, FP64 convergence to 1e-8 Eh). Times exclude integrals and the
+    fit; the learned time includes the guess."""
+    from hafnium_amd.clusters import atomic_density_guess, build_system, run_cluster
+
+    torch = _torch()
+    cache: dict = {}
+    rows = []
+    for seed, n in zip(seeds, sizes):
+        system = build_system(water_cluster(int(n), int(seed)), keep_integrals=True)
+        sad = atomic_density_guess(system.molecule, cache=cache)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ref, _ = run_cluster(int(n), int(seed), system=system, guess=sad, precision=precision)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        start, info = learned_guess(system, model, sad, precision=precision)
+        got, _ = run_cluster(int(n), int(seed), system=system, guess=start, precision=precision)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        row = {"seed": int(seed), "n_waters": int(n), "sad_iterations": ref.n_iterations,
+               "learned_iterations": got.n_iterations, "sad_seconds": t1 - t0, "learned_seconds": t2 - t1,
+               "used": info["used"], "energy_diff": abs(got.energy - ref.energy)}
+        rows.append(row)
+        log(f"[eval] {len(rows)}/{len(seeds)} {row}", flush=True)
+        del system
+    summary = {"count": len(rows), "precision": precision,
+               "mean_sad_iterations": float(np.mean([r["sad_iterations"] for r in rows])),
+               "mean_learned_iterations": float(np.mean([r["learned_iterations"] for r in rows])),
+               "sad_seconds": float(sum(r["sad_seconds"] for r in rows)),
+               "learned_seconds": float(sum(r["learned_seconds"] for r in rows)),
+               "fallbacks": sum(1 for r in rows if r["used"] != "learned"),
+               "max_energy_diff": float(max(r["energy_diff"] for r in rows)), "clusters": rows}
+    return summary
+
+
 # ---------------------------------------------------------------- command line
 
 def _main(argv=None) -> int:
@@ -554,7 +591,25 @@ def _main(argv=None) -> int:
     t.add_argument("--epochs", type=int, default=300)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--batch", type=int, default=256)
+    e = sub.add_parser("evaluate", help="held-out clusters: atomic vs learned guess -> JSON summary")
+    e.add_argument("--model", default=MODEL_PATH)
+    e.add_argument("--count", type=int, default=100)
+    e.add_argument("--sizes", default="2-16")
+    e.add_argument("--seed0", type=int, default=500000)
+    e.add_argument("--precision", default="fp64")
+    e.add_argument("--out", required=True)
     args = p.parse_args(argv)
+    if args.cmd == "evaluate":
+        import json
+
+        lo, hi = (int(v) for v in args.sizes.split("-"))
+        sizes = np.random.default_rng(args.seed0).integers(lo, hi + 1, args.count)
+        seeds = args.seed0 + np.arange(args.count)
+        summary = evaluate(LearnedGuessModel.load(args.model), seeds, sizes, precision=args.precision)
+        with open(args.out, "w") as fh:
+            json.dump(summary, fh, indent=1)
+        print(json.dumps({k: v for k, v in summary.items() if k != "clusters"}), flush=True)
+        return 0
     if args.cmd == "generate":
         lo, hi = (int(v) for v in args.sizes.split("-"))
         rng = np.random.default_rng(args.seed0)
