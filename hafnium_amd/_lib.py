@@ -52,6 +52,7 @@ SIGNATURES = [
     ("hfjk_bind", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp]),
     ("hfjk_workspace_bytes", _c_int, [_vp, _c_int, _c_int, ctypes.POINTER(_c_size)]),
     ("hfjk_coulomb", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_coulomb_charges", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_basis_nfunc", _c_int, [_vp, ctypes.POINTER(_c_int)]),
@@ -84,7 +85,9 @@ SIGNATURES = [
     ("hfjk_dsyev_batched_work", _c_ll, [_c_int, _c_int]),
     ("hfjk_dsyev_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp]),
     ("hfjk_ddot_batched", _c_int, [_vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_ll, _vp, _vp]),
+    ("hfjk_history_dots_batched", _c_int, [_vp, _c_int, _c_ll, _c_int, _vp, _c_int, _vp, _vp]),
     ("hfjk_antisymmetrize_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp]),
+    ("hfjk_symmetrize_batched", _c_int, [_vp, _c_int, _c_ll, _vp, _vp]),
     ("hfjk_orthogonalizer_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
     ("hfjk_shift_discarded_batched", _c_int, [_vp, _c_int, _c_int, _vp, ctypes.c_double, _vp, _vp]),
     ("hfjk_mask_columns_batched", _c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),

@@ -36,9 +36,12 @@ class TuningResult:
 
 
 def _device_name(jk) -> str:
+    """Architecture and compute-unit count (the marketing name reads differently under a profiler)."""
     import torch
 
-    return torch.cuda.get_device_properties(jk.device).name or "gpu"
+    props = torch.cuda.get_device_properties(jk.device)
+    arch = (getattr(props, "gcnArchName", "") or "gpu").split(":")[0]
+    return f"{arch}x{props.multi_processor_count}"
 
 
 def tuning_key(jk, nocc: int, precision: str) -> str:

@@ -109,7 +109,7 @@ class _Start:
 class BatchedRhf:
     """Host integrals of N molecules -> fitted tensors in one batch, padded FP64 state on the GPU."""
 
-    def __init__(self, inputs, *, device=None, eigensolver: str = "jacobi") -> None:
+    def __init__(self, inputs, *, device=None, eigensolver: str = "divide") -> None:
         from hafnium_amd.batch import BatchedCoulombExchange
         from hafnium_amd.coulomb_exchange import _Context
         from hafnium_amd.linalg import Dense
@@ -140,6 +140,8 @@ class BatchedRhf:
             b = torch.from_numpy(np.ascontiguousarray(x.three_center).reshape(x.naux, -1)).to(self.device)
             _lib.check(self.lib.hfjk_dtrsm_lower(self._ctx.handle, x.naux, x.nao * x.nao, _p(m), _p(b),
                                                  self._stream()))
+            # (B + B^T) / 2: removes the triangular solve's rounding asymmetry (see density_fitting.df_tensor).
+            _lib.check(self.lib.hfjk_symmetrize_batched(self._ctx.handle, x.nao, x.naux, _p(b), self._stream()))
             self.jk.set_tensor(i, b.view(x.naux, x.nao, x.nao))
             del b
             self.factors.append(m)
@@ -336,9 +338,10 @@ class BatchedRhf:
         cnt, nm, nn, m = self.count, self.nm, self._nn, options.diis_size
         fh = torch.zeros((m, cnt, nm, nm), dtype=torch.float64, device=self.device)
         gh = torch.zeros_like(fh)
-        scal = torch.empty(3 * cnt + cnt * m * m, dtype=torch.float64, device=self.device)
+        scal = torch.empty(3 * cnt + cnt * m, dtype=torch.float64, device=self.device)
         terms_dev = scal[:3 * cnt].view(3, cnt)
-        ov_dev = scal[3 * cnt:].view(cnt, m, m)
+        row_dev = scal[3 * cnt:].view(cnt, m)
+        ov = np.zeros((cnt, m, m))  # 2 <G_a, G_b> per molecule, one row/column refreshed per iteration
         precision = options.precision
         density = start.density
         orbitals = None
@@ -357,12 +360,15 @@ class BatchedRhf:
             mark("fock")
             self.gradient(fock, density, gh[slot])
             mark("gradient")
-            # Every molecule's DIIS overlaps <G_a, G_b> over the stored slots, one batched GEMM.
-            self._gemm(False, True, m, m, nn, gh, cnt * nn, nn, gh, cnt * nn, nn, ov_dev, m, m * m)
+            # The newest error's overlaps with every slot, for every molecule, in one launch.
+            _lib.check(self.lib.hfjk_history_dots_batched(self._ctx.handle, cnt, nn, m, _p(gh), slot, _p(row_dev),
+                                                          self._stream()))
             host = scal.cpu().numpy()
             mark("overlaps_transfer")
             terms = start.terms if index == 0 else host[:3 * cnt].reshape(3, cnt)
-            ov = 2.0 * host[3 * cnt:].reshape(cnt, m, m)
+            row = 2.0 * host[3 * cnt:].reshape(cnt, m)
+            ov[:, slot, :] = row
+            ov[:, :, slot] = row
             energy = self.energies(terms)
             gnorm = np.sqrt(np.maximum(ov[:, slot, slot], 0.0))
             change = energy - previous
@@ -445,8 +451,9 @@ def atomic_start(batch: BatchedRhf, atom_densities: dict, precision: str = "fp64
 
 
 def learned_start(batch: BatchedRhf, model, sad: _Start, precision: str = "fp64") -> _Start:
-    """F0 = h + J[d] - K[D_atoms] from predicted fitted coefficients d; the start is the aufbau density
-    of F0 where its energy is not above the atomic start's, the atomic start elsewhere."""
+    """F0 = h + J[d] - K[D_atoms] from predicted fitted coefficients d. Per molecule the start is the
+    aufbau density of F0 where its energy is not above that of the aufbau density of the atomic
+    start's Fock matrix (both idempotent: a variational comparison), and the latter elsewhere."""
     torch = _torch()
     t0 = time.perf_counter()
     coeffs = model.predict_many([x.molecule for x in batch.inputs])
@@ -463,10 +470,11 @@ def learned_start(batch: BatchedRhf, model, sad: _Start, precision: str = "fp64"
     guess = torch.empty_like(batch.h)
     batch._combine(1.0, sad.fock, -2.0, sad.info["coulomb"], 1.0, j_pad, guess)
     learned = batch.start_from_fock(guess, precision)
+    fallback = batch.start_from_fock(sad.fock, precision)
     e_learned = batch.energies(learned.terms)
-    e_sad = batch.energies(sad.terms)
+    e_sad = batch.energies(fallback.terms)
     flags = e_learned <= e_sad
-    chosen = batch.select(flags, learned, sad)
+    chosen = batch.select(flags, learned, fallback)
     chosen.info = {"learned": int(flags.sum()), "fallback": int((~flags).sum()),
                    "energy_learned": e_learned.tolist(), "energy_sad": e_sad.tolist(),
                    "seconds": time.perf_counter() - t0}

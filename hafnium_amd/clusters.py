@@ -103,11 +103,18 @@ def atomic_density_guess(molecule: Molecule, *, basis: str = ORBITAL_BASIS, aux:
 
 def run_cluster(n_waters: int, seed: int, *, precision: str = "fp64", guess=None, energy_tolerance: float = 1e-8,
                 gradient_tolerance: float = 1e-6, switch_threshold: float = 1e-5, device=None, system=None):
-    """Converge (H2O)_n; returns ``(result, system)`` with the SCF wall time in ``result.seconds``."""
+    """Converge (H2O)_n; returns ``(result, system)`` with the SCF wall time in ``result.seconds``.
+
+    ``guess``: a density, or a start of ``learned_guess.learned_guess`` (density, orbitals and the
+    first Fock build)."""
     if system is None:
         system = build_system(water_cluster(n_waters, seed), precision=precision, device=device)
     opts = ScfOptions(precision=precision, energy_tolerance=energy_tolerance, gradient_tolerance=gradient_tolerance,
                       switch_threshold=switch_threshold)
     _torch().cuda.synchronize()
-    result = solve_scf(system.hamiltonian, guess=guess, options=opts)
+    if hasattr(guess, "first_build"):
+        result = solve_scf(system.hamiltonian, guess=guess.density, guess_orbitals=guess.orbitals,
+                           first_build=guess.first_build, options=opts)
+    else:
+        result = solve_scf(system.hamiltonian, guess=guess, options=opts)
     return result, system

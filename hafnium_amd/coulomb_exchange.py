@@ -244,6 +244,22 @@ class MI355XCoulombExchange:
                                               self._stream()))
         return self._out_like(out, density)
 
+    def coulomb_from_charges(self, charges, *, precision: str | None = None):
+        """``J = sum_P charges[P] B[P]`` for a device float64 vector of naux charges (no reference
+        counterpart; the Coulomb matrix of a predicted fitted density)."""
+        torch = _torch()
+        prec = self._precision if precision is None else _lib.precision_code(precision)
+        if not (isinstance(charges, torch.Tensor) and charges.dtype == torch.float64
+                and charges.numel() == self._naux and charges.device == self._device):
+            raise InputError(f"charges are a float64 tensor of {self._naux} values on {self._device}")
+        q = charges.contiguous()
+        self._ensure_copy(prec)
+        out = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
+        ws, nbytes = self._workspace_for(1, prec)
+        _lib.check(self._ctx.lib.hfjk_coulomb_charges(self._ctx.handle, _ptr(q), _ptr(out), prec, ws, nbytes,
+                                                      self._stream()))
+        return out
+
     def exchange(self, density: Any, *, precision: str | None = None):
         """``K = sum_P B[P] D B[P]`` (coulomb_exchange.py:184-190) for a symmetric density.
 

@@ -9,6 +9,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <string>
 
 #include "hfjk_internal.h"
@@ -78,6 +79,25 @@ __global__ __launch_bounds__(kT) void batched_dot_kernel(long long len, const do
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
+// out[i][a] = <hist[newest][i], hist[a][i]> over len elements, hist [nslots][count][len];
+// one workgroup per (instance, slot), fixed summation order.
+__global__ __launch_bounds__(kT) void history_dots_kernel(long long len, int count, const double* __restrict__ hist,
+                                                          int newest, double* __restrict__ out) {
+  __shared__ double red[kT];
+  const int i = blockIdx.x, a = blockIdx.y;
+  const double* x = hist + ((long long)newest * count + i) * len;
+  const double* y = hist + ((long long)a * count + i) * len;
+  double s = 0.0;
+  for (long long e = threadIdx.x; e < len; e += kT) s += x[e] * y[e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(long long)i * gridDim.y + a] = red[0];
+}
+
 __global__ __launch_bounds__(kT) void batched_antisym_kernel(int n, const double* __restrict__ m,
                                                              double* __restrict__ out) {
   const long long nn = (long long)n * n;
@@ -86,6 +106,20 @@ __global__ __launch_bounds__(kT) void batched_antisym_kernel(int n, const double
   for (long long e = (long long)blockIdx.x * kT + threadIdx.x; e < nn; e += (long long)gridDim.x * kT) {
     const int r = (int)(e / n), c = (int)(e % n);
     oi[e] = mi[e] - mi[(long long)c * n + r];
+  }
+}
+
+// a_i <- (a_i + a_i^T) / 2 in place: one thread per pair (r < c) writes both entries.
+__global__ __launch_bounds__(kT) void batched_sym_kernel(int n, double* __restrict__ a) {
+  const long long nn = (long long)n * n;
+  double* ai = a + (long long)blockIdx.y * nn;
+  for (long long e = (long long)blockIdx.x * kT + threadIdx.x; e < nn; e += (long long)gridDim.x * kT) {
+    const int r = (int)(e / n), c = (int)(e % n);
+    if (r < c) {
+      const double v = 0.5 * (ai[e] + ai[(long long)c * n + r]);
+      ai[e] = v;
+      ai[(long long)c * n + r] = v;
+    }
   }
 }
 
@@ -225,6 +259,18 @@ int hfjk_ddot_batched(hfjk_ctx* ctx, int count, long long len, const double* x, 
   });
 }
 
+int hfjk_history_dots_batched(hfjk_ctx* ctx, int count, long long len, int nslots, const double* hist, int newest,
+                              double* out, void* stream) {
+  return bguard(ctx, [&] {
+    check_count(count);
+    if (len < 0 || nslots < 1 || nslots > 64 || newest < 0 || newest >= nslots || hist == nullptr || out == nullptr)
+      bfail(HFJK_ERR_INPUT, "hfjk_history_dots_batched: bad input");
+    hipLaunchKernelGGL(history_dots_kernel, dim3(count, nslots), dim3(kT), 0, (hipStream_t)stream, len, count, hist,
+                       newest, out);
+    bcheck_hip(hipGetLastError(), "history_dots launch");
+  });
+}
+
 int hfjk_antisymmetrize_batched(hfjk_ctx* ctx, int n, int count, const double* m, double* out, void* stream) {
   return bguard(ctx, [&] {
     check_count(count);
@@ -233,6 +279,19 @@ int hfjk_antisymmetrize_batched(hfjk_ctx* ctx, int n, int count, const double* m
     hipLaunchKernelGGL(batched_antisym_kernel, dim3(blocks_for((long long)n * n), count), dim3(kT), 0,
                        (hipStream_t)stream, n, m, out);
     bcheck_hip(hipGetLastError(), "batched_antisym launch");
+  });
+}
+
+int hfjk_symmetrize_batched(hfjk_ctx* ctx, int n, long long count, double* a, void* stream) {
+  return bguard(ctx, [&] {
+    if (n < 1 || count < 0 || a == nullptr) bfail(HFJK_ERR_INPUT, "hfjk_symmetrize_batched: bad input");
+    // Chunks of at most 65535 matrices per launch (grid.y limit).
+    for (long long at = 0; at < count; at += 65535) {
+      const unsigned chunk = (unsigned)std::min<long long>(65535, count - at);
+      hipLaunchKernelGGL(batched_sym_kernel, dim3(blocks_for((long long)n * n), chunk), dim3(kT), 0,
+                         (hipStream_t)stream, n, a + at * ((long long)n * n));
+      bcheck_hip(hipGetLastError(), "batched_sym launch");
+    }
   });
 }
 

@@ -420,6 +420,34 @@ int hfjk_workspace_bytes(hfjk_ctx* ctx, int nocc, int precision, size_t* bytes) 
   });
 }
 
+int hfjk_coulomb_charges(hfjk_ctx* ctx, const double* charges, double* coulomb, int precision, void* workspace,
+                         size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_precision(precision);
+    check_bound(ctx, precision);
+    if (charges == nullptr || coulomb == nullptr) fail(HFJK_ERR_INPUT, "null pointer");
+    const Layout L = make_layout(ctx, 1, precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32);
+    if (workspace_bytes < L.c_end || workspace == nullptr)
+      fail(HFJK_ERR_WORKSPACE, "workspace smaller than the coulomb needs (" + std::to_string(L.c_end) + " bytes)");
+    if (((uintptr_t)workspace & 255) != 0) fail(HFJK_ERR_WORKSPACE, "workspace must be 256-byte aligned");
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    const void* b = bound_tensor(ctx, precision);
+    double* jp = at<double>(workspace, L.cjp);
+    {
+      Stage st(ctx, s, "coulomb_accumulate");
+      check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, charges, jp,
+                                                s),
+                "coulomb accumulate launch");
+    }
+    {
+      Stage st(ctx, s, "coulomb_unpack");
+      check_hip(hfjk::launch_coulomb_unpack(jp, L.nsplit_j, L.p, ctx->nao, coulomb, s), "coulomb unpack launch");
+    }
+  });
+}
+
 int hfjk_coulomb(hfjk_ctx* ctx, const double* density, double* coulomb, int precision, void* workspace,
                  size_t workspace_bytes, void* stream) {
   return guarded([&] {

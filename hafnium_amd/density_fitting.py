@@ -13,6 +13,8 @@ reference's.
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from hafnium_amd.errors import InputError
@@ -24,8 +26,17 @@ def _torch():
     return torch
 
 
-def df_tensor(dense, metric, three_center):
-    """``L^-1 (P|uv)`` ``[aux, ao, ao]`` as a device tensor."""
+def df_tensor(dense, metric, three_center, *, return_factor: bool = False):
+    """``L^-1 (P|uv)`` ``[aux, ao, ao]`` as a device tensor, symmetrised in (u, v).
+
+    The triangular solve treats the (u, v) and (v, u) columns separately, so
+    its rounding leaves an asymmetry of order 1e-12 relative on larger
+    molecules; (B + B^T) / 2 removes it (the packed storage keeps one
+    triangle, and its symmetry check is exact). ``return_factor`` also
+    returns the Cholesky factor (lower triangle of an [aux, aux] device
+    tensor)."""
+    from hafnium_amd import _lib
+
     torch = _torch()
     m = torch.as_tensor(np.asarray(metric) if isinstance(metric, np.ndarray) else metric,
                         dtype=torch.float64).to(dense.device).contiguous()
@@ -37,16 +48,25 @@ def df_tensor(dense, metric, three_center):
     if t.ndim != 3 or t.shape[0] != naux or t.shape[1] != t.shape[2]:
         raise InputError("three-centre integrals have shape [aux, ao, ao]")
     n = t.shape[1]
-    return dense.cholesky_solve(m, t.reshape(naux, n * n)).reshape(naux, n, n)
+    factor = dense.cholesky(m)
+    out = dense.lower_solve(factor, t.reshape(naux, n * n)).reshape(naux, n, n)
+    _lib.check(dense._lib.hfjk_symmetrize_batched(dense._ctx.handle, n, naux, ctypes.c_void_p(out.data_ptr()),
+                                                 dense._stream()))
+    return (out, factor) if return_factor else out
 
 
 def fitted_coulomb_exchange(metric, three_center, *, precision: str = "fp64", device=None):
-    """An ``MI355XCoulombExchange`` of the fitted tensor built on the GPU."""
+    """An ``MI355XCoulombExchange`` of the fitted tensor built on the GPU; the metric's Cholesky
+    factor stays on it as ``metric_factor`` (used to turn fitted coefficients into charges)."""
     from hafnium_amd.coulomb_exchange import MI355XCoulombExchange, _Context
     from hafnium_amd.linalg import Dense
 
     torch = _torch()
     dev = torch.device(device) if device is not None else torch.device("cuda", 0)
     ctx = _Context(dev.index or 0)
-    b = df_tensor(Dense(ctx, dev), metric, three_center)
-    return MI355XCoulombExchange(b, precision=precision, device=dev)
+    dense = Dense(ctx, dev)
+    b, factor = df_tensor(dense, metric, three_center, return_factor=True)
+    out = MI355XCoulombExchange(b, precision=precision, device=dev)
+    out.metric_factor = factor
+    out.fit_dense = dense
+    return out

@@ -23,7 +23,8 @@ squared error is the Coulomb self-energy of each atom's density error.
 Guess: ``F0 = h + J[d] - K[D_sad]`` (Coulomb from the predicted density,
 exchange from the superposition of atomic densities); the aufbau density
 of F0 starts the SCF. When its energy is above the energy of the atomic
-guess itself, the SCF starts from the atomic guess instead (the fallback).
+guess's own aufbau density (both idempotent, so the comparison is
+variational), the SCF starts from the atomic guess instead (the fallback).
 
 Training data: converged FP64 SCF densities of seeded water clusters
 produced by this library's own GPU SCF (``generate_dataset``).
@@ -407,13 +408,24 @@ def cluster_record(n_waters: int, seed: int, sad_cache: dict, *, device=None) ->
             "local": local, "energy": result.energy, "iterations": result.n_iterations}
 
 
-def generate_dataset(seeds, sizes, path: str, *, device=None, log=print) -> dict:
-    """Converged FP64 densities of ``water_cluster(sizes[i], seeds[i])`` as per-element arrays in ``path``."""
+def generate_dataset(seeds, sizes, path: str, *, device=None, log=print, checkpoint_every: int = 100) -> dict:
+    """Converged FP64 densities of ``water_cluster(sizes[i], seeds[i])`` as per-element arrays in ``path``
+    (rewritten every ``checkpoint_every`` clusters, so an interrupted run keeps what it finished)."""
     sad_cache: dict = {}
     feats = {z: [] for z in ELEMENTS}
     targets = {z: [] for z in ELEMENTS}
     cluster_of = {z: [] for z in ELEMENTS}
     meta = []
+
+    def save():
+        arrays = {"meta": np.asarray(meta, dtype=np.float64)}
+        for z in ELEMENTS:
+            arrays[f"x_{z}"] = np.concatenate(feats[z])
+            arrays[f"y_{z}"] = np.concatenate(targets[z])
+            arrays[f"cluster_{z}"] = np.asarray(cluster_of[z], dtype=np.int32)
+        np.savez_compressed(path, **arrays)
+        return arrays
+
     t0 = time.perf_counter()
     for index, (seed, n) in enumerate(zip(seeds, sizes)):
         rec = cluster_record(int(n), int(seed), sad_cache, device=device)
@@ -425,19 +437,15 @@ def generate_dataset(seeds, sizes, path: str, *, device=None, log=print) -> dict
         meta.append((int(seed), int(n), rec["energy"], rec["iterations"]))
         log(f"[data] {index + 1}/{len(seeds)} n={n} seed={seed} E={rec['energy']:.10f} it={rec['iterations']} "
             f"t={time.perf_counter() - t0:.0f}s", flush=True)
-    arrays = {"meta": np.asarray(meta, dtype=np.float64)}
-    for z in ELEMENTS:
-        arrays[f"x_{z}"] = np.concatenate(feats[z])
-        arrays[f"y_{z}"] = np.concatenate(targets[z])
-        arrays[f"cluster_{z}"] = np.asarray(cluster_of[z], dtype=np.int32)
-    np.savez_compressed(path, **arrays)
-    return arrays
+        if checkpoint_every and (index + 1) % checkpoint_every == 0:
+            save()
+    return save()
 
 
 # ---------------------------------------------------------------- training
 
-def train(data: dict, *, aux_name: str = "def2-universal-jkfit", epochs: int = 300, batch: int = 256,
-          lr: float = 1e-3, weight_decay: float = 1e-5, validation_fraction: float = 0.1, seed: int = 0,
+def train(data: dict, *, aux_name: str = "def2-universal-jkfit", epochs: int = 60, batch: int = 256,
+          lr: float = 1e-3, weight_decay: float = 1e-2, validation_fraction: float = 0.1, seed: int = 0,
           device=None, log=print) -> tuple["LearnedGuessModel", dict]:
     """Train one network per element on the GPU; returns the model with the best validation loss."""
     from hafnium_amd.mlp import Mlp
@@ -507,35 +515,72 @@ def train(data: dict, *, aux_name: str = "def2-universal-jkfit", epochs: int = 3
 
 # ---------------------------------------------------------------- the guess
 
-def learned_guess(system, model: LearnedGuessModel, sad_density: np.ndarray, *, precision: str = "fp64"):
-    """Starting density (D_alpha, device) from the predicted fitted density, with the atomic-guess fallback.
+@dataclass
+class GuessStart:
+    """A starting point for ``solve_scf``: density, its occupied orbitals and its Fock build."""
 
-    Returns ``(density, info)``; ``info["used"]`` is "learned" or "sad".
-    """
-    from hafnium_amd.scf import _density, _energy_terms, _Setup, ScfOptions
+    density: object
+    orbitals: object
+    first_build: tuple  # (fock, terms) of hamiltonian.fock(density, orbitals)
+
+
+def _aufbau_start(h, setup, fock_guess, precision):
+    from hafnium_amd.scf import _density, _energy_terms
+
+    _, c = setup.solve(fock_guess)
+    density, orbitals = _density(h.dense, c, n_alpha=h.n_alpha)
+    fock, terms = h.fock(density, orbitals, precision)
+    return GuessStart(density, orbitals, (fock, terms)), _energy_terms(h, terms.cpu().numpy()).total
+
+
+def atomic_iterate_start(system, sad_density: np.ndarray, *, precision: str = "fp64") -> GuessStart:
+    """The fallback start alone: the aufbau density of the atomic guess's Fock matrix."""
+    from hafnium_amd.scf import _Setup, ScfOptions
 
     torch = _torch()
-    if system.three_center is None:
+    h = system.hamiltonian
+    sad = torch.from_numpy(np.ascontiguousarray(sad_density)).to(h.device)
+    fock, _ = h.fock(sad, None, precision)
+    return _aufbau_start(h, _Setup(h, ScfOptions()), fock, precision)[0]
+
+
+def learned_guess(system, model: LearnedGuessModel, sad_density: np.ndarray, *, precision: str = "fp64"):
+    """Start for ``run_cluster`` from the predicted fitted density, with the atomic-guess fallback.
+
+    F_sad = h + J[2 D_sad] - K[D_sad] is the Fock matrix of the atomic guess;
+    F0 = h + J[d] - K[D_sad] uses the predicted coefficients d instead, with
+    J[d] = sum_P q_P B[P], q = L^-1 (P|Q) d, on the GPU. The aufbau densities
+    of F0 and of F_sad are both idempotent, so their energies compare
+    variationally: the lower one starts the SCF (the first is the learned
+    start, the second the atomic guess's first iterate, i.e. the fallback),
+    with its Fock build handed on as the SCF's first iteration.
+    Returns ``(GuessStart, info)``; ``info["used"]`` is "learned" or "sad".
+    """
+    from hafnium_amd.scf import _Setup, ScfOptions
+
+    torch = _torch()
+    if system.metric is None:
         raise InputError("build the system with keep_integrals=True for the learned guess")
     h = system.hamiltonian
+    tb = h.two_body
+    dn = h.dense
     t0 = time.perf_counter()
     coeffs = model.predict(system.molecule)
-    j_pred = torch.from_numpy(coulomb_of_coefficients(system.three_center, coeffs)).to(h.device)
+    v = torch.from_numpy(system.metric @ coeffs).to(h.device).view(-1, 1).contiguous()
+    charges = tb.fit_dense.lower_solve(tb.metric_factor, v).view(-1)
+    j_pred = tb.coulomb_from_charges(charges, precision=precision)
     sad = torch.from_numpy(np.ascontiguousarray(sad_density)).to(h.device)
-    k_sad = h.two_body.exchange(sad, precision=precision)
-    dn = h.dense
+    j_sad = tb.coulomb(sad, precision=precision)
+    k_sad = tb.exchange(sad, precision=precision)
+    f_sad = dn.combine(1.0, h.core, 2.0, j_sad, -1.0, k_sad)
     f0 = dn.combine(1.0, h.core, 1.0, j_pred, -1.0, k_sad)
     setup = _Setup(h, ScfOptions())
-    _, c = setup.solve(f0)
-    density, orbitals = _density(dn, c, n_alpha=h.n_alpha)
-    _, terms_l = h.fock(density, orbitals, precision)
-    _, terms_s = h.fock(sad, None, precision)
-    e_learned = _energy_terms(h, terms_l.cpu().numpy()).total
-    e_sad = _energy_terms(h, terms_s.cpu().numpy()).total
+    learned, e_learned = _aufbau_start(h, setup, f0, precision)
+    fallback, e_sad = _aufbau_start(h, setup, f_sad, precision)
     use = e_learned <= e_sad
     info = {"used": "learned" if use else "sad", "energy_learned": e_learned, "energy_sad": e_sad,
             "seconds": time.perf_counter() - t0}
-    return (density if use else sad), info
+    return (learned if use else fallback), info
 
 
 def evaluate(model: "LearnedGuessModel", seeds, sizes, *, precision: str = "fp64", log=print) -> dict:
@@ -559,15 +604,23 @@ def evaluate(model: "LearnedGuessModel", seeds, sizes, *, precision: str = "fp64
         got, _ = run_cluster(int(n), int(seed), system=system, guess=start, precision=precision)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
+        fb, _ = run_cluster(int(n), int(seed), system=system, guess=atomic_iterate_start(system, sad),
+                            precision=precision)
         row = {"seed": int(seed), "n_waters": int(n), "sad_iterations": ref.n_iterations,
-               "learned_iterations": got.n_iterations, "sad_seconds": t1 - t0, "learned_seconds": t2 - t1,
-               "used": info["used"], "energy_diff": abs(got.energy - ref.energy)}
+               "learned_iterations": got.n_iterations, "fallback_iterations": fb.n_iterations,
+               "sad_seconds": t1 - t0, "learned_seconds": t2 - t1, "used": info["used"],
+               "start_error_learned": info["energy_learned"] - ref.energy,
+               "start_error_fallback": info["energy_sad"] - ref.energy,
+               "energy_diff": max(abs(got.energy - ref.energy), abs(fb.energy - ref.energy))}
         rows.append(row)
         log(f"[eval] {len(rows)}/{len(seeds)} {row}", flush=True)
         del system
     summary = {"count": len(rows), "precision": precision,
                "mean_sad_iterations": float(np.mean([r["sad_iterations"] for r in rows])),
                "mean_learned_iterations": float(np.mean([r["learned_iterations"] for r in rows])),
+               "mean_fallback_only_iterations": float(np.mean([r["fallback_iterations"] for r in rows])),
+               "median_start_error_learned": float(np.median([r["start_error_learned"] for r in rows])),
+               "median_start_error_fallback": float(np.median([r["start_error_fallback"] for r in rows])),
                "sad_seconds": float(sum(r["sad_seconds"] for r in rows)),
                "learned_seconds": float(sum(r["learned_seconds"] for r in rows)),
                "fallbacks": sum(1 for r in rows if r["used"] != "learned"),
@@ -588,8 +641,9 @@ def _main(argv=None) -> int:
     t = sub.add_parser("train", help="dataset .npz -> model .npz")
     t.add_argument("--data", required=True)
     t.add_argument("--out", required=True)
-    t.add_argument("--epochs", type=int, default=300)
+    t.add_argument("--epochs", type=int, default=60)
     t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--weight-decay", type=float, default=1e-2)
     t.add_argument("--batch", type=int, default=256)
     e = sub.add_parser("evaluate", help="held-out clusters: atomic vs learned guess -> JSON summary")
     e.add_argument("--model", default=MODEL_PATH)
@@ -619,7 +673,8 @@ def _main(argv=None) -> int:
     else:
         with np.load(args.data, allow_pickle=False) as f:
             data = {k: f[k] for k in f.files}
-        model, report = train(data, epochs=args.epochs, lr=args.lr, batch=args.batch)
+        model, report = train(data, epochs=args.epochs, lr=args.lr, batch=args.batch,
+                              weight_decay=args.weight_decay)
         model.save(args.out)
         for z, r in report.items():
             print(f"[train] element {z}: {r}", flush=True)

@@ -90,14 +90,24 @@ class Dense:
 
     def cholesky_solve(self, metric, rhs):
         """``L^-1 rhs`` with ``metric = L L^T`` (``rhs`` ``[m, k]``, overwritten copy returned)."""
-        self._check(metric, rhs)
+        return self.lower_solve(self.cholesky(metric), rhs)
+
+    def cholesky(self, metric):
+        """``L`` (lower triangle of the returned copy; the strict upper triangle keeps the input)."""
+        self._check(metric)
         m = metric.shape[0]
         fac = metric.clone()
         _lib.check(self._lib.hfjk_dpotrf(self._ctx.handle, m, _p(fac), _p(self._info), self._stream()))
         if self.info() != 0:
             raise BackendError(f"the metric is not positive definite (dpotrf info {self.info()})")
+        return fac
+
+    def lower_solve(self, factor, rhs):
+        """``L^-1 rhs`` for the lower triangle ``L`` of ``factor`` (``rhs`` ``[m, k]``; a solved copy)."""
+        self._check(factor, rhs)
         out = rhs.clone()
-        _lib.check(self._lib.hfjk_dtrsm_lower(self._ctx.handle, m, rhs.shape[1], _p(fac), _p(out), self._stream()))
+        _lib.check(self._lib.hfjk_dtrsm_lower(self._ctx.handle, factor.shape[0], rhs.shape[1], _p(factor), _p(out),
+                                              self._stream()))
         return out
 
     def combine(self, a: float, x, b: float = 0.0, y=None, c: float = 0.0, z=None, out=None):

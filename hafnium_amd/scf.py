@@ -290,7 +290,7 @@ def _density(dense, c, occupations=None, n_alpha=None):
 
 
 def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOptions | None = None,
-              occupation: str = "aufbau") -> ScfResult:
+              occupation: str = "aufbau", guess_orbitals=None, first_build=None) -> ScfResult:
     """Converge the closed-shell state (driver.py:154-213) with the precision policy.
 
     ``guess``: a density ``[n, n]`` (D_alpha) or ``None`` for the averaged
@@ -298,6 +298,10 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
     (``AufbauOccupation``) or "averaged" (``AveragedOccupation``: degenerate
     frontier groups share the electrons, for spherical atoms;
     occupation.py:130-159).
+    ``guess_orbitals`` (``[n, n_alpha]`` with guess = C C^T) selects the
+    occupied-form exchange for the first build; ``first_build`` =
+    ``(fock, terms)`` of ``hamiltonian.fock(guess, ...)`` already computed by
+    the caller is used as the first iteration's build.
     Raises ConvergenceError after ``max_iterations`` Fock builds; its
     ``last`` holds the result.
     """
@@ -332,6 +336,12 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
             from hafnium_amd.errors import StructureError
 
             raise StructureError(f"the guess acts on {tuple(density.shape)}, the Hamiltonian on {hamiltonian.n_ao}")
+        if guess_orbitals is not None:
+            orbitals = torch.as_tensor(guess_orbitals, dtype=torch.float64).to(hamiltonian.device).contiguous()
+            if tuple(orbitals.shape) != (hamiltonian.n_ao, n_alpha):
+                raise InputError("guess orbitals are [n_ao, n_alpha]")
+    if first_build is not None and guess is None:
+        raise InputError("a first build needs the guess density it was built from")
     precision = options.precision
     focks: list = []
     errors: list = []
@@ -341,7 +351,10 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
     previous = None
     for index in range(options.max_iterations):
         t_it = time.perf_counter()
-        fock, terms = hamiltonian.fock(density, orbitals, precision)
+        if index == 0 and first_build is not None:
+            fock, terms = first_build
+        else:
+            fock, terms = hamiltonian.fock(density, orbitals, precision)
         grad = setup.gradient(fock, density)
         # One host transfer per iteration: energy terms, gradient norm, DIIS overlaps.
         focks = (focks + [fock])[-options.diis_size:]

@@ -118,6 +118,13 @@ HFJK_API int hfjk_workspace_bytes(hfjk_ctx *ctx, int nocc, int precision, size_t
 HFJK_API int hfjk_coulomb(hfjk_ctx *ctx, const double *density, double *coulomb, int precision,
                  void *workspace, size_t workspace_bytes, void *stream);
 
+/* J = sum_P charges[P] B[P] for given charges (device float64 [naux]): the
+ * Coulomb matrix of a density known only by its charges against B, e.g. a
+ * predicted fitted density d (charges = L^-1 (P|Q) d, (P|Q) = L L^T). Same
+ * workspace as hfjk_coulomb. No counterpart in the reference. */
+HFJK_API int hfjk_coulomb_charges(hfjk_ctx *ctx, const double *charges, double *coulomb, int precision,
+                                  void *workspace, size_t workspace_bytes, void *stream);
+
 /* K = sum_P (B[P] C) S (B[P] C)^T with C = orbitals [nao][nocc] and
  * S = diag(signs) (signs: device float64 [nocc] of +1/-1, or NULL for all
  * +1). The occupied-orbital form of DensityFittedCoulombExchange.exchange
@@ -277,9 +284,18 @@ HFJK_API int hfjk_dsyev_batched(hfjk_ctx *ctx, int n, int count, double *a, doub
 /* out[i] = <x_i, y_i> over len elements, fixed summation order. */
 HFJK_API int hfjk_ddot_batched(hfjk_ctx *ctx, int count, long long len, const double *x, long long stride_x,
                                const double *y, long long stride_y, double *out, void *stream);
+/* DIIS overlaps of the newest stored error with every slot: hist is
+ * [nslots][count][len]; out[i][a] = <hist[newest][i], hist[a][i]> (fixed
+ * summation order; nslots <= 64). */
+HFJK_API int hfjk_history_dots_batched(hfjk_ctx *ctx, int count, long long len, int nslots, const double *hist,
+                                       int newest, double *out, void *stream);
 /* out_i = m_i - m_i^T for count [n][n] matrices at stride n*n. */
 HFJK_API int hfjk_antisymmetrize_batched(hfjk_ctx *ctx, int n, int count, const double *m, double *out,
                                          void *stream);
+/* a_i <- (a_i + a_i^T) / 2 in place for count [n][n] matrices at stride n*n
+ * (removes the rounding asymmetry of a fitted tensor L^-1 (P|uv), whose
+ * packed storage keeps one triangle). */
+HFJK_API int hfjk_symmetrize_batched(hfjk_ctx *ctx, int n, long long count, double *a, void *stream);
 /* Canonical orthogonaliser from eigenpairs of the overlaps (rows of vecs):
  * x_i[u][j] = vecs_i[j][u] / sqrt(vals_i[j]) where vals_i[j] > threshold, else 0;
  * kept[i][j] = 1 / 0 accordingly (orthogonalization.py:16-42 of the reference). */

@@ -54,15 +54,17 @@ def test_learned_guess_starts_a_converging_scf(tiny_model, tmp_path):
     np.testing.assert_allclose(loaded.predict(system.molecule), model.predict(system.molecule), rtol=1e-6,
                                atol=1e-6)
     sad = atomic_density_guess(system.molecule)
-    density, info = lg.learned_guess(system, loaded, sad)
+    start, info = lg.learned_guess(system, loaded, sad)
     s = system.hamiltonian.overlap
-    d = density
-    # An aufbau density: tr(D S) = n_alpha and D S D = D.
-    if info["used"] == "learned":
-        assert float((d @ s).trace()) == pytest.approx(system.hamiltonian.n_alpha, abs=1e-8)
-        assert float((d @ s @ d - d).abs().max()) < 1e-8
+    d = start.density
+    # Either start is an aufbau density: tr(D S) = n_alpha and D S D = D.
+    assert float((d @ s).trace()) == pytest.approx(system.hamiltonian.n_alpha, abs=1e-8)
+    assert float((d @ s @ d - d).abs().max()) < 1e-8
+    # The handed-on first build is the Fock matrix of that density.
+    fock, _ = system.hamiltonian.fock(start.density, start.orbitals)
+    assert float((fock - start.first_build[0]).abs().max()) < 1e-12
     ref, _ = run_cluster(3, 777, system=system, guess=sad)
-    got, _ = run_cluster(3, 777, system=system, guess=density)
+    got, _ = run_cluster(3, 777, system=system, guess=start)
     assert got.energy == pytest.approx(ref.energy, abs=SAME_STATE_TOL)
 
 
@@ -78,30 +80,42 @@ def test_fallback_to_the_atomic_guess(tiny_model):
                                    for z, em in model.elements.items()}, model.aux_name)
     system = build_system(water_cluster(2, seed=778), keep_integrals=True)
     sad = atomic_density_guess(system.molecule)
-    density, info = lg.learned_guess(system, broken, sad)
+    start, info = lg.learned_guess(system, broken, sad)
     assert info["used"] == "sad"
     assert info["energy_learned"] > info["energy_sad"]
-    np.testing.assert_array_equal(density.cpu().numpy(), sad)
+    fallback = lg.atomic_iterate_start(system, sad)
+    np.testing.assert_allclose(start.density.cpu().numpy(), fallback.density.cpu().numpy(), atol=1e-12)
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "hafnium_amd", "data",
                                                     "learned_guess_water.npz")),
                     reason="no trained model stored in hafnium_amd/data")
-def test_stored_model_saves_iterations_on_held_out_clusters():
+def test_stored_model_improves_the_start_on_held_out_clusters():
     from hafnium_amd import learned_guess as lg
     from hafnium_amd.clusters import atomic_density_guess, build_system, run_cluster
     from hafnium_amd.molecule import water_cluster
 
     model = lg.LearnedGuessModel.load()
     cache: dict = {}
-    sad_its, learned_its = [], []
-    for n, seed in [(4, 900001), (6, 900002), (8, 900003), (10, 900004)]:
+    sad_its, fallback_its, learned_its, used, gaps = [], [], [], [], []
+    for n, seed in [(4, 900001), (6, 900002), (8, 900003), (10, 900004), (12, 900005), (14, 900006)]:
         system = build_system(water_cluster(n, seed), keep_integrals=True)
         sad = atomic_density_guess(system.molecule, cache=cache)
         ref, _ = run_cluster(n, seed, system=system, guess=sad)
-        density, info = lg.learned_guess(system, model, sad)
-        got, _ = run_cluster(n, seed, system=system, guess=density)
-        assert got.energy == pytest.approx(ref.energy, abs=SAME_STATE_TOL)
+        fb, _ = run_cluster(n, seed, system=system, guess=lg.atomic_iterate_start(system, sad))
+        start, info = lg.learned_guess(system, model, sad)
+        got, _ = run_cluster(n, seed, system=system, guess=start)
+        for r in (fb, got):
+            assert r.energy == pytest.approx(ref.energy, abs=SAME_STATE_TOL)
         sad_its.append(ref.n_iterations)
+        fallback_its.append(fb.n_iterations)
         learned_its.append(got.n_iterations)
+        used.append(info["used"])
+        gaps.append((info["energy_learned"] - ref.energy, info["energy_sad"] - ref.energy))
+    # What the network adds over the method without it: a start of lower (variational) energy than
+    # the atomic guess's own first iterate on most clusters, and fewer iterations than the SCF from
+    # the atomic density; against the first iterate alone the iteration counts are on par.
+    assert used.count("learned") >= len(used) - 1, (used, gaps)
+    assert np.median([g[0] for g in gaps]) < np.median([g[1] for g in gaps]), gaps
     assert np.mean(learned_its) < np.mean(sad_its), (learned_its, sad_its)
+    assert np.mean(learned_its) <= np.mean(fallback_its) + 0.5, (learned_its, fallback_its)
