@@ -37,6 +37,7 @@ hafnium/scf/atoms.py:76-119).
 from __future__ import annotations
 
 import argparse
+import functools
 import math
 import os
 import sys
@@ -127,6 +128,7 @@ def solid_harmonics(l: int, vectors: np.ndarray) -> np.ndarray:
 
 
 _ROT_POINTS = None
+_ROT_PINV: dict = {}
 
 
 def rotation_matrices(rot: np.ndarray, lmax: int) -> list[np.ndarray]:
@@ -134,18 +136,26 @@ def rotation_matrices(rot: np.ndarray, lmax: int) -> list[np.ndarray]:
 
     A density ``sum_m c_m S_m(r)`` equals ``sum_m c'_m S_m(rot^T r)`` when
     ``c = D c'``. Solved by least squares on 64 fixed directions (exact: the
-    harmonics of one l span a rotation-invariant space).
+    harmonics of one l span a rotation-invariant space); the pseudo-inverse
+    of the fixed directions' harmonics is computed once per l.
     """
+    return [d[0] for d in rotation_matrices_many(np.asarray(rot)[None], lmax)]
+
+
+def rotation_matrices_many(rots: np.ndarray, lmax: int) -> list[np.ndarray]:
+    """:func:`rotation_matrices` of many rotations ``[k, 3, 3]`` at once: a list over l of ``[k, 2l+1, 2l+1]``."""
     global _ROT_POINTS
     if _ROT_POINTS is None:
         pts = np.random.default_rng(12345).standard_normal((64, 3))
         _ROT_POINTS = pts / np.linalg.norm(pts, axis=1, keepdims=True)
+    k = rots.shape[0]
+    rotated = np.einsum("pi,kij->kpj", _ROT_POINTS, rots).reshape(-1, 3)
     out = []
     for l in range(lmax + 1):
-        y = solid_harmonics(l, _ROT_POINTS)
-        y_rot = solid_harmonics(l, _ROT_POINTS @ rot)
-        d, *_ = np.linalg.lstsq(y, y_rot, rcond=None)
-        out.append(d)
+        if l not in _ROT_PINV:
+            _ROT_PINV[l] = np.linalg.pinv(solid_harmonics(l, _ROT_POINTS))
+        y = solid_harmonics(l, rotated).reshape(k, len(_ROT_POINTS), 2 * l + 1)
+        out.append(np.einsum("mp,kpn->kmn", _ROT_PINV[l], y))
     return out
 
 
@@ -180,26 +190,29 @@ def descriptors(molecule: Molecule) -> np.ndarray:
     centres = np.linspace(R_MIN, R_MAX, N_RADIAL)
     width = centres[1] - centres[0]
     nat = len(charges)
-    feats = np.zeros((nat, len(ELEMENTS), N_RADIAL, (L_DESCRIPTOR + 1) ** 2))
-    for a in range(nat):
-        d = pos - pos[a]
-        r = np.linalg.norm(d, axis=1)
-        mask = (r > 1e-8) & (r < R_MAX)
-        if not np.any(mask):
-            continue
-        local = (d[mask] @ rots[a]) / r[mask, None]
-        cut = 0.5 * (np.cos(np.pi * r[mask] / R_MAX) + 1.0)
-        radial = np.exp(-0.5 * ((r[mask, None] - centres[None, :]) / width) ** 2) * cut[:, None]
-        ang = np.concatenate([solid_harmonics(l, local) for l in range(L_DESCRIPTOR + 1)], axis=1)
-        for e_i, z in enumerate(ELEMENTS):
-            sel = charges[mask] == z
-            if np.any(sel):
-                feats[a, e_i] = radial[sel].T @ ang[sel]
+    n_ang = (L_DESCRIPTOR + 1) ** 2
+    feats = np.zeros((nat, len(ELEMENTS), N_RADIAL * n_ang))
+    # All (atom a, neighbour b) pairs within the cutoff at once.
+    d = pos[None, :, :] - pos[:, None, :]
+    r = np.linalg.norm(d, axis=2)
+    ia, ib = np.nonzero((r > 1e-8) & (r < R_MAX))
+    if len(ia) == 0:
+        return feats.reshape(nat, -1)
+    rp = r[ia, ib]
+    local = np.einsum("pk,pkj->pj", d[ia, ib], rots[ia]) / rp[:, None]
+    cut = 0.5 * (np.cos(np.pi * rp / R_MAX) + 1.0)
+    radial = np.exp(-0.5 * ((rp[:, None] - centres[None, :]) / width) ** 2) * cut[:, None]
+    ang = np.concatenate([solid_harmonics(l, local) for l in range(L_DESCRIPTOR + 1)], axis=1)
+    outer = (radial[:, :, None] * ang[:, None, :]).reshape(len(ia), -1)
+    for e_i, z in enumerate(ELEMENTS):
+        sel = charges[ib] == z
+        np.add.at(feats[:, e_i], ia[sel], outer[sel])
     return feats.reshape(nat, -1)
 
 
 # ---------------------------------------------------------------- coefficient layout
 
+@functools.lru_cache(maxsize=None)
 def shell_blocks(aux_name: str, z: int):
     """``([(start, l), ...], n)``: the (2l+1)-blocks of element z's auxiliary functions."""
     blocks = []
@@ -211,15 +224,26 @@ def shell_blocks(aux_name: str, z: int):
     return blocks, at
 
 
+@functools.lru_cache(maxsize=None)
 def _lmax(aux_name: str) -> int:
     return max(sh["l"] for z in ELEMENTS for sh in load_basis(aux_name)[str(z)])
 
 
-def _rotate(coeffs, blocks, dmats, inverse=False):
-    out = coeffs.copy()
-    for start, l in blocks:
-        d = dmats[l].T if inverse else dmats[l]
-        out[start:start + 2 * l + 1] = d @ coeffs[start:start + 2 * l + 1]
+@functools.lru_cache(maxsize=None)
+def _block_columns(aux_name: str, z: int) -> dict:
+    """``{l: [n_blocks_l, 2l+1]}`` coefficient columns of element z's (2l+1)-blocks."""
+    by_l: dict = {}
+    for start, l in shell_blocks(aux_name, z)[0]:
+        by_l.setdefault(l, []).append(np.arange(start, start + 2 * l + 1))
+    return {l: np.asarray(v) for l, v in by_l.items()}
+
+
+def _rotate_atoms(coeffs: np.ndarray, dms: list, columns: dict, inverse: bool = False) -> np.ndarray:
+    """Rotate every (2l+1)-block of ``coeffs [atoms, n]`` by the atoms' ``D^l`` (``dms[l]`` ``[atoms, 2l+1, 2l+1]``)."""
+    out = np.array(coeffs, dtype=np.float64, copy=True)
+    for l, cols in columns.items():
+        d = dms[l].transpose(0, 2, 1) if inverse else dms[l]
+        out[:, cols] = np.einsum("aij,abj->abi", d, out[:, cols])
     return out
 
 
@@ -237,31 +261,39 @@ def to_local(molecule: Molecule, coeffs: np.ndarray, aux_name: str) -> dict:
     The local coefficients c' of an atom satisfy c = D(R) c' (``rotation_matrices``),
     so c' = D^T c (D is orthogonal).
     """
-    rots = water_frames(molecule)
     offs = atom_offsets(molecule, aux_name)
-    lmax = _lmax(aux_name)
-    out: dict = {}
-    for a, z in enumerate(molecule.charges):
-        blocks, n = shell_blocks(aux_name, z)
-        dm = rotation_matrices(rots[a], lmax)
-        out.setdefault(z, []).append(_rotate(coeffs[offs[a]:offs[a] + n], blocks, dm, inverse=True))
-    return {z: np.asarray(v) for z, v in out.items()}
+    charges = np.asarray(molecule.charges)
+    dms = _frame_rotations(molecule, aux_name)
+    out = {}
+    for z in sorted(set(molecule.charges)):
+        atoms = np.nonzero(charges == z)[0]
+        n = shell_blocks(aux_name, z)[1]
+        lab = np.stack([coeffs[offs[a]:offs[a] + n] for a in atoms])
+        out[z] = _rotate_atoms(lab, [d[atoms // 3] for d in dms], _block_columns(aux_name, z), inverse=True)
+    return out
+
+
+def _frame_rotations(molecule: Molecule, aux_name: str) -> list:
+    """``rotation_matrices`` of every water's frame (its three atoms share it): a list over l of
+    ``[waters, 2l+1, 2l+1]``."""
+    rots = water_frames(molecule)
+    return rotation_matrices_many(rots[0::3], _lmax(aux_name))
 
 
 def to_lab(molecule: Molecule, local: dict, aux_name: str) -> np.ndarray:
     """Inverse of :func:`to_local`."""
-    rots = water_frames(molecule)
     offs = atom_offsets(molecule, aux_name)
-    lmax = _lmax(aux_name)
+    charges = np.asarray(molecule.charges)
+    dms = _frame_rotations(molecule, aux_name)
     total = offs[-1] + shell_blocks(aux_name, molecule.charges[-1])[1]
     out = np.zeros(total)
-    seen: dict = {}
-    for a, z in enumerate(molecule.charges):
-        blocks, n = shell_blocks(aux_name, z)
-        i = seen.get(z, 0)
-        out[offs[a]:offs[a] + n] = _rotate(np.asarray(local[z][i], dtype=np.float64), blocks,
-                                           rotation_matrices(rots[a], lmax))
-        seen[z] = i + 1
+    for z in sorted(set(molecule.charges)):
+        atoms = np.nonzero(charges == z)[0]
+        n = shell_blocks(aux_name, z)[1]
+        lab = _rotate_atoms(np.asarray(local[z], dtype=np.float64)[:len(atoms)], [d[atoms // 3] for d in dms],
+                            _block_columns(aux_name, z))
+        for k, a in enumerate(atoms):
+            out[offs[a]:offs[a] + n] = lab[k]
     return out
 
 
