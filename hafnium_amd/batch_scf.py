@@ -13,8 +13,9 @@ Padded and discarded directions of the orthogonal basis get a large
 diagonal shift, so they sort above every occupied level and never mix in.
 
 Precision policy: all molecules share the arithmetic of the J/K build; it
-switches to FP64 (and resets DIIS) once every unconverged molecule has its
-gradient below ``switch_threshold`` or has stalled (``mixed_phase_stalled``).
+switches to FP64 (and resets DIIS) once at least half of the unconverged
+molecules have their gradient below ``switch_threshold`` or have stalled
+(``mixed_phase_stalled``); switching early only moves molecules to FP64.
 Convergence of a molecule (|dE| < energy_tolerance and gradient below
 gradient_tolerance) is only declared on FP64 iterations; its energy and
 iteration count are recorded at that iteration while the batch runs on.
@@ -383,11 +384,13 @@ class BatchedRhf:
             if all(r is not None for r in results):
                 break
             if precision != "fp64":
+                # Batch-wide switch once most unconverged molecules are below the threshold or have
+                # stalled at the reduced-precision floor (FP64 iterations never hurt the others).
                 pending = [i for i in range(cnt) if results[i] is None]
                 mixed = [[g for (_, g, p) in history[i] if p != "fp64"] for i in pending]
-                if all(g[-1] < options.switch_threshold
-                       or mixed_phase_stalled(g, options.stall_window, options.stall_ratio) for g in mixed) \
-                        or len(mixed[0]) >= options.max_iterations // 2:
+                ready = sum(1 for g in mixed if g[-1] < options.switch_threshold
+                            or mixed_phase_stalled(g, options.stall_window, options.stall_ratio))
+                if 2 * ready >= len(mixed) or len(mixed[0]) >= options.max_iterations // 2:
                     precision = "fp64"
                     if options.reset_diis_on_switch:
                         valid = [slot]
