@@ -418,7 +418,7 @@ def main(argv=None):
         tuning = None
         if not args.no_tune:
             tuned = autotune(jk, c, mode, cache_path=args.tune_cache)
-            tuning = {"x_bn": tuned.x_bn, "syrk_splits": tuned.syrk_splits, "x_ms": tuned.x_ms,
+            tuning = {"x_bn": tuned.x_bn, "x_bm": tuned.x_bm, "syrk_splits": tuned.syrk_splits, "x_ms": tuned.x_ms,
                       "syrk_ms": tuned.syrk_ms, "seconds": round(tuned.seconds, 2)}
             log(f"{mode}: tuned {tuning}")
         for _ in range(args.warmup):
@@ -474,7 +474,11 @@ def main(argv=None):
         achieved = x_flops / (x_ms * 1e-3) / 1e12
         peak = PEAK_TFLOPS[head]
         tile = (main_entry.get("tuning") or {}).get("x_bn") or (256 if o > 128 else 128)
-        kernel = {"fp64": f"xform_kernel<double, {tile}>", "fp32": f"xform_kernel<float, {tile}>",
+        rows = (main_entry.get("tuning") or {}).get("x_bm") or 64
+        if tile > 128:
+            rows = 64
+        w2 = "_w2" if (tile, rows) == (128, 128) else ""  # the FP64 128 x 128 tile's two-wave variant
+        kernel = {"fp64": f"xform_kernel{w2}<double, {tile}, {rows}>", "fp32": f"xform_kernel<float, {tile}, {rows}>",
                   "bf16x3": f"xform_bf16x3_kernel<{tile}>"}[head]
         roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved, "peak": peak,
                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(kernel, n, q, o),

@@ -23,6 +23,7 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 TUNE_X_BN = 1
 TUNE_X16_BN = 2
 TUNE_SYRK_SPLITS = 3
+TUNE_X_BM = 4
 
 OK = 0
 ERR_INPUT = 1

@@ -1,8 +1,9 @@
 """Autotuner of the exchange kernels' tile schedule.
 
 For a bound tensor, an occupied width and a precision it times every
-candidate column tile of the half-transform and every candidate SYRK split
-count with the library's HIP-event stage timers, keeps the fastest of each
+candidate tile of the half-transform (column tile, and for FP64/FP32 the
+64- or 128-row tile) and every candidate SYRK split count with the
+library's HIP-event stage timers, keeps the fastest of each
 (the two kernels are independent: the split count only moves the SYRK, the
 tile only the half-transform) and leaves them set on the context. Results are
 cached in a JSON file keyed by device, precision and shape, so a later run
@@ -33,6 +34,7 @@ class TuningResult:
     syrk_ms: dict = field(default_factory=dict)
     seconds: float = 0.0
     from_cache: bool = False
+    x_bm: int = 64
 
 
 def _device_name(jk) -> str:
@@ -81,14 +83,19 @@ def autotune(jk, orbitals, precision: str = "fp64", *, repeats: int = 2, cache_p
         if cached:
             jk.set_tuning(tile_param, cached["x_bn"])
             jk.set_tuning(_lib.TUNE_SYRK_SPLITS, cached["syrk_splits"])
+            bm = int(cached.get("x_bm", 64))
+            if precision != "bf16x3":
+                jk.set_tuning(_lib.TUNE_X_BM, bm)
             return TuningResult(key, cached["x_bn"], cached["syrk_splits"], cached.get("x_ms", {}),
-                                cached.get("syrk_ms", {}), 0.0, True)
+                                cached.get("syrk_ms", {}), 0.0, True, bm)
 
     start = time.perf_counter()
     ldo = -(-nocc // 32) * 32
-    tiles = [t for t in TILE_CANDIDATES if t // 2 < ldo or t == 32]
+    tiles = [(t, 64) for t in TILE_CANDIDATES if t // 2 < ldo or t == 32]
+    if precision != "bf16x3":
+        tiles += [(t, 128) for t, _ in tiles if t <= 128]
     splits = _split_candidates(jk)
-    x_ms: dict[int, float] = {}
+    x_ms: dict = {}
     syrk_ms: dict[int, float] = {}
     jk.profile(True)
     try:
@@ -96,7 +103,9 @@ def autotune(jk, orbitals, precision: str = "fp64", *, repeats: int = 2, cache_p
         # Half-transform tile, with the default split count.
         jk.set_tuning(_lib.TUNE_SYRK_SPLITS, 0)
         for tile in tiles:
-            jk.set_tuning(tile_param, tile)
+            jk.set_tuning(tile_param, tile[0])
+            if precision != "bf16x3":
+                jk.set_tuning(_lib.TUNE_X_BM, tile[1])
             jk.exchange_occupied(orbitals, precision=precision)  # warm-up
             torch.cuda.synchronize(jk.device)
             jk.stage_times()
@@ -104,7 +113,9 @@ def autotune(jk, orbitals, precision: str = "fp64", *, repeats: int = 2, cache_p
                 jk.exchange_occupied(orbitals, precision=precision)
             x_ms[tile] = _stage_ms(jk.stage_times(), "exchange_half_transform")
         best_tile = min(x_ms, key=x_ms.get)
-        jk.set_tuning(tile_param, best_tile)
+        jk.set_tuning(tile_param, best_tile[0])
+        if precision != "bf16x3":
+            jk.set_tuning(_lib.TUNE_X_BM, best_tile[1])
         for split in splits:
             jk.set_tuning(_lib.TUNE_SYRK_SPLITS, split)
             jk.exchange_occupied(orbitals, precision=precision)
@@ -118,8 +129,10 @@ def autotune(jk, orbitals, precision: str = "fp64", *, repeats: int = 2, cache_p
         jk.set_tuning(_lib.TUNE_SYRK_SPLITS, best_split)
     finally:
         jk.profile(False)
-    result = TuningResult(key, best_tile, best_split, {str(k): v for k, v in x_ms.items()},
-                          {str(k): v for k, v in syrk_ms.items()}, time.perf_counter() - start)
+    # Tile labels: "BN" for 64-row tiles, "BNx128" for 128-row tiles.
+    label = lambda t: str(t[0]) if t[1] == 64 else f"{t[0]}x{t[1]}"  # noqa: E731
+    result = TuningResult(key, best_tile[0], best_split, {label(k): v for k, v in x_ms.items()},
+                          {str(k): v for k, v in syrk_ms.items()}, time.perf_counter() - start, False, best_tile[1])
     if cache_path:
         _store(cache_path, key, result)
     return result
@@ -134,8 +147,8 @@ def _store(path: str, key: str, result: TuningResult) -> None:
                 data = json.load(fh)
         except (OSError, ValueError):
             data = {}
-    data[key] = {"x_bn": result.x_bn, "syrk_splits": result.syrk_splits, "x_ms": result.x_ms,
-                 "syrk_ms": result.syrk_ms}
+    data[key] = {"x_bn": result.x_bn, "x_bm": result.x_bm, "syrk_splits": result.syrk_splits,
+                 "x_ms": result.x_ms, "syrk_ms": result.syrk_ms}
     tmp = path + ".tmp"
     with open(tmp, "w") as fh:
         json.dump(data, fh, indent=1)

@@ -257,7 +257,8 @@ void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double*
                                                    ctx->x16_bn, s),
                 "half transform launch");
     else
-      check_hip(hfjk::launch_half_transform(b, precision, L.p, ctx->naux, ctx->nao, cp, L.ldo, x, ctx->x_bn, s),
+      check_hip(hfjk::launch_half_transform(b, precision, L.p, ctx->naux, ctx->nao, cp, L.ldo, x, ctx->x_bn,
+                                            ctx->x_bm == 0 ? 64 : ctx->x_bm, s),
                 "half transform launch");
   }
   {
@@ -570,6 +571,11 @@ int hfjk_set_tuning(hfjk_ctx* ctx, int param, int value) {
         if (value > 65536) fail(HFJK_ERR_INPUT, "at most 65536 SYRK splits");
         ctx->syrk_splits = value;
         break;
+      case HFJK_TUNE_X_BM:
+        if (value != 0 && value != 64 && value != 128)
+          fail(HFJK_ERR_INPUT, "the half-transform row tile is 0, 64 or 128");
+        ctx->x_bm = value;
+        break;
       default:
         fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
@@ -584,6 +590,7 @@ int hfjk_get_tuning(hfjk_ctx* ctx, int param, int* value) {
       case HFJK_TUNE_X_BN: *value = ctx->x_bn; break;
       case HFJK_TUNE_X16_BN: *value = ctx->x16_bn; break;
       case HFJK_TUNE_SYRK_SPLITS: *value = ctx->syrk_splits; break;
+      case HFJK_TUNE_X_BM: *value = ctx->x_bm; break;
       default: fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
   });

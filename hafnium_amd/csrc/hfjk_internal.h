@@ -78,6 +78,7 @@ struct hfjk_ctx {
   int x_bn = 0;
   int x16_bn = 0;
   int syrk_splits = 0;
+  int x_bm = 0;  // 0 = 64
 };
 
 namespace hfjk {
@@ -134,9 +135,10 @@ hipError_t launch_coulomb_unpack(const double* partial, int nsplit, long long p,
 
 // Exchange (jk.hip).
 hipError_t launch_orbitals(const double* c, int nao, int nocc, int precision, void* out, hipStream_t s);
-// bn = column tile (32, 64, 128, 256) or 0 for the default for ldo.
+// bn = column tile (32, 64, 128, 256) or 0 for the default for ldo; bm = row
+// tile (64 or 128; 128 applies to column tiles up to 128).
 hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao,
-                                 const void* c, int ldo, void* x, int bn, hipStream_t s);
+                                 const void* c, int ldo, void* x, int bn, int bm, hipStream_t s);
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
                        int nsplit, void* partial, hipStream_t s);
 hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, int nao, double* out,

@@ -121,22 +121,26 @@ struct XShape {
 
 // ------------------------------------------------------------------ FP64 / FP32 half-transform
 
-// X[q][m][i] = sum_k B[q][m][k] C[k][i]; workgroup tile 64 rows (two layout
-// tile rows r = 2 mb, 2 mb + 1) x BN columns; one layout tile column kt per
-// k step.
-template <typename T, int BN>
+// X[q][m][i] = sum_k B[q][m][k] C[k][i]; workgroup tile BM rows (BM / 32
+// layout tile rows r = (BM / 32) mb + h) x BN columns; one layout tile
+// column kt per k step. BM = 128 doubles the MFMAs per LDS read and halves
+// the staged bytes per flop against BM = 64, at two workgroups per CU.
+template <typename T, int BN, int BM = 64>
 struct XformSmem {
-  T At[2][kTile][kTile + 1];
+  T At[BM / kTile][kTile][kTile + 1];
   T Cs[kTile][BN + 1];
 };
 
-// One 64 x BN tile (row block mb, column block nb) of X_q = B_q C for one
-// aux function: bq = its packed B, xq = its X rows. The whole workgroup
-// calls it; LDS is free again when it returns.
-template <typename T, int BN>
+// One BM x BN tile (row block mb, column block nb) of X_q = B_q C for one
+// aux function: bq = its packed B, xq = its X rows (xrows of them). The
+// whole workgroup calls it; LDS is free again when it returns.
+template <typename T, int BN, int BM = 64>
 __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, const T* __restrict__ c, int ldo,
-                                           T* __restrict__ xq, int mb, int nb, XformSmem<T, BN>& sm) {
+                                           T* __restrict__ xq, int xrows, int mb, int nb,
+                                           XformSmem<T, BN, BM>& sm) {
   typedef XShape<BN> S;
+  constexpr int NTL = BM / kTile;                  // layout tiles per row block
+  constexpr int MI = BM / S::WM / 16;              // 16-row blocks per wave
   constexpr int CCH = kTile * BN / 2 / kThreads;  // 2-element chunks of the C tile per thread
   auto& At = sm.At;
   auto& Cs = sm.Cs;
@@ -147,23 +151,23 @@ __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, con
   const int wm = wave % S::WM;
   const int wn = wave / S::WM;
 
-  typename Op<T>::acc_t acc[S::MI][S::NI];
+  typename Op<T>::acc_t acc[MI][S::NI];
 #pragma unroll
-  for (int mi = 0; mi < S::MI; ++mi)
+  for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
     for (int ni = 0; ni < S::NI; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
 
-  // Prefetch registers: 4 consecutive elements of each of the two tiles,
+  // Prefetch registers: 4 consecutive elements of each of the NTL tiles,
   // and CCH 2-element chunks of the C tile.
-  typename Op<T>::vec2 ra[2][2];
+  typename Op<T>::vec2 ra[NTL][2];
   typename Op<T>::vec2 rc[CCH];
 
   auto load = [&](int kt) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r = 2 * mb + h;
+    for (int h = 0; h < NTL; ++h) {
+      const int r = NTL * mb + h;
       if (r < nt) {
         const long long off = (r >= kt ? tidx(r, kt) : tidx(kt, r)) * kTileElems + t * 4;
         const typename Op<T>::vec2* src = (const typename Op<T>::vec2*)(bq + off);
@@ -189,7 +193,7 @@ __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, con
     const int a = (t * 4) / kTile;
     const int bc = (t * 4) % kTile;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NTL; ++h) {
       At[h][a][bc + 0] = ra[h][0].x;
       At[h][a][bc + 1] = ra[h][0].y;
       At[h][a][bc + 2] = ra[h][1].x;
@@ -212,16 +216,16 @@ __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, con
     if (kt + 1 < nt) load(kt + 1);
     // Tile h is stored as (r, kt) when r >= kt (read A[i][k] = At[h][i][k]) and
     // as (kt, r) otherwise (read A[i][k] = At[h][k][i]).
-    bool direct[2];
+    bool direct[NTL];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) direct[h] = (2 * mb + h) >= kt;
+    for (int h = 0; h < NTL; ++h) direct[h] = (NTL * mb + h) >= kt;
 #pragma unroll
     for (int kk = 0; kk < kTile; kk += 4) {
       const int k = kk + (lane >> 4);
-      T af[S::MI], bf[S::NI];
+      T af[MI], bf[S::NI];
 #pragma unroll
-      for (int mi = 0; mi < S::MI; ++mi) {
-        const int row = wm * (64 / S::WM) + mi * 16 + (lane & 15);
+      for (int mi = 0; mi < MI; ++mi) {
+        const int row = wm * (BM / S::WM) + mi * 16 + (lane & 15);
         const int h = row >> 5;
         const int i = row & 31;
         af[mi] = direct[h] ? At[h][i][k] : At[h][k][i];
@@ -229,7 +233,7 @@ __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, con
 #pragma unroll
       for (int ni = 0; ni < S::NI; ++ni) bf[ni] = Cs[k][wn * (BN / S::WN) + ni * 16 + (lane & 15)];
 #pragma unroll
-      for (int mi = 0; mi < S::MI; ++mi)
+      for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
         for (int ni = 0; ni < S::NI; ++ni) acc[mi][ni] = Op<T>::mma(af[mi], bf[ni], acc[mi][ni]);
     }
@@ -240,31 +244,46 @@ __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, con
     }
   }
 #pragma unroll
-  for (int mi = 0; mi < S::MI; ++mi)
+  for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
     for (int ni = 0; ni < S::NI; ++ni) {
       const int col = n0 + wn * (BN / S::WN) + ni * 16 + (lane & 15);
       if (col < ldo) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = mb * kBlock + wm * (64 / S::WM) + mi * 16 + Op<T>::row(lane, r);
-          xq[(long long)row * ldo + col] = acc[mi][ni][r];
+          const int row = mb * BM + wm * (BM / S::WM) + mi * 16 + Op<T>::row(lane, r);
+          if (row < xrows) xq[(long long)row * ldo + col] = acc[mi][ni][r];
         }
       }
     }
 }
 
-template <typename T, int BN>
+template <typename T, int BN, int BM>
 __global__ __launch_bounds__(kThreads) void xform_kernel(const T* __restrict__ b, long long p, int nt,
                                                           const T* __restrict__ c, int ldo, T* __restrict__ x,
                                                           int xrows, int qbase, int nblocks_n, int nblocks_m,
                                                           int nwg) {
-  __shared__ XformSmem<T, BN> sm;
+  __shared__ XformSmem<T, BN, BM> sm;
   const int logical = xcd_remap(blockIdx.x, nwg);
   const int nb = logical % nblocks_n;
   const int mb = (logical / nblocks_n) % nblocks_m;
   const int q = qbase + logical / (nblocks_n * nblocks_m);
-  xform_tile<T, BN>(b + (long long)q * p, nt, c, ldo, x + (long long)q * xrows * ldo, mb, nb, sm);
+  xform_tile<T, BN, BM>(b + (long long)q * p, nt, c, ldo, x + (long long)q * xrows * ldo, xrows, mb, nb, sm);
+}
+
+// The same kernel held to 256 registers per lane (two waves per SIMD): the
+// 128 x 128 FP64 tile needs 258 otherwise.
+template <typename T, int BN, int BM>
+__global__ __launch_bounds__(kThreads, 2) void xform_kernel_w2(const T* __restrict__ b, long long p, int nt,
+                                                                const T* __restrict__ c, int ldo, T* __restrict__ x,
+                                                                int xrows, int qbase, int nblocks_n, int nblocks_m,
+                                                                int nwg) {
+  __shared__ XformSmem<T, BN, BM> sm;
+  const int logical = xcd_remap(blockIdx.x, nwg);
+  const int nb = logical % nblocks_n;
+  const int mb = (logical / nblocks_n) % nblocks_m;
+  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  xform_tile<T, BN, BM>(b + (long long)q * p, nt, c, ldo, x + (long long)q * xrows * ldo, xrows, mb, nb, sm);
 }
 
 // ------------------------------------------------------------------ BF16x3 half-transform
@@ -758,18 +777,22 @@ __global__ __launch_bounds__(kThreads) void syrk_reduce_kernel(const T* __restri
   }
 }
 
-template <typename T, int BN>
+template <typename T, int BN, int BM, bool W2 = false>
 hipError_t launch_x_t(const T* b, long long p, int naux, int nao, const T* c, int ldo, T* x, hipStream_t s) {
   const int nt = n_tiles(nao);
   const int xrows = x_rows(nao);
   const int nbn = (ldo + BN - 1) / BN;
-  const int nbm = xrows / kBlock;
+  const int nbm = (xrows + BM - 1) / BM;
   const int per = std::max(1, (1 << 24) / (nbn * nbm));  // keep a launch below 2^32 work-items
   for (int base = 0; base < naux; base += per) {
     const int count = std::min(per, naux - base);
     const int nwg = nbn * nbm * count;
-    hipLaunchKernelGGL((xform_kernel<T, BN>), dim3((unsigned)nwg), dim3(kThreads), 0, s, b, p, nt, c, ldo, x, xrows,
-                       base, nbn, nbm, nwg);
+    if constexpr (W2)
+      hipLaunchKernelGGL((xform_kernel_w2<T, BN, BM>), dim3((unsigned)nwg), dim3(kThreads), 0, s, b, p, nt, c, ldo,
+                         x, xrows, base, nbn, nbm, nwg);
+    else
+      hipLaunchKernelGGL((xform_kernel<T, BN, BM>), dim3((unsigned)nwg), dim3(kThreads), 0, s, b, p, nt, c, ldo, x,
+                         xrows, base, nbn, nbm, nwg);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
@@ -823,25 +846,28 @@ static int pick_bn(int bn, int ldo) {
   return ldo <= 32 ? 32 : ldo <= 64 ? 64 : ldo <= 128 ? 128 : 256;
 }
 
-hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao, const void* c, int ldo,
-                                 void* x, int bn, hipStream_t s) {
-  bn = pick_bn(bn, ldo);
-  if (precision == HFJK_FP64) {
-    const double* bb = (const double*)b;
-    const double* cc = (const double*)c;
-    double* xx = (double*)x;
-    if (bn == 32) return launch_x_t<double, 32>(bb, p, naux, nao, cc, ldo, xx, s);
-    if (bn == 64) return launch_x_t<double, 64>(bb, p, naux, nao, cc, ldo, xx, s);
-    if (bn == 128) return launch_x_t<double, 128>(bb, p, naux, nao, cc, ldo, xx, s);
-    return launch_x_t<double, 256>(bb, p, naux, nao, cc, ldo, xx, s);
+template <typename T>
+hipError_t launch_x_typed(const T* b, long long p, int naux, int nao, const T* c, int ldo, T* x, int bn, int bm,
+                          hipStream_t s) {
+  // 128-row tiles for column tiles up to 128 (a 128 x 256 tile would not fit the register file).
+  if (bm == 128 && bn <= 128) {
+    if (bn == 32) return launch_x_t<T, 32, 128>(b, p, naux, nao, c, ldo, x, s);
+    if (bn == 64) return launch_x_t<T, 64, 128>(b, p, naux, nao, c, ldo, x, s);
+    // FP64 128 x 128 is held to two waves per SIMD (see xform_kernel_w2).
+    return launch_x_t<T, 128, 128, sizeof(T) == 8>(b, p, naux, nao, c, ldo, x, s);
   }
-  const float* bb = (const float*)b;
-  const float* cc = (const float*)c;
-  float* xx = (float*)x;
-  if (bn == 32) return launch_x_t<float, 32>(bb, p, naux, nao, cc, ldo, xx, s);
-  if (bn == 64) return launch_x_t<float, 64>(bb, p, naux, nao, cc, ldo, xx, s);
-  if (bn == 128) return launch_x_t<float, 128>(bb, p, naux, nao, cc, ldo, xx, s);
-  return launch_x_t<float, 256>(bb, p, naux, nao, cc, ldo, xx, s);
+  if (bn == 32) return launch_x_t<T, 32, 64>(b, p, naux, nao, c, ldo, x, s);
+  if (bn == 64) return launch_x_t<T, 64, 64>(b, p, naux, nao, c, ldo, x, s);
+  if (bn == 128) return launch_x_t<T, 128, 64>(b, p, naux, nao, c, ldo, x, s);
+  return launch_x_t<T, 256, 64>(b, p, naux, nao, c, ldo, x, s);
+}
+
+hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao, const void* c, int ldo,
+                                 void* x, int bn, int bm, hipStream_t s) {
+  bn = pick_bn(bn, ldo);
+  if (precision == HFJK_FP64)
+    return launch_x_typed<double>((const double*)b, p, naux, nao, (const double*)c, ldo, (double*)x, bn, bm, s);
+  return launch_x_typed<float>((const float*)b, p, naux, nao, (const float*)c, ldo, (float*)x, bn, bm, s);
 }
 
 hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
@@ -918,7 +944,7 @@ __global__ __launch_bounds__(kThreads) void batch_xform_kernel(BatchView v, cons
     const int mb = (int)((local / in.nbn) % in.nbm);
     const int q = (int)(local / ((long long)in.nbn * in.nbm));
     xform_tile<T, BN>(b + in.b_off + (long long)q * in.p, in.nt, cbase + in.cp_off, in.ldo,
-                      xbase + in.x_off + (long long)q * in.xrows * in.ldo, mb, nb, sm);
+                      xbase + in.x_off + (long long)q * in.xrows * in.ldo, in.xrows, mb, nb, sm);
   }
 }
 

@@ -243,12 +243,15 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * HFJK_TUNE_X_BN: column tile of the FP64/FP32 half-transform (32, 64, 128, 256).
  * HFJK_TUNE_X16_BN: column tile of the BF16x3 half-transform (32, 64, 128, 256).
  * HFJK_TUNE_SYRK_SPLITS: aux-slice splits of the SYRK (rounded up to a multiple of 8).
+ * HFJK_TUNE_X_BM: row tile of the FP64/FP32 half-transform (64, 128; 128 is
+ * used with column tiles up to 128).
  * Set by the project's autotuner (hafnium_amd/autotune.py); the workspace size
  * depends on HFJK_TUNE_SYRK_SPLITS, so query it after tuning. No counterpart
  * in the reference. */
 #define HFJK_TUNE_X_BN 1
 #define HFJK_TUNE_X16_BN 2
 #define HFJK_TUNE_SYRK_SPLITS 3
+#define HFJK_TUNE_X_BM 4
 HFJK_API int hfjk_set_tuning(hfjk_ctx *ctx, int param, int value);
 HFJK_API int hfjk_get_tuning(hfjk_ctx *ctx, int param, int *value);
 

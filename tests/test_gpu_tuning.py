@@ -33,6 +33,25 @@ def test_all_tiles_and_splits_agree_with_the_oracle(precision):
         jk.set_tuning(param, 48)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("nao", [33, 150, 257])
+def test_row_tiles_agree_with_the_oracle(precision, nao):
+    """128-row half-transform tiles, including row blocks that stick out past the padded X rows."""
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    b, c, _ = synthetic.jk_inputs(91, 7, nao, min(nao, 70))
+    expect = jk_ref.exchange_occupied(b, c)
+    jk = MI355XCoulombExchange(b, precision=precision)
+    for rows in (64, 128):
+        for tile in (32, 64, 128, 256):
+            jk.set_tuning(_lib.TUNE_X_BM, rows)
+            jk.set_tuning(_lib.TUNE_X_BN, tile)
+            k = jk.exchange_occupied(c)
+            assert jk_ref.max_relative_error(k, expect) < TOL[precision], (rows, tile)
+    with pytest.raises(ValueError):
+        jk.set_tuning(_lib.TUNE_X_BM, 96)
+
+
 def test_autotune_sets_a_valid_schedule(tmp_path):
     from hafnium_amd.autotune import autotune
     from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
