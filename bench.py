@@ -99,10 +99,29 @@ def cpu_baseline(jk, wl, slice_q):
         blas_name = f"{blas.get('name')} {blas.get('version')}"
     except Exception:  # pragma: no cover - informational only
         blas_name = "unknown"
+    threads = cores
+    try:
+        from threadpoolctl import threadpool_info
+
+        pools = [p for p in threadpool_info() if p.get("user_api") == "blas"]
+        if pools:
+            threads = int(pools[0]["num_threads"])
+    except Exception:  # pragma: no cover - informational only
+        pass
+    cpu_model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:  # pragma: no cover - informational only
+        pass
     return {
         "value": 1.0 / t_full,
         "unit": "builds/s",
-        "cores": cores,
+        "cores": threads,
+        "cores_available": cores,
         "kind": "port",
         "sample": (f"density-form J+K (reference formula, NumPy oracle) on aux functions 0..{slice_q - 1} "
                    f"of the same seeded tensor, 1 warm-up + median of 2, scaled by naux/{slice_q}; "
@@ -110,7 +129,7 @@ def cpu_baseline(jk, wl, slice_q):
         "seconds_per_build": t_full,
         "occupied_form_seconds_per_build": t_occ,
         "blas": blas_name,
-        "cpu": platform.processor() or platform.machine(),
+        "cpu": cpu_model,
         "numpy": np.__version__,
     }
 
@@ -224,6 +243,9 @@ def bench_clusters(args, dev):
         if model is not None:
             runs += [("learned_fp64", "fp64", True), ("learned_mixed", args.cluster_mixed, True)]
         for label, precision, learned in runs:
+            two_body = system.hamiltonian.two_body
+            two_body.profile(True)  # HIP events around every J/K stage (no host synchronisation)
+            two_body.stage_times()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             start, info = guess, None
@@ -231,8 +253,11 @@ def bench_clusters(args, dev):
                 start, info = lg.learned_guess(system, model, guess, precision=precision)
             result, _ = run_cluster(n, 100 + n, system=system, guess=start, precision=precision)
             torch.cuda.synchronize()
-            entry[label] = {"seconds": time.perf_counter() - t0, "iterations": result.n_iterations,
-                            "energy": result.energy,
+            seconds = time.perf_counter() - t0
+            jk_ms = sum(ms for _, ms in two_body.stage_times())
+            two_body.profile(False)
+            entry[label] = {"seconds": seconds, "iterations": result.n_iterations,
+                            "energy": result.energy, "jk_share": jk_ms * 1e-3 / seconds,
                             "mixed_iterations": sum(1 for r in result.history if r.precision != "fp64")}
             if info is not None:
                 entry[label]["guess"] = info["used"]
@@ -336,6 +361,15 @@ def bench_scf_batch(args, dev):
             entry["learned_starts"] = info["learned"]
             entry["fallbacks"] = info["fallback"]
             entry["guess_seconds"] = info["seconds"]
+        if label == "batched_fp64":
+            # One more (untimed) run with a device synchronisation after every stage: the split of an
+            # iteration between the batched J/K ("fock"), the eigensolver and the rest.
+            prof = {}
+            batch.run(atomic_start(batch, cache, precision),
+                      ScfOptions(precision=precision, energy_tolerance=1e-8, gradient_tolerance=1e-6), profile=prof)
+            its = max(r.iterations for r in results)
+            entry["stage_ms_per_iteration"] = {k: round(1e3 * v / its, 3) for k, v in prof.items()}
+            entry["jk_share"] = prof.get("fock", 0.0) / max(sum(prof.values()), 1e-12)
         out[label] = entry
         log(f"{label}: {entry}")
     del batch
