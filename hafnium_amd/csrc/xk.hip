@@ -337,18 +337,17 @@ __device__ __forceinline__ void xform16_tile(const uint16_t* __restrict__ hq, co
   uint2 rah[4], ral[4];
   uint4 rch[CCH], rcl[CCH];
 
-  // Loads are unconditional (from a valid clamped address) and masked by a
-  // select, which keeps the prefetch arrays in registers.
+  // Loads are unconditional (from a valid clamped address); the masks (bitwise,
+  // which keeps the prefetch arrays in registers) are applied in store(), so
+  // the wait for the loads falls at the end of the k step, not at its start.
+  const bool own = r_own < nt;
   auto load = [&](int kt) {
-    const bool own = r_own < nt;
     const int rc = own ? r_own : 0;
     const long long base = (rc >= kt ? tidx(rc, kt) : tidx(kt, rc)) * kTileElems + a0 * kTile + b0;
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
-      const uint2 h = *(const uint2*)(hq + base + rr * kTile);
-      const uint2 l = *(const uint2*)(lq + base + rr * kTile);
-      rah[rr] = mask2(own, h);
-      ral[rr] = mask2(own, l);
+      rah[rr] = *(const uint2*)(hq + base + rr * kTile);
+      ral[rr] = *(const uint2*)(lq + base + rr * kTile);
     }
 #pragma unroll
     for (int v = 0; v < CCH; ++v) {
@@ -358,13 +357,23 @@ __device__ __forceinline__ void xform16_tile(const uint16_t* __restrict__ hq, co
       const int col = n0 + jj;
       const bool ok = chunk < BN * 4 && col < ldo;
       const long long off = ok ? (long long)col * npad + kt * kTile + k8 : 0;
-      const uint4 h = *(const uint4*)(cthi + off);
-      const uint4 l = *(const uint4*)(ctlo + off);
-      rch[v] = mask4(ok, h);
-      rcl[v] = mask4(ok, l);
+      rch[v] = *(const uint4*)(cthi + off);
+      rcl[v] = *(const uint4*)(ctlo + off);
     }
   };
   auto store = [&](int kt) {
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      rah[rr] = mask2(own, rah[rr]);
+      ral[rr] = mask2(own, ral[rr]);
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;
+      const bool ok = chunk < BN * 4 && n0 + (chunk >> 2) < ldo;
+      rch[v] = mask4(ok, rch[v]);
+      rcl[v] = mask4(ok, rcl[v]);
+    }
     const bool direct = r_own >= kt;  // also true for r_own >= nt (zeros)
     if (direct) {
       // element (a, b) is B[m = r*32 + a][k = kt*32 + b]
@@ -520,7 +529,11 @@ __device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int q
       for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
 
   // 128 rows x 32 columns per operand = 2048 2-element chunks: 8 per thread.
+  // The prefetch only copies loaded values into registers; the row masks and
+  // the column signs are applied in store(), so the wait for the loads falls
+  // at the end of the k step, behind its MFMAs, not at its start.
   typename Op<T>::vec2 ra[8], rb[8];
+  T sg[2];
   const int nchunk = ldo / kTile;
   const int mrows = min(kSyrkTile, xrows - mb * kSyrkTile);
   const int nrows = min(kSyrkTile, xrows - nb * kSyrkTile);
@@ -532,18 +545,12 @@ __device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int q
       const int chunk = t + kThreads * v;
       const int row = chunk >> 4;
       const int c2 = (chunk & 15) * 2;
-      const bool okm = row < mrows;
-      const bool okn = row < nrows;
-      const typename Op<T>::vec2 a = *(const typename Op<T>::vec2*)(xm + (long long)(okm ? row : 0) * ldo + c2);
-      const typename Op<T>::vec2 bb = *(const typename Op<T>::vec2*)(xn + (long long)(okn ? row : 0) * ldo + c2);
-      const T sm = okm ? (T)1 : (T)0;
-      const T sn0 = okn ? (T)signs[i0 + c2] : (T)0;
-      const T sn1 = okn ? (T)signs[i0 + c2 + 1] : (T)0;
-      ra[v].x = a.x * sm;
-      ra[v].y = a.y * sm;
-      rb[v].x = bb.x * sn0;
-      rb[v].y = bb.y * sn1;
+      ra[v] = *(const typename Op<T>::vec2*)(xm + (long long)(row < mrows ? row : 0) * ldo + c2);
+      rb[v] = *(const typename Op<T>::vec2*)(xn + (long long)(row < nrows ? row : 0) * ldo + c2);
     }
+    // Every chunk of a thread has the same column pair c2 = (t & 15) * 2.
+    sg[0] = (T)signs[i0 + (t & 15) * 2];
+    sg[1] = (T)signs[i0 + (t & 15) * 2 + 1];
   };
   auto store = [&]() {
 #pragma unroll
@@ -551,10 +558,12 @@ __device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int q
       const int chunk = t + kThreads * v;
       const int row = chunk >> 4;
       const int c2 = (chunk & 15) * 2;
-      As[row][c2] = ra[v].x;
-      As[row][c2 + 1] = ra[v].y;
-      Bs[row][c2] = rb[v].x;
-      Bs[row][c2 + 1] = rb[v].y;
+      const T sm = row < mrows ? (T)1 : (T)0;
+      const T sn = row < nrows ? (T)1 : (T)0;
+      As[row][c2] = ra[v].x * sm;
+      As[row][c2 + 1] = ra[v].y * sm;
+      Bs[row][c2] = rb[v].x * (sn * sg[0]);
+      Bs[row][c2 + 1] = rb[v].y * (sn * sg[1]);
     }
   };
 
@@ -642,34 +651,43 @@ __device__ __forceinline__ void syrk16_tile(const uint16_t* __restrict__ xhi, co
     for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
 
   // 128 rows x 32 bf16 per plane = 512 16-byte chunks: 2 per thread per plane.
+  // As in the FP64/FP32 SYRK, the prefetch only copies: row masks and sign
+  // flips are applied in store(), at the end of the k step.
   uint4 rah[2], ral[2], rbh[2], rbl[2];
+  double rsg[8];
   const int nchunk = ldo / kTile;
   const int mrows = min(kSyrkTile, xrows - mb * kSyrkTile);
   const int nrows = min(kSyrkTile, xrows - nb * kSyrkTile);
+  // Every chunk of a thread has the same column block c8 = (t & 3) * 8.
+  const int c8 = (t & 3) * 8;
   auto load = [&](int q, int i0) {
     const long long om = ((long long)q * xrows + mb * kSyrkTile) * ldo + i0;
     const long long on = ((long long)q * xrows + nb * kSyrkTile) * ldo + i0;
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
-      const int chunk = t + kThreads * v;
-      const int row = chunk >> 2;
-      const int c8 = (chunk & 3) * 8;
+      const int row = (t + kThreads * v) >> 2;
+      const long long am = om + (long long)(row < mrows ? row : 0) * ldo + c8;
+      const long long an = on + (long long)(row < nrows ? row : 0) * ldo + c8;
+      rah[v] = *(const uint4*)(xhi + am);
+      ral[v] = *(const uint4*)(xlo + am);
+      rbh[v] = *(const uint4*)(xhi + an);
+      rbl[v] = *(const uint4*)(xlo + an);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) rsg[e] = signs[i0 + c8 + e];
+  };
+  auto store = [&]() {
+    // A negative sign flips the sign bit of the element in both planes.
+    const uint32_t f0 = (rsg[0] < 0.0 ? 0x8000u : 0u) | (rsg[1] < 0.0 ? 0x80000000u : 0u);
+    const uint32_t f1 = (rsg[2] < 0.0 ? 0x8000u : 0u) | (rsg[3] < 0.0 ? 0x80000000u : 0u);
+    const uint32_t f2 = (rsg[4] < 0.0 ? 0x8000u : 0u) | (rsg[5] < 0.0 ? 0x80000000u : 0u);
+    const uint32_t f3 = (rsg[6] < 0.0 ? 0x8000u : 0u) | (rsg[7] < 0.0 ? 0x80000000u : 0u);
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int row = (t + kThreads * v) >> 2;
       const bool okm = row < mrows;
       const bool okn = row < nrows;
-      const long long am = om + (long long)(okm ? row : 0) * ldo + c8;
-      const long long an = on + (long long)(okn ? row : 0) * ldo + c8;
-      const uint4 mh = *(const uint4*)(xhi + am);
-      const uint4 ml = *(const uint4*)(xlo + am);
-      uint4 h = *(const uint4*)(xhi + an);
-      uint4 l = *(const uint4*)(xlo + an);
-      rah[v] = mask4(okm, mh);
-      ral[v] = mask4(okm, ml);
-      // A negative sign flips the sign bit of the element in both planes.
-      const double* sg = signs + i0 + c8;
-      const uint32_t f0 = (sg[0] < 0.0 ? 0x8000u : 0u) | (sg[1] < 0.0 ? 0x80000000u : 0u);
-      const uint32_t f1 = (sg[2] < 0.0 ? 0x8000u : 0u) | (sg[3] < 0.0 ? 0x80000000u : 0u);
-      const uint32_t f2 = (sg[4] < 0.0 ? 0x8000u : 0u) | (sg[5] < 0.0 ? 0x80000000u : 0u);
-      const uint32_t f3 = (sg[6] < 0.0 ? 0x8000u : 0u) | (sg[7] < 0.0 ? 0x80000000u : 0u);
+      uint4 h = rbh[v], l = rbl[v];
       h.x ^= f0;
       h.y ^= f1;
       h.z ^= f2;
@@ -678,20 +696,10 @@ __device__ __forceinline__ void syrk16_tile(const uint16_t* __restrict__ xhi, co
       l.y ^= f1;
       l.z ^= f2;
       l.w ^= f3;
-      rbh[v] = mask4(okn, h);
-      rbl[v] = mask4(okn, l);
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int v = 0; v < 2; ++v) {
-      const int chunk = t + kThreads * v;
-      const int row = chunk >> 2;
-      const int c8 = (chunk & 3) * 8;
-      *(uint4*)&Ah[row][c8] = rah[v];
-      *(uint4*)&Al[row][c8] = ral[v];
-      *(uint4*)&Bh[row][c8] = rbh[v];
-      *(uint4*)&Bl[row][c8] = rbl[v];
+      *(uint4*)&Ah[row][c8] = mask4(okm, rah[v]);
+      *(uint4*)&Al[row][c8] = mask4(okm, ral[v]);
+      *(uint4*)&Bh[row][c8] = mask4(okn, h);
+      *(uint4*)&Bl[row][c8] = mask4(okn, l);
     }
   };
 
