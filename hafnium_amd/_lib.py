@@ -24,6 +24,7 @@ TUNE_X_BN = 1
 TUNE_X16_BN = 2
 TUNE_SYRK_SPLITS = 3
 TUNE_X_BM = 4
+TUNE_SYRK_KIND = 5
 
 OK = 0
 ERR_INPUT = 1

@@ -263,7 +263,8 @@ void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double*
   }
   {
     Stage st(ctx, s, "exchange_syrk");
-    check_hip(hfjk::launch_syrk(x, precision, ctx->naux, ctx->nao, L.ldo, sg, L.nsplit_k, part, s), "syrk launch");
+    check_hip(hfjk::launch_syrk(x, precision, ctx->naux, ctx->nao, L.ldo, sg, signs != nullptr, L.nsplit_k, part,
+                                ctx->syrk_kind, s), "syrk launch");
   }
   {
     Stage st(ctx, s, "exchange_reduce");
@@ -576,6 +577,10 @@ int hfjk_set_tuning(hfjk_ctx* ctx, int param, int value) {
           fail(HFJK_ERR_INPUT, "the half-transform row tile is 0, 64 or 128");
         ctx->x_bm = value;
         break;
+      case HFJK_TUNE_SYRK_KIND:
+        if (value > 2) fail(HFJK_ERR_INPUT, "the SYRK kernel kind is 0, 1 or 2");
+        ctx->syrk_kind = value;
+        break;
       default:
         fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
@@ -591,6 +596,7 @@ int hfjk_get_tuning(hfjk_ctx* ctx, int param, int* value) {
       case HFJK_TUNE_X16_BN: *value = ctx->x16_bn; break;
       case HFJK_TUNE_SYRK_SPLITS: *value = ctx->syrk_splits; break;
       case HFJK_TUNE_X_BM: *value = ctx->x_bm; break;
+      case HFJK_TUNE_SYRK_KIND: *value = ctx->syrk_kind; break;
       default: fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
   });

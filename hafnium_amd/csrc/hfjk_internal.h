@@ -79,6 +79,7 @@ struct hfjk_ctx {
   int x16_bn = 0;
   int syrk_splits = 0;
   int x_bm = 0;  // 0 = 64
+  int syrk_kind = 0;  // 0 = default, 1 = register-staged, 2 = direct LDS loads
 };
 
 namespace hfjk {
@@ -139,8 +140,12 @@ hipError_t launch_orbitals(const double* c, int nao, int nocc, int precision, vo
 // tile (64 or 128; 128 applies to column tiles up to 128).
 hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao,
                                  const void* c, int ldo, void* x, int bn, int bm, hipStream_t s);
+// kind: 0 = default, 1 = register-staged FP64/FP32 kernel, 2 = direct LDS
+// loads (FP64/FP32 with ldo <= 2048; otherwise the register-staged kernel runs).
+// is_signed = false promises signs[i] = 1 for every occupied column, which
+// lets the direct-LDS kernel skip them.
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
-                       int nsplit, void* partial, hipStream_t s);
+                       bool is_signed, int nsplit, void* partial, int kind, hipStream_t s);
 hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, int nao, double* out,
                               hipStream_t s);
 hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* signs_out, hipStream_t s);

@@ -623,6 +623,198 @@ __global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x,
                partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile), sm);
 }
 
+// ---------------------------------------------- SYRK over direct LDS loads
+//
+// FP64 / FP32 SYRK whose operand rows go from global memory straight into
+// LDS (global_load_lds_dwordx4, gfx950), through a ring of kDmaStages stages:
+// the loads of the next two k steps are in flight while one step computes
+// and the first operands of the following step are read, no registers
+// stage them, and each k step has one barrier.
+//
+// A stage holds kDmaK consecutive k of the 128 rows of both operands as
+// 1 KB chunks (one load instruction of one wave; lane l fills bytes
+// [16 l, 16 l + 16)): chunk = 16 rows x 4 groups of 16 bytes along k, lane l
+// carrying row l % 16 and group l / 16. The MFMA side reads a chunk back
+// with one 16-byte LDS read per lane, lane l again getting row l % 16, group
+// l / 16 (a contiguous, conflict-free 1 KB read). Element e of the group
+// feeds MFMA e: that MFMA sums over k = E (l / 16) + e, a permutation of the
+// chunk's k that both operands share, so the E MFMAs together sum all of it.
+constexpr int kDmaStages = 4;
+constexpr int kDmaMaxLdo = 2048;
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+template <typename T>
+struct SyrkDma {
+  static constexpr int E = 16 / (int)sizeof(T);       // elements per 16-byte group
+  static constexpr int CHUNK = 1024 / (int)sizeof(T);  // elements per 1 KB chunk
+  static constexpr int KC = 4 * E;                      // k per chunk
+  static constexpr int KS = 2 * KC;                     // k per stage (two chunks per 16-row block)
+  static constexpr int STAGE = 32 * CHUNK;              // 2 operands x 8 row blocks x 2 chunks
+  typedef T unit __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+
+// s_waitcnt vmcnt(VM) lgkmcnt(0) (gfx9 encoding: vmcnt in bits 3:0 and 15:14).
+template <int VM>
+__device__ __forceinline__ void wait_vm_lgkm0() {
+  static_assert(VM >= 0 && VM < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((VM & 15) | ((VM >> 4) << 14) | (0x7 << 4));
+}
+
+// Workgroup barrier without a memory fence (a fence would wait for every
+// load in flight, the stages ahead included). The clobber keeps the compiler
+// from moving memory accesses across it.
+__device__ __forceinline__ void barrier_nofence() { asm volatile("s_barrier" ::: "memory"); }
+
+// SIGNED = false: every sign is +1 on the occupied columns (the padded
+// columns of X are zero), so the sign table is neither built nor applied.
+template <typename T, bool SIGNED>
+__global__ __launch_bounds__(kThreads, 1) void syrk_dma_kernel(const T* __restrict__ x, int naux, int xrows, int ldo,
+                                                               const double* __restrict__ signs, int nsplit,
+                                                               int npairs, T* __restrict__ partial) {
+  typedef SyrkDma<T> D;
+  typedef typename D::unit unit;
+  constexpr int E = D::E;
+  __shared__ __attribute__((aligned(16))) T st0[D::STAGE];
+  __shared__ __attribute__((aligned(16))) T st1[D::STAGE];
+  __shared__ __attribute__((aligned(16))) T st2[D::STAGE];
+  __shared__ __attribute__((aligned(16))) T st3[D::STAGE];
+  __shared__ __attribute__((aligned(16))) T sgn[kDmaMaxLdo];
+
+  int s, pair;
+  syrk_coords(blockIdx.x, npairs, s, pair);
+  int mb, nb;
+  tile_of(pair, mb, nb);
+  const int qb = (int)((long long)s * naux / nsplit);
+  const int qe = (int)((long long)(s + 1) * naux / nsplit);
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+
+  if (SIGNED)
+    for (int i = t; i < ldo; i += kThreads) sgn[i] = (T)signs[i];
+
+  // Byte offsets (within one aux function's X, from k = i0) of this lane's
+  // group in the wave's 8 chunks: chunks 8 wave .. 8 wave + 7 of the 32 of a
+  // stage (0-15 operand A = row block mb, 16-31 operand B = row block nb;
+  // chunk = 2 * (16-row block) + k half). Rows past xrows (the last 128-row
+  // block may hold only 64) read row 0 instead: their products land in K
+  // rows and columns past nao, which are never read.
+  int off[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const int ch = wave * 8 + v;
+    const int blk = (ch >> 4) ? nb : mb;
+    int row = blk * kSyrkTile + ((ch >> 1) & 7) * 16 + (lane & 15);
+    row = row < xrows ? row : 0;
+    off[v] = (row * ldo + (ch & 1) * D::KC + (lane >> 4) * E) * (int)sizeof(T);
+  }
+  const int nch = ldo / D::KS;
+  const int steps = (qe - qb) * nch;
+  auto issue = [&](int step, T* stage) {
+    const int qq = step / nch;
+    const T* xq = x + (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * D::KS;
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+      __builtin_amdgcn_global_load_lds((const void*)((const char*)xq + (unsigned)off[v]),
+                                       (lds_void_t*)(stage + (wave * 8 + v) * D::CHUNK), 16, 0, 0);
+  };
+
+  typename Op<T>::acc_t acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
+
+  // Operands of one chunk (16 rows x KC k per 16-row block) of a stage, with
+  // the column signs applied to the B side.
+  auto read_chunk = [&](const T* stage, int step, int c, unit (&a)[4], unit (&b)[4]) {
+    const int i0 = (step % nch) * D::KS;
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) a[mi] = *(const unit*)(stage + ((((wm * 4 + mi) * 2 + c) * 64 + lane) * E));
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+      b[ni] = *(const unit*)(stage + D::STAGE / 2 + ((((wn * 4 + ni) * 2 + c) * 64 + lane) * E));
+    if (SIGNED) {
+      const unit sg = *(const unit*)(sgn + i0 + c * D::KC + (lane >> 4) * E);
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) b[ni] *= sg;
+    }
+  };
+  auto mma_chunk = [&](const unit (&a)[4], const unit (&b)[4]) {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = Op<T>::mma(a[mi][e], b[ni][e], acc[mi][ni]);
+  };
+
+  // Step k: wait until this wave's loads of stage k + 1 have landed (those
+  // of stage k + 2 stay in flight) and every LDS read issued so far has
+  // returned; after the barrier every wave's loads of stages k and k + 1 are
+  // in LDS and nobody reads stage k - 1 any more, so its buffer takes the
+  // loads of step k + 3. The first chunk of stage k is already in registers
+  // (read during step k - 1); the step reads its second chunk, runs the
+  // first chunk's MFMAs, reads the first chunk of stage k + 1, then runs the
+  // second chunk's MFMAs, so no MFMA waits for an LDS read after the barrier.
+  // Past the last step the loads repeat the last stage into buffers nobody
+  // reads and the MFMAs are skipped, so there are only the seven load sites
+  // below, each naming its stage buffer statically: the compiler then tells
+  // the stages being read from the ones being loaded and adds no wait of its
+  // own.
+  unit a0[4], b0[4], a1[4], b1[4];
+  auto step_body = [&](int k, const T* cur, const T* next, T* free) {
+    // Keep the previous step's MFMAs ahead of the wait: issued before it,
+    // they keep the matrix pipe busy while the wave waits at the barrier.
+    __builtin_amdgcn_sched_barrier(0);
+    wait_vm_lgkm0<8>();
+    barrier_nofence();
+    issue(min(k + 3, steps - 1), free);
+    if (k < steps) {
+      read_chunk(cur, k, 1, a1, b1);
+      mma_chunk(a0, b0);
+      read_chunk(next, k + 1, 0, a0, b0);
+      mma_chunk(a1, b1);
+    }
+  };
+
+  __syncthreads();  // sign table written
+  if (steps > 0) {
+    issue(0, st0);
+    issue(min(1, steps - 1), st1);
+    issue(min(2, steps - 1), st2);
+    wait_vm_lgkm0<16>();
+    barrier_nofence();
+    read_chunk(st0, 0, 0, a0, b0);
+    for (int k = 0; k < steps; k += kDmaStages) {
+      step_body(k, st0, st1, st3);
+      step_body(k + 1, st1, st2, st0);
+      step_body(k + 2, st2, st3, st1);
+      step_body(k + 3, st3, st0, st2);
+    }
+    // No load may still be writing this workgroup's LDS when it exits.
+    wait_vm_lgkm0<0>();
+  }
+
+  T* dst = partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile);
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const int col = wn * 64 + ni * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = wm * 64 + mi * 16 + Op<T>::row(lane, r);
+        dst[row * kSyrkTile + col] = acc[mi][ni][r];
+      }
+    }
+}
+
 struct Syrk16Smem {
   uint16_t Ah[kSyrkTile][kRowK];
   uint16_t Al[kSyrkTile][kRowK];
@@ -895,12 +1087,28 @@ hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux
   return launch_x16_t<256>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
 }
 
-hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs, int nsplit,
-                       void* partial, hipStream_t s) {
+hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
+                       bool is_signed, int nsplit, void* partial, int kind, hipStream_t s) {
   const int xrows = x_rows(nao);
   const int npairs = syrk_pairs(nao);
   const unsigned grid = (unsigned)(npairs * nsplit);
-  if (precision == HFJK_FP64)
+  // The direct-load kernel keeps per-lane element offsets within one aux
+  // function's X in 32 bits and the sign table in LDS.
+  const bool dma = kind != 1 && precision != HFJK_BF16X3 && ldo <= kDmaMaxLdo &&
+                   (long long)xrows * ldo < (1LL << 31);
+  if (dma && precision == HFJK_FP64 && is_signed)
+    hipLaunchKernelGGL((syrk_dma_kernel<double, true>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
+                       xrows, ldo, signs, nsplit, npairs, (double*)partial);
+  else if (dma && precision == HFJK_FP64)
+    hipLaunchKernelGGL((syrk_dma_kernel<double, false>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
+                       xrows, ldo, signs, nsplit, npairs, (double*)partial);
+  else if (dma && is_signed)
+    hipLaunchKernelGGL((syrk_dma_kernel<float, true>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux, xrows,
+                       ldo, signs, nsplit, npairs, (float*)partial);
+  else if (dma)
+    hipLaunchKernelGGL((syrk_dma_kernel<float, false>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux, xrows,
+                       ldo, signs, nsplit, npairs, (float*)partial);
+  else if (precision == HFJK_FP64)
     hipLaunchKernelGGL(syrk_kernel<double>, dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux, xrows, ldo,
                        signs, nsplit, npairs, (double*)partial);
   else if (precision == HFJK_FP32)

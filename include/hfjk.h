@@ -245,6 +245,10 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * HFJK_TUNE_SYRK_SPLITS: aux-slice splits of the SYRK (rounded up to a multiple of 8).
  * HFJK_TUNE_X_BM: row tile of the FP64/FP32 half-transform (64, 128; 128 is
  * used with column tiles up to 128).
+ * HFJK_TUNE_SYRK_KIND: FP64/FP32 SYRK kernel: 1 = operands staged through
+ * registers into one LDS buffer, 2 = operands loaded straight into a ring of
+ * LDS stages (occupied widths up to 2048); 0 = the library's choice (2 where
+ * it applies).
  * Set by the project's autotuner (hafnium_amd/autotune.py); the workspace size
  * depends on HFJK_TUNE_SYRK_SPLITS, so query it after tuning. No counterpart
  * in the reference. */
@@ -252,6 +256,7 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
 #define HFJK_TUNE_X16_BN 2
 #define HFJK_TUNE_SYRK_SPLITS 3
 #define HFJK_TUNE_X_BM 4
+#define HFJK_TUNE_SYRK_KIND 5
 HFJK_API int hfjk_set_tuning(hfjk_ctx *ctx, int param, int value);
 HFJK_API int hfjk_get_tuning(hfjk_ctx *ctx, int param, int *value);
 

@@ -66,3 +66,53 @@ def test_autotune_sets_a_valid_schedule(tmp_path):
     assert again.from_cache and (again.x_bn, again.syrk_splits) == (result.x_bn, result.syrk_splits)
     k = jk.exchange_occupied(c)
     assert jk_ref.max_relative_error(k, jk_ref.exchange_occupied(b, c)) < BF16X3_REL_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("naux,nao,nocc,splits", [
+    (9, 150, 70, 8),     # last 128-row block holds 64 rows of X
+    (5, 257, 33, 16),    # more splits than aux functions: empty splits
+    (1, 40, 3, 8),       # one step per split (prologue repeats the last stage)
+    (23, 300, 140, 8),   # several bulk iterations and a partial one
+])
+def test_syrk_kernels_agree_with_the_oracle(precision, naux, nao, nocc, splits):
+    """The register-staged and the direct-LDS SYRK give the oracle's K."""
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    b, c, _ = synthetic.jk_inputs(17, naux, nao, nocc)
+    expect = jk_ref.exchange_occupied(b, c)
+    jk = MI355XCoulombExchange(b, precision=precision)
+    jk.set_tuning(_lib.TUNE_SYRK_SPLITS, splits)
+    for kind in (1, 2):
+        jk.set_tuning(_lib.TUNE_SYRK_KIND, kind)
+        k = jk.exchange_occupied(c)
+        assert jk_ref.max_relative_error(k, expect) < TOL[precision], kind
+    with pytest.raises(ValueError):
+        jk.set_tuning(_lib.TUNE_SYRK_KIND, 3)
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_syrk_kernels_apply_signs(kind):
+    """Signed columns (density form of an indefinite matrix) through both SYRK kernels."""
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    naux, nao, nocc = 11, 130, 45
+    b, c, _ = synthetic.jk_inputs(29, naux, nao, nocc)
+    signs = np.where(np.arange(nocc) % 3 == 0, -1.0, 1.0)
+    expect = jk_ref.exchange(b, (c * signs) @ c.T)
+    jk = MI355XCoulombExchange(b, precision="fp64")
+    jk.set_tuning(_lib.TUNE_SYRK_KIND, kind)
+    k = jk.exchange_occupied(c, signs)
+    assert jk_ref.max_relative_error(k, expect) < FP64_REL_TOL
+
+
+def test_wide_occupied_block_uses_the_register_kernel():
+    """Occupied widths beyond the direct-LDS kernel's sign table still give the oracle's K."""
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    naux, nao, nocc = 2, 2100, 2050
+    b, c, _ = synthetic.jk_inputs(3, naux, nao, nocc)
+    expect = jk_ref.exchange_occupied(b, c)
+    jk = MI355XCoulombExchange(b, precision="fp64")
+    k = jk.exchange_occupied(c)
+    assert jk_ref.max_relative_error(k, expect) < FP64_REL_TOL
