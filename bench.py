@@ -2,10 +2,14 @@
 
 Workload: nao=1500, naux=5000, nocc=250, seed 1 (B[Q,m,n] = z exp(-|m-n|/20)
 from the package's seeded generator, generated directly in HBM; C_occ the Q
-factor of a seeded Gaussian; D = C C^T). One step = one J + K build:
-J = coulomb(D) (density form, two HBM passes over B) and K =
-exchange_occupied(C) (half-transform on MFMA + SYRK on MFMA). The timed
-region holds exactly `--steps` builds with every input resident in HBM.
+factor of a seeded Gaussian; D = C C^T). One step = one J + K build of D
+through the combined call jk_occupied(C) (coulomb_exchange_matrices of the
+reference for an aufbau density): half-transform X = B C on MFMA, SYRK on
+MFMA, the Coulomb charges gamma_P = sum C o X_P from X (one pass over X),
+then J = sum_P gamma_P B_P (one HBM pass over B). The timed region holds
+exactly `--steps` builds with every input resident in HBM. The same build
+as two separate calls, coulomb(D) (two passes over B) + exchange_occupied(C),
+is timed after it and reported as "separate_calls".
 
 `value` is the FP64 builds/s. The FP32-MFMA and BF16x3 modes are timed the
 same way and reported under "modes" with their errors against the FP64
@@ -456,23 +460,37 @@ def main(argv=None):
                       "syrk_ms": tuned.syrk_ms, "seconds": round(tuned.seconds, 2)}
             log(f"{mode}: tuned {tuning}")
         for _ in range(args.warmup):
-            jk.coulomb(d)
-            jk.exchange_occupied(c)
+            jk.jk_occupied(c)
         torch.cuda.synchronize()
         jk.profile(True)
         torch.cuda.synchronize()
         start = time.perf_counter()
         for _ in range(args.steps):
-            j = jk.coulomb(d)
-            k = jk.exchange_occupied(c)
+            j, k = jk.jk_occupied(c)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - start
         stages = jk.stage_times()
+        # The same build as two calls (density-form J, occupied-form K).
+        jk.coulomb(d)
+        jk.exchange_occupied(c)
+        torch.cuda.synchronize()
+        start_sep = time.perf_counter()
+        for _ in range(args.steps):
+            j_sep = jk.coulomb(d)
+            k_sep = jk.exchange_occupied(c)
+        torch.cuda.synchronize()
+        elapsed_sep = time.perf_counter() - start_sep
+        stages_sep = jk.stage_times()
         jk.profile(False)
-        per_stage = {}
-        for name, ms in stages:
-            per_stage.setdefault(name, []).append(ms)
-        avg = {name: sum(v) / len(v) for name, v in per_stage.items()}
+
+        def stage_means(stage_list):
+            per_stage = {}
+            for name, ms in stage_list:
+                per_stage.setdefault(name, []).append(ms)
+            return {name: sum(v) / len(v) for name, v in per_stage.items()}
+
+        avg = stage_means(stages)
+        avg_sep = stage_means(stages_sep)
         j_np, k_np = j.cpu().numpy(), k.cpu().numpy()
         d_np = d.cpu().numpy()
         entry = {
@@ -480,6 +498,13 @@ def main(argv=None):
             "ms_per_build": 1000.0 * elapsed / args.steps,
             "stage_ms": {k_: round(v, 4) for k_, v in avg.items()},
             "tuning": tuning,
+            "separate_calls": {
+                "builds_per_s": args.steps / elapsed_sep,
+                "ms_per_build": 1000.0 * elapsed_sep / args.steps,
+                "stage_ms": {k_: round(v, 4) for k_, v in avg_sep.items()},
+                "max_rel_diff_J": jk_ref.max_relative_error(j_sep.cpu().numpy(), j.cpu().numpy()),
+                "max_rel_diff_K": jk_ref.max_relative_error(k_sep.cpu().numpy(), k.cpu().numpy()),
+            },
         }
         if mode == "fp64":
             reference = (j_np, k_np)
@@ -528,7 +553,7 @@ def main(argv=None):
 
     batched = None
     if not args.no_batched and args.workload == "large":
-        del jk, c, d, j, k
+        del jk, c, d, j, k, j_sep, k_sep
         torch.cuda.empty_cache()
         batched = bench_batched(args, dev)
 
@@ -556,7 +581,8 @@ def main(argv=None):
         "dtype": "f64" if head == "fp64" else head,
         "data": "synthetic (seeded Philox generator, generated in HBM)",
         "config": {"workload": f"single DF-J/K build nao={n} naux={q} nocc={o} seed={wl['seed']}",
-                   "nao": n, "naux": q, "nocc": o, "step": "J (density form) + K (occupied form)"},
+                   "nao": n, "naux": q, "nocc": o,
+                   "step": "J + K of D = C C^T in one call (charges from X)"},
         "flops_per_build": f_build,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -57,6 +57,7 @@ SIGNATURES = [
     ("hfjk_coulomb_charges", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_jk_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_basis_nfunc", _c_int, [_vp, ctypes.POINTER(_c_int)]),
     ("hfjk_int1e", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
     ("hfjk_int2c2e", _c_int, [_vp, _vp]),

@@ -304,6 +304,33 @@ class MI355XCoulombExchange:
             self._stream()))
         return self._out_like(out, template)
 
+    def jk_occupied(self, orbitals: Any, *, precision: str | None = None):
+        """``(J, K)`` of ``D = C C^T`` for ``orbitals`` ``[ao, occ]`` in one call.
+
+        ``coulomb_exchange_matrices`` (coulomb_exchange.py:397-404) for an
+        aufbau density: equal to ``(coulomb(C C^T), exchange_occupied(C))``,
+        with the Coulomb charges formed from the half-transformed ``X = B C``
+        so the fitted tensor is read once less.
+        """
+        torch = _torch()
+        prec = self._precision if precision is None else _lib.precision_code(precision)
+        template = orbitals
+        if isinstance(orbitals, np.ndarray):
+            orbitals = torch.from_numpy(np.ascontiguousarray(orbitals, dtype=np.float64))
+        if orbitals.is_complex():
+            raise BackendError("complex classes are not implemented on MI355X")
+        if orbitals.ndim != 2 or orbitals.shape[0] != self._nao:
+            raise InputError(f"the orbitals have shape {tuple(orbitals.shape)}, expected [{self._nao}, occ]")
+        nocc = int(orbitals.shape[1])
+        c = orbitals.to(device=self._device, dtype=torch.float64).contiguous()
+        self._ensure_copy(prec)
+        j = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
+        k = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
+        ws, nbytes = self._workspace_for(max(nocc, 1), prec)
+        _lib.check(self._ctx.lib.hfjk_jk_occ(self._ctx.handle, _ptr(c), nocc, _ptr(j), _ptr(k), prec, ws, nbytes,
+                                             self._stream()))
+        return self._out_like(j, template), self._out_like(k, template)
+
     # --------------------------------------------------------------- tuning
 
     def set_tuning(self, param: int, value: int) -> None:

@@ -112,6 +112,10 @@ struct Layout {
   size_t cw, cgp, cg, cjp, c_end;
   // exchange region
   size_t xc, xs, xx, xp, x_end;
+  // charges of hfjk_jk_occ formed from X, after both regions: row-group
+  // parts [naux][charge_groups], gamma [naux], and an FP32 padded orbital
+  // copy for BF16X3 (whose exchange keeps only the bf16 planes of C^T)
+  size_t qt, qg, qc, q_end;
   size_t total;
   int nsplit_j, nsplit_k, ldo, xrows, npairs;
   long long p, ntiles;
@@ -157,7 +161,14 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   const size_t tile2 = (size_t)hfjk::syrk_tile() * hfjk::syrk_tile();
   L.xp = off; off = align_up(off + (size_t)L.nsplit_k * L.npairs * tile2 * es);
   L.x_end = off;
-  L.total = std::max(L.c_end, L.x_end);
+  // Past both regions: the Coulomb partials of hfjk_jk_occ reuse the
+  // exchange region while the charges are still being read.
+  off = std::max(L.c_end, L.x_end);
+  L.qt = off; off = align_up(off + (size_t)naux * hfjk::charge_groups(n) * 8);
+  L.qg = off; off = align_up(off + (size_t)naux * 8);
+  L.qc = off; off = align_up(off + npad * L.ldo * 4);
+  L.q_end = off;
+  L.total = L.q_end;
   return L;
 }
 
@@ -487,6 +498,62 @@ int hfjk_exchange_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, const dou
       return;
     }
     run_exchange(ctx, orbitals, nocc, signs, exchange, precision, workspace, L, s);
+  });
+}
+
+int hfjk_jk_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, double* coulomb, double* exchange, int precision,
+                void* workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_precision(precision);
+    check_bound(ctx, precision);
+    if (nocc < 0) fail(HFJK_ERR_INPUT, "negative nocc");
+    if (coulomb == nullptr || exchange == nullptr) fail(HFJK_ERR_INPUT, "null output pointer");
+    if (nocc > 0 && orbitals == nullptr) fail(HFJK_ERR_INPUT, "null orbitals pointer");
+    const int kprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;
+    const Layout L = make_layout(ctx, std::max(nocc, 1), kprec);
+    if (workspace_bytes < L.q_end || workspace == nullptr)
+      fail(HFJK_ERR_WORKSPACE, "workspace smaller than the combined build needs (" + std::to_string(L.q_end) +
+                                   " bytes)");
+    if (((uintptr_t)workspace & 255) != 0) fail(HFJK_ERR_WORKSPACE, "workspace must be 256-byte aligned");
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = sizeof(double) * ctx->nao * ctx->nao;
+    if (nocc == 0) {
+      check_hip(hipMemsetAsync(coulomb, 0, bytes, s), "hipMemsetAsync");
+      check_hip(hipMemsetAsync(exchange, 0, bytes, s), "hipMemsetAsync");
+      return;
+    }
+    double* parts = at<double>(workspace, L.qt);
+    double* gamma = at<double>(workspace, L.qg);
+    run_exchange(ctx, orbitals, nocc, nullptr, exchange, precision, workspace, L, s);
+    {
+      // X and the padded orbitals are still in the workspace: gamma_P =
+      // sum C o X_P, one pass over X.
+      Stage st(ctx, s, "coulomb_gamma");
+      const void* cpad = at<void>(workspace, L.xc);
+      if (precision == HFJK_BF16X3) {
+        check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, HFJK_FP32, at<void>(workspace, L.qc), s),
+                  "orbitals launch");
+        cpad = at<void>(workspace, L.qc);
+      }
+      check_hip(hfjk::launch_charges_from_x(at<void>(workspace, L.xx), precision, ctx->naux, ctx->nao, L.ldo, cpad,
+                                            parts, gamma, s),
+                "charges launch");
+    }
+    // The exchange's X and partial tiles are dead from here on; the Coulomb
+    // partials reuse their space (same stream, so after the charges).
+    const void* b = bound_tensor(ctx, precision);
+    double* jp = at<double>(workspace, L.cjp);
+    {
+      Stage st(ctx, s, "coulomb_accumulate");
+      check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, gamma, jp, s),
+                "coulomb accumulate launch");
+    }
+    {
+      Stage st(ctx, s, "coulomb_unpack");
+      check_hip(hfjk::launch_coulomb_unpack(jp, L.nsplit_j, L.p, ctx->nao, coulomb, s), "coulomb unpack launch");
+    }
   });
 }
 

@@ -155,6 +155,13 @@ hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* s
 hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s);
 hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
                                         void* x, int bn, hipStream_t s);
+// Coulomb charges of D = C C^T from the half-transformed X (xk.hip):
+// gamma[q] = sum_{m, i} X_q[m][i] cpad[m][i], cpad = the zero-padded
+// orbitals [npad][ldo] of launch_orbitals (FP64 when X is FP64, FP32 for
+// FP32 and BF16x3 X); parts holds naux * charge_groups(nao) doubles.
+int charge_groups(int nao);
+hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao, int ldo, const void* cpad,
+                                 double* parts, double* gamma, hipStream_t s);
 // Lower-triangle 128 x 128 output tiles of the SYRK and the tile edge.
 int syrk_pairs(int nao);
 int syrk_tile();

@@ -24,6 +24,7 @@
 // tiles stored transposed are transposed in registers (4 x 4 blocks) on their
 // way into a [row][k] LDS tile.
 #include <algorithm>
+#include <type_traits>
 
 #include "hfjk_internal.h"
 
@@ -469,6 +470,137 @@ __global__ __launch_bounds__(kThreads) void xform_bf16x3_kernel(
   const long long xbase = (long long)q * xrows * ldo;
   xform16_tile<BN>(bhi + (long long)q * p, blo + (long long)q * p, nt, cthi, ctlo, npad, ldo, xhi + xbase,
                    xlo + xbase, xrows, mb, nb, sm);
+}
+
+// ------------------------------------------------------------------ Coulomb charges from X
+//
+// gamma_q = <B_q, C C^T> = sum_{m, i} X_q[m][i] C[m][i]: once X is
+// half-transformed, the charges of D = C C^T cost one pass over X instead of
+// one over B. C is the zero-padded orbital copy in X's row layout
+// ([npad][ldo]; FP64 for X in FP64, FP32 otherwise). Workgroup (q, g) sums
+// rows [kChargeRows g, kChargeRows (g + 1)) of X_q into parts[q][g];
+// charge_reduce_kernel adds the parts in a fixed order.
+// F: 0 = X in FP64, 1 = FP32, 2 = BF16x3 (hi plane, then lo plane `plane`
+// elements later).
+constexpr int kChargeRows = 128;
+
+// V consecutive X values from element idx (16 bytes of the plane, or of each
+// plane for BF16x3), as doubles.
+template <int F>
+struct XVec {
+  static constexpr int V = F == 0 ? 2 : F == 1 ? 4 : 8;
+  static __device__ __forceinline__ void load(const void* __restrict__ x, long long plane, long long idx,
+                                              double (&out)[V]) {
+    if constexpr (F == 0) {
+      const double2 v = *(const double2*)((const double*)x + idx);
+      out[0] = v.x;
+      out[1] = v.y;
+    } else if constexpr (F == 1) {
+      const float4 v = *(const float4*)((const float*)x + idx);
+      out[0] = v.x;
+      out[1] = v.y;
+      out[2] = v.z;
+      out[3] = v.w;
+    } else {
+      const uint16_t* h = (const uint16_t*)x;
+      const uint4 hv = *(const uint4*)(h + idx);
+      const uint4 lv = *(const uint4*)(h + idx + plane);
+      const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w};
+      const uint32_t lw[4] = {lv.x, lv.y, lv.z, lv.w};
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const uint16_t hb = (uint16_t)(hw[e >> 1] >> ((e & 1) * 16));
+        const uint16_t lb = (uint16_t)(lw[e >> 1] >> ((e & 1) * 16));
+        out[e] = (double)f32_of(hb) + (double)f32_of(lb);
+      }
+    }
+  }
+};
+
+// V consecutive padded orbital values from element idx, as doubles.
+template <typename TC, int V>
+__device__ __forceinline__ void load_orbitals(const TC* __restrict__ c, long long idx, double (&out)[V]) {
+  if constexpr (sizeof(TC) == 8) {
+#pragma unroll
+    for (int e = 0; e < V; e += 2) {
+      const double2 v = *(const double2*)(c + idx + e);
+      out[e] = v.x;
+      out[e + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < V; e += 4) {
+      const float4 v = *(const float4*)(c + idx + e);
+      out[e] = v.x;
+      out[e + 1] = v.y;
+      out[e + 2] = v.z;
+      out[e + 3] = v.w;
+    }
+  }
+}
+
+template <int F>
+__global__ __launch_bounds__(kThreads) void charges_from_x_kernel(const void* __restrict__ x, long long plane, int nao,
+                                                                   int xrows, int ldo, const void* __restrict__ cpad,
+                                                                   int ngroups, double* __restrict__ parts) {
+  typedef typename std::conditional<F == 0, double, float>::type TC;
+  const TC* __restrict__ c = (const TC*)cpad;
+  const int q = blockIdx.x / ngroups;
+  const int g = blockIdx.x - q * ngroups;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const long long xq = (long long)q * xrows * ldo;
+  const int m_end = min(nao, (g + 1) * kChargeRows);
+  constexpr int V = XVec<F>::V;
+  double part = 0.0;
+  // Rows [m_begin, m_end) of X_q are one contiguous range of
+  // (m_end - m_begin) * ldo elements: the workgroup streams it in 16-byte
+  // groups (ldo is a multiple of 32, so no group straddles two rows), group
+  // f of thread t being f = t + kThreads j. The padding is zero, so the loop
+  // body has no branch and the loads of successive groups overlap.
+  const int m_begin = g * kChargeRows;
+  const int per_row = ldo / V;
+  const int ngroups_blk = (m_end - m_begin) * per_row;
+  const int step_m = kThreads / per_row;
+  const int step_g = kThreads - step_m * per_row;
+  int m = m_begin + (int)threadIdx.x / per_row;
+  int gi = (int)threadIdx.x % per_row;  // group within the row
+#pragma unroll 4
+  for (int f = threadIdx.x; f < ngroups_blk; f += kThreads) {
+    const int i0 = gi * V;
+    double xv[V], cv[V];
+    XVec<F>::load(x, plane, xq + (long long)m * ldo + i0, xv);
+    load_orbitals<TC, V>(c, (long long)m * ldo + i0, cv);
+#pragma unroll
+    for (int e = 0; e < V; ++e) part += xv[e] * cv[e];
+    m += step_m;
+    gi += step_g;
+    if (gi >= per_row) {
+      gi -= per_row;
+      ++m;
+    }
+  }
+  __shared__ double red[kThreads / 64];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) sum += red[w];
+    parts[blockIdx.x] = sum;
+  }
+}
+
+// gamma[q] = sum_g parts[q][g], in a fixed order.
+__global__ __launch_bounds__(kThreads) void charge_reduce_kernel(const double* __restrict__ parts, int naux,
+                                                                  int ngroups, double* __restrict__ gamma) {
+  const int q = blockIdx.x * kThreads + threadIdx.x;
+  if (q >= naux) return;
+  double sum = 0.0;
+  for (int g = 0; g < ngroups; ++g) sum += parts[(long long)q * ngroups + g];
+  gamma[q] = sum;
 }
 
 // Orbitals [nao][nocc] float64 -> transposed split planes Ct[ldo][npad]
@@ -1029,6 +1161,31 @@ hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out,
   uint16_t* hi = (uint16_t*)out;
   hipLaunchKernelGGL(orbitals_bf16x3_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, c, nao, nocc, npad, ldo, hi,
                      hi + n);
+  return hipGetLastError();
+}
+
+int charge_groups(int nao) { return (nao + kChargeRows - 1) / kChargeRows; }
+
+hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao, int ldo, const void* cpad,
+                                 double* parts, double* gamma, hipStream_t s) {
+  const int xrows = x_rows(nao);
+  const int ng = charge_groups(nao);
+  const long long plane = (long long)naux * xrows * ldo;
+  // Grids stay below 2^31 workgroups for any bound tensor (naux * ng).
+  const dim3 grid((unsigned)((long long)naux * ng));
+  if (precision == HFJK_FP64)
+    hipLaunchKernelGGL(charges_from_x_kernel<0>, grid, dim3(kThreads), 0, s, x, plane, nao, xrows, ldo, cpad, ng,
+                       parts);
+  else if (precision == HFJK_FP32)
+    hipLaunchKernelGGL(charges_from_x_kernel<1>, grid, dim3(kThreads), 0, s, x, plane, nao, xrows, ldo, cpad, ng,
+                       parts);
+  else
+    hipLaunchKernelGGL(charges_from_x_kernel<2>, grid, dim3(kThreads), 0, s, x, plane, nao, xrows, ldo, cpad, ng,
+                       parts);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(charge_reduce_kernel, dim3((unsigned)((naux + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                     parts, naux, ng, gamma);
   return hipGetLastError();
 }
 

@@ -223,10 +223,11 @@ class RestrictedHamiltonian:
         """
         torch = _torch()
         dn = self.dense
-        j1 = self.two_body.coulomb(density, precision=precision)
         if orbitals is not None:
-            k = self.two_body.exchange_occupied(orbitals, precision=precision)
+            # D = C C^T: J and K in one call (coulomb_exchange.py:397-404).
+            j1, k = self.two_body.jk_occupied(orbitals, precision=precision)
         else:
+            j1 = self.two_body.coulomb(density, precision=precision)
             k = self.two_body.exchange(density, precision=precision)
         fock = dn.combine(1.0, self.core, 2.0, j1, -1.0, k)
         terms = torch.empty(3, dtype=torch.float64, device=self.device)

@@ -133,6 +133,15 @@ HFJK_API int hfjk_exchange_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, 
                       double *exchange, int precision, void *workspace, size_t workspace_bytes,
                       void *stream);
 
+/* J and K of the density D = C C^T of C = orbitals [nao][nocc] in one call:
+ * coulomb_exchange_matrices (coulomb_exchange.py:397-404) for an aufbau
+ * density, i.e. hfjk_coulomb(D) and hfjk_exchange_occ(C, NULL). The charges
+ * gamma_P = <B[P], D> = sum C o (B[P] C) are formed from the half-transformed
+ * X the exchange leaves in the workspace, one pass over X instead of one over
+ * B. Workspace: hfjk_workspace_bytes(nocc). */
+HFJK_API int hfjk_jk_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, double *coulomb, double *exchange,
+                         int precision, void *workspace, size_t workspace_bytes, void *stream);
+
 /* K = sum_P B[P] D B[P] for a symmetric (possibly indefinite) density:
  * D = V diag(l) V^T by rocSOLVER dsyevd, eigenpairs with |l| <= 1e-12 max|l|
  * dropped, C = V |l|^1/2, S = sign(l), then hfjk_exchange_occ. Drop-in for

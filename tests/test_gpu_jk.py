@@ -92,6 +92,45 @@ def test_ragged_shapes_match_the_oracle(torch, precision, naux, nao, nocc):
     assert rel(jk.exchange(d), k_ref) < tol
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("naux,nao,nocc", [(1, 1, 1), (2, 33, 3), (11, 65, 40), (3, 130, 130), (17, 97, 70),
+                                           (3, 300, 300), (9, 129, 17)])
+def test_combined_build_matches_the_oracle(torch, precision, naux, nao, nocc):
+    """J and K of D = C C^T in one call (charges formed from the half-transformed X)."""
+    b, c, d = synthetic.jk_inputs(200 + nao, naux, nao, nocc)
+    jk = builder(b, precision=precision)
+    j, k = jk.jk_occupied(c)
+    assert rel(j, jk_ref.coulomb(b, d)) < TOL[precision]
+    assert rel(k, jk_ref.exchange(b, d)) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_combined_build_over_every_tile_choice(torch, precision):
+    """The charges from X give the same J for every half-transform tiling."""
+    from hafnium_amd import _lib
+
+    b, c, d = synthetic.jk_inputs(41, 6, 150, 90)
+    jk = builder(b, precision=precision)
+    j_ref = jk_ref.coulomb(b, d)
+    for rows in (64, 128):
+        for tile in (32, 64, 128, 256):
+            jk.set_tuning(_lib.TUNE_X_BM, rows)
+            jk.set_tuning(_lib.TUNE_X_BN, tile)
+            j, _ = jk.jk_occupied(c)
+            assert rel(j, j_ref) < TOL[precision], (rows, tile)
+
+
+def test_combined_build_is_deterministic_and_equals_the_separate_calls(torch):
+    b, c, d = synthetic.jk_inputs(13, 40, 90, 20)
+    jk = builder(b)
+    j1, k1 = jk.jk_occupied(c)
+    j2, k2 = jk.jk_occupied(c)
+    np.testing.assert_array_equal(j1, j2)
+    np.testing.assert_array_equal(k1, k2)
+    np.testing.assert_array_equal(k1, jk.exchange_occupied(c))
+    assert rel(j1, jk.coulomb(d)) < 1e-13
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_indefinite_and_nonsymmetric_densities(torch, precision):
     rng = np.random.default_rng(5)
