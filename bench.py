@@ -214,8 +214,10 @@ def bench_clusters(args, dev):
     learned guess was not trained on), cc-pVDZ with def2-universal-jkfit,
     converged to |dE| < 1e-8 Eh and a gradient below 1e-6. Runs per cluster:
     plain FP64 from the superposition-of-atomic-densities guess (the
-    baseline), the mixed policy from the same guess, and, when a trained
-    model is stored, FP64 and mixed from the learned guess. The SCF wall time
+    baseline), the mixed policy from the same guess, FP64 from the
+    superposition of water densities (the isolated water's orbitals rotated
+    into every water's frame), and, when a trained model is stored, FP64 and
+    mixed from the learned guess. The SCF wall time
     excludes the integrals and the fit, which are the same for every run; the
     learned runs include the guess itself (descriptors, network, J of the
     predicted density, K of the atomic guess, one diagonalisation and the two
@@ -232,6 +234,7 @@ def bench_clusters(args, dev):
 
     model = lg.LearnedGuessModel.load(device=dev) if os.path.exists(lg.MODEL_PATH) and not args.no_learned else None
     cache = {}
+    water = lg.water_orbitals(device=dev, cache=cache)
     rows = []
     for n in [int(x) for x in args.cluster_sizes.split(",") if x]:
         mol = water_cluster(n, 100 + n)
@@ -243,7 +246,7 @@ def bench_clusters(args, dev):
             lg.learned_guess(system, model, guess)
         entry = {"n_waters": n, "nao": system.timings["nao"], "naux": system.timings["naux"],
                  "integrals_s": round(system.timings["integrals_s"], 3), "fit_s": round(system.timings["fit_s"], 3)}
-        runs = [("fp64", "fp64", False), ("mixed", args.cluster_mixed, False)]
+        runs = [("fp64", "fp64", False), ("mixed", args.cluster_mixed, False), ("smd_fp64", "fp64", "molecular")]
         if model is not None:
             runs += [("learned_fp64", "fp64", True), ("learned_mixed", args.cluster_mixed, True)]
         for label, precision, learned in runs:
@@ -253,7 +256,10 @@ def bench_clusters(args, dev):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             start, info = guess, None
-            if learned:
+            if learned == "molecular":
+                dens, orbs = lg.molecular_factors(mol, water)
+                start = lg.GuessStart(dens, orbs, None)
+            elif learned:
                 start, info = lg.learned_guess(system, model, guess, precision=precision)
             result, _ = run_cluster(n, 100 + n, system=system, guess=start, precision=precision)
             torch.cuda.synchronize()
@@ -285,14 +291,15 @@ def bench_clusters(args, dev):
 
 def bench_scf_batch(args, dev):
     """Time to converge a set of water clusters to 1e-8 Eh: one molecule at a time (plain FP64 from
-    the atomic-density guess, the baseline) against the batched driver (FP64; the mixed policy; the
-    mixed policy from the learned guess when a trained model is stored).
+    the atomic-density guess, the baseline) against the batched driver from the atomic start (FP64;
+    the mixed policy), from the superposition of water densities (FP64), and from the learned guess
+    on either base when a trained model is stored (FP64; mixed on the atomic base).
 
     The set cycles through ``--batch-scf-sizes`` with seeds 300000 + i (not used for training).
     Integrals and the fit are excluded from every timing (they are the same work for all runs); the
-    atomic densities are computed once per element beforehand and shared. Timings include the
-    starts: the atomic start's Fock build, and for the learned runs the descriptors, the network,
-    J of the predicted densities, one diagonalisation and the fallback test.
+    atomic densities and the isolated water's orbitals are computed once beforehand and shared.
+    Timings include the starts: the start's Fock build, and for the learned runs the descriptors,
+    the network, J of the predicted densities, one diagonalisation and the fallback test.
     """
     import os
 
@@ -300,7 +307,7 @@ def bench_scf_batch(args, dev):
     import torch
 
     from hafnium_amd import learned_guess as lg
-    from hafnium_amd.batch_scf import BatchedRhf, atomic_start, learned_start, molecule_inputs
+    from hafnium_amd.batch_scf import BatchedRhf, atomic_start, learned_start, molecular_start, molecule_inputs
     from hafnium_amd.clusters import atomic_density_guess, build_system, run_cluster
     from hafnium_amd.molecule import water_cluster
     from hafnium_amd.scf import ScfOptions
@@ -313,6 +320,7 @@ def bench_scf_batch(args, dev):
     cache = {}
     for x in inputs:
         atomic_density_guess(x.molecule, device=dev, cache=cache)
+    water = lg.water_orbitals(device=dev, cache=cache)
     serial, t_serial = [], 0.0
     for x, (n, seed) in zip(inputs, specs):
         system = build_system(x.molecule, device=dev, precomputed=(x.overlap, x.core, x.metric, x.three_center))
@@ -331,29 +339,34 @@ def bench_scf_batch(args, dev):
     t_fit = time.perf_counter() - t0
     mixed = args.cluster_mixed
     model = lg.LearnedGuessModel.load(device=dev) if os.path.exists(lg.MODEL_PATH) and not args.no_learned else None
-    runs = [("batched_fp64", "fp64", False), ("batched_mixed", mixed, False)]
+    runs = [("batched_fp64", "fp64", "atomic", False), ("batched_mixed", mixed, "atomic", False),
+            ("batched_smd_fp64", "fp64", "molecular", False)]
     if model is not None:
-        runs += [("batched_learned_fp64", "fp64", True), ("batched_learned_mixed", mixed, True)]
+        runs += [("batched_learned_fp64", "fp64", "atomic", True), ("batched_learned_mixed", mixed, "atomic", True),
+                 ("batched_learned_smd_fp64", "fp64", "molecular", True)]
 
-    def solve(precision, learned):
+    def solve(precision, base, learned):
         opts = ScfOptions(precision=precision, energy_tolerance=1e-8, gradient_tolerance=1e-6)
-        start = atomic_start(batch, cache, precision)
+        if base == "molecular":
+            start = molecular_start(batch, water, precision)
+        else:
+            start = atomic_start(batch, cache, precision)
         info = None
         if learned:
             start = learned_start(batch, model, start, precision)
             info = start.info
         return batch.run(start, opts), info
 
-    for _, precision, learned in runs:  # warm-up: plans, tensor copies, network buffers
-        solve(precision, learned)
+    for _, precision, base, learned in runs:  # warm-up: plans, tensor copies, network buffers
+        solve(precision, base, learned)
     ref = np.array([r.energy for r in serial])
     out = {"clusters": [{"n_waters": n, "seed": seed} for n, seed in specs], "count": len(specs),
            "integrals_s": t_integrals, "batched_fit_s": t_fit,
            "serial_fp64": {"seconds": t_serial, "mean_iterations": float(np.mean([r.n_iterations for r in serial]))}}
-    for label, precision, learned in runs:
+    for label, precision, base, learned in runs:
         torch.cuda.synchronize()
         t = time.perf_counter()
-        results, info = solve(precision, learned)
+        results, info = solve(precision, base, learned)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
         energies = np.array([r.energy for r in results])

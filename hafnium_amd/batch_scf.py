@@ -453,10 +453,20 @@ def atomic_start(batch: BatchedRhf, atom_densities: dict, precision: str = "fp64
     return batch.start_from_factors([p[0] for p in pairs], [p[1] for p in pairs], precision)
 
 
+def molecular_start(batch: BatchedRhf, water_orbitals: np.ndarray, precision: str = "fp64") -> _Start:
+    """Superposition of water densities: each water's block is the isolated reference water's
+    occupied orbitals rotated into its frame (``learned_guess.molecular_factors``)."""
+    from hafnium_amd.learned_guess import molecular_factors
+
+    pairs = [molecular_factors(x.molecule, water_orbitals) for x in batch.inputs]
+    return batch.start_from_factors([p[0] for p in pairs], [p[1] for p in pairs], precision)
+
+
 def learned_start(batch: BatchedRhf, model, sad: _Start, precision: str = "fp64") -> _Start:
-    """F0 = h + J[d] - K[D_atoms] from predicted fitted coefficients d. Per molecule the start is the
-    aufbau density of F0 where its energy is not above that of the aufbau density of the atomic
-    start's Fock matrix (both idempotent: a variational comparison), and the latter elsewhere."""
+    """F0 = h + J[d] - K[D_base] from predicted fitted coefficients d, D_base the density of the
+    given start (``sad``: atomic or molecular superposition). Per molecule the start is the aufbau
+    density of F0 where its energy is not above that of the aufbau density of the base start's Fock
+    matrix (both idempotent: a variational comparison), and the latter elsewhere."""
     torch = _torch()
     t0 = time.perf_counter()
     coeffs = model.predict_many([x.molecule for x in batch.inputs])

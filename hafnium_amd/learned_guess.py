@@ -545,6 +545,69 @@ def train(data: dict, *, aux_name: str = "def2-universal-jkfit", epochs: int = 6
     return LearnedGuessModel(elements, aux_name, device=dev), report
 
 
+# ---------------------------------------------------------------- superposition of water densities
+
+WATER_OH_ANGSTROM = 0.96
+WATER_ANGLE_DEG = 104.5
+
+
+def reference_water() -> Molecule:
+    """The cluster generator's rigid water in its own frame: O at the origin, the
+    bisector along z, H1 - H2 along x (the frame :func:`water_frames` builds)."""
+    half = math.radians(WATER_ANGLE_DEG) / 2.0
+    s, c = WATER_OH_ANGSTROM * math.sin(half), WATER_OH_ANGSTROM * math.cos(half)
+    return Molecule.from_angstrom(["O", "H", "H"], np.array([[0.0, 0.0, 0.0], [s, 0.0, c], [-s, 0.0, c]]))
+
+
+def water_orbitals(*, basis: str = "cc-pvdz", aux: str = "def2-universal-jkfit", device=None,
+                   cache: dict | None = None) -> np.ndarray:
+    """Converged occupied orbitals ``[n_ao(H2O), 5]`` of the isolated reference water (FP64 SCF)."""
+    from hafnium_amd.clusters import atomic_density_guess, build_system
+    from hafnium_amd.scf import ScfOptions, solve_scf
+
+    key = ("water-orbitals", basis, aux)
+    if cache is not None and key in cache:
+        return cache[key]
+    mol = reference_water()
+    system = build_system(mol, basis=basis, aux=aux, device=device)
+    guess = atomic_density_guess(mol, basis=basis, aux=aux, device=device, cache=cache)
+    result = solve_scf(system.hamiltonian, guess=guess,
+                       options=ScfOptions(energy_tolerance=1e-11, gradient_tolerance=1e-8))
+    c = result.mo_coefficients[:, :system.hamiltonian.n_alpha].cpu().numpy().copy()
+    if cache is not None:
+        cache[key] = c
+    return c
+
+
+@functools.lru_cache(maxsize=None)
+def _orbital_rows(basis: str) -> list:
+    """Per atom of a water (O, H, H): ``(row offset, {l: columns})`` of its orbital-basis blocks."""
+    out, at = [], 0
+    for z in (8, 1, 1):
+        out.append((at, _block_columns(basis, z)))
+        at += shell_blocks(basis, z)[1]
+    return out
+
+
+def molecular_factors(molecule: Molecule, c_ref: np.ndarray, basis: str = "cc-pvdz"):
+    """Superposition of water densities: ``(D, C)`` with C block-diagonal over the waters, block k =
+    the reference water's occupied orbitals rotated into water k's frame (c_lab = D(R) c_local per
+    (2l+1)-block, as for the fitted coefficients), and D = C C^T."""
+    rots = water_frames(molecule)[0::3]
+    nw, (nw_ao, nocc) = len(rots), c_ref.shape
+    lmax = max(max(cols) for _, cols in _orbital_rows(basis))
+    dms = rotation_matrices_many(rots, lmax)
+    blocks = np.repeat(c_ref[None], nw, axis=0)  # [waters, n_ao(H2O), nocc]
+    for offset, cols in _orbital_rows(basis):
+        for l, c in cols.items():
+            rows = offset + c  # [n_blocks_l, 2l+1]
+            blocks[:, rows, :] = np.einsum("wij,wbjo->wbio", dms[l], blocks[:, rows, :])
+    factors = np.zeros((nw * nw_ao, nw * nocc))
+    for k in range(nw):
+        factors[k * nw_ao:(k + 1) * nw_ao, k * nocc:(k + 1) * nocc] = blocks[k]
+    return factors @ factors.T, factors
+
+
 # ---------------------------------------------------------------- the guess
 
 @dataclass

@@ -46,6 +46,24 @@ def test_batched_fp64_matches_serial(setup, eigensolver):
         assert abs(got.iterations - ref.n_iterations) <= 2
 
 
+def test_molecular_start_reaches_the_same_state_in_fewer_iterations(setup):
+    """The superposition of water densities converges to the atomic start's FP64 state, and
+    in no more iterations on any molecule."""
+    from hafnium_amd.batch_scf import atomic_start, molecular_start
+    from hafnium_amd.learned_guess import water_orbitals
+    from hafnium_amd.scf import ScfOptions
+
+    batch, serial, cache, _ = setup
+    opts = ScfOptions(energy_tolerance=1e-8, gradient_tolerance=1e-6)
+    atomic = batch.run(atomic_start(batch, cache), opts)
+    molecular = batch.run(molecular_start(batch, water_orbitals(cache=cache)), opts)
+    for got, base, ref in zip(molecular, atomic, serial):
+        assert got.converged
+        assert got.energy == pytest.approx(ref.energy, abs=SAME_STATE_TOL)
+        assert got.iterations <= base.iterations
+    assert sum(r.iterations for r in molecular) < sum(r.iterations for r in atomic)
+
+
 def test_batched_mixed_precision_reaches_the_fp64_state(setup):
     from hafnium_amd.batch_scf import atomic_start
     from hafnium_amd.scf import ScfOptions

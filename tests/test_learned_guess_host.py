@@ -127,3 +127,31 @@ def test_whitening_measures_the_coulomb_self_energy():
     delta = c - mean
     np.testing.assert_allclose(np.sum(y ** 2, axis=1), np.einsum("ai,ij,aj->a", delta, g, delta) / 0.25,
                                rtol=1e-10)
+
+
+def test_molecular_factors_rotate_orbitals_with_the_integral_engine():
+    """Rotating a water's orbitals into its frame preserves their overlaps: for any C,
+    C_k^T S_kk C_k equals C^T S_ref C (S_kk = water k's own overlap block). An unrotated
+    copy does not, and the reference water is mapped onto itself."""
+    from hafnium_amd import integrals
+    from hafnium_amd.molecule import BasisOnAtoms
+
+    ref = lg.reference_water()
+    s_ref = integrals.one_electron(BasisOnAtoms("cc-pvdz", ref), ref)[0]
+    c = np.random.default_rng(1).standard_normal((s_ref.shape[0], 5))
+    gram = c.T @ s_ref @ c
+    mol = water_cluster(3, seed=11)
+    s = integrals.one_electron(BasisOnAtoms("cc-pvdz", mol), mol)[0]
+    dens, fac = lg.molecular_factors(mol, c)
+    n = s_ref.shape[0]
+    assert fac.shape == (3 * n, 15)
+    np.testing.assert_allclose(dens, fac @ fac.T)
+    for k in range(3):
+        blk = fac[k * n:(k + 1) * n, 5 * k:5 * k + 5]
+        skk = s[k * n:(k + 1) * n, k * n:(k + 1) * n]
+        assert np.abs(blk.T @ skk @ blk - gram).max() < 1e-10 * np.abs(gram).max()
+        assert np.abs(fac[k * n:(k + 1) * n, :5 * k]).max(initial=0.0) == 0.0
+    unrotated = s[n:2 * n, n:2 * n]
+    assert np.abs(c.T @ unrotated @ c - gram).max() > 1e-3 * np.abs(gram).max()
+    _, self_fac = lg.molecular_factors(ref, c)
+    np.testing.assert_allclose(self_fac, c, atol=1e-12)
