@@ -112,10 +112,11 @@ struct Layout {
   size_t cw, cgp, cg, cjp, c_end;
   // exchange region
   size_t xc, xs, xx, xp, x_end;
-  // charges of hfjk_jk_occ formed from X, after both regions: row-group
-  // parts [naux][charge_groups], gamma [naux], and an FP32 padded orbital
-  // copy for BF16X3 (whose exchange keeps only the bf16 planes of C^T)
-  size_t qt, qg, qc, q_end;
+  // the Coulomb half of hfjk_jk_occ, after both regions (it runs while the
+  // SYRK still reads X): row-group parts of the charges [naux][charge_groups],
+  // gamma [naux], an FP32 padded orbital copy for BF16X3 (whose exchange keeps
+  // only the bf16 planes of C^T), and the J partials
+  size_t qt, qg, qc, qj, q_end;
   size_t total;
   int nsplit_j, nsplit_k, ldo, xrows, npairs;
   long long p, ntiles;
@@ -161,12 +162,11 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   const size_t tile2 = (size_t)hfjk::syrk_tile() * hfjk::syrk_tile();
   L.xp = off; off = align_up(off + (size_t)L.nsplit_k * L.npairs * tile2 * es);
   L.x_end = off;
-  // Past both regions: the Coulomb partials of hfjk_jk_occ reuse the
-  // exchange region while the charges are still being read.
   off = std::max(L.c_end, L.x_end);
   L.qt = off; off = align_up(off + (size_t)naux * hfjk::charge_groups(n) * 8);
   L.qg = off; off = align_up(off + (size_t)naux * 8);
   L.qc = off; off = align_up(off + npad * L.ldo * 4);
+  L.qj = off; off = align_up(off + (size_t)L.nsplit_j * L.p * 8);
   L.q_end = off;
   L.total = L.q_end;
   return L;
@@ -241,8 +241,9 @@ void run_coulomb(hfjk_ctx* ctx, const double* density, double* out, int precisio
   }
 }
 
-void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double* signs, double* out, int precision,
-                  void* ws, const Layout& L, hipStream_t s) {
+// Orbitals (and signs) into the workspace and the half-transform X = B C.
+void exchange_front(hfjk_ctx* ctx, const double* orbitals, int nocc, const double* signs, int precision, void* ws,
+                    const Layout& L, hipStream_t s) {
   // FP64 and FP32 run the same MFMA kernels on their tensor copy; BF16X3
   // runs the split-product kernels on the bf16 hi/lo planes. The FP32 and
   // BF16X3 buffers have the same sizes (4 bytes per element).
@@ -251,7 +252,6 @@ void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double*
   void* cp = at<void>(ws, L.xc);
   double* sg = at<double>(ws, L.xs);
   void* x = at<void>(ws, L.xx);
-  void* part = at<void>(ws, L.xp);
   const bool split = precision == HFJK_BF16X3;
   {
     Stage st(ctx, s, "exchange_prepare");
@@ -272,9 +272,17 @@ void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double*
                                             ctx->x_bm == 0 ? 64 : ctx->x_bm, s),
                 "half transform launch");
   }
+}
+
+// K = sum_P X_P S X_P^T from the X that exchange_front left in the workspace.
+void exchange_back(hfjk_ctx* ctx, bool is_signed, double* out, int precision, void* ws, const Layout& L,
+                   hipStream_t s) {
+  double* sg = at<double>(ws, L.xs);
+  void* x = at<void>(ws, L.xx);
+  void* part = at<void>(ws, L.xp);
   {
     Stage st(ctx, s, "exchange_syrk");
-    check_hip(hfjk::launch_syrk(x, precision, ctx->naux, ctx->nao, L.ldo, sg, signs != nullptr, L.nsplit_k, part,
+    check_hip(hfjk::launch_syrk(x, precision, ctx->naux, ctx->nao, L.ldo, sg, is_signed, L.nsplit_k, part,
                                 ctx->syrk_kind, s), "syrk launch");
   }
   {
@@ -282,6 +290,12 @@ void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double*
     const int rprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;  // BF16X3 partials are FP32
     check_hip(hfjk::launch_syrk_reduce(part, rprec, L.nsplit_k, ctx->nao, out, s), "syrk reduce launch");
   }
+}
+
+void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double* signs, double* out, int precision,
+                  void* ws, const Layout& L, hipStream_t s) {
+  exchange_front(ctx, orbitals, nocc, signs, precision, ws, L, s);
+  exchange_back(ctx, signs != nullptr, out, precision, ws, L, s);
 }
 
 rocblas_handle blas_handle(hfjk_ctx* ctx) {
@@ -352,6 +366,9 @@ int hfjk_destroy(hfjk_ctx* ctx) {
       if (t.stop) (void)hipEventDestroy(t.stop);
     }
     if (ctx->blas != nullptr) rocblas_destroy_handle((rocblas_handle)ctx->blas);
+    if (ctx->fork != nullptr) (void)hipEventDestroy(ctx->fork);
+    if (ctx->join != nullptr) (void)hipEventDestroy(ctx->join);
+    if (ctx->side != nullptr) (void)hipStreamDestroy(ctx->side);
     delete ctx;
   });
 }
@@ -526,34 +543,46 @@ int hfjk_jk_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, double* coulomb
     }
     double* parts = at<double>(workspace, L.qt);
     double* gamma = at<double>(workspace, L.qg);
-    run_exchange(ctx, orbitals, nocc, nullptr, exchange, precision, workspace, L, s);
+    double* jp = at<double>(workspace, L.qj);
+    if (ctx->side == nullptr) {
+      check_hip(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+      check_hip(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+      check_hip(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming), "hipEventCreateWithFlags");
+    }
+    hipStream_t side = ctx->side;
+    exchange_front(ctx, orbitals, nocc, nullptr, precision, workspace, L, s);
+    // The Coulomb half needs only X and the padded orbitals: it runs on the
+    // side stream (HBM-bound) while the SYRK (MFMA-bound) runs on s. Its
+    // buffers (charges, J partials) lie outside the regions the SYRK uses.
+    check_hip(hipEventRecord(ctx->fork, s), "hipEventRecord");
+    check_hip(hipStreamWaitEvent(side, ctx->fork, 0), "hipStreamWaitEvent");
     {
-      // X and the padded orbitals are still in the workspace: gamma_P =
-      // sum C o X_P, one pass over X.
-      Stage st(ctx, s, "coulomb_gamma");
+      // gamma_P = sum C o X_P, one pass over X.
+      Stage st(ctx, side, "coulomb_gamma");
       const void* cpad = at<void>(workspace, L.xc);
       if (precision == HFJK_BF16X3) {
-        check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, HFJK_FP32, at<void>(workspace, L.qc), s),
+        check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, HFJK_FP32, at<void>(workspace, L.qc), side),
                   "orbitals launch");
         cpad = at<void>(workspace, L.qc);
       }
       check_hip(hfjk::launch_charges_from_x(at<void>(workspace, L.xx), precision, ctx->naux, ctx->nao, L.ldo, cpad,
-                                            parts, gamma, s),
+                                            parts, gamma, side),
                 "charges launch");
     }
-    // The exchange's X and partial tiles are dead from here on; the Coulomb
-    // partials reuse their space (same stream, so after the charges).
     const void* b = bound_tensor(ctx, precision);
-    double* jp = at<double>(workspace, L.cjp);
     {
-      Stage st(ctx, s, "coulomb_accumulate");
-      check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, gamma, jp, s),
+      Stage st(ctx, side, "coulomb_accumulate");
+      check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, gamma, jp,
+                                                side),
                 "coulomb accumulate launch");
     }
     {
-      Stage st(ctx, s, "coulomb_unpack");
-      check_hip(hfjk::launch_coulomb_unpack(jp, L.nsplit_j, L.p, ctx->nao, coulomb, s), "coulomb unpack launch");
+      Stage st(ctx, side, "coulomb_unpack");
+      check_hip(hfjk::launch_coulomb_unpack(jp, L.nsplit_j, L.p, ctx->nao, coulomb, side), "coulomb unpack launch");
     }
+    check_hip(hipEventRecord(ctx->join, side), "hipEventRecord");
+    exchange_back(ctx, false, exchange, precision, workspace, L, s);
+    check_hip(hipStreamWaitEvent(s, ctx->join, 0), "hipStreamWaitEvent");
   });
 }
 

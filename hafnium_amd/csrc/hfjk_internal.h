@@ -80,6 +80,11 @@ struct hfjk_ctx {
   int syrk_splits = 0;
   int x_bm = 0;  // 0 = 64
   int syrk_kind = 0;  // 0 = default, 1 = register-staged, 2 = direct LDS loads
+  // Second stream of hfjk_jk_occ (default priority, created on first use):
+  // the Coulomb half runs there beside the SYRK, fenced by fork/join events.
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr;
+  hipEvent_t join = nullptr;
 };
 
 namespace hfjk {
