@@ -674,7 +674,7 @@ int hfjk_set_tuning(hfjk_ctx* ctx, int param, int value) {
         ctx->x_bm = value;
         break;
       case HFJK_TUNE_SYRK_KIND:
-        if (value > 2) fail(HFJK_ERR_INPUT, "the SYRK kernel kind is 0, 1 or 2");
+        if (value > 3) fail(HFJK_ERR_INPUT, "the SYRK kernel kind is 0, 1, 2 or 3");
         ctx->syrk_kind = value;
         break;
       default:

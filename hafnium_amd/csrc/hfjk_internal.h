@@ -79,7 +79,7 @@ struct hfjk_ctx {
   int x16_bn = 0;
   int syrk_splits = 0;
   int x_bm = 0;  // 0 = 64
-  int syrk_kind = 0;  // 0 = default, 1 = register-staged, 2 = direct LDS loads
+  int syrk_kind = 0;  // 0 = default, 1 = register-staged, 2/3 = direct LDS loads (two / one workgroups per CU)
   // Second stream of hfjk_jk_occ (default priority, created on first use):
   // the Coulomb half runs there beside the SYRK, fenced by fork/join events.
   hipStream_t side = nullptr;
@@ -146,7 +146,9 @@ hipError_t launch_orbitals(const double* c, int nao, int nocc, int precision, vo
 hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao,
                                  const void* c, int ldo, void* x, int bn, int bm, hipStream_t s);
 // kind: 0 = default, 1 = register-staged FP64/FP32 kernel, 2 = direct LDS
-// loads (FP64/FP32 with ldo <= 2048; otherwise the register-staged kernel runs).
+// loads (FP64/FP32 with ldo <= 2048; otherwise the register-staged kernel
+// runs; two workgroups per CU for unsigned columns), 3 = direct LDS loads with
+// one workgroup per CU.
 // is_signed = false promises signs[i] = 1 for every occupied column, which
 // lets the direct-LDS kernel skip them.
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,

@@ -763,7 +763,7 @@ __global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x,
 // and the first operands of the following step are read, no registers
 // stage them, and each k step has one barrier.
 //
-// A stage holds kDmaK consecutive k of the 128 rows of both operands as
+// A stage holds CH chunks of k of the 128 rows of both operands as
 // 1 KB chunks (one load instruction of one wave; lane l fills bytes
 // [16 l, 16 l + 16)): chunk = 16 rows x 4 groups of 16 bytes along k, lane l
 // carrying row l % 16 and group l / 16. The MFMA side reads a chunk back
@@ -771,18 +771,25 @@ __global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x,
 // l / 16 (a contiguous, conflict-free 1 KB read). Element e of the group
 // feeds MFMA e: that MFMA sums over k = E (l / 16) + e, a permutation of the
 // chunk's k that both operands share, so the E MFMAs together sum all of it.
+//
+// CH = 2 (32 KB stages, 128 KB ring): one workgroup per CU, so one wave per
+// SIMD, which measured at most 86 % of the FP64 MFMA rate even without any
+// memory traffic. CH = 1 (16 KB stages, 64 KB ring, no sign table): two
+// workgroups per CU, two waves per SIMD that fill each other's barrier and
+// LDS-read gaps (99 % in the same measurement).
 constexpr int kDmaStages = 4;
 constexpr int kDmaMaxLdo = 2048;
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-template <typename T>
+template <typename T, int CH>
 struct SyrkDma {
   static constexpr int E = 16 / (int)sizeof(T);       // elements per 16-byte group
   static constexpr int CHUNK = 1024 / (int)sizeof(T);  // elements per 1 KB chunk
   static constexpr int KC = 4 * E;                      // k per chunk
-  static constexpr int KS = 2 * KC;                     // k per stage (two chunks per 16-row block)
-  static constexpr int STAGE = 32 * CHUNK;              // 2 operands x 8 row blocks x 2 chunks
+  static constexpr int KS = CH * KC;                    // k per stage (CH chunks per 16-row block)
+  static constexpr int STAGE = 16 * CH * CHUNK;         // 2 operands x 8 row blocks x CH chunks
+  static constexpr int LOADS = 4 * CH;                  // load instructions per wave per stage
   typedef T unit __attribute__((ext_vector_type(16 / sizeof(T))));
 };
 
@@ -800,18 +807,19 @@ __device__ __forceinline__ void barrier_nofence() { asm volatile("s_barrier" :::
 
 // SIGNED = false: every sign is +1 on the occupied columns (the padded
 // columns of X are zero), so the sign table is neither built nor applied.
-template <typename T, bool SIGNED>
-__global__ __launch_bounds__(kThreads, 1) void syrk_dma_kernel(const T* __restrict__ x, int naux, int xrows, int ldo,
-                                                               const double* __restrict__ signs, int nsplit,
-                                                               int npairs, T* __restrict__ partial) {
-  typedef SyrkDma<T> D;
+template <typename T, bool SIGNED, int CH>
+__global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
+    const T* __restrict__ x, int naux, int xrows, int ldo, const double* __restrict__ signs, int nsplit, int npairs,
+    T* __restrict__ partial) {
+  typedef SyrkDma<T, CH> D;
   typedef typename D::unit unit;
   constexpr int E = D::E;
+  constexpr int LOADS = D::LOADS;
   __shared__ __attribute__((aligned(16))) T st0[D::STAGE];
   __shared__ __attribute__((aligned(16))) T st1[D::STAGE];
   __shared__ __attribute__((aligned(16))) T st2[D::STAGE];
   __shared__ __attribute__((aligned(16))) T st3[D::STAGE];
-  __shared__ __attribute__((aligned(16))) T sgn[kDmaMaxLdo];
+  __shared__ __attribute__((aligned(16))) T sgn[SIGNED ? kDmaMaxLdo : 1];
 
   int s, pair;
   syrk_coords(blockIdx.x, npairs, s, pair);
@@ -829,19 +837,20 @@ __global__ __launch_bounds__(kThreads, 1) void syrk_dma_kernel(const T* __restri
     for (int i = t; i < ldo; i += kThreads) sgn[i] = (T)signs[i];
 
   // Byte offsets (within one aux function's X, from k = i0) of this lane's
-  // group in the wave's 8 chunks: chunks 8 wave .. 8 wave + 7 of the 32 of a
-  // stage (0-15 operand A = row block mb, 16-31 operand B = row block nb;
-  // chunk = 2 * (16-row block) + k half). Rows past xrows (the last 128-row
-  // block may hold only 64) read row 0 instead: their products land in K
-  // rows and columns past nao, which are never read.
-  int off[8];
+  // group in the wave's LOADS chunks: chunks LOADS wave .. LOADS wave +
+  // LOADS - 1 of the 16 CH of a stage (the first 8 CH operand A = row block
+  // mb, the rest operand B = row block nb; chunk = CH * (16-row block) + k
+  // part). Rows past xrows (the last 128-row block may hold only 64) read
+  // row 0 instead: their products land in K rows and columns past nao, which
+  // are never read.
+  int off[LOADS];
 #pragma unroll
-  for (int v = 0; v < 8; ++v) {
-    const int ch = wave * 8 + v;
-    const int blk = (ch >> 4) ? nb : mb;
-    int row = blk * kSyrkTile + ((ch >> 1) & 7) * 16 + (lane & 15);
+  for (int v = 0; v < LOADS; ++v) {
+    const int ch = wave * LOADS + v;
+    const int blk = ch >= 8 * CH ? nb : mb;
+    int row = blk * kSyrkTile + ((ch / CH) % 8) * 16 + (lane & 15);
     row = row < xrows ? row : 0;
-    off[v] = (row * ldo + (ch & 1) * D::KC + (lane >> 4) * E) * (int)sizeof(T);
+    off[v] = (row * ldo + (ch % CH) * D::KC + (lane >> 4) * E) * (int)sizeof(T);
   }
   const int nch = ldo / D::KS;
   const int steps = (qe - qb) * nch;
@@ -849,9 +858,9 @@ __global__ __launch_bounds__(kThreads, 1) void syrk_dma_kernel(const T* __restri
     const int qq = step / nch;
     const T* xq = x + (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * D::KS;
 #pragma unroll
-    for (int v = 0; v < 8; ++v)
+    for (int v = 0; v < LOADS; ++v)
       __builtin_amdgcn_global_load_lds((const void*)((const char*)xq + (unsigned)off[v]),
-                                       (lds_void_t*)(stage + (wave * 8 + v) * D::CHUNK), 16, 0, 0);
+                                       (lds_void_t*)(stage + (wave * LOADS + v) * D::CHUNK), 16, 0, 0);
   };
 
   typename Op<T>::acc_t acc[4][4];
@@ -867,10 +876,10 @@ __global__ __launch_bounds__(kThreads, 1) void syrk_dma_kernel(const T* __restri
   auto read_chunk = [&](const T* stage, int step, int c, unit (&a)[4], unit (&b)[4]) {
     const int i0 = (step % nch) * D::KS;
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi) a[mi] = *(const unit*)(stage + ((((wm * 4 + mi) * 2 + c) * 64 + lane) * E));
+    for (int mi = 0; mi < 4; ++mi) a[mi] = *(const unit*)(stage + ((((wm * 4 + mi) * CH + c) * 64 + lane) * E));
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni)
-      b[ni] = *(const unit*)(stage + D::STAGE / 2 + ((((wn * 4 + ni) * 2 + c) * 64 + lane) * E));
+      b[ni] = *(const unit*)(stage + D::STAGE / 2 + ((((wn * 4 + ni) * CH + c) * 64 + lane) * E));
     if (SIGNED) {
       const unit sg = *(const unit*)(sgn + i0 + c * D::KC + (lane >> 4) * E);
 #pragma unroll
@@ -891,27 +900,38 @@ __global__ __launch_bounds__(kThreads, 1) void syrk_dma_kernel(const T* __restri
   // returned; after the barrier every wave's loads of stages k and k + 1 are
   // in LDS and nobody reads stage k - 1 any more, so its buffer takes the
   // loads of step k + 3. The first chunk of stage k is already in registers
-  // (read during step k - 1); the step reads its second chunk, runs the
-  // first chunk's MFMAs, reads the first chunk of stage k + 1, then runs the
-  // second chunk's MFMAs, so no MFMA waits for an LDS read after the barrier.
+  // (read during step k - 1). With CH = 2 the step reads its second chunk,
+  // runs the first chunk's MFMAs, reads the first chunk of stage k + 1, then
+  // runs the second chunk's MFMAs; with CH = 1 it reads stage k + 1's chunk
+  // into the other register set while the MFMAs of stage k run. Either way
+  // no MFMA waits for an LDS read after the barrier.
   // Past the last step the loads repeat the last stage into buffers nobody
   // reads and the MFMAs are skipped, so there are only the seven load sites
   // below, each naming its stage buffer statically: the compiler then tells
   // the stages being read from the ones being loaded and adds no wait of its
   // own.
   unit a0[4], b0[4], a1[4], b1[4];
-  auto step_body = [&](int k, const T* cur, const T* next, T* free) {
+  // odd: which register set holds stage k's first chunk (CH = 1 alternates).
+  auto step_body = [&](int k, const T* cur, const T* next, T* free, bool odd) {
     // Keep the previous step's MFMAs ahead of the wait: issued before it,
     // they keep the matrix pipe busy while the wave waits at the barrier.
     __builtin_amdgcn_sched_barrier(0);
-    wait_vm_lgkm0<8>();
+    wait_vm_lgkm0<LOADS>();
     barrier_nofence();
     issue(min(k + 3, steps - 1), free);
     if (k < steps) {
-      read_chunk(cur, k, 1, a1, b1);
-      mma_chunk(a0, b0);
-      read_chunk(next, k + 1, 0, a0, b0);
-      mma_chunk(a1, b1);
+      if constexpr (CH == 2) {
+        read_chunk(cur, k, 1, a1, b1);
+        mma_chunk(a0, b0);
+        read_chunk(next, k + 1, 0, a0, b0);
+        mma_chunk(a1, b1);
+      } else if (!odd) {
+        read_chunk(next, k + 1, 0, a1, b1);
+        mma_chunk(a0, b0);
+      } else {
+        read_chunk(next, k + 1, 0, a0, b0);
+        mma_chunk(a1, b1);
+      }
     }
   };
 
@@ -920,14 +940,14 @@ __global__ __launch_bounds__(kThreads, 1) void syrk_dma_kernel(const T* __restri
     issue(0, st0);
     issue(min(1, steps - 1), st1);
     issue(min(2, steps - 1), st2);
-    wait_vm_lgkm0<16>();
+    wait_vm_lgkm0<2 * LOADS>();
     barrier_nofence();
     read_chunk(st0, 0, 0, a0, b0);
     for (int k = 0; k < steps; k += kDmaStages) {
-      step_body(k, st0, st1, st3);
-      step_body(k + 1, st1, st2, st0);
-      step_body(k + 2, st2, st3, st1);
-      step_body(k + 3, st3, st0, st2);
+      step_body(k, st0, st1, st3, false);
+      step_body(k + 1, st1, st2, st0, true);
+      step_body(k + 2, st2, st3, st1, false);
+      step_body(k + 3, st3, st0, st2, true);
     }
     // No load may still be writing this workgroup's LDS when it exits.
     wait_vm_lgkm0<0>();
@@ -1253,18 +1273,26 @@ hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo,
   // function's X in 32 bits and the sign table in LDS.
   const bool dma = kind != 1 && precision != HFJK_BF16X3 && ldo <= kDmaMaxLdo &&
                    (long long)xrows * ldo < (1LL << 31);
+  // Signed columns need the LDS sign table: the one-workgroup-per-CU ring.
+  const bool two_wg = kind != 3 && !is_signed;
   if (dma && precision == HFJK_FP64 && is_signed)
-    hipLaunchKernelGGL((syrk_dma_kernel<double, true>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
+    hipLaunchKernelGGL((syrk_dma_kernel<double, true, 2>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
+                       xrows, ldo, signs, nsplit, npairs, (double*)partial);
+  else if (dma && precision == HFJK_FP64 && two_wg)
+    hipLaunchKernelGGL((syrk_dma_kernel<double, false, 1>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
                        xrows, ldo, signs, nsplit, npairs, (double*)partial);
   else if (dma && precision == HFJK_FP64)
-    hipLaunchKernelGGL((syrk_dma_kernel<double, false>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
+    hipLaunchKernelGGL((syrk_dma_kernel<double, false, 2>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
                        xrows, ldo, signs, nsplit, npairs, (double*)partial);
   else if (dma && is_signed)
-    hipLaunchKernelGGL((syrk_dma_kernel<float, true>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux, xrows,
-                       ldo, signs, nsplit, npairs, (float*)partial);
+    hipLaunchKernelGGL((syrk_dma_kernel<float, true, 2>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux,
+                       xrows, ldo, signs, nsplit, npairs, (float*)partial);
+  else if (dma && two_wg)
+    hipLaunchKernelGGL((syrk_dma_kernel<float, false, 1>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux,
+                       xrows, ldo, signs, nsplit, npairs, (float*)partial);
   else if (dma)
-    hipLaunchKernelGGL((syrk_dma_kernel<float, false>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux, xrows,
-                       ldo, signs, nsplit, npairs, (float*)partial);
+    hipLaunchKernelGGL((syrk_dma_kernel<float, false, 2>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux,
+                       xrows, ldo, signs, nsplit, npairs, (float*)partial);
   else if (precision == HFJK_FP64)
     hipLaunchKernelGGL(syrk_kernel<double>, dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux, xrows, ldo,
                        signs, nsplit, npairs, (double*)partial);

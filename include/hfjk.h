@@ -256,8 +256,9 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * used with column tiles up to 128).
  * HFJK_TUNE_SYRK_KIND: FP64/FP32 SYRK kernel: 1 = operands staged through
  * registers into one LDS buffer, 2 = operands loaded straight into a ring of
- * LDS stages (occupied widths up to 2048); 0 = the library's choice (2 where
- * it applies).
+ * LDS stages (occupied widths up to 2048; two workgroups per CU unless the
+ * columns carry signs), 3 = the same ring with one workgroup per CU; 0 = the
+ * library's choice (2 where it applies).
  * Set by the project's autotuner (hafnium_amd/autotune.py); the workspace size
  * depends on HFJK_TUNE_SYRK_SPLITS, so query it after tuning. No counterpart
  * in the reference. */

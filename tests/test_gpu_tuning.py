@@ -76,22 +76,22 @@ def test_autotune_sets_a_valid_schedule(tmp_path):
     (23, 300, 140, 8),   # several bulk iterations and a partial one
 ])
 def test_syrk_kernels_agree_with_the_oracle(precision, naux, nao, nocc, splits):
-    """The register-staged and the direct-LDS SYRK give the oracle's K."""
+    """The register-staged and both direct-LDS SYRKs (two and one workgroups per CU) give the oracle's K."""
     from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
 
     b, c, _ = synthetic.jk_inputs(17, naux, nao, nocc)
     expect = jk_ref.exchange_occupied(b, c)
     jk = MI355XCoulombExchange(b, precision=precision)
     jk.set_tuning(_lib.TUNE_SYRK_SPLITS, splits)
-    for kind in (1, 2):
+    for kind in (1, 2, 3):
         jk.set_tuning(_lib.TUNE_SYRK_KIND, kind)
         k = jk.exchange_occupied(c)
         assert jk_ref.max_relative_error(k, expect) < TOL[precision], kind
     with pytest.raises(ValueError):
-        jk.set_tuning(_lib.TUNE_SYRK_KIND, 3)
+        jk.set_tuning(_lib.TUNE_SYRK_KIND, 4)
 
 
-@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("kind", [1, 2, 3])
 def test_syrk_kernels_apply_signs(kind):
     """Signed columns (density form of an indefinite matrix) through both SYRK kernels."""
     from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
