@@ -404,6 +404,8 @@ def main(argv=None):
     ap.add_argument("--cpu-slice", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="use the library's default tiles")
+    ap.add_argument("--no-full-copy", action="store_true",
+                    help="keep only the packed FP64 tensor (no full copy for the direct-LDS half-transform)")
     ap.add_argument("--tune-cache", default=None,
                     help="JSON file of tuned tiles: read when it holds this workload, written after tuning "
                          "(a profiled run then launches exactly the timed run's kernels)")
@@ -458,6 +460,16 @@ def main(argv=None):
     torch.cuda.synchronize()
     log(f"generated B ({q}x{n}x{n}, packed {jk.packed_tensor().numel() * 8 / 1e9:.1f} GB fp64) in "
         f"{time.perf_counter() - t0:.2f} s")
+    # The full row-major FP64 copy of B (both triangles) for the direct-LDS half-transform, when it
+    # fits beside the packed tensor, its FP32/BF16 copies and the workspace (a 60 GB margin).
+    npad = -(-n // 32) * 32
+    full_bytes = 8 * q * npad * npad
+    use_full = not args.no_full_copy and torch.cuda.mem_get_info(dev)[0] > full_bytes + 60e9
+    if use_full:
+        t1 = time.perf_counter()
+        jk.expand_full()
+        torch.cuda.synchronize()
+        log(f"full FP64 copy of B ({full_bytes / 1e9:.1f} GB) in {time.perf_counter() - t1:.2f} s")
 
     f_build = flops_per_build(n, q, o)
     results = {}
@@ -552,11 +564,15 @@ def main(argv=None):
         w2 = "_w2" if (tile, rows) == (128, 128) else ""  # the FP64 128 x 128 tile's two-wave variant
         kernel = {"fp64": f"xform_kernel{w2}<double, {tile}, {rows}>", "fp32": f"xform_kernel<float, {tile}, {rows}>",
                   "bf16x3": f"xform_bf16x3_kernel<{tile}>"}[head]
+        # B bytes per launch: the packed lower triangle, or with the full copy both triangles.
+        b_bytes = q * n * (n + 1) / 2 * (8 if head == "fp64" else 4)
+        if head == "fp64" and use_full:
+            kernel = "xform_dma_kernel<1>"
+            b_bytes = q * n * n * 8.0
         roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved, "peak": peak,
                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(kernel, n, q, o),
                     "algorithmic_flops_per_launch": x_flops,
-                    "algorithmic_bytes_per_launch": q * n * (n + 1) / 2 * (8 if head == "fp64" else 4)
-                    + n * q * o * (8 if head == "fp64" else 4)}
+                    "algorithmic_bytes_per_launch": b_bytes + n * q * o * (8 if head == "fp64" else 4)}
         if syrk_ms:
             roofline["syrk_achieved"] = (q * n * n * o) / (syrk_ms * 1e-3) / 1e12
 
@@ -595,7 +611,8 @@ def main(argv=None):
         "data": "synthetic (seeded Philox generator, generated in HBM)",
         "config": {"workload": f"single DF-J/K build nao={n} naux={q} nocc={o} seed={wl['seed']}",
                    "nao": n, "naux": q, "nocc": o,
-                   "step": "J + K of D = C C^T in one call (charges from X)"},
+                   "step": "J + K of D = C C^T in one call (charges from X)",
+                   "fp64_b_storage": "packed tiles + full row-major copy" if use_full else "packed tiles"},
         "flops_per_build": f_build,
         "roofline": roofline,
         "cpu_baseline": cpu,

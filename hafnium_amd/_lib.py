@@ -25,6 +25,7 @@ TUNE_X16_BN = 2
 TUNE_SYRK_SPLITS = 3
 TUNE_X_BM = 4
 TUNE_SYRK_KIND = 5
+TUNE_X_FULL = 6
 
 OK = 0
 ERR_INPUT = 1
@@ -58,6 +59,8 @@ SIGNATURES = [
     ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_jk_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_expand_full", _c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp]),
+    ("hfjk_bind_full", _c_int, [_vp, _vp]),
     ("hfjk_basis_nfunc", _c_int, [_vp, ctypes.POINTER(_c_int)]),
     ("hfjk_int1e", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
     ("hfjk_int2c2e", _c_int, [_vp, _vp]),

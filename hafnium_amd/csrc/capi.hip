@@ -110,8 +110,8 @@ struct Layout {
   size_t dens_a, dens_w, dens_e, dens_info, dens_stats, dens_c, dens_signs, dens_end;
   // coulomb region (shares the base of the exchange region)
   size_t cw, cgp, cg, cjp, c_end;
-  // exchange region
-  size_t xc, xs, xx, xp, x_end;
+  // exchange region (xct: C^T in FP64 for the half-transform from the full copy)
+  size_t xc, xs, xx, xp, xct, x_end;
   // the Coulomb half of hfjk_jk_occ, after both regions (it runs while the
   // SYRK still reads X): row-group parts of the charges [naux][charge_groups],
   // gamma [naux], an FP32 padded orbital copy for BF16X3 (whose exchange keeps
@@ -161,6 +161,7 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   L.xx = off; off = align_up(off + (size_t)naux * L.xrows * L.ldo * es);
   const size_t tile2 = (size_t)hfjk::syrk_tile() * hfjk::syrk_tile();
   L.xp = off; off = align_up(off + (size_t)L.nsplit_k * L.npairs * tile2 * es);
+  L.xct = off; off = align_up(off + npad * L.ldo * 8);
   L.x_end = off;
   off = std::max(L.c_end, L.x_end);
   L.qt = off; off = align_up(off + (size_t)naux * hfjk::charge_groups(n) * 8);
@@ -253,17 +254,24 @@ void exchange_front(hfjk_ctx* ctx, const double* orbitals, int nocc, const doubl
   double* sg = at<double>(ws, L.xs);
   void* x = at<void>(ws, L.xx);
   const bool split = precision == HFJK_BF16X3;
+  // FP64 with a bound full copy of B: the half-transform on the direct-LDS ring.
+  const bool full = precision == HFJK_FP64 && ctx->b64_full != nullptr && ctx->x_full == 0;
+  double* ct = at<double>(ws, L.xct);
   {
     Stage st(ctx, s, "exchange_prepare");
     if (split)
       check_hip(hfjk::launch_orbitals_bf16x3(orbitals, ctx->nao, nocc, cp, s), "orbitals launch");
     else
       check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, precision, cp, s), "orbitals launch");
+    if (full) check_hip(hfjk::launch_orbitals_t(orbitals, ctx->nao, nocc, ct, s), "orbitals launch");
     check_hip(hfjk::launch_occ_signs(signs, nocc, L.ldo, sg, s), "signs launch");
   }
   {
     Stage st(ctx, s, "exchange_half_transform");
-    if (split)
+    if (full)
+      check_hip(hfjk::launch_half_transform_full(ctx->b64_full, ctx->naux, ctx->nao, ct, L.ldo, (double*)x, s),
+                "half transform launch");
+    else if (split)
       check_hip(hfjk::launch_half_transform_bf16x3((const uint16_t*)b, L.p, ctx->naux, ctx->nao, cp, L.ldo, x,
                                                    ctx->x16_bn, s),
                 "half transform launch");
@@ -435,6 +443,32 @@ int hfjk_bind(hfjk_ctx* ctx, int naux, int nao, const double* b64, const float* 
     ctx->b64 = b64;
     ctx->b32 = b32;
     ctx->b16 = b16;
+    ctx->b64_full = nullptr;  // a full copy belongs to the tensor it was expanded from
+  });
+}
+
+int hfjk_expand_full(hfjk_ctx* ctx, const double* packed, int naux, int nao, double* full, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (naux <= 0 || nao <= 0) fail(HFJK_ERR_INPUT, "the fitted tensor has shape [aux, ao, ao] with aux, ao > 0");
+    if (packed == nullptr || full == nullptr) fail(HFJK_ERR_INPUT, "null pointer");
+    const long long npad = (long long)hfjk::n_tiles(nao) * hfjk::kTile;
+    if (npad * npad * 8 >= (1LL << 31)) fail(HFJK_ERR_INPUT, "the full copy needs nao below 16384");
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    Stage st(ctx, s, "expand_full");
+    check_hip(hfjk::launch_expand_full(packed, hfjk::packed_elems(nao), naux, nao, full, s), "expand launch");
+  });
+}
+
+int hfjk_bind_full(hfjk_ctx* ctx, const double* full) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (full != nullptr && ctx->naux <= 0) fail(HFJK_ERR_INPUT, "bind the packed tensor first (hfjk_bind)");
+    if (full != nullptr && (long long)hfjk::n_tiles(ctx->nao) * hfjk::kTile * hfjk::n_tiles(ctx->nao) * hfjk::kTile * 8 >=
+                               (1LL << 31))
+      fail(HFJK_ERR_INPUT, "the full copy needs nao below 16384");
+    ctx->b64_full = full;
   });
 }
 
@@ -677,6 +711,10 @@ int hfjk_set_tuning(hfjk_ctx* ctx, int param, int value) {
         if (value > 3) fail(HFJK_ERR_INPUT, "the SYRK kernel kind is 0, 1, 2 or 3");
         ctx->syrk_kind = value;
         break;
+      case HFJK_TUNE_X_FULL:
+        if (value > 1) fail(HFJK_ERR_INPUT, "HFJK_TUNE_X_FULL is 0 (use a bound full copy) or 1 (ignore it)");
+        ctx->x_full = value;
+        break;
       default:
         fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
@@ -693,6 +731,7 @@ int hfjk_get_tuning(hfjk_ctx* ctx, int param, int* value) {
       case HFJK_TUNE_SYRK_SPLITS: *value = ctx->syrk_splits; break;
       case HFJK_TUNE_X_BM: *value = ctx->x_bm; break;
       case HFJK_TUNE_SYRK_KIND: *value = ctx->syrk_kind; break;
+      case HFJK_TUNE_X_FULL: *value = ctx->x_full; break;
       default: fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
   });

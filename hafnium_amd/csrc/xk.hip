@@ -805,12 +805,14 @@ __device__ __forceinline__ void wait_vm_lgkm0() {
 // from moving memory accesses across it.
 __device__ __forceinline__ void barrier_nofence() { asm volatile("s_barrier" ::: "memory"); }
 
-// SIGNED = false: every sign is +1 on the occupied columns (the padded
-// columns of X are zero), so the sign table is neither built nor applied.
-template <typename T, bool SIGNED, int CH>
-__global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
-    const T* __restrict__ x, int naux, int xrows, int ldo, const double* __restrict__ signs, int nsplit, int npairs,
-    T* __restrict__ partial) {
+// The ring: acc += sum over `steps` k steps of (A rows) x (B rows)^T, the
+// 128 rows of each operand read at step k from src(k, operand) plus this
+// lane's byte offsets off[] (operand A = chunks of waves 0-1, B = waves 2-3).
+// With SIGNED, the B operand's k values are scaled by sgn[sign_at(k) + k'].
+template <typename T, bool SIGNED, int CH, typename Src, typename SignAt>
+__device__ __forceinline__ void dma_ring(const Src& src, const int (&off)[4 * CH], int steps,
+                                         const T* __restrict__ sgn, const SignAt& sign_at,
+                                         typename Op<T>::acc_t (&acc)[4][4]) {
   typedef SyrkDma<T, CH> D;
   typedef typename D::unit unit;
   constexpr int E = D::E;
@@ -819,69 +821,30 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
   __shared__ __attribute__((aligned(16))) T st1[D::STAGE];
   __shared__ __attribute__((aligned(16))) T st2[D::STAGE];
   __shared__ __attribute__((aligned(16))) T st3[D::STAGE];
-  __shared__ __attribute__((aligned(16))) T sgn[SIGNED ? kDmaMaxLdo : 1];
-
-  int s, pair;
-  syrk_coords(blockIdx.x, npairs, s, pair);
-  int mb, nb;
-  tile_of(pair, mb, nb);
-  const int qb = (int)((long long)s * naux / nsplit);
-  const int qe = (int)((long long)(s + 1) * naux / nsplit);
-  const int t = threadIdx.x;
-  const int lane = t & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = wave & 1;
   const int wn = wave >> 1;
+  const int operand = wave >> 1;  // the chunks of waves 0-1 are A's, of waves 2-3 B's
 
-  if (SIGNED)
-    for (int i = t; i < ldo; i += kThreads) sgn[i] = (T)signs[i];
-
-  // Byte offsets (within one aux function's X, from k = i0) of this lane's
-  // group in the wave's LOADS chunks: chunks LOADS wave .. LOADS wave +
-  // LOADS - 1 of the 16 CH of a stage (the first 8 CH operand A = row block
-  // mb, the rest operand B = row block nb; chunk = CH * (16-row block) + k
-  // part). Rows past xrows (the last 128-row block may hold only 64) read
-  // row 0 instead: their products land in K rows and columns past nao, which
-  // are never read.
-  int off[LOADS];
-#pragma unroll
-  for (int v = 0; v < LOADS; ++v) {
-    const int ch = wave * LOADS + v;
-    const int blk = ch >= 8 * CH ? nb : mb;
-    int row = blk * kSyrkTile + ((ch / CH) % 8) * 16 + (lane & 15);
-    row = row < xrows ? row : 0;
-    off[v] = (row * ldo + (ch % CH) * D::KC + (lane >> 4) * E) * (int)sizeof(T);
-  }
-  const int nch = ldo / D::KS;
-  const int steps = (qe - qb) * nch;
   auto issue = [&](int step, T* stage) {
-    const int qq = step / nch;
-    const T* xq = x + (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * D::KS;
+    const T* base = src(step, operand);
 #pragma unroll
     for (int v = 0; v < LOADS; ++v)
-      __builtin_amdgcn_global_load_lds((const void*)((const char*)xq + (unsigned)off[v]),
+      __builtin_amdgcn_global_load_lds((const void*)((const char*)base + (unsigned)off[v]),
                                        (lds_void_t*)(stage + (wave * LOADS + v) * D::CHUNK), 16, 0, 0);
   };
-
-  typename Op<T>::acc_t acc[4][4];
-#pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
 
   // Operands of one chunk (16 rows x KC k per 16-row block) of a stage, with
   // the column signs applied to the B side.
   auto read_chunk = [&](const T* stage, int step, int c, unit (&a)[4], unit (&b)[4]) {
-    const int i0 = (step % nch) * D::KS;
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) a[mi] = *(const unit*)(stage + ((((wm * 4 + mi) * CH + c) * 64 + lane) * E));
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni)
       b[ni] = *(const unit*)(stage + D::STAGE / 2 + ((((wn * 4 + ni) * CH + c) * 64 + lane) * E));
-    if (SIGNED) {
-      const unit sg = *(const unit*)(sgn + i0 + c * D::KC + (lane >> 4) * E);
+    if constexpr (SIGNED) {
+      const unit sg = *(const unit*)(sgn + sign_at(step) + c * D::KC + (lane >> 4) * E);
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni) b[ni] *= sg;
     }
@@ -935,7 +898,6 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
     }
   };
 
-  __syncthreads();  // sign table written
   if (steps > 0) {
     issue(0, st0);
     issue(min(1, steps - 1), st1);
@@ -949,9 +911,72 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
       step_body(k + 2, st2, st3, st1, false);
       step_body(k + 3, st3, st0, st2, true);
     }
-    // No load may still be writing this workgroup's LDS when it exits.
+    // No load may still be writing this workgroup's LDS when it returns.
     wait_vm_lgkm0<0>();
   }
+}
+
+// SIGNED = false: every sign is +1 on the occupied columns (the padded
+// columns of X are zero), so the sign table is neither built nor applied.
+template <typename T, bool SIGNED, int CH>
+__global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
+    const T* __restrict__ x, int naux, int xrows, int ldo, const double* __restrict__ signs, int nsplit, int npairs,
+    T* __restrict__ partial) {
+  typedef SyrkDma<T, CH> D;
+  constexpr int E = D::E;
+  constexpr int LOADS = D::LOADS;
+  __shared__ __attribute__((aligned(16))) T sgn[SIGNED ? kDmaMaxLdo : 1];
+
+  int s, pair;
+  syrk_coords(blockIdx.x, npairs, s, pair);
+  int mb, nb;
+  tile_of(pair, mb, nb);
+  const int qb = (int)((long long)s * naux / nsplit);
+  const int qe = (int)((long long)(s + 1) * naux / nsplit);
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+
+  if (SIGNED)
+    for (int i = t; i < ldo; i += kThreads) sgn[i] = (T)signs[i];
+
+  // Byte offsets (within one aux function's X, from k = i0) of this lane's
+  // group in the wave's LOADS chunks: chunks LOADS wave .. LOADS wave +
+  // LOADS - 1 of the 16 CH of a stage (the first 8 CH operand A = row block
+  // mb, the rest operand B = row block nb; chunk = CH * (16-row block) + k
+  // part). Rows past xrows (the last 128-row block may hold only 64) read
+  // row 0 instead: their products land in K rows and columns past nao, which
+  // are never read.
+  int off[LOADS];
+#pragma unroll
+  for (int v = 0; v < LOADS; ++v) {
+    const int ch = wave * LOADS + v;
+    const int blk = ch >= 8 * CH ? nb : mb;
+    int row = blk * kSyrkTile + ((ch / CH) % 8) * 16 + (lane & 15);
+    row = row < xrows ? row : 0;
+    off[v] = (row * ldo + (ch % CH) * D::KC + (lane >> 4) * E) * (int)sizeof(T);
+  }
+  const int nch = ldo / D::KS;
+  const int steps = (qe - qb) * nch;
+  // Step k covers k values i0 = (k % nch) KS of aux function qb + k / nch.
+  auto src = [&](int step, int) -> const T* {
+    const int qq = step / nch;
+    return x + (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * D::KS;
+  };
+  auto sign_at = [&](int step) { return (step % nch) * D::KS; };
+
+  typename Op<T>::acc_t acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
+
+  __syncthreads();  // sign table written
+  dma_ring<T, SIGNED, CH>(src, off, steps, sgn, sign_at, acc);
 
   T* dst = partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile);
 #pragma unroll
@@ -965,6 +990,98 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
         dst[row * kSyrkTile + col] = acc[mi][ni][r];
       }
     }
+}
+
+// FP64 half-transform on the ring, from a full row-major copy of B
+// (bfull[q][npad][npad], both triangles) and C^T (ct[ldo][npad], zero past
+// nocc and past nao): X_q[m][i] = sum_k B_q[m][k] C^T[i][k], one 128 x 128
+// tile (row block mb, column block nb) of one aux function per workgroup.
+// Both operands are rows along k, as in the SYRK, so the ring is the same.
+// X rows past npad (xrows may exceed it by up to 32) are written as zeros.
+template <int CH>
+__global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void xform_dma_kernel(
+    const double* __restrict__ bfull, int npad, const double* __restrict__ ct, int ldo, double* __restrict__ x,
+    int xrows, int qbase, int nblocks_n, int nblocks_m, int nwg) {
+  typedef SyrkDma<double, CH> D;
+  constexpr int E = D::E;
+  constexpr int LOADS = D::LOADS;
+  const int logical = xcd_remap(blockIdx.x, nwg);
+  const int nb = logical % nblocks_n;
+  const int mb = (logical / nblocks_n) % nblocks_m;
+  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+
+  // Lane offsets: operand A rows m of B_q (clamped below npad), operand B
+  // rows i of C^T (clamped below ldo); both have row stride npad.
+  int off[LOADS];
+#pragma unroll
+  for (int v = 0; v < LOADS; ++v) {
+    const int ch = wave * LOADS + v;
+    const bool is_b = ch >= 8 * CH;
+    int row = (is_b ? nb : mb) * kSyrkTile + ((ch / CH) % 8) * 16 + (lane & 15);
+    row = row < (is_b ? ldo : npad) ? row : 0;
+    off[v] = (row * npad + (ch % CH) * D::KC + (lane >> 4) * E) * 8;
+  }
+  const double* bq = bfull + (long long)q * npad * npad;
+  auto src = [&](int step, int operand) -> const double* { return (operand ? ct : bq) + step * D::KS; };
+  auto no_sign = [](int) { return 0; };
+
+  typename Op<double>::acc_t acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = 0.0;
+
+  dma_ring<double, false, CH>(src, off, npad / D::KS, (const double*)nullptr, no_sign, acc);
+
+  double* xq = x + (long long)q * xrows * ldo;
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const int col = nb * kSyrkTile + wn * 64 + ni * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = mb * kSyrkTile + wm * 64 + mi * 16 + Op<double>::row(lane, r);
+        if (row < xrows && col < ldo) xq[(long long)row * ldo + col] = row < npad ? acc[mi][ni][r] : 0.0;
+      }
+    }
+}
+
+// Full row-major copy of tiled packed B: out[q][r][c] = B_q[r][c] for
+// r, c < npad (both triangles; zero past nao, as the packed padding).
+__global__ __launch_bounds__(kThreads) void expand_full_kernel(const double* __restrict__ packed, long long p,
+                                                                int naux, int npad, double* __restrict__ out) {
+  const long long per = (long long)npad * npad;
+  const long long total = per * naux;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const long long q = idx / per;
+    const int rc = (int)(idx - q * per);
+    const int r = rc / npad, c = rc % npad;
+    const int tr = r / kTile, tc = c / kTile;
+    const long long src = tr >= tc ? tidx(tr, tc) * kTileElems + (r % kTile) * kTile + (c % kTile)
+                                   : tidx(tc, tr) * kTileElems + (c % kTile) * kTile + (r % kTile);
+    out[idx] = packed[q * p + src];
+  }
+}
+
+// C^T in FP64: ct[i][k] = C[k][i] for k < nao, i < nocc, zero elsewhere
+// ([ldo][npad]).
+__global__ __launch_bounds__(kThreads) void orbitals_t_kernel(const double* __restrict__ c, int nao, int nocc,
+                                                               int npad, int ldo, double* __restrict__ ct) {
+  const long long total = (long long)npad * ldo;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int i = (int)(idx / npad);
+    const int k = (int)(idx % npad);
+    ct[idx] = (k < nao && i < nocc) ? c[(long long)k * nocc + i] : 0.0;
+  }
 }
 
 struct Syrk16Smem {
@@ -1173,6 +1290,39 @@ hipError_t launch_x16_t(const uint16_t* bhi, const uint16_t* blo, long long p, i
 }
 
 }  // namespace
+
+hipError_t launch_expand_full(const double* packed, long long p, int naux, int nao, double* out, hipStream_t s) {
+  const int npad = n_tiles(nao) * kTile;
+  hipLaunchKernelGGL(expand_full_kernel, dim3(grid_for((long long)naux * npad * npad)), dim3(kThreads), 0, s, packed,
+                     p, naux, npad, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_orbitals_t(const double* c, int nao, int nocc, double* ct, hipStream_t s) {
+  const int npad = n_tiles(nao) * kTile;
+  const int ldo = occ_ld(nocc);
+  hipLaunchKernelGGL(orbitals_t_kernel, dim3(grid_for((long long)npad * ldo)), dim3(kThreads), 0, s, c, nao, nocc,
+                     npad, ldo, ct);
+  return hipGetLastError();
+}
+
+hipError_t launch_half_transform_full(const double* bfull, int naux, int nao, const double* ct, int ldo, double* x,
+                                      hipStream_t s) {
+  const int npad = n_tiles(nao) * kTile;
+  const int xrows = x_rows(nao);
+  const int nbn = (ldo + kSyrkTile - 1) / kSyrkTile;
+  const int nbm = (xrows + kSyrkTile - 1) / kSyrkTile;
+  const int per = std::max(1, (1 << 24) / (nbn * nbm));  // keep a launch below 2^32 work-items
+  for (int base = 0; base < naux; base += per) {
+    const int count = std::min(per, naux - base);
+    const int nwg = nbn * nbm * count;
+    hipLaunchKernelGGL((xform_dma_kernel<1>), dim3((unsigned)nwg), dim3(kThreads), 0, s, bfull, npad, ct, ldo, x,
+                       xrows, base, nbn, nbm, nwg);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
 
 hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s) {
   const int npad = n_tiles(nao) * kTile;

@@ -108,6 +108,17 @@ HFJK_API int hfjk_split(hfjk_ctx *ctx, const double *packed, long long count, in
 HFJK_API int hfjk_bind(hfjk_ctx *ctx, int naux, int nao, const double *b64, const float *b32,
               const uint16_t *b16);
 
+/* Optional full row-major FP64 copy of the bound tensor, for the FP64
+ * half-transform: hfjk_expand_full writes full[q][npad][npad] (npad =
+ * 32 ceil(nao / 32), both triangles, zero past nao; 8 naux npad^2 bytes) from
+ * the tiled packed tensor; hfjk_bind_full borrows it (NULL unbinds;
+ * hfjk_bind unbinds it too). With it the FP64 half-transform streams both
+ * operands straight into LDS (twice the memory of the packed tensor for about
+ * ten percent less time). No counterpart in the reference (its B is always
+ * full, [naux, nao, nao]: coulomb_exchange.py:124-145). */
+HFJK_API int hfjk_expand_full(hfjk_ctx *ctx, const double *packed, int naux, int nao, double *full, void *stream);
+HFJK_API int hfjk_bind_full(hfjk_ctx *ctx, const double *full);
+
 /* Workspace bytes that hfjk_coulomb / hfjk_exchange_occ /
  * hfjk_exchange_density need for the bound tensor, `nocc` occupied columns
  * (use nao for hfjk_exchange_density) and `precision`. */
@@ -259,6 +270,8 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * LDS stages (occupied widths up to 2048; two workgroups per CU unless the
  * columns carry signs), 3 = the same ring with one workgroup per CU; 0 = the
  * library's choice (2 where it applies).
+ * HFJK_TUNE_X_FULL: 0 = the FP64 half-transform uses the full copy of B when
+ * one is bound (hfjk_bind_full), 1 = it ignores it.
  * Set by the project's autotuner (hafnium_amd/autotune.py); the workspace size
  * depends on HFJK_TUNE_SYRK_SPLITS, so query it after tuning. No counterpart
  * in the reference. */
@@ -267,6 +280,7 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
 #define HFJK_TUNE_SYRK_SPLITS 3
 #define HFJK_TUNE_X_BM 4
 #define HFJK_TUNE_SYRK_KIND 5
+#define HFJK_TUNE_X_FULL 6
 HFJK_API int hfjk_set_tuning(hfjk_ctx *ctx, int param, int value);
 HFJK_API int hfjk_get_tuning(hfjk_ctx *ctx, int param, int *value);
 
