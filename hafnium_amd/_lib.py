@@ -59,8 +59,8 @@ SIGNATURES = [
     ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_jk_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
-    ("hfjk_expand_full", _c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp]),
-    ("hfjk_bind_full", _c_int, [_vp, _vp]),
+    ("hfjk_expand_full", _c_int, [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp]),
+    ("hfjk_bind_full", _c_int, [_vp, _c_int, _vp]),
     ("hfjk_basis_nfunc", _c_int, [_vp, ctypes.POINTER(_c_int)]),
     ("hfjk_int1e", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
     ("hfjk_int2c2e", _c_int, [_vp, _vp]),

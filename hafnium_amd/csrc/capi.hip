@@ -254,22 +254,25 @@ void exchange_front(hfjk_ctx* ctx, const double* orbitals, int nocc, const doubl
   double* sg = at<double>(ws, L.xs);
   void* x = at<void>(ws, L.xx);
   const bool split = precision == HFJK_BF16X3;
-  // FP64 with a bound full copy of B: the half-transform on the direct-LDS ring.
-  const bool full = precision == HFJK_FP64 && ctx->b64_full != nullptr && ctx->x_full == 0;
-  double* ct = at<double>(ws, L.xct);
+  // FP64 / FP32 with a bound full copy of B: the half-transform on the direct-LDS ring.
+  const void* bfull = precision == HFJK_FP64 ? (const void*)ctx->b64_full
+                      : precision == HFJK_FP32 ? (const void*)ctx->b32_full
+                                               : nullptr;
+  const bool full = bfull != nullptr && ctx->x_full == 0;
+  void* ct = at<void>(ws, L.xct);
   {
     Stage st(ctx, s, "exchange_prepare");
     if (split)
       check_hip(hfjk::launch_orbitals_bf16x3(orbitals, ctx->nao, nocc, cp, s), "orbitals launch");
     else
       check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, precision, cp, s), "orbitals launch");
-    if (full) check_hip(hfjk::launch_orbitals_t(orbitals, ctx->nao, nocc, ct, s), "orbitals launch");
+    if (full) check_hip(hfjk::launch_orbitals_t(orbitals, ctx->nao, nocc, precision, ct, s), "orbitals launch");
     check_hip(hfjk::launch_occ_signs(signs, nocc, L.ldo, sg, s), "signs launch");
   }
   {
     Stage st(ctx, s, "exchange_half_transform");
     if (full)
-      check_hip(hfjk::launch_half_transform_full(ctx->b64_full, ctx->naux, ctx->nao, ct, L.ldo, (double*)x, s),
+      check_hip(hfjk::launch_half_transform_full(bfull, precision, ctx->naux, ctx->nao, ct, L.ldo, x, s),
                 "half transform launch");
     else if (split)
       check_hip(hfjk::launch_half_transform_bf16x3((const uint16_t*)b, L.p, ctx->naux, ctx->nao, cp, L.ldo, x,
@@ -444,12 +447,15 @@ int hfjk_bind(hfjk_ctx* ctx, int naux, int nao, const double* b64, const float* 
     ctx->b32 = b32;
     ctx->b16 = b16;
     ctx->b64_full = nullptr;  // a full copy belongs to the tensor it was expanded from
+    ctx->b32_full = nullptr;
   });
 }
 
-int hfjk_expand_full(hfjk_ctx* ctx, const double* packed, int naux, int nao, double* full, void* stream) {
+int hfjk_expand_full(hfjk_ctx* ctx, const double* packed, int naux, int nao, int precision, void* full,
+                     void* stream) {
   return guarded([&] {
     check_ctx(ctx);
+    if (precision != HFJK_FP64 && precision != HFJK_FP32) fail(HFJK_ERR_INPUT, "full copies are FP64 or FP32");
     if (naux <= 0 || nao <= 0) fail(HFJK_ERR_INPUT, "the fitted tensor has shape [aux, ao, ao] with aux, ao > 0");
     if (packed == nullptr || full == nullptr) fail(HFJK_ERR_INPUT, "null pointer");
     const long long npad = (long long)hfjk::n_tiles(nao) * hfjk::kTile;
@@ -457,18 +463,23 @@ int hfjk_expand_full(hfjk_ctx* ctx, const double* packed, int naux, int nao, dou
     begin_call(ctx);
     hipStream_t s = (hipStream_t)stream;
     Stage st(ctx, s, "expand_full");
-    check_hip(hfjk::launch_expand_full(packed, hfjk::packed_elems(nao), naux, nao, full, s), "expand launch");
+    check_hip(hfjk::launch_expand_full(packed, hfjk::packed_elems(nao), naux, nao, precision, full, s),
+              "expand launch");
   });
 }
 
-int hfjk_bind_full(hfjk_ctx* ctx, const double* full) {
+int hfjk_bind_full(hfjk_ctx* ctx, int precision, const void* full) {
   return guarded([&] {
     check_ctx(ctx);
+    if (precision != HFJK_FP64 && precision != HFJK_FP32) fail(HFJK_ERR_INPUT, "full copies are FP64 or FP32");
     if (full != nullptr && ctx->naux <= 0) fail(HFJK_ERR_INPUT, "bind the packed tensor first (hfjk_bind)");
     if (full != nullptr && (long long)hfjk::n_tiles(ctx->nao) * hfjk::kTile * hfjk::n_tiles(ctx->nao) * hfjk::kTile * 8 >=
                                (1LL << 31))
       fail(HFJK_ERR_INPUT, "the full copy needs nao below 16384");
-    ctx->b64_full = full;
+    if (precision == HFJK_FP64)
+      ctx->b64_full = (const double*)full;
+    else
+      ctx->b32_full = (const float*)full;
   });
 }
 

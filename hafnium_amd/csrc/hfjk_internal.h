@@ -80,8 +80,10 @@ struct hfjk_ctx {
   int syrk_splits = 0;
   int x_bm = 0;  // 0 = 64
   int syrk_kind = 0;  // 0 = default, 1 = register-staged, 2/3 = direct LDS loads (two / one workgroups per CU)
-  // Borrowed full row-major FP64 copy of B ([naux][npad][npad]) and its use (0 = when bound, 1 = never).
+  // Borrowed full row-major copies of B ([naux][npad][npad], FP64 and FP32) and their use (0 = when
+  // bound, 1 = never).
   const double* b64_full = nullptr;
+  const float* b32_full = nullptr;
   int x_full = 0;
   // Second stream of hfjk_jk_occ (default priority, created on first use):
   // the Coulomb half runs there beside the SYRK, fenced by fork/join events.
@@ -172,14 +174,16 @@ hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux
 int charge_groups(int nao);
 hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao, int ldo, const void* cpad,
                                  double* parts, double* gamma, hipStream_t s);
-// FP64 half-transform from a full row-major copy of B (xk.hip):
-// launch_expand_full writes bfull[q][npad][npad] (npad = 32 ceil(nao / 32))
-// from the tiled packed tensor; launch_orbitals_t writes C^T [ldo][npad];
+// FP64 / FP32 half-transform from a full row-major copy of B (xk.hip):
+// launch_expand_full writes bfull[q][npad][npad] in the precision's type
+// (npad = 32 ceil(nao / 32)) from the tiled packed FP64 tensor;
+// launch_orbitals_t writes C^T [ldo][npad] in that type;
 // launch_half_transform_full computes X = B C on the direct-LDS ring.
-hipError_t launch_expand_full(const double* packed, long long p, int naux, int nao, double* out, hipStream_t s);
-hipError_t launch_orbitals_t(const double* c, int nao, int nocc, double* ct, hipStream_t s);
-hipError_t launch_half_transform_full(const double* bfull, int naux, int nao, const double* ct, int ldo, double* x,
-                                      hipStream_t s);
+hipError_t launch_expand_full(const double* packed, long long p, int naux, int nao, int precision, void* out,
+                              hipStream_t s);
+hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, void* ct, hipStream_t s);
+hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
+                                      void* x, hipStream_t s);
 // Lower-triangle 128 x 128 output tiles of the SYRK and the tile edge.
 int syrk_pairs(int nao);
 int syrk_tile();

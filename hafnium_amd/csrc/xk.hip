@@ -992,17 +992,17 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
     }
 }
 
-// FP64 half-transform on the ring, from a full row-major copy of B
+// FP64 / FP32 half-transform on the ring, from a full row-major copy of B
 // (bfull[q][npad][npad], both triangles) and C^T (ct[ldo][npad], zero past
 // nocc and past nao): X_q[m][i] = sum_k B_q[m][k] C^T[i][k], one 128 x 128
 // tile (row block mb, column block nb) of one aux function per workgroup.
 // Both operands are rows along k, as in the SYRK, so the ring is the same.
 // X rows past npad (xrows may exceed it by up to 32) are written as zeros.
-template <int CH>
+template <typename T, int CH>
 __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void xform_dma_kernel(
-    const double* __restrict__ bfull, int npad, const double* __restrict__ ct, int ldo, double* __restrict__ x,
-    int xrows, int qbase, int nblocks_n, int nblocks_m, int nwg) {
-  typedef SyrkDma<double, CH> D;
+    const T* __restrict__ bfull, int npad, const T* __restrict__ ct, int ldo, T* __restrict__ x, int xrows,
+    int qbase, int nblocks_n, int nblocks_m, int nwg) {
+  typedef SyrkDma<T, CH> D;
   constexpr int E = D::E;
   constexpr int LOADS = D::LOADS;
   const int logical = xcd_remap(blockIdx.x, nwg);
@@ -1023,23 +1023,23 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void xform_dma_kernel(
     const bool is_b = ch >= 8 * CH;
     int row = (is_b ? nb : mb) * kSyrkTile + ((ch / CH) % 8) * 16 + (lane & 15);
     row = row < (is_b ? ldo : npad) ? row : 0;
-    off[v] = (row * npad + (ch % CH) * D::KC + (lane >> 4) * E) * 8;
+    off[v] = (row * npad + (ch % CH) * D::KC + (lane >> 4) * E) * (int)sizeof(T);
   }
-  const double* bq = bfull + (long long)q * npad * npad;
-  auto src = [&](int step, int operand) -> const double* { return (operand ? ct : bq) + step * D::KS; };
+  const T* bq = bfull + (long long)q * npad * npad;
+  auto src = [&](int step, int operand) -> const T* { return (operand ? ct : bq) + step * D::KS; };
   auto no_sign = [](int) { return 0; };
 
-  typename Op<double>::acc_t acc[4][4];
+  typename Op<T>::acc_t acc[4][4];
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = 0.0;
+      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
 
-  dma_ring<double, false, CH>(src, off, npad / D::KS, (const double*)nullptr, no_sign, acc);
+  dma_ring<T, false, CH>(src, off, npad / D::KS, (const T*)nullptr, no_sign, acc);
 
-  double* xq = x + (long long)q * xrows * ldo;
+  T* xq = x + (long long)q * xrows * ldo;
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
@@ -1047,16 +1047,17 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void xform_dma_kernel(
       const int col = nb * kSyrkTile + wn * 64 + ni * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = mb * kSyrkTile + wm * 64 + mi * 16 + Op<double>::row(lane, r);
-        if (row < xrows && col < ldo) xq[(long long)row * ldo + col] = row < npad ? acc[mi][ni][r] : 0.0;
+        const int row = mb * kSyrkTile + wm * 64 + mi * 16 + Op<T>::row(lane, r);
+        if (row < xrows && col < ldo) xq[(long long)row * ldo + col] = row < npad ? acc[mi][ni][r] : (T)0;
       }
     }
 }
 
-// Full row-major copy of tiled packed B: out[q][r][c] = B_q[r][c] for
+// Full row-major copy of tiled packed B in T: out[q][r][c] = B_q[r][c] for
 // r, c < npad (both triangles; zero past nao, as the packed padding).
+template <typename T>
 __global__ __launch_bounds__(kThreads) void expand_full_kernel(const double* __restrict__ packed, long long p,
-                                                                int naux, int npad, double* __restrict__ out) {
+                                                                int naux, int npad, T* __restrict__ out) {
   const long long per = (long long)npad * npad;
   const long long total = per * naux;
   for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
@@ -1067,20 +1068,21 @@ __global__ __launch_bounds__(kThreads) void expand_full_kernel(const double* __r
     const int tr = r / kTile, tc = c / kTile;
     const long long src = tr >= tc ? tidx(tr, tc) * kTileElems + (r % kTile) * kTile + (c % kTile)
                                    : tidx(tc, tr) * kTileElems + (c % kTile) * kTile + (r % kTile);
-    out[idx] = packed[q * p + src];
+    out[idx] = (T)packed[q * p + src];
   }
 }
 
-// C^T in FP64: ct[i][k] = C[k][i] for k < nao, i < nocc, zero elsewhere
+// C^T in T: ct[i][k] = C[k][i] for k < nao, i < nocc, zero elsewhere
 // ([ldo][npad]).
+template <typename T>
 __global__ __launch_bounds__(kThreads) void orbitals_t_kernel(const double* __restrict__ c, int nao, int nocc,
-                                                               int npad, int ldo, double* __restrict__ ct) {
+                                                               int npad, int ldo, T* __restrict__ ct) {
   const long long total = (long long)npad * ldo;
   for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
        idx += (long long)gridDim.x * kThreads) {
     const int i = (int)(idx / npad);
     const int k = (int)(idx % npad);
-    ct[idx] = (k < nao && i < nocc) ? c[(long long)k * nocc + i] : 0.0;
+    ct[idx] = (T)((k < nao && i < nocc) ? c[(long long)k * nocc + i] : 0.0);
   }
 }
 
@@ -1291,23 +1293,30 @@ hipError_t launch_x16_t(const uint16_t* bhi, const uint16_t* blo, long long p, i
 
 }  // namespace
 
-hipError_t launch_expand_full(const double* packed, long long p, int naux, int nao, double* out, hipStream_t s) {
+hipError_t launch_expand_full(const double* packed, long long p, int naux, int nao, int precision, void* out,
+                              hipStream_t s) {
   const int npad = n_tiles(nao) * kTile;
-  hipLaunchKernelGGL(expand_full_kernel, dim3(grid_for((long long)naux * npad * npad)), dim3(kThreads), 0, s, packed,
-                     p, naux, npad, out);
+  const dim3 grid(grid_for((long long)naux * npad * npad));
+  if (precision == HFJK_FP64)
+    hipLaunchKernelGGL(expand_full_kernel<double>, grid, dim3(kThreads), 0, s, packed, p, naux, npad, (double*)out);
+  else
+    hipLaunchKernelGGL(expand_full_kernel<float>, grid, dim3(kThreads), 0, s, packed, p, naux, npad, (float*)out);
   return hipGetLastError();
 }
 
-hipError_t launch_orbitals_t(const double* c, int nao, int nocc, double* ct, hipStream_t s) {
+hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, void* ct, hipStream_t s) {
   const int npad = n_tiles(nao) * kTile;
   const int ldo = occ_ld(nocc);
-  hipLaunchKernelGGL(orbitals_t_kernel, dim3(grid_for((long long)npad * ldo)), dim3(kThreads), 0, s, c, nao, nocc,
-                     npad, ldo, ct);
+  const dim3 grid(grid_for((long long)npad * ldo));
+  if (precision == HFJK_FP64)
+    hipLaunchKernelGGL(orbitals_t_kernel<double>, grid, dim3(kThreads), 0, s, c, nao, nocc, npad, ldo, (double*)ct);
+  else
+    hipLaunchKernelGGL(orbitals_t_kernel<float>, grid, dim3(kThreads), 0, s, c, nao, nocc, npad, ldo, (float*)ct);
   return hipGetLastError();
 }
 
-hipError_t launch_half_transform_full(const double* bfull, int naux, int nao, const double* ct, int ldo, double* x,
-                                      hipStream_t s) {
+hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
+                                      void* x, hipStream_t s) {
   const int npad = n_tiles(nao) * kTile;
   const int xrows = x_rows(nao);
   const int nbn = (ldo + kSyrkTile - 1) / kSyrkTile;
@@ -1316,8 +1325,12 @@ hipError_t launch_half_transform_full(const double* bfull, int naux, int nao, co
   for (int base = 0; base < naux; base += per) {
     const int count = std::min(per, naux - base);
     const int nwg = nbn * nbm * count;
-    hipLaunchKernelGGL((xform_dma_kernel<1>), dim3((unsigned)nwg), dim3(kThreads), 0, s, bfull, npad, ct, ldo, x,
-                       xrows, base, nbn, nbm, nwg);
+    if (precision == HFJK_FP64)
+      hipLaunchKernelGGL((xform_dma_kernel<double, 1>), dim3((unsigned)nwg), dim3(kThreads), 0, s,
+                         (const double*)bfull, npad, (const double*)ct, ldo, (double*)x, xrows, base, nbn, nbm, nwg);
+    else
+      hipLaunchKernelGGL((xform_dma_kernel<float, 1>), dim3((unsigned)nwg), dim3(kThreads), 0, s,
+                         (const float*)bfull, npad, (const float*)ct, ldo, (float*)x, xrows, base, nbn, nbm, nwg);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
   }

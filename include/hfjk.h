@@ -108,16 +108,18 @@ HFJK_API int hfjk_split(hfjk_ctx *ctx, const double *packed, long long count, in
 HFJK_API int hfjk_bind(hfjk_ctx *ctx, int naux, int nao, const double *b64, const float *b32,
               const uint16_t *b16);
 
-/* Optional full row-major FP64 copy of the bound tensor, for the FP64
- * half-transform: hfjk_expand_full writes full[q][npad][npad] (npad =
- * 32 ceil(nao / 32), both triangles, zero past nao; 8 naux npad^2 bytes) from
- * the tiled packed tensor; hfjk_bind_full borrows it (NULL unbinds;
- * hfjk_bind unbinds it too). With it the FP64 half-transform streams both
- * operands straight into LDS (twice the memory of the packed tensor for about
- * ten percent less time). No counterpart in the reference (its B is always
- * full, [naux, nao, nao]: coulomb_exchange.py:124-145). */
-HFJK_API int hfjk_expand_full(hfjk_ctx *ctx, const double *packed, int naux, int nao, double *full, void *stream);
-HFJK_API int hfjk_bind_full(hfjk_ctx *ctx, const double *full);
+/* Optional full row-major copies of the bound tensor, for the FP64 and
+ * FP32 half-transforms: hfjk_expand_full writes full[q][npad][npad] in
+ * precision HFJK_FP64 (double) or HFJK_FP32 (float) (npad = 32 ceil(nao / 32),
+ * both triangles, zero past nao) from the tiled packed FP64 tensor;
+ * hfjk_bind_full borrows it for that precision (NULL unbinds; hfjk_bind
+ * unbinds both). With it the half-transform of that precision streams both
+ * operands straight into LDS (twice the memory of the packed copy for
+ * 12-25 % less time). No counterpart in the reference (its B is always full,
+ * [naux, nao, nao]: coulomb_exchange.py:124-145). */
+HFJK_API int hfjk_expand_full(hfjk_ctx *ctx, const double *packed, int naux, int nao, int precision, void *full,
+                              void *stream);
+HFJK_API int hfjk_bind_full(hfjk_ctx *ctx, int precision, const void *full);
 
 /* Workspace bytes that hfjk_coulomb / hfjk_exchange_occ /
  * hfjk_exchange_density need for the bound tensor, `nocc` occupied columns
@@ -270,8 +272,8 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * LDS stages (occupied widths up to 2048; two workgroups per CU unless the
  * columns carry signs), 3 = the same ring with one workgroup per CU; 0 = the
  * library's choice (2 where it applies).
- * HFJK_TUNE_X_FULL: 0 = the FP64 half-transform uses the full copy of B when
- * one is bound (hfjk_bind_full), 1 = it ignores it.
+ * HFJK_TUNE_X_FULL: 0 = the FP64/FP32 half-transform uses the full copy of B
+ * of its precision when one is bound (hfjk_bind_full), 1 = it ignores it.
  * Set by the project's autotuner (hafnium_amd/autotune.py); the workspace size
  * depends on HFJK_TUNE_SYRK_SPLITS, so query it after tuning. No counterpart
  * in the reference. */

@@ -226,28 +226,30 @@ def test_mixed_copies_of_a_tensor_beyond_two_to_the_32_elements(torch):
     assert np.all(np.abs(value - x) <= 2.0**-15 * np.abs(x) + 1e-300)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("naux,nao,nocc", [(1, 1, 1), (3, 33, 5), (7, 97, 70), (5, 150, 140), (4, 257, 33),
                                            (2, 300, 300)])
-def test_full_copy_half_transform_matches_the_oracle(torch, naux, nao, nocc):
-    """FP64 half-transform on the direct-LDS ring from the full copy of B: X rows past npad,
+def test_full_copy_half_transform_matches_the_oracle(torch, precision, naux, nao, nocc):
+    """Half-transform on the direct-LDS ring from the full copy of B: X rows past npad,
     one to three column blocks, partial row blocks."""
     from hafnium_amd import _lib
 
     b, c, d = synthetic.jk_inputs(300 + nao, naux, nao, nocc)
-    jk = builder(b)
+    jk = builder(b, precision=precision)
     k_packed = jk.exchange_occupied(c)
-    jk.expand_full()
+    jk.expand_full(precision)
     npad = -(-nao // 32) * 32
-    full = jk._bfull.view(naux, npad, npad).cpu().numpy()
+    full = jk._full[_lib.precision_code(precision)].view(naux, npad, npad).cpu().numpy()
     expect_full = np.zeros((naux, npad, npad))
     expect_full[:, :nao, :nao] = b
-    np.testing.assert_array_equal(full, expect_full)
+    np.testing.assert_array_equal(full, expect_full.astype(full.dtype))
     k = jk.exchange_occupied(c)
     j, k2 = jk.jk_occupied(c)
-    assert rel(k, jk_ref.exchange(b, d)) < FP64_REL_TOL
-    assert rel(k2, jk_ref.exchange(b, d)) < FP64_REL_TOL
-    assert rel(j, jk_ref.coulomb(b, d)) < FP64_REL_TOL
-    assert rel(k, k_packed) < 1e-13
+    tol = TOL[precision]
+    assert rel(k, jk_ref.exchange(b, d)) < tol
+    assert rel(k2, jk_ref.exchange(b, d)) < tol
+    assert rel(j, jk_ref.coulomb(b, d)) < tol
+    assert rel(k, k_packed) < (1e-13 if precision == "fp64" else 1e-5)
     jk.set_tuning(_lib.TUNE_X_FULL, 1)
     np.testing.assert_array_equal(jk.exchange_occupied(c), k_packed)
     jk.set_tuning(_lib.TUNE_X_FULL, 0)
@@ -262,6 +264,8 @@ def test_full_copy_survives_rebinding_and_can_be_dropped(torch):
     k_full = jk.exchange_occupied(c)
     jk.exchange_occupied(c, precision="fp32")  # the FP32 copy rebinds the tensors
     np.testing.assert_array_equal(jk.exchange_occupied(c), k_full)
+    with pytest.raises(InputError):
+        jk.expand_full("bf16x3")
     jk.drop_full()
     assert rel(jk.exchange_occupied(c), k_full) < 1e-13
     assert rel(k_full, jk_ref.exchange(b, d)) < FP64_REL_TOL
