@@ -470,6 +470,14 @@ def main(argv=None):
         jk.expand_full()
         torch.cuda.synchronize()
         log(f"full FP64 copy of B ({full_bytes / 1e9:.1f} GB) in {time.perf_counter() - t1:.2f} s")
+    # The same for the FP32-MFMA mode (half the bytes), when it still fits.
+    use_full32 = (not args.no_full_copy and "fp32" in modes
+                  and torch.cuda.mem_get_info(dev)[0] > full_bytes / 2 + 60e9)
+    if use_full32:
+        t1 = time.perf_counter()
+        jk.expand_full("fp32")
+        torch.cuda.synchronize()
+        log(f"full FP32 copy of B ({full_bytes / 2e9:.1f} GB) in {time.perf_counter() - t1:.2f} s")
 
     f_build = flops_per_build(n, q, o)
     results = {}
@@ -567,8 +575,11 @@ def main(argv=None):
         # B bytes per launch: the packed lower triangle, or with the full copy both triangles.
         b_bytes = q * n * (n + 1) / 2 * (8 if head == "fp64" else 4)
         if head == "fp64" and use_full:
-            kernel = "xform_dma_kernel<1>"
+            kernel = "xform_dma_kernel<double, 1>"
             b_bytes = q * n * n * 8.0
+        elif head == "fp32" and use_full32:
+            kernel = "xform_dma_kernel<float, 1>"
+            b_bytes = q * n * n * 4.0
         roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved, "peak": peak,
                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(kernel, n, q, o),
                     "algorithmic_flops_per_launch": x_flops,
@@ -612,7 +623,9 @@ def main(argv=None):
         "config": {"workload": f"single DF-J/K build nao={n} naux={q} nocc={o} seed={wl['seed']}",
                    "nao": n, "naux": q, "nocc": o,
                    "step": "J + K of D = C C^T in one call (charges from X)",
-                   "fp64_b_storage": "packed tiles + full row-major copy" if use_full else "packed tiles"},
+                   "fp64_b_storage": "packed tiles + full row-major copy" if use_full else "packed tiles",
+                   "fp32_b_storage": ("packed tiles + full row-major copy" if use_full32 else "packed tiles")
+                   if "fp32" in modes else None},
         "flops_per_build": f_build,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -26,6 +26,7 @@ TUNE_SYRK_SPLITS = 3
 TUNE_X_BM = 4
 TUNE_SYRK_KIND = 5
 TUNE_X_FULL = 6
+TUNE_GENERAL_CHUNK = 7
 
 OK = 0
 ERR_INPUT = 1

@@ -93,10 +93,10 @@ class MI355XCoulombExchange:
             raise InputError("the fitted tensor has shape [aux, ao, ao] with aux, ao > 0")
         if isinstance(df_tensor, np.ndarray):
             if np.iscomplexobj(df_tensor):
-                raise BackendError("complex classes are not implemented on MI355X")
+                raise BackendError("complex fitted tensors (k-point classes) are not implemented on MI355X")
             df_tensor = torch.from_numpy(np.ascontiguousarray(df_tensor, dtype=np.float64))
         if df_tensor.is_complex():
-            raise BackendError("complex classes are not implemented on MI355X")
+            raise BackendError("complex fitted tensors (k-point classes) are not implemented on MI355X")
         self._device = torch.device(device) if device is not None else (
             df_tensor.device if df_tensor.is_cuda else torch.device("cuda", 0)
         )
@@ -244,18 +244,20 @@ class MI355XCoulombExchange:
     def device(self):
         return self._device
 
-    def _matrix_in(self, array: Any, name: str):
+    def _density_parts(self, array: Any):
+        """``(real, imaginary)`` parts of a density as device float64 ``[ao, ao]`` tensors; the
+        imaginary part is None for a real density."""
         torch = _torch()
         n = self._nao
         if isinstance(array, np.ndarray):
-            if np.iscomplexobj(array):
-                raise BackendError("complex classes are not implemented on MI355X")
-            array = torch.from_numpy(np.ascontiguousarray(array, dtype=np.float64))
-        if array.is_complex():
-            raise BackendError("complex classes are not implemented on MI355X")
+            array = torch.from_numpy(np.ascontiguousarray(
+                array, dtype=np.complex128 if np.iscomplexobj(array) else np.float64))
         if tuple(array.shape) != (n, n):
-            raise InputError(f"the {name} has shape {tuple(array.shape)}, the integrals act on {n} atomic orbitals")
-        return array.to(device=self._device, dtype=torch.float64).contiguous()
+            raise InputError(f"the density has shape {tuple(array.shape)}, the integrals act on {n} atomic orbitals")
+        if array.is_complex():
+            z = array.to(device=self._device, dtype=torch.complex128)
+            return z.real.contiguous(), z.imag.contiguous()
+        return array.to(device=self._device, dtype=torch.float64).contiguous(), None
 
     @staticmethod
     def _out_like(value, template):
@@ -263,16 +265,28 @@ class MI355XCoulombExchange:
             return value.cpu().numpy()
         return value
 
-    def coulomb(self, density: Any, *, precision: str | None = None):
-        """``J = sum_P B[P] <B[P], D>`` (coulomb_exchange.py:176-182)."""
+    def _coulomb_real(self, d, prec: int):
         torch = _torch()
-        prec = self._precision if precision is None else _lib.precision_code(precision)
-        d = self._matrix_in(density, "density")
-        self._ensure_copy(prec)
         out = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
         ws, nbytes = self._workspace_for(1, prec)
         _lib.check(self._ctx.lib.hfjk_coulomb(self._ctx.handle, _ptr(d), _ptr(out), prec, ws, nbytes,
                                               self._stream()))
+        return out
+
+    def coulomb(self, density: Any, *, precision: str | None = None):
+        """``J = sum_P B[P] <B[P], D>`` (coulomb_exchange.py:176-182).
+
+        Any real or complex density: J depends on the symmetric part of D
+        only (B[P] is symmetric), and a complex density gives
+        ``J[Re D] + i J[Im D]``, real for a Hermitian one up to rounding.
+        """
+        torch = _torch()
+        prec = self._precision if precision is None else _lib.precision_code(precision)
+        re, im = self._density_parts(density)
+        self._ensure_copy(prec)
+        out = self._coulomb_real(re, prec)
+        if im is not None:
+            out = torch.complex(out, self._coulomb_real(im, prec))
         return self._out_like(out, density)
 
     def coulomb_from_charges(self, charges, *, precision: str | None = None):
@@ -291,36 +305,55 @@ class MI355XCoulombExchange:
                                                       self._stream()))
         return out
 
-    def exchange(self, density: Any, *, precision: str | None = None):
-        """``K = sum_P B[P] D B[P]`` (coulomb_exchange.py:184-190) for a symmetric density.
-
-        The density is factorised on the GPU (``D = C_+ C_+^T - C_- C_-^T``)
-        and the occupied-form kernels run on the factors, so indefinite and
-        fractionally occupied densities give the density-form result.
-        """
+    def _exchange_real(self, d, prec: int):
         torch = _torch()
-        prec = self._precision if precision is None else _lib.precision_code(precision)
-        d = self._matrix_in(density, "density")
-        self._ensure_copy(prec)
         out = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
         ws, nbytes = self._workspace_for(self._nao, prec)
         _lib.check(self._ctx.lib.hfjk_exchange_density(self._ctx.handle, _ptr(d), _ptr(out), prec, ws, nbytes,
                                                        self._stream()))
+        return out
+
+    def exchange(self, density: Any, *, precision: str | None = None):
+        """``K = sum_P B[P] D B[P]`` (coulomb_exchange.py:184-190) for any real or complex density.
+
+        A symmetric density is factorised on the GPU (``D = C_+ C_+^T -
+        C_- C_-^T``) and the occupied-form kernels run on the factors in the
+        chosen precision, so indefinite and fractionally occupied densities
+        give the density-form result. A non-symmetric real density takes the
+        density form itself (two FP64 DGEMMs per chunk of aux functions; other
+        precisions raise ``BackendError``). A complex density gives
+        ``K[Re D] + i K[Im D]``: for a Hermitian one the real part is
+        symmetric and the imaginary part antisymmetric.
+        """
+        torch = _torch()
+        prec = self._precision if precision is None else _lib.precision_code(precision)
+        re, im = self._density_parts(density)
+        self._ensure_copy(prec)
+        out = self._exchange_real(re, prec)
+        if im is not None:
+            out = torch.complex(out, self._exchange_real(im, prec))
         return self._out_like(out, density)
+
+    def _orbitals_in(self, orbitals: Any):
+        """Real ``[ao, occ]`` orbitals as a device float64 tensor."""
+        torch = _torch()
+        if isinstance(orbitals, np.ndarray):
+            if np.iscomplexobj(orbitals):
+                raise BackendError("complex orbitals are not implemented on MI355X; pass the density to exchange()")
+            orbitals = torch.from_numpy(np.ascontiguousarray(orbitals, dtype=np.float64))
+        if orbitals.is_complex():
+            raise BackendError("complex orbitals are not implemented on MI355X; pass the density to exchange()")
+        if orbitals.ndim != 2 or orbitals.shape[0] != self._nao:
+            raise InputError(f"the orbitals have shape {tuple(orbitals.shape)}, expected [{self._nao}, occ]")
+        return orbitals.to(device=self._device, dtype=torch.float64).contiguous()
 
     def exchange_occupied(self, orbitals: Any, signs: Any = None, *, precision: str | None = None):
         """``K = sum_P (B[P] C) diag(signs) (B[P] C)^T`` for ``orbitals`` ``[ao, occ]``."""
         torch = _torch()
         prec = self._precision if precision is None else _lib.precision_code(precision)
         template = orbitals
-        if isinstance(orbitals, np.ndarray):
-            orbitals = torch.from_numpy(np.ascontiguousarray(orbitals, dtype=np.float64))
-        if orbitals.is_complex():
-            raise BackendError("complex classes are not implemented on MI355X")
-        if orbitals.ndim != 2 or orbitals.shape[0] != self._nao:
-            raise InputError(f"the orbitals have shape {tuple(orbitals.shape)}, expected [{self._nao}, occ]")
-        nocc = int(orbitals.shape[1])
-        c = orbitals.to(device=self._device, dtype=torch.float64).contiguous()
+        c = self._orbitals_in(orbitals)
+        nocc = int(c.shape[1])
         s = None
         if signs is not None:
             s = torch.as_tensor(np.asarray(signs, dtype=np.float64) if isinstance(signs, (list, np.ndarray))
@@ -346,14 +379,8 @@ class MI355XCoulombExchange:
         torch = _torch()
         prec = self._precision if precision is None else _lib.precision_code(precision)
         template = orbitals
-        if isinstance(orbitals, np.ndarray):
-            orbitals = torch.from_numpy(np.ascontiguousarray(orbitals, dtype=np.float64))
-        if orbitals.is_complex():
-            raise BackendError("complex classes are not implemented on MI355X")
-        if orbitals.ndim != 2 or orbitals.shape[0] != self._nao:
-            raise InputError(f"the orbitals have shape {tuple(orbitals.shape)}, expected [{self._nao}, occ]")
-        nocc = int(orbitals.shape[1])
-        c = orbitals.to(device=self._device, dtype=torch.float64).contiguous()
+        c = self._orbitals_in(orbitals)
+        nocc = int(c.shape[1])
         self._ensure_copy(prec)
         j = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
         k = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)

@@ -318,6 +318,101 @@ rocblas_handle blas_handle(hfjk_ctx* ctx) {
   return (rocblas_handle)ctx->blas;
 }
 
+void check_blas(rocblas_status status, const char* what) {
+  if (status != rocblas_status_success)
+    fail(HFJK_ERR_SOLVER, std::string(what) + " failed (rocblas status " + std::to_string((int)status) + ")");
+}
+
+// out = (accumulate ? out : 0) + sum over slices of part[slice][i].
+__global__ void sum_slices_kernel(const double* __restrict__ part, int nslices, long long count, int accumulate,
+                                  double* __restrict__ out) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256) {
+    double v = accumulate ? out[i] : 0.0;
+    for (int sl = 0; sl < nslices; ++sl) v += part[(long long)sl * count + i];
+    out[i] = v;
+  }
+}
+
+// K = sum_P B_P D B_P for any real D in FP64 (the density form of
+// coulomb_exchange.py:184-190), for densities the factorisation does not
+// cover. Per chunk of aux functions: F = full rows of B_P (npad x npad,
+// symmetric, zero padding), G_P = D F_P (one strided-batched DGEMM with D
+// shared by the batch), then sum_{P,w} F_P[w][u] G_P[w][v] over k = (P, w)
+// (F_P[w][u] = F_P[u][w]), split into slices of P by a second
+// strided-batched DGEMM (a lone n x n output with k = naux npad would fill
+// few CUs) and summed into K. Row-major arrays are read as the column-major
+// transposes rocBLAS sees. 4 naux npad^3 flops.
+void exchange_general(hfjk_ctx* ctx, const double* density, double* out, void* ws, const Layout& L,
+                      hipStream_t s) {
+  if (ctx->b64 == nullptr) fail(HFJK_ERR_BACKEND, "a non-symmetric density needs the bound FP64 tensor");
+  const int n = ctx->nao;
+  const int npad = hfjk::n_tiles(n) * hfjk::kTile;
+  const size_t nn = (size_t)npad * npad;
+  const size_t kk = (size_t)n * n;
+  const size_t mat = align_up(nn * 8);
+  // At most 64 slices of at most 2^26 partial elements in all.
+  const int max_slices = (int)std::min<size_t>({(size_t)64, std::max<size_t>(1, ((size_t)1 << 26) / kk),
+                                                (size_t)ctx->naux});
+  const size_t parts_bytes = align_up((size_t)max_slices * kk * 8);
+  const size_t avail = L.total - L.dens_end;
+  const bool have_full = ctx->b64_full != nullptr && ctx->x_full == 0;
+  // D padded, the slice partials, then the G and F chunks (F only without a bound full copy).
+  const size_t per_q = (have_full ? 1 : 2) * nn * 8;
+  if (avail < mat + parts_bytes + per_q + 512)
+    fail(HFJK_ERR_WORKSPACE, "workspace too small for the general exchange");
+  int chunk = (int)std::min<size_t>((size_t)ctx->naux, (avail - mat - parts_bytes - 512) / per_q);
+  if (ctx->general_chunk > 0) chunk = std::min(chunk, ctx->general_chunk);
+  double* dpad = at<double>(ws, L.dens_end);
+  double* parts = at<double>(ws, L.dens_end + mat);
+  double* g = at<double>(ws, L.dens_end + mat + parts_bytes);
+  double* f = at<double>(ws, L.dens_end + mat + parts_bytes + align_up((size_t)chunk * nn * 8));
+  rocblas_handle h = blas_handle(ctx);
+  check_blas(rocblas_set_stream(h, s), "rocblas_set_stream");
+  check_blas(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host), "rocblas_set_pointer_mode");
+  check_hip(hfjk::launch_orbitals(density, n, n, HFJK_FP64, dpad, s), "density pad launch");
+  const double one = 1.0, zero = 0.0;
+  for (int q0 = 0; q0 < ctx->naux; q0 += chunk) {
+    const int pc = std::min(chunk, ctx->naux - q0);
+    const double* fq = have_full ? ctx->b64_full + (size_t)q0 * nn : f;
+    if (!have_full) {
+      Stage st(ctx, s, "general_expand");
+      check_hip(hfjk::launch_expand_full(ctx->b64 + (size_t)q0 * L.p, L.p, pc, n, HFJK_FP64, f, s),
+                "expand launch");
+    }
+    {
+      // Column-major G_P^T = F_P D^T, i.e. row-major G_P = D F_P.
+      Stage st(ctx, s, "general_dgemm_half");
+      check_blas(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, npad, npad, npad,
+                                               &one, fq, npad, (rocblas_stride)nn, dpad, npad, 0, &zero, g, npad,
+                                               (rocblas_stride)nn, pc),
+                 "rocblas_dgemm_strided_batched");
+    }
+    // Slices of ps aux functions; the last may be shorter.
+    const int ps = (pc + std::min(max_slices, pc) - 1) / std::min(max_slices, pc);
+    const int nfull = pc / ps, rem = pc - nfull * ps;
+    {
+      // Column-major partial^T[v][u] = sum_k G[v][k] F[u][k] over the slice's k = (P, w).
+      Stage st(ctx, s, "general_dgemm_sum");
+      const rocblas_stride sk = (rocblas_stride)ps * nn;
+      check_blas(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, n, n,
+                                               ps * npad, &one, g, npad, sk, fq, npad, sk, &zero, parts, n,
+                                               (rocblas_stride)kk, nfull),
+                 "rocblas_dgemm_strided_batched");
+      if (rem > 0)
+        check_blas(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, rem * npad, &one,
+                                 g + (size_t)nfull * sk, npad, fq + (size_t)nfull * sk, npad, &zero,
+                                 parts + (size_t)nfull * kk, n),
+                   "rocblas_dgemm");
+    }
+    {
+      Stage st(ctx, s, "general_sum_slices");
+      hipLaunchKernelGGL(sum_slices_kernel, dim3(hfjk::grid_for((long long)kk)), dim3(256), 0, s, parts,
+                         nfull + (rem > 0 ? 1 : 0), (long long)kk, q0 > 0 ? 1 : 0, out);
+      check_hip(hipGetLastError(), "sum slices launch");
+    }
+  }
+}
+
 __global__ void density_stats_kernel(const double* __restrict__ d, int n, unsigned long long* __restrict__ stats) {
   double asym = 0.0, mag = 0.0;
   const long long total = (long long)n * n;
@@ -651,12 +746,29 @@ int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange
     unsigned long long* stats = at<unsigned long long>(workspace, L.dens_stats);
     double* cf = at<double>(workspace, L.dens_c);
     double* sg = at<double>(workspace, L.dens_signs);
+    const long long nn = (long long)n * n;
+    unsigned long long host_stats[2];
     {
-      Stage st(ctx, s, "density_eigh");
+      Stage st(ctx, s, "density_stats");
       check_hip(hipMemsetAsync(stats, 0, 2 * sizeof(unsigned long long), s), "hipMemsetAsync");
-      const long long nn = (long long)n * n;
       hipLaunchKernelGGL(density_stats_kernel, dim3(hfjk::grid_for(nn)), dim3(256), 0, s, density, n, stats);
       check_hip(hipGetLastError(), "density stats launch");
+    }
+    check_hip(hipMemcpyAsync(host_stats, stats, sizeof(host_stats), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    double asym, mag;
+    std::memcpy(&asym, &host_stats[0], sizeof(double));
+    std::memcpy(&mag, &host_stats[1], sizeof(double));
+    if (!(asym <= 1e-12 * (mag > 0 ? mag : 1.0))) {
+      // Not symmetric: the two-DGEMM density form, FP64 only.
+      if (precision != HFJK_FP64)
+        fail(HFJK_ERR_BACKEND, "a non-symmetric density (max |D - D^T| = " + std::to_string(asym) +
+                                   ") is contracted in FP64 only");
+      exchange_general(ctx, density, exchange, workspace, L, s);
+      return;
+    }
+    {
+      Stage st(ctx, s, "density_eigh");
       check_hip(hipMemcpyAsync(a, density, sizeof(double) * nn, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
       rocblas_handle h = blas_handle(ctx);
       if (rocblas_set_stream(h, s) != rocblas_status_success) fail(HFJK_ERR_SOLVER, "rocblas_set_stream failed");
@@ -667,17 +779,10 @@ int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange
         fail(HFJK_ERR_SOLVER, "rocsolver_dsyevd failed to launch");
     }
     std::vector<double> vals(n);
-    unsigned long long host_stats[2];
     rocblas_int host_info = 0;
     check_hip(hipMemcpyAsync(vals.data(), w, sizeof(double) * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-    check_hip(hipMemcpyAsync(host_stats, stats, sizeof(host_stats), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
     check_hip(hipMemcpyAsync(&host_info, info, sizeof(host_info), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
-    double asym, mag;
-    std::memcpy(&asym, &host_stats[0], sizeof(double));
-    std::memcpy(&mag, &host_stats[1], sizeof(double));
-    if (!(asym <= 1e-12 * (mag > 0 ? mag : 1.0)))
-      fail(HFJK_ERR_INPUT, "the density is not symmetric (max |D - D^T| = " + std::to_string(asym) + ")");
     if (host_info != 0) fail(HFJK_ERR_SOLVER, "rocsolver_dsyevd did not converge (info " + std::to_string(host_info) + ")");
     double top = 0.0;
     for (double v : vals) top = std::max(top, std::fabs(v));
@@ -726,6 +831,9 @@ int hfjk_set_tuning(hfjk_ctx* ctx, int param, int value) {
         if (value > 1) fail(HFJK_ERR_INPUT, "HFJK_TUNE_X_FULL is 0 (use a bound full copy) or 1 (ignore it)");
         ctx->x_full = value;
         break;
+      case HFJK_TUNE_GENERAL_CHUNK:
+        ctx->general_chunk = value;
+        break;
       default:
         fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
@@ -743,6 +851,7 @@ int hfjk_get_tuning(hfjk_ctx* ctx, int param, int* value) {
       case HFJK_TUNE_X_BM: *value = ctx->x_bm; break;
       case HFJK_TUNE_SYRK_KIND: *value = ctx->syrk_kind; break;
       case HFJK_TUNE_X_FULL: *value = ctx->x_full; break;
+      case HFJK_TUNE_GENERAL_CHUNK: *value = ctx->general_chunk; break;
       default: fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
   });

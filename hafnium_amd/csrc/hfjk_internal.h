@@ -85,6 +85,7 @@ struct hfjk_ctx {
   const double* b64_full = nullptr;
   const float* b32_full = nullptr;
   int x_full = 0;
+  int general_chunk = 0;  // aux functions per chunk of the non-symmetric exchange (0 = fit)
   // Second stream of hfjk_jk_occ (default priority, created on first use):
   // the Coulomb half runs there beside the SYRK, fenced by fork/join events.
   hipStream_t side = nullptr;

@@ -155,12 +155,18 @@ HFJK_API int hfjk_exchange_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, 
 HFJK_API int hfjk_jk_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, double *coulomb, double *exchange,
                          int precision, void *workspace, size_t workspace_bytes, void *stream);
 
-/* K = sum_P B[P] D B[P] for a symmetric (possibly indefinite) density:
- * D = V diag(l) V^T by rocSOLVER dsyevd, eigenpairs with |l| <= 1e-12 max|l|
- * dropped, C = V |l|^1/2, S = sign(l), then hfjk_exchange_occ. Drop-in for
+/* K = sum_P B[P] D B[P] for any real density [nao][nao]. Drop-in for
  * DensityFittedCoulombExchange.exchange (coulomb_exchange.py:184-190).
- * Synchronises `stream` once (the retained rank sizes the launch). The
- * density must be symmetric to 1e-12 relative (checked: HFJK_ERR_INPUT). */
+ * Symmetric to 1e-12 relative (possibly indefinite): D = V diag(l) V^T by
+ * rocSOLVER dsyevd, eigenpairs with |l| <= 1e-12 max|l| dropped,
+ * C = V |l|^1/2, S = sign(l), then hfjk_exchange_occ in `precision`.
+ * Otherwise the density form itself in FP64 (precision must be HFJK_FP64,
+ * else HFJK_ERR_BACKEND): per chunk of aux functions the full rows F_P of
+ * B (from the bound full FP64 copy, or expanded into the workspace),
+ * G_P = D F_P by a strided-batched DGEMM and K += sum_P F_P G_P by one DGEMM.
+ * A complex density is two calls, on its real and imaginary parts.
+ * Synchronises `stream` once or twice (the symmetry check, the retained
+ * rank). */
 HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double *exchange, int precision,
                           void *workspace, size_t workspace_bytes, void *stream);
 
@@ -274,6 +280,8 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * library's choice (2 where it applies).
  * HFJK_TUNE_X_FULL: 0 = the FP64/FP32 half-transform uses the full copy of B
  * of its precision when one is bound (hfjk_bind_full), 1 = it ignores it.
+ * HFJK_TUNE_GENERAL_CHUNK: most aux functions per chunk of the non-symmetric
+ * exchange of hfjk_exchange_density (0 = as many as the workspace holds).
  * Set by the project's autotuner (hafnium_amd/autotune.py); the workspace size
  * depends on HFJK_TUNE_SYRK_SPLITS, so query it after tuning. No counterpart
  * in the reference. */
@@ -283,6 +291,7 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
 #define HFJK_TUNE_X_BM 4
 #define HFJK_TUNE_SYRK_KIND 5
 #define HFJK_TUNE_X_FULL 6
+#define HFJK_TUNE_GENERAL_CHUNK 7
 HFJK_API int hfjk_set_tuning(hfjk_ctx *ctx, int param, int value);
 HFJK_API int hfjk_get_tuning(hfjk_ctx *ctx, int param, int *value);
 

@@ -84,8 +84,9 @@ def looped_coulomb_exchange(eri: np.ndarray, density: np.ndarray):
     For tiny ``n`` only: it is the brute-force check of the contractions.
     """
     n = eri.shape[0]
-    j = np.zeros((n, n))
-    k = np.zeros((n, n))
+    dtype = np.result_type(eri, density)
+    j = np.zeros((n, n), dtype=dtype)
+    k = np.zeros((n, n), dtype=dtype)
     for u in range(n):
         for v in range(n):
             sj = 0.0

@@ -141,8 +141,48 @@ def test_indefinite_and_nonsymmetric_densities(torch, precision):
     assert rel(jk.exchange(sym), jk_ref.exchange(b, sym)) < TOL[precision]
     # J depends on the symmetric part only, as the oracle's full contraction.
     assert rel(jk.coulomb(a), jk_ref.coulomb(b, a)) < TOL[precision]
-    with pytest.raises(InputError, match="not symmetric"):
-        jk.exchange(a)
+    if precision == "fp64":
+        assert rel(jk.exchange(a), jk_ref.exchange(b, a)) < FP64_REL_TOL
+    else:
+        with pytest.raises(BackendError, match="FP64 only"):
+            jk.exchange(a)
+
+
+@pytest.mark.parametrize("naux,nao,chunk,full", [(5, 1, 0, False), (7, 33, 3, False), (9, 70, 0, True),
+                                                 (6, 70, 4, True), (4, 130, 1, False)])
+def test_nonsymmetric_and_complex_densities_take_the_density_form(torch, naux, nao, chunk, full):
+    """Densities of the reference's tests (tests/unit/test_coulomb_exchange.py:45-105): random
+    Hermitian, real non-symmetric, and the off-diagonal spin block of a Hermitian spinor density,
+    over one or several aux chunks, with and without the full FP64 copy of B."""
+    from hafnium_amd import _lib
+
+    rng = np.random.default_rng(nao + naux)
+    b = synthetic.tensor_full(11 + nao, naux, nao)
+    jk = builder(b)
+    if full:
+        jk.expand_full()
+    jk.set_tuning(_lib.TUNE_GENERAL_CHUNK, chunk)
+    nonsym = rng.standard_normal((nao, nao))
+    assert rel(jk.exchange(nonsym), jk_ref.exchange(b, nonsym)) < FP64_REL_TOL
+    z = rng.standard_normal((2 * nao, 2 * nao)) + 1j * rng.standard_normal((2 * nao, 2 * nao))
+    spinor = z + z.conj().T
+    herm, off = spinor[:nao, :nao], spinor[:nao, nao:]
+    k = jk.exchange(herm)
+    j = jk.coulomb(herm)
+    assert k.dtype == np.complex128 and j.dtype == np.complex128
+    assert rel(k, jk_ref.exchange(b, herm)) < FP64_REL_TOL
+    assert rel(j, jk_ref.coulomb(b, herm)) < FP64_REL_TOL
+    # Both matrices are Hermitian for a Hermitian density; J is real.
+    assert rel(k, k.conj().T) < 1e-14
+    assert np.max(np.abs(j.imag)) <= 1e-13 * np.max(np.abs(j.real))
+    assert rel(jk.exchange(off), jk_ref.exchange(b, off)) < FP64_REL_TOL
+    # Torch in, torch out, on the device.
+    kt = jk.exchange(torch.from_numpy(off).cuda())
+    assert kt.is_cuda and kt.dtype == torch.complex128
+    np.testing.assert_array_equal(kt.cpu().numpy(), jk.exchange(off))
+    # A complex density with a zero imaginary part gives the real result.
+    sym = nonsym + nonsym.T
+    np.testing.assert_array_equal(jk.exchange(sym.astype(np.complex128)).real, jk.exchange(sym))
 
 
 def test_signed_orbitals_give_the_difference_of_exchanges(torch):
@@ -199,7 +239,9 @@ def test_malformed_inputs_are_rejected(torch):
     with pytest.raises(InputError, match="atomic orbitals"):
         jk.coulomb(np.eye(7))
     with pytest.raises(BackendError, match="complex"):
-        jk.exchange(np.eye(6, dtype=complex))
+        jk.exchange_occupied(np.eye(6, dtype=complex))
+    with pytest.raises(InputError, match="atomic orbitals"):
+        jk.exchange(np.eye(5, dtype=complex))
     with pytest.raises(InputError):
         jk.exchange_occupied(np.ones((5, 2)))
 

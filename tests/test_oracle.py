@@ -72,6 +72,15 @@ def test_density_form_matches_the_looped_four_index_definitions(water):
     nonsym = rng.standard_normal((7, 7))
     _, k_ns = jk_ref.looped_coulomb_exchange(eri, nonsym)
     np.testing.assert_allclose(jk_ref.exchange(b, nonsym), k_ns, atol=STRUCTURE_TOL)
+    # A complex Hermitian density (tests/unit/test_coulomb_exchange.py:45-51 of the reference):
+    # J of it is real, K is Hermitian.
+    z = rng.standard_normal((7, 7)) + 1j * rng.standard_normal((7, 7))
+    herm = z + z.conj().T
+    j_z, k_z = jk_ref.looped_coulomb_exchange(eri, herm)
+    np.testing.assert_allclose(jk_ref.coulomb(b, herm), j_z, atol=STRUCTURE_TOL)
+    np.testing.assert_allclose(jk_ref.exchange(b, herm), k_z, atol=STRUCTURE_TOL)
+    np.testing.assert_allclose(np.imag(j_z), 0.0, atol=STRUCTURE_TOL)
+    np.testing.assert_allclose(k_z, k_z.conj().T, atol=STRUCTURE_TOL)
 
 
 def test_fitted_contractions_approximate_the_exact_ones(water):
