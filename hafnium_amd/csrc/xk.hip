@@ -1054,21 +1054,39 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void xform_dma_kernel(
 }
 
 // Full row-major copy of tiled packed B in T: out[q][r][c] = B_q[r][c] for
-// r, c < npad (both triangles; zero past nao, as the packed padding).
+// r, c < npad (both triangles; zero past nao, as the packed padding). One
+// workgroup per stored 32 x 32 tile (tr >= tc) of one aux function, in a
+// grid-stride loop: the tile is read once, contiguously, written to block
+// (tr, tc) and, through LDS, transposed into block (tc, tr), so every global
+// access is a run of 32 consecutive elements.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void expand_full_kernel(const double* __restrict__ packed, long long p,
-                                                                int naux, int npad, T* __restrict__ out) {
-  const long long per = (long long)npad * npad;
-  const long long total = per * naux;
-  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
-       idx += (long long)gridDim.x * kThreads) {
-    const long long q = idx / per;
-    const int rc = (int)(idx - q * per);
-    const int r = rc / npad, c = rc % npad;
-    const int tr = r / kTile, tc = c / kTile;
-    const long long src = tr >= tc ? tidx(tr, tc) * kTileElems + (r % kTile) * kTile + (c % kTile)
-                                   : tidx(tc, tr) * kTileElems + (c % kTile) * kTile + (r % kTile);
-    out[idx] = (T)packed[q * p + src];
+                                                                int naux, int ntp, int npad, T* __restrict__ out) {
+  __shared__ T tile[kTile][kTile + 1];
+  const long long jobs = (long long)naux * ntp;
+  for (long long job = blockIdx.x; job < jobs; job += gridDim.x) {
+    const int q = (int)(job / ntp);
+    const int t = (int)(job - (long long)q * ntp);
+    int tr = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((tr + 1) * (tr + 2) / 2 <= t) ++tr;
+    while (tr * (tr + 1) / 2 > t) --tr;
+    const int tc = t - tr * (tr + 1) / 2;
+    const double* src = packed + q * p + (long long)t * kTileElems;
+    T* oq = out + (long long)q * npad * npad;
+    for (int e = threadIdx.x; e < kTileElems; e += kThreads) {
+      const int i = e / kTile, j = e % kTile;
+      const T v = (T)src[e];
+      tile[i][j] = v;
+      oq[(long long)(tr * kTile + i) * npad + tc * kTile + j] = v;
+    }
+    if (tr != tc) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < kTileElems; e += kThreads) {
+        const int i = e / kTile, j = e % kTile;
+        oq[(long long)(tc * kTile + i) * npad + tr * kTile + j] = tile[j][i];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1296,11 +1314,14 @@ hipError_t launch_x16_t(const uint16_t* bhi, const uint16_t* blo, long long p, i
 hipError_t launch_expand_full(const double* packed, long long p, int naux, int nao, int precision, void* out,
                               hipStream_t s) {
   const int npad = n_tiles(nao) * kTile;
-  const dim3 grid(grid_for((long long)naux * npad * npad));
+  const int ntp = (int)n_tile_pairs(nao);
+  const dim3 grid((unsigned)std::min<long long>((long long)naux * ntp, 1 << 20));
   if (precision == HFJK_FP64)
-    hipLaunchKernelGGL(expand_full_kernel<double>, grid, dim3(kThreads), 0, s, packed, p, naux, npad, (double*)out);
+    hipLaunchKernelGGL(expand_full_kernel<double>, grid, dim3(kThreads), 0, s, packed, p, naux, ntp, npad,
+                       (double*)out);
   else
-    hipLaunchKernelGGL(expand_full_kernel<float>, grid, dim3(kThreads), 0, s, packed, p, naux, npad, (float*)out);
+    hipLaunchKernelGGL(expand_full_kernel<float>, grid, dim3(kThreads), 0, s, packed, p, naux, ntp, npad,
+                       (float*)out);
   return hipGetLastError();
 }
 
