@@ -323,6 +323,18 @@ class BatchedRhf:
 
         ``profile`` (a dict) receives wall seconds per stage, measured with a device
         synchronisation after each stage (which slows the run slightly)."""
+        steps = self.iterate(start, options, profile=profile)
+        while True:
+            try:
+                next(steps)
+            except StopIteration as stop:
+                return _finish(stop.value, raise_on_failure)
+
+    def iterate(self, start: _Start, options: ScfOptions | None = None, *, profile: dict | None = None):
+        """The SCF loop as a generator: it yields once per iteration, after the iteration's GPU work
+        is enqueued on the current stream and before the host waits for its result, and returns one
+        ``BatchResult`` per molecule (unconverged ones flagged). Resume it with that stream current
+        (``run_pipelined`` interleaves several batches on their own streams this way)."""
         torch = _torch()
         options = ScfOptions() if options is None else options
         clock = [time.perf_counter()]
@@ -364,6 +376,7 @@ class BatchedRhf:
             # The newest error's overlaps with every slot, for every molecule, in one launch.
             _lib.check(self.lib.hfjk_history_dots_batched(self._ctx.handle, cnt, nn, m, _p(gh), slot, _p(row_dev),
                                                           self._stream()))
+            yield index
             host = scal.cpu().numpy()
             mark("overlaps_transfer")
             terms = start.terms if index == 0 else host[:3 * cnt].reshape(3, cnt)
@@ -418,10 +431,61 @@ class BatchedRhf:
                 last = history[i][-1]
                 results[i] = BatchResult(last[0], False, len(history[i]), total, history[i])
             out.append(results[i])
-        if raise_on_failure and not all(r.converged for r in out):
-            bad = [i for i, r in enumerate(out) if not r.converged]
-            raise ConvergenceError(f"molecules {bad} did not converge in {options.max_iterations} iterations", out)
         return out
+
+
+def _finish(out, raise_on_failure: bool):
+    if raise_on_failure and not all(r.converged for r in out):
+        bad = [i for i, r in enumerate(out) if not r.converged]
+        raise ConvergenceError(f"molecules {bad} did not converge", out)
+    return out
+
+
+def run_pipelined(groups, options: ScfOptions | None = None, *, raise_on_failure: bool = True):
+    """Converge several batches at once, each on its own stream, interleaving their iterations.
+
+    ``groups`` is a list of ``(BatchedRhf, start)`` pairs; returns one result list per group. While
+    the host waits for one batch's energies and DIIS overlaps, the other batches' enqueued work
+    runs: the batched eigensolver of one (latency-bound kernels that leave most CUs idle) overlaps
+    the J/K build of another (MFMA-bound). Each batch iterates exactly as in ``BatchedRhf.run``.
+    """
+    torch = _torch()
+    if not groups:
+        raise InputError("at least one batch")
+    device = groups[0][0].device
+    origin = torch.cuda.current_stream(device)
+    streams = [torch.cuda.Stream(device=batch.device) for batch, _ in groups]
+    for stream in streams:
+        stream.wait_stream(origin)  # the starts were computed on the caller's stream
+    steps = [batch.iterate(start, options) for batch, start in groups]
+    outs: list = [None] * len(groups)
+    live = list(range(len(groups)))
+    while live:
+        for g in list(live):
+            with torch.cuda.stream(streams[g]):
+                try:
+                    next(steps[g])
+                except StopIteration as stop:
+                    outs[g] = stop.value
+                    live.remove(g)
+    for stream in streams:
+        origin.wait_stream(stream)
+    return [_finish(out, raise_on_failure) for out in outs]
+
+
+def balanced_groups(sizes, count: int) -> list[list[int]]:
+    """Indices of ``sizes`` dealt into ``count`` groups of about equal total cost (largest first,
+    each to the group with the least nao^4 so far)."""
+    if count < 1:
+        raise InputError("at least one group")
+    order = sorted(range(len(sizes)), key=lambda i: -sizes[i])
+    groups: list[list[int]] = [[] for _ in range(min(count, len(sizes)))]
+    cost = [0.0] * len(groups)
+    for i in order:
+        g = min(range(len(groups)), key=lambda k: cost[k])
+        groups[g].append(i)
+        cost[g] += float(sizes[i]) ** 4
+    return [sorted(g) for g in groups]
 
 
 # ---------------------------------------------------------------- guesses

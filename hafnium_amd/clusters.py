@@ -9,9 +9,8 @@ library's engine, the fitted tensor by the GPU Cholesky route, and
 ``atomic_density_guess`` mirrors the reference's superposition of atomic
 densities (hafnium/scf/atoms.py:55-119): one spin-averaged spherical atomic
 RHF per element (``AveragedOccupation`` with half the electrons per spin),
-the atomic blocks placed on the diagonal. The reference converges the atoms
-with exact four-centre integrals; here they use the same density fitting as
-the molecule, which changes the guess slightly and not the converged state.
+the atomic blocks placed on the diagonal, each atom converged with exact
+four-centre integrals as in the reference.
 """
 
 from __future__ import annotations
@@ -79,17 +78,29 @@ def build_system(molecule: Molecule, *, basis: str = ORBITAL_BASIS, aux: str = A
                          m if keep_integrals else None, tc if keep_integrals else None, aux)
 
 
-def atomic_density_guess(molecule: Molecule, *, basis: str = ORBITAL_BASIS, aux: str = AUX_BASIS, device=None,
+def atomic_density_guess(molecule: Molecule, *, basis: str = ORBITAL_BASIS, device=None,
                          cache: dict | None = None):
-    """Block-diagonal superposition of spin-averaged atomic densities (D_alpha)."""
+    """Block-diagonal superposition of spin-averaged atomic densities (D_alpha).
+
+    As the reference's ``atomic_density_guess`` (hafnium/scf/atoms.py:76-119, ``_spherical_atom``
+    :55-73): every element is converged once as the neutral atom at the origin in its own shells,
+    restricted, with its electrons averaged over both spins and over degenerate levels
+    (occupation "averaged"), with exact four-centre integrals and the default SCF options; the
+    atomic densities go on the diagonal in the order of the atoms. ``cache`` maps the atomic
+    number to the atom's density (one basis per cache).
+    """
+    from hafnium_amd.density_fitting import exact_coulomb_exchange
+
     cache = {} if cache is None else cache
     blocks = []
     for z in molecule.charges:
         if z not in cache:
             atom = Molecule((z,), np.zeros((1, 3)))
-            system = build_system(atom, basis=basis, aux=aux, device=device)
-            result = solve_scf(system.hamiltonian, occupation="averaged",
-                               options=ScfOptions(energy_tolerance=1e-9, gradient_tolerance=1e-5))
+            orb = BasisOnAtoms(basis, atom)
+            s, t, v = integrals.one_electron(orb, atom)
+            two_body = exact_coulomb_exchange(integrals.four_center(orb), device=device)
+            h = RestrictedHamiltonian(s, t + v, two_body, z / 2.0, 0.0)
+            result = solve_scf(h, occupation="averaged", options=ScfOptions())
             cache[z] = result.density.cpu().numpy()
         blocks.append(cache[z])
     n = sum(b.shape[0] for b in blocks)

@@ -491,6 +491,141 @@ void triple(const Shell& A, const Shell& B, const Shell& C, const std::vector<do
             }
 }
 
+// Spherical transform of one axis of a 4-index block: dims[axis] goes from
+// S.ncontr * ncart(S.l) Cartesian to S.ncontr * (2 l + 1) spherical entries.
+void sph_axis(const std::vector<double>& in, int dims[4], int axis, const Shell& S, std::vector<double>& out) {
+  const int nc = ncart(S.l), ns = 2 * S.l + 1;
+  const auto& t = sph_table(S.l);
+  long long inner = 1, outer = 1;
+  for (int k = axis + 1; k < 4; ++k) inner *= dims[k];
+  for (int k = 0; k < axis; ++k) outer *= dims[k];
+  const int nout = S.ncontr * ns;
+  out.assign((size_t)(outer * nout * inner), 0.0);
+  for (long long o = 0; o < outer; ++o)
+    for (int c = 0; c < S.ncontr; ++c)
+      for (int m = 0; m < ns; ++m) {
+        double* dst = &out[(size_t)((o * nout + c * ns + m) * inner)];
+        for (int u = 0; u < nc; ++u) {
+          const double w = t[m * nc + u];
+          if (w == 0.0) continue;
+          const double* src = &in[(size_t)((o * dims[axis] + c * nc + u) * inner)];
+          for (long long i = 0; i < inner; ++i) dst[i] += w * src[i];
+        }
+      }
+  dims[axis] = nout;
+}
+
+// (ab|cd) for one shell quartet, spherical and normalised, written into
+// out[((a * n + b) * n + c) * n + d] for all eight index permutations.
+void quartet(const Shell& A, const Shell& B, const Shell& C, const Shell& D, const std::vector<double>& norm, int n,
+             double* out) {
+  const Shell* S[4] = {&A, &B, &C, &D};
+  std::vector<int> cx[4], cy[4], cz[4];
+  int nc[4], np[4];
+  for (int k = 0; k < 4; ++k) {
+    cart_list(S[k]->l, cx[k], cy[k], cz[k]);
+    nc[k] = ncart(S[k]->l);
+    np[k] = (int)S[k]->exps.size();
+  }
+  int dims[4];
+  for (int k = 0; k < 4; ++k) dims[k] = S[k]->ncontr * nc[k];
+  std::vector<double> cart((size_t)dims[0] * dims[1] * dims[2] * dims[3], 0.0);
+  std::vector<double> prim((size_t)nc[0] * nc[1] * nc[2] * nc[3]);
+  double ab[3], cd[3];
+  for (int k = 0; k < 3; ++k) {
+    ab[k] = A.c[k] - B.c[k];
+    cd[k] = C.c[k] - D.c[k];
+  }
+  const double ab2 = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+  const double cd2 = cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2];
+  const int L = A.l + B.l + C.l + D.l;
+  Herm h[3], g[3];
+  RTab rt;
+  for (int i = 0; i < np[0]; ++i)
+    for (int j = 0; j < np[1]; ++j) {
+      const double a = A.exps[i], b = B.exps[j], p = a + b;
+      if (std::exp(-a * b / p * ab2) < kPairCut) continue;
+      for (int k = 0; k < 3; ++k) hermite(A.l, B.l, a, b, ab[k], h[k]);
+      double P[3];
+      for (int k = 0; k < 3; ++k) P[k] = (a * A.c[k] + b * B.c[k]) / p;
+      for (int k2 = 0; k2 < np[2]; ++k2)
+        for (int l2 = 0; l2 < np[3]; ++l2) {
+          const double c = C.exps[k2], d = D.exps[l2], q = c + d;
+          if (std::exp(-c * d / q * cd2) < kPairCut) continue;
+          for (int k = 0; k < 3; ++k) hermite(C.l, D.l, c, d, cd[k], g[k]);
+          double Q[3];
+          for (int k = 0; k < 3; ++k) Q[k] = (c * C.c[k] + d * D.c[k]) / q;
+          hermite_coulomb(L, p * q / (p + q), P[0] - Q[0], P[1] - Q[1], P[2] - Q[2], rt);
+          const double pref = 2.0 * std::pow(kPi, 2.5) / (p * q * std::sqrt(p + q));
+          for (int u = 0; u < nc[0]; ++u)
+            for (int v = 0; v < nc[1]; ++v)
+              for (int w = 0; w < nc[2]; ++w)
+                for (int x = 0; x < nc[3]; ++x) {
+                  double sum = 0.0;
+                  for (int t1 = 0; t1 <= cx[0][u] + cx[1][v]; ++t1) {
+                    const double e1 = h[0].get(cx[0][u], cx[1][v], t1);
+                    if (e1 == 0.0) continue;
+                    for (int u1 = 0; u1 <= cy[0][u] + cy[1][v]; ++u1) {
+                      const double e2 = e1 * h[1].get(cy[0][u], cy[1][v], u1);
+                      if (e2 == 0.0) continue;
+                      for (int v1 = 0; v1 <= cz[0][u] + cz[1][v]; ++v1) {
+                        const double e3 = e2 * h[2].get(cz[0][u], cz[1][v], v1);
+                        if (e3 == 0.0) continue;
+                        for (int t2 = 0; t2 <= cx[2][w] + cx[3][x]; ++t2) {
+                          const double f1 = g[0].get(cx[2][w], cx[3][x], t2);
+                          if (f1 == 0.0) continue;
+                          for (int u2 = 0; u2 <= cy[2][w] + cy[3][x]; ++u2) {
+                            const double f2 = f1 * g[1].get(cy[2][w], cy[3][x], u2);
+                            if (f2 == 0.0) continue;
+                            for (int v2 = 0; v2 <= cz[2][w] + cz[3][x]; ++v2) {
+                              const double f3 = f2 * g[2].get(cz[2][w], cz[3][x], v2);
+                              if (f3 == 0.0) continue;
+                              const double sg = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
+                              sum += e3 * f3 * sg * rt.get(t1 + t2, u1 + u2, v1 + v2);
+                            }
+                          }
+                        }
+                      }
+                    }
+                  }
+                  prim[(((size_t)u * nc[1] + v) * nc[2] + w) * nc[3] + x] = pref * sum;
+                }
+          for (int ca = 0; ca < A.ncontr; ++ca)
+            for (int cb = 0; cb < B.ncontr; ++cb)
+              for (int cc = 0; cc < C.ncontr; ++cc)
+                for (int cdd = 0; cdd < D.ncontr; ++cdd) {
+                  const double wgt = A.coefs[(size_t)ca * np[0] + i] * B.coefs[(size_t)cb * np[1] + j] *
+                                     C.coefs[(size_t)cc * np[2] + k2] * D.coefs[(size_t)cdd * np[3] + l2];
+                  if (wgt == 0.0) continue;
+                  for (int u = 0; u < nc[0]; ++u)
+                    for (int v = 0; v < nc[1]; ++v)
+                      for (int w = 0; w < nc[2]; ++w)
+                        for (int x = 0; x < nc[3]; ++x)
+                          cart[(((size_t)(ca * nc[0] + u) * dims[1] + cb * nc[1] + v) * dims[2] + cc * nc[2] + w) *
+                                   dims[3] +
+                               cdd * nc[3] + x] += wgt * prim[(((size_t)u * nc[1] + v) * nc[2] + w) * nc[3] + x];
+                }
+        }
+    }
+  std::vector<double> tmp;
+  for (int k = 0; k < 4; ++k) {
+    sph_axis(cart, dims, k, *S[k], tmp);
+    cart.swap(tmp);
+  }
+  for (int fa = 0; fa < dims[0]; ++fa)
+    for (int fb = 0; fb < dims[1]; ++fb)
+      for (int fc = 0; fc < dims[2]; ++fc)
+        for (int fd = 0; fd < dims[3]; ++fd) {
+          const int a = A.offset + fa, b = B.offset + fb, c = C.offset + fc, d = D.offset + fd;
+          const double val = cart[(((size_t)fa * dims[1] + fb) * dims[2] + fc) * dims[3] + fd] * norm[a] * norm[b] *
+                             norm[c] * norm[d];
+          const size_t N = (size_t)n;
+          const int idx[8][4] = {{a, b, c, d}, {b, a, c, d}, {a, b, d, c}, {b, a, d, c},
+                                 {c, d, a, b}, {d, c, a, b}, {c, d, b, a}, {d, c, b, a}};
+          for (const auto& q : idx) out[((q[0] * N + q[1]) * N + q[2]) * N + q[3]] = val;
+        }
+}
+
 template <typename F>
 int iguard(F&& body) {
   try {
@@ -600,6 +735,29 @@ int hfjk_int2c2e(const hfjk_basis* aux, double* metric) {
           metric[(size_t)i * n + j] = val;
           metric[(size_t)j * n + i] = val;
         }
+    });
+  });
+}
+
+int hfjk_int2e(const hfjk_basis* basis, double* out) {
+  return iguard([&] {
+    int n = 0;
+    const std::vector<Shell> sh = make_shells(basis, n);
+    if (out == nullptr) throw std::invalid_argument("null output pointer");
+    const std::vector<double> nm = norms(sh, n);
+    // Unique quartets: shell pairs (s1 >= s2) and pair index p12 >= p34.
+    const long long ns = (long long)sh.size();
+    const long long npair = ns * (ns + 1) / 2;
+    std::vector<std::pair<int, int>> pairs;
+    for (int s1 = 0; s1 < ns; ++s1)
+      for (int s2 = 0; s2 <= s1; ++s2) pairs.emplace_back(s1, s2);
+    const long long total = npair * (npair + 1) / 2;
+    parallel_for(total, [&](long long idx) {
+      long long p12 = (long long)((std::sqrt(8.0 * idx + 1.0) - 1.0) / 2.0);
+      while (p12 * (p12 + 1) / 2 > idx) --p12;
+      while ((p12 + 1) * (p12 + 2) / 2 <= idx) ++p12;
+      const long long p34 = idx - p12 * (p12 + 1) / 2;
+      quartet(sh[pairs[p12].first], sh[pairs[p12].second], sh[pairs[p34].first], sh[pairs[p34].second], nm, n, out);
     });
   });
 }

@@ -70,3 +70,45 @@ def fitted_coulomb_exchange(metric, three_center, *, precision: str = "fp64", de
     out.metric_factor = factor
     out.fit_dense = dense
     return out
+
+
+EXACT_RANK_THRESHOLD = 1e-13
+"""Eigenvalues of the [n^2, n^2] integral matrix at or below this fraction of the largest are
+treated as its null space (the antisymmetric products and one-centre linear dependencies, which
+sit at the rounding level, about 1e-16 relative)."""
+
+
+def exact_coulomb_exchange(eri, *, precision: str = "fp64", device=None):
+    """An ``MI355XCoulombExchange`` that contracts with exact four-centre integrals.
+
+    The drop-in for the reference's ``ExactCoulombExchange(eri)``
+    (hafnium/scf/coulomb_exchange.py:74-121: ``J = einsum("uvwx,wx", eri, D)``,
+    ``K = einsum("uwxv,wx", eri, D)``). The integral matrix ``V[(uv), (wx)]`` is
+    symmetric positive semidefinite, so ``V = sum_P l_P u_P u_P^T`` (rocSOLVER dsyevd on the
+    GPU) and ``B[P] = sqrt(l_P) u_P`` over the eigenvalues above ``EXACT_RANK_THRESHOLD`` of the
+    largest is an exact factor (``sum_P B[P,uv] B[P,wx] = (uv|wx)``): the DF kernels then give the
+    exact J and K, to the threshold. For atoms and small molecules (``n^4`` doubles).
+    """
+    from hafnium_amd import _lib
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange, _Context
+    from hafnium_amd.linalg import Dense
+
+    torch = _torch()
+    dev = torch.device(device) if device is not None else torch.device("cuda", 0)
+    e = torch.as_tensor(np.asarray(eri) if isinstance(eri, np.ndarray) else eri, dtype=torch.float64)
+    n = e.shape[0]
+    if e.ndim != 4 or tuple(e.shape) != (n, n, n, n):
+        raise InputError("the four-centre integrals have shape [ao, ao, ao, ao]")
+    ctx = _Context(dev.index or 0)
+    dense = Dense(ctx, dev)
+    w, v = dense.eigh(e.to(dev).reshape(n * n, n * n).contiguous())
+    if dense.info() != 0:
+        raise InputError(f"the integral matrix could not be diagonalised (dsyevd info {dense.info()})")
+    top = float(w.abs().max())
+    keep = torch.nonzero(w > EXACT_RANK_THRESHOLD * top).flatten()
+    if float(w.min()) < -1e-10 * top:
+        raise InputError("the four-centre integrals are not a positive semidefinite Coulomb matrix")
+    b = (v[:, keep] * w[keep].sqrt()).T.contiguous().reshape(-1, n, n)
+    _lib.check(dense._lib.hfjk_symmetrize_batched(dense._ctx.handle, n, b.shape[0], ctypes.c_void_p(b.data_ptr()),
+                                                 dense._stream()))
+    return MI355XCoulombExchange(b, precision=precision, device=dev)

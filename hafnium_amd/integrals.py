@@ -4,8 +4,9 @@ The producer of the SCF inputs: S, T, V (the reference's
 ``IntegralEngine.overlap/kinetic/nuclear_attraction``,
 hafnium/integrals/engine.py:108-377) and the metric ``(P|Q)`` and
 three-centre ``(P|uv)`` of ``DensityFitting.build``
-(hafnium/integrals/density_fitting.py:58-85), computed by the library's
-McMurchie-Davidson engine.
+(hafnium/integrals/density_fitting.py:58-85), and the exact four-centre
+``(uv|wx)`` of small systems (the spherical atoms of the density guess),
+computed by the library's McMurchie-Davidson engine.
 """
 
 from __future__ import annotations
@@ -49,6 +50,18 @@ def three_center(aux: BasisOnAtoms, orbital: BasisOnAtoms):
     out = np.empty((aux.n_functions, orbital.n_functions, orbital.n_functions))
     a, o = aux.c_struct(), orbital.c_struct()
     _lib.check(lib.hfjk_int3c2e(ctypes.byref(a), ctypes.byref(o), _ptr(out)))
+    return out
+
+
+def four_center(basis: BasisOnAtoms):
+    """``(uv|wx)`` ``[n, n, n, n]`` in chemists' notation (the reference's ``two_electron.eri``,
+    hafnium/integrals/two_electron.py:26-48, without screening). ``n^4`` doubles: for atoms and
+    small molecules."""
+    lib = _lib.load()
+    n = basis.n_functions
+    out = np.empty((n, n, n, n))
+    b = basis.c_struct()
+    _lib.check(lib.hfjk_int2e(ctypes.byref(b), _ptr(out)))
     return out
 
 

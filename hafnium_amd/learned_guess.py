@@ -570,7 +570,7 @@ def water_orbitals(*, basis: str = "cc-pvdz", aux: str = "def2-universal-jkfit",
         return cache[key]
     mol = reference_water()
     system = build_system(mol, basis=basis, aux=aux, device=device)
-    guess = atomic_density_guess(mol, basis=basis, aux=aux, device=device, cache=cache)
+    guess = atomic_density_guess(mol, basis=basis, device=device, cache=cache)
     result = solve_scf(system.hamiltonian, guess=guess,
                        options=ScfOptions(energy_tolerance=1e-11, gradient_tolerance=1e-8))
     c = result.mo_coefficients[:, :system.hamiltonian.n_alpha].cpu().numpy().copy()

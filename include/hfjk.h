@@ -235,6 +235,12 @@ HFJK_API int hfjk_int1e(const hfjk_basis *basis, int natom, const double *charge
 HFJK_API int hfjk_int2c2e(const hfjk_basis *aux, double *metric);
 /* (P|uv) [naux][n][n]. */
 HFJK_API int hfjk_int3c2e(const hfjk_basis *aux, const hfjk_basis *orbital, double *out);
+/* (uv|wx) [n][n][n][n], chemists' notation: the four-centre integrals of the
+ * reference's two_electron.eri / IntegralEngine.coulomb_four_center
+ * (hafnium/integrals/two_electron.py:26-41, engine.py:280) that
+ * ExactCoulombExchange holds (scf/coulomb_exchange.py:74-121), e.g. for the
+ * spherical atoms of the density guess (scf/atoms.py:55-73). n^4 doubles. */
+HFJK_API int hfjk_int2e(const hfjk_basis *basis, double *out);
 
 /* ---- FP64 dense steps of the SCF around the J/K build ------------------
  * Replace the NumPy/array-API calls of the reference's SCF driver

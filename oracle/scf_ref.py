@@ -181,8 +181,17 @@ def rhf(
     energy_tolerance: float = 1e-10,
     gradient_tolerance: float = 1e-6,
     diis_size: int = 8,
+    occupation: str = "aufbau",
 ) -> RhfResult:
-    """Converge a closed-shell RHF state (driver.py:154-213, :307-367)."""
+    """Converge a closed-shell RHF state (driver.py:154-213, :307-367).
+
+    ``occupation`` "aufbau" fills whole levels; "averaged" spreads ``n_alpha`` (possibly
+    fractional) evenly over degenerate groups in every iteration, the reference's
+    ``AveragedOccupation`` (occupation.py:342-370) used for the spherical atoms of its density
+    guess (atoms.py:55-73)."""
+    if occupation not in ("aufbau", "averaged"):
+        raise ValueError("occupation is 'aufbau' or 'averaged'")
+    fill = fill_whole_levels if occupation == "aufbau" else fill_degenerate_groups
     x = canonical_orthogonalizer(overlap)
 
     def solve(f):
@@ -211,11 +220,11 @@ def rhf(
             break
         diis.push(fock, grad)
         e, c = solve(diis.extrapolate())
-        occ = fill_whole_levels(e, n_alpha)
+        occ = fill(e, n_alpha)
         density = (c * occ[None, :]) @ c.T
     final = fock if converged else diis.focks[-1]
     e, c = solve(final)
-    occ = fill_whole_levels(e, n_alpha)
+    occ = fill(e, n_alpha)
     density = (c * occ[None, :]) @ c.T
     fock, terms = fock_and_energies(core, j_of, k_of, density, nuclear_repulsion)
     return RhfResult(

@@ -133,3 +133,27 @@ def test_learned_start_falls_back_per_molecule(setup):
     results = batch.run(start, ScfOptions(energy_tolerance=1e-8, gradient_tolerance=1e-6))
     for got, ref in zip(results, serial):
         assert got.energy == pytest.approx(ref.energy, abs=SAME_STATE_TOL)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_pipelined_batches_match_serial(setup, precision):
+    """Two batches on their own streams, iterations interleaved (one batch's eigensolver runs while
+    the other's J/K build does), reach the serial FP64 states; each group as a single batch gives
+    the same energies and iteration counts."""
+    from hafnium_amd.batch_scf import BatchedRhf, atomic_start, balanced_groups, run_pipelined
+    from hafnium_amd.scf import ScfOptions
+
+    _, serial, cache, inputs = setup
+    groups = balanced_groups([x.nao for x in inputs], 2)
+    assert sorted(i for g in groups for i in g) == list(range(len(inputs)))
+    batches = [BatchedRhf([inputs[i] for i in g]) for g in groups]
+    opts = ScfOptions(precision=precision, energy_tolerance=1e-8, gradient_tolerance=1e-6)
+    outs = run_pipelined([(b, atomic_start(b, cache, precision)) for b in batches], opts)
+    alone = [b.run(atomic_start(b, cache, precision), opts) for b in batches]
+    for g, out, ref_out in zip(groups, outs, alone):
+        for i, got, ref in zip(g, out, ref_out):
+            assert got.converged
+            assert got.energy == pytest.approx(serial[i].energy, abs=SAME_STATE_TOL)
+            assert got.energy == pytest.approx(ref.energy, abs=1e-10)
+            assert got.iterations == ref.iterations
+

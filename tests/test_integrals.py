@@ -10,7 +10,7 @@ import pytest
 from hafnium_amd import integrals, molecule
 from oracle import scf_ref
 from tests.conftest import GOLDEN
-from tests.tolerances import ENERGY_TOL
+from tests.tolerances import ENERGY_TOL, FITTED_INTEGRAL_TOL
 
 INTEGRAL_TOL = 1e-12  # the reference's tests/tolerances.py INTEGRAL_TOL
 
@@ -49,6 +49,40 @@ def test_density_fitting_integrals_match_the_stored_metric_and_three_centre(wate
     tc = integrals.three_center(aux, orb)
     np.testing.assert_allclose(m, water["metric"], atol=INTEGRAL_TOL * 1000, rtol=1e-12)
     np.testing.assert_allclose(tc, water["three_center"], atol=INTEGRAL_TOL * 10)
+
+
+def test_four_centre_integrals_match_the_stored_eri(water, water_mol):
+    """(uv|wx) [7, 7, 7, 7] of water in STO-3G against the reference's stored PySCF values, and the
+    eight-fold permutational symmetry."""
+    orb = molecule.BasisOnAtoms("sto-3g", water_mol)
+    eri = integrals.four_center(orb)
+    np.testing.assert_allclose(eri, water["eri"], atol=INTEGRAL_TOL)
+    for perm in ((1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1)):
+        np.testing.assert_array_equal(eri, eri.transpose(perm))
+
+
+def test_four_centre_integrals_of_an_oxygen_atom():
+    """cc-pVDZ oxygen (general contractions, d functions): the [n^2, n^2] matrix (uv|wx) is
+    symmetric positive semidefinite, independent of where the atom sits, and approximated by the
+    density fit of the (separately pinned) metric and three-centre integrals to the reference's
+    fitted-integral tolerance (tests/unit/test_coulomb_exchange.py:54-62 makes the same comparison
+    for water)."""
+    out = []
+    for position in (np.zeros((1, 3)), np.array([[0.3, -1.2, 2.5]])):
+        atom = molecule.Molecule((8,), position)
+        orb = molecule.BasisOnAtoms("cc-pvdz", atom)
+        aux = molecule.BasisOnAtoms("def2-universal-jkfit", atom)
+        n = orb.n_functions
+        eri = integrals.four_center(orb)
+        mat = eri.reshape(n * n, n * n)
+        np.testing.assert_allclose(mat, mat.T, atol=1e-14)
+        w = np.linalg.eigvalsh(mat)
+        assert w.min() > -1e-12 * w.max()
+        b = scf_ref.df_tensor(integrals.metric(aux), integrals.three_center(aux, orb), "cholesky")
+        fitted = np.einsum("Puv,Pwx->uvwx", b, b)
+        np.testing.assert_allclose(fitted, eri, atol=FITTED_INTEGRAL_TOL)
+        out.append(eri)
+    np.testing.assert_allclose(out[1], out[0], atol=1e-14)
 
 
 def test_df_rhf_energy_from_own_integrals(water, water_mol):

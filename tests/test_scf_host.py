@@ -55,3 +55,13 @@ def test_mixed_phase_stall_detection():
     assert not scf.mixed_phase_stalled([1e-1, 1e-2, 2e-2, 5e-3], 2, 0.7)
     with pytest.raises(InputError):
         scf.ScfOptions(stall_ratio=0.0)
+
+
+def test_balanced_groups_deal_by_cost():
+    from hafnium_amd.batch_scf import balanced_groups
+
+    groups = balanced_groups([48, 288, 96, 240, 144, 192], 2)
+    assert sorted(i for g in groups for i in g) == list(range(6))
+    cost = [sum([48, 288, 96, 240, 144, 192][i] ** 4 for i in g) for g in groups]
+    assert max(cost) / min(cost) < 1.6
+    assert balanced_groups([10], 3) == [[0]]
