@@ -440,9 +440,14 @@ def cluster_record(n_waters: int, seed: int, sad_cache: dict, *, device=None) ->
             "local": local, "energy": result.energy, "iterations": result.n_iterations}
 
 
-def generate_dataset(seeds, sizes, path: str, *, device=None, log=print, checkpoint_every: int = 100) -> dict:
+def generate_dataset(seeds, sizes, path: str, *, device=None, log=print, checkpoint_every: int = 100,
+                     keep_features: bool = True) -> dict:
     """Converged FP64 densities of ``water_cluster(sizes[i], seeds[i])`` as per-element arrays in ``path``
-    (rewritten every ``checkpoint_every`` clusters, so an interrupted run keeps what it finished)."""
+    (rewritten every ``checkpoint_every`` clusters, so an interrupted run keeps what it finished).
+
+    Without ``keep_features`` only the targets and the (seed, size) of every cluster are stored;
+    :func:`with_features` recomputes the descriptors from them (they depend on the geometry only).
+    """
     sad_cache: dict = {}
     feats = {z: [] for z in ELEMENTS}
     targets = {z: [] for z in ELEMENTS}
@@ -452,7 +457,8 @@ def generate_dataset(seeds, sizes, path: str, *, device=None, log=print, checkpo
     def save():
         arrays = {"meta": np.asarray(meta, dtype=np.float64)}
         for z in ELEMENTS:
-            arrays[f"x_{z}"] = np.concatenate(feats[z])
+            if keep_features:
+                arrays[f"x_{z}"] = np.concatenate(feats[z])
             arrays[f"y_{z}"] = np.concatenate(targets[z])
             arrays[f"cluster_{z}"] = np.asarray(cluster_of[z], dtype=np.int32)
         np.savez_compressed(path, **arrays)
@@ -472,6 +478,40 @@ def generate_dataset(seeds, sizes, path: str, *, device=None, log=print, checkpo
         if checkpoint_every and (index + 1) % checkpoint_every == 0:
             save()
     return save()
+
+
+def merge_datasets(parts: list[dict]) -> dict:
+    """One dataset from several of :func:`generate_dataset` (cluster indices renumbered)."""
+    if not parts:
+        raise InputError("no dataset to merge")
+    out = {"meta": np.concatenate([p["meta"] for p in parts])}
+    for z in ELEMENTS:
+        offsets = np.cumsum([0] + [int(p["meta"].shape[0]) for p in parts[:-1]])
+        out[f"y_{z}"] = np.concatenate([p[f"y_{z}"] for p in parts])
+        out[f"cluster_{z}"] = np.concatenate([p[f"cluster_{z}"] + o for p, o in zip(parts, offsets)]).astype(np.int32)
+        if all(f"x_{z}" in p for p in parts):
+            out[f"x_{z}"] = np.concatenate([p[f"x_{z}"] for p in parts])
+    return out
+
+
+def with_features(data: dict) -> dict:
+    """``data`` with the per-element descriptors, recomputed from each cluster's (seed, size) in
+    ``meta`` where the dataset does not store them (same order as the targets)."""
+    if all(f"x_{z}" in data for z in ELEMENTS):
+        return data
+    feats = {z: [] for z in ELEMENTS}
+    for seed, n in data["meta"][:, :2].astype(np.int64):
+        mol = water_cluster(int(n), int(seed))
+        x = descriptors(mol).astype(np.float32)
+        charges = np.asarray(mol.charges)
+        for z in ELEMENTS:
+            feats[z].append(x[charges == z])
+    out = dict(data)
+    for z in ELEMENTS:
+        out[f"x_{z}"] = np.concatenate(feats[z])
+        if out[f"x_{z}"].shape[0] != out[f"y_{z}"].shape[0]:
+            raise InputError("descriptors and targets do not line up")
+    return out
 
 
 # ---------------------------------------------------------------- training
@@ -732,9 +772,11 @@ def _main(argv=None) -> int:
     g.add_argument("--count", type=int, default=400)
     g.add_argument("--sizes", default="2-8", help="water counts drawn uniformly from this range")
     g.add_argument("--seed0", type=int, default=100000)
+    g.add_argument("--range", default=None, help="start:stop slice of the count clusters to converge here")
+    g.add_argument("--no-features", action="store_true", help="store targets only (descriptors recomputed)")
     g.add_argument("--out", required=True)
-    t = sub.add_parser("train", help="dataset .npz -> model .npz")
-    t.add_argument("--data", required=True)
+    t = sub.add_parser("train", help="dataset .npz files -> model .npz")
+    t.add_argument("--data", required=True, nargs="+")
     t.add_argument("--out", required=True)
     t.add_argument("--epochs", type=int, default=60)
     t.add_argument("--lr", type=float, default=1e-3)
@@ -764,10 +806,14 @@ def _main(argv=None) -> int:
         rng = np.random.default_rng(args.seed0)
         sizes = rng.integers(lo, hi + 1, args.count)
         seeds = args.seed0 + np.arange(args.count)
-        generate_dataset(seeds, sizes, args.out)
+        start, stop = (0, args.count) if args.range is None else (int(v) for v in args.range.split(":"))
+        generate_dataset(seeds[start:stop], sizes[start:stop], args.out, keep_features=not args.no_features)
     else:
-        with np.load(args.data, allow_pickle=False) as f:
-            data = {k: f[k] for k in f.files}
+        parts = []
+        for path in args.data:
+            with np.load(path, allow_pickle=False) as f:
+                parts.append({k: f[k] for k in f.files})
+        data = with_features(merge_datasets(parts))
         model, report = train(data, epochs=args.epochs, lr=args.lr, batch=args.batch,
                               weight_decay=args.weight_decay)
         model.save(args.out)

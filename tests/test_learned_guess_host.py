@@ -155,3 +155,34 @@ def test_molecular_factors_rotate_orbitals_with_the_integral_engine():
     assert np.abs(c.T @ unrotated @ c - gram).max() > 1e-3 * np.abs(gram).max()
     _, self_fac = lg.molecular_factors(ref, c)
     np.testing.assert_allclose(self_fac, c, atol=1e-12)
+
+
+def test_datasets_merge_and_recompute_their_descriptors():
+    """Chunks of a dataset stored without descriptors merge into one, with cluster indices
+    renumbered, and the descriptors recomputed from (seed, size) equal the direct ones."""
+    from hafnium_amd import learned_guess as lg
+    from hafnium_amd.molecule import water_cluster
+
+    specs = [(2, 11), (3, 12), (1, 13)]
+
+    def part(rows, first):
+        out = {"meta": np.array([[seed, n, -1.0, 10] for n, seed in rows], dtype=np.float64)}
+        for z in lg.ELEMENTS:
+            ys, cl = [], []
+            for k, (n, seed) in enumerate(rows):
+                count = int(np.sum(np.asarray(water_cluster(n, seed).charges) == z))
+                ys.append(np.full((count, 3), first + k, dtype=np.float64))
+                cl += [k] * count
+            out[f"y_{z}"] = np.concatenate(ys)
+            out[f"cluster_{z}"] = np.asarray(cl, dtype=np.int32)
+        return out
+
+    merged = lg.with_features(lg.merge_datasets([part(specs[:2], 0), part(specs[2:], 2)]))
+    assert merged["meta"].shape[0] == 3
+    for z in lg.ELEMENTS:
+        np.testing.assert_array_equal(merged[f"y_{z}"][:, 0], merged[f"cluster_{z}"])
+        direct = []
+        for n, seed in specs:
+            mol = water_cluster(n, seed)
+            direct.append(lg.descriptors(mol)[np.asarray(mol.charges) == z].astype(np.float32))
+        np.testing.assert_array_equal(merged[f"x_{z}"], np.concatenate(direct))
