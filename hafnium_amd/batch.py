@@ -201,9 +201,11 @@ class BatchedCoulombExchange:
 
     # ------------------------------------------------------------- run
 
-    def run(self, orbitals, densities, *, precision: str = "fp64", tile: int = 0, out=None, noccs=None):
+    def run(self, orbitals, densities=None, *, precision: str = "fp64", tile: int = 0, out=None, noccs=None):
         """One batched J/K build; returns flat ``(J, K)`` buffers (see :meth:`matrix`).
-        ``orbitals`` follow the layout of ``noccs`` (default: the batch's own counts)."""
+        ``orbitals`` follow the layout of ``noccs`` (default: the batch's own counts). Without
+        ``densities`` each instance's density is ``C C^T`` of its orbitals and its Coulomb charges
+        are formed from the half-transformed ``X = B C`` (the fitted tensor is read once less)."""
         torch = _torch()
         code = _lib.precision_code(precision)
         handle, _, _ = self.plan(precision, tile, noccs)
@@ -214,7 +216,8 @@ class BatchedCoulombExchange:
         j, k = out
         _lib.check(self._ctx.lib.hfjk_batch_run(
             handle, ctypes.c_void_p(tensor.data_ptr()), ctypes.c_void_p(orbitals.data_ptr()),
-            ctypes.c_void_p(densities.data_ptr()), ctypes.c_void_p(j.data_ptr()), ctypes.c_void_p(k.data_ptr()),
+            ctypes.c_void_p(densities.data_ptr()) if densities is not None else None,
+            ctypes.c_void_p(j.data_ptr()), ctypes.c_void_p(k.data_ptr()),
             self._stream()))
         return j, k
 
@@ -234,7 +237,7 @@ class BatchedCoulombExchange:
             ctypes.c_void_p(out.data_ptr()), self._stream()))
         return out
 
-    def capture(self, orbitals, densities, *, precision: str = "fp64", tile: int = 0):
+    def capture(self, orbitals, densities=None, *, precision: str = "fp64", tile: int = 0):
         """A hipGraph (``torch.cuda.CUDAGraph``) of one batched build and its output buffers.
 
         The plan and tensor copy are prepared before capture; the captured work

@@ -182,9 +182,8 @@ class BatchedRhf:
             if kept[i] < x.n_alpha:
                 raise InputError(f"molecule {i}: {x.n_alpha} occupied orbitals do not fit {kept[i]} functions")
         # Work buffers.
-        self._dc = torch.empty(self.jk.matrix_elems, dtype=torch.float64, device=self.device)
-        self._jc = torch.empty_like(self._dc)
-        self._kc = torch.empty_like(self._dc)
+        self._jc = torch.empty(self.jk.matrix_elems, dtype=torch.float64, device=self.device)
+        self._kc = torch.empty_like(self._jc)
         self._cc = torch.empty(self.jk.orbital_elems, dtype=torch.float64, device=self.device)
         self._jpad, self._kpad = zeros(cnt, nm, nm), zeros(cnt, nm, nm)
         self._t1, self._t2 = zeros(cnt, nm, nm), zeros(cnt, nm, nm)
@@ -224,9 +223,10 @@ class BatchedRhf:
 
     # ------------------------------------------------------------ SCF steps
     def fock(self, density, orbitals_compact, precision, out, terms, noccs=None):
-        """F = h + 2 J[D] - K[D] into ``out``; terms[0:3] (device rows) = tr(D h), tr(D J), tr(D K)."""
-        self._copy(self._nao, self._nao, self._moffs, self._max_mat, density, self.nm, self._nn, self._dc, True)
-        self.jk.run(orbitals_compact, self._dc, precision=precision, out=(self._jc, self._kc), noccs=noccs)
+        """F = h + 2 J[D] - K[D] into ``out`` for D = C C^T of ``orbitals_compact`` (the padded
+        ``density`` is that product; J's charges come from the build's half-transform);
+        terms[0:3] (device rows) = tr(D h), tr(D J), tr(D K)."""
+        self.jk.run(orbitals_compact, None, precision=precision, out=(self._jc, self._kc), noccs=noccs)
         self._copy(self._nao, self._nao, self._moffs, self._max_mat, self._jpad, self.nm, self._nn, self._jc, False)
         self._copy(self._nao, self._nao, self._moffs, self._max_mat, self._kpad, self.nm, self._nn, self._kc, False)
         self._combine(1.0, self.h, 2.0, self._jpad, -1.0, self._kpad, out)

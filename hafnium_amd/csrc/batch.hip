@@ -168,6 +168,41 @@ __global__ __launch_bounds__(kThreads) void b_charges_load_kernel(BatchView v, c
   BATCH_END
 }
 
+// gamma[q] = <B[q], C C^T> = sum_{mu < nao, i < nocc} C[mu][i] X[q][mu][i] from the
+// half-transformed X of the exchange (mode layout; BF16X3: hi + lo planes)
+// and the caller's FP64 orbitals, one aux function per item.
+template <int PREC>
+__global__ __launch_bounds__(kThreads) void b_charges_x_kernel(BatchView v, const double* __restrict__ c,
+                                                               const void* __restrict__ xbase, long long x_plane,
+                                                               double* __restrict__ ws64) {
+  __shared__ double red[kThreads / 64];
+  BATCH_LOOP(kPhCharges)
+  const double* ci = c + in.c_off;
+  const long long xq = in.x_off + local * in.xrows * in.ldo;
+  const long long total = (long long)in.nao * in.nocc;
+  double s = 0.0;
+  for (long long e = threadIdx.x; e < total; e += kThreads) {
+    const int mu = (int)(e / in.nocc), i = (int)(e - (long long)mu * in.nocc);
+    const long long at = xq + (long long)mu * in.ldo + i;
+    double x;
+    if (PREC == HFJK_FP64) {
+      x = ((const double*)xbase)[at];
+    } else if (PREC == HFJK_FP32) {
+      x = (double)((const float*)xbase)[at];
+    } else {
+      const uint16_t* xh = (const uint16_t*)xbase;
+      x = (double)(__uint_as_float(((uint32_t)xh[at]) << 16) + __uint_as_float(((uint32_t)xh[at + x_plane]) << 16));
+    }
+    s += ci[e] * x;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) ws64[in.g_off + local] = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  BATCH_END
+}
+
 // Packed J of one tile: sum_q gamma[q] B[q][tile], one tile per item.
 template <int PREC>
 __global__ __launch_bounds__(kThreads) void b_coulomb_kernel(BatchView v, const void* __restrict__ b, long long plane,
@@ -264,6 +299,24 @@ hipError_t launch_batch_coulomb(BatchView v, int precision, const void* b, long 
   return hipGetLastError();
 }
 
+hipError_t launch_batch_coulomb_occ(BatchView v, int precision, const void* b, long long b_plane, const double* c,
+                                    const void* xbase, long long x_plane, double* ws64, double* j, int grid,
+                                    hipStream_t s) {
+  const dim3 g((unsigned)grid), blk(kThreads);
+  if (precision == HFJK_FP64) {
+    hipLaunchKernelGGL(b_charges_x_kernel<HFJK_FP64>, g, blk, 0, s, v, c, xbase, x_plane, ws64);
+    hipLaunchKernelGGL(b_coulomb_kernel<HFJK_FP64>, g, blk, 0, s, v, b, b_plane, ws64);
+  } else if (precision == HFJK_FP32) {
+    hipLaunchKernelGGL(b_charges_x_kernel<HFJK_FP32>, g, blk, 0, s, v, c, xbase, x_plane, ws64);
+    hipLaunchKernelGGL(b_coulomb_kernel<HFJK_FP32>, g, blk, 0, s, v, b, b_plane, ws64);
+  } else {
+    hipLaunchKernelGGL(b_charges_x_kernel<HFJK_BF16X3>, g, blk, 0, s, v, c, xbase, x_plane, ws64);
+    hipLaunchKernelGGL(b_coulomb_kernel<HFJK_BF16X3>, g, blk, 0, s, v, b, b_plane, ws64);
+  }
+  hipLaunchKernelGGL(b_coulomb_unpack_kernel, g, blk, 0, s, v, ws64, j);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_coulomb_charges(BatchView v, int precision, const void* b, long long b_plane,
                                         const double* charges, double* ws64, double* j, int grid, hipStream_t s) {
   const dim3 g((unsigned)grid), blk(kThreads);
@@ -331,7 +384,6 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
   bt->tile = tile != 0 ? tile : (max_ldo <= 32 ? 32 : max_ldo <= 64 ? 64 : 128);
   const int rows = hfjk::batch_tile_rows(precision);
   const int es = precision == HFJK_FP64 ? 8 : precision == HFJK_FP32 ? 4 : 2;  // bf16: per plane
-  const long long tile2 = (long long)hfjk::syrk_tile() * hfjk::syrk_tile();
   long long w64 = 0, cp = 0, xx = 0, kp = 0, qq = 0;
   for (int i = 0; i < ninst; ++i) {
     const hfjk_batch_instance& src = instances[i];
@@ -350,7 +402,16 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
     in.ldo = hfjk::occ_ld(src.nocc);
     in.nt = hfjk::n_tiles(src.nao);
     in.xrows = hfjk::x_rows(src.nao);
-    in.npairs = hfjk::syrk_pairs(src.nao);
+    // SYRK tile: 64-row blocks where they cut the computed lower-triangle area below 80 % of the
+    // 128-row tiling (small instances); the BF16X3 SYRK has 128-row tiles only.
+    {
+      const long long b64 = (in.xrows + 63) / 64, b128 = (in.xrows + 127) / 128;
+      const long long area64 = b64 * (b64 + 1) / 2 * 64 * 64, area128 = b128 * (b128 + 1) / 2 * 128 * 128;
+      in.st = (precision != HFJK_BF16X3 && 5 * area64 < 4 * area128) ? 64 : hfjk::syrk_tile();
+      const long long nb = (in.xrows + in.st - 1) / in.st;
+      in.npairs = (int)(nb * (nb + 1) / 2);
+      in.pad_ = 0;
+    }
     in.nsplit = (int)std::max(1LL, std::min(64LL, ((long long)src.naux * in.ldo + 8191) / 8192));
     in.nbm = (in.xrows + rows - 1) / rows;
     in.nbn = (in.ldo + bt->tile - 1) / bt->tile;
@@ -375,7 +436,7 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
     in.x_off = xx;
     xx += (long long)src.naux * in.xrows * in.ldo + 64;
     in.kp_off = kp;
-    kp += (long long)in.nsplit * in.npairs * tile2;
+    kp += (long long)in.nsplit * in.npairs * in.st * in.st;
   }
   bt->ws64_total = w64;
   bt->cp_total = cp;
@@ -397,6 +458,7 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
     items[hfjk::kPhXform] = (long long)in.naux * in.nbm * in.nbn;
     items[hfjk::kPhSyrk] = (long long)in.npairs * in.nsplit;
     items[hfjk::kPhSyrkReduce] = (nn + hfjk::kThreads - 1) / hfjk::kThreads;
+    items[hfjk::kPhCharges] = in.naux;
     for (int ph = 0; ph < hfjk::kPhCount; ++ph) {
       const size_t base = (size_t)ph * (ninst + 1);
       bt->prefix[base + i + 1] = bt->prefix[base + i] + items[ph];
@@ -484,8 +546,8 @@ int hfjk_batch_run(hfjk_batch* batch, const void* tensor, const double* orbitals
     hfjk_internal_set_error("the batch is not bound to a workspace (hfjk_batch_bind)");
     return HFJK_ERR_INPUT;
   }
-  if (tensor == nullptr || orbitals == nullptr || density == nullptr || coulomb == nullptr || exchange == nullptr) {
-    hfjk_internal_set_error("hfjk_batch_run needs the tensor, orbitals, densities and both outputs");
+  if (tensor == nullptr || orbitals == nullptr || coulomb == nullptr || exchange == nullptr) {
+    hfjk_internal_set_error("hfjk_batch_run needs the tensor, orbitals and both outputs");
     return HFJK_ERR_INPUT;
   }
   hipStream_t s = (hipStream_t)stream;
@@ -502,13 +564,17 @@ int hfjk_batch_run(hfjk_batch* batch, const void* tensor, const double* orbitals
   const int prec = batch->precision;
   hipError_t e = hipSetDevice(batch->ctx->device);
   if (e == hipSuccess) e = hfjk::launch_batch_reset(v, s);
-  if (e == hipSuccess)
+  if (e == hipSuccess && density != nullptr)
     e = hfjk::launch_batch_coulomb(v, prec, tensor, batch->b_total, density, ws64, coulomb, batch->grid_elem, s);
   if (e == hipSuccess)
     e = hfjk::launch_batch_orbitals(v, prec, orbitals, ws64, cbase, batch->cp_total, batch->grid_elem, s);
   if (e == hipSuccess)
     e = hfjk::launch_batch_xform(v, prec, batch->tile, tensor, batch->b_total, cbase, batch->cp_total, xbase,
                                  batch->x_total, batch->grid_mfma, s);
+  // Without densities D = C C^T: the charges come from X (one pass over X instead of one over B).
+  if (e == hipSuccess && density == nullptr)
+    e = hfjk::launch_batch_coulomb_occ(v, prec, tensor, batch->b_total, orbitals, xbase, batch->x_total, ws64,
+                                       coulomb, batch->grid_elem, s);
   if (e == hipSuccess)
     e = hfjk::launch_batch_syrk(v, prec, xbase, batch->x_total, ws64, kbase, batch->grid_mfma, s);
   if (e == hipSuccess) e = hfjk::launch_batch_syrk_reduce(v, prec, kbase, exchange, batch->grid_elem, s);

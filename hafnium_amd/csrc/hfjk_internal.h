@@ -196,6 +196,7 @@ struct BatchInst {
   int nao, naux, nocc, ldo;
   int nt, xrows, npairs, nsplit;
   int nbm, nbn;       // half-transform row / column blocks at the batch tile
+  int st, pad_;       // SYRK tile edge (64 or 128) and padding
   long long p;        // packed elements per aux function
   long long b_off;    // batch tensor (element units of the bound copy)
   long long c_off;    // orbitals [nao][nocc] (float64)
@@ -221,6 +222,7 @@ enum BatchPhase {
   kPhXform,
   kPhSyrk,
   kPhSyrkReduce,
+  kPhCharges,  // charges from the half-transformed X (D = C C^T)
   kPhCount
 };
 
@@ -236,6 +238,9 @@ struct BatchView {
 // Launchers of the batched persistent kernels (batch.hip, xk.hip). `grid`
 // is a multiple of 8 workgroups; mode buffers are typed by `precision`.
 hipError_t launch_batch_reset(BatchView v, hipStream_t s);
+hipError_t launch_batch_coulomb_occ(BatchView v, int precision, const void* b, long long b_plane, const double* c,
+                                    const void* xbase, long long x_plane, double* ws64, double* j, int grid,
+                                    hipStream_t s);
 hipError_t launch_batch_coulomb_charges(BatchView v, int precision, const void* b, long long b_plane,
                                         const double* charges, double* ws64, double* j, int grid, hipStream_t s);
 hipError_t launch_batch_coulomb(BatchView v, int precision, const void* b, long long b_plane, const double* d,

@@ -638,13 +638,16 @@ struct SyrkSmem {
   T Bs[kSyrkTile][kTile + 1];
 };
 
-// FP64 / FP32: one 128 x 128 tile (row block mb, column block nb) of
-// sum_{q in [qb, qe)} X_q S X_q^T into dst[128][128]; x = the instance's X.
-template <typename T>
+// FP64 / FP32: one TS x TS tile (row block mb, column block nb; TS = 128, or
+// 64 for the small instances of a batch) of sum_{q in [qb, qe)} X_q S X_q^T
+// into dst[TS][TS]; x = the instance's X.
+template <typename T, int TS = kSyrkTile>
 __device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int qe, int xrows, int ldo,
                                           const double* __restrict__ signs, int mb, int nb, T* __restrict__ dst,
                                           SyrkSmem<T>& sm) {
-  constexpr int MI = 4, NI = 4;  // wave tile 64 x 64
+  static_assert(TS == 64 || TS == 128, "SYRK tiles are 64 or 128 rows");
+  constexpr int MI = TS / 32, NI = TS / 32;  // wave tile TS/2 x TS/2
+  constexpr int NV = TS / 16;                // 2-element chunks per thread and operand
   auto& As = sm.As;
   auto& Bs = sm.Bs;
   const int t = threadIdx.x;
@@ -660,20 +663,20 @@ __device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int q
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
 
-  // 128 rows x 32 columns per operand = 2048 2-element chunks: 8 per thread.
-  // The prefetch only copies loaded values into registers; the row masks and
-  // the column signs are applied in store(), so the wait for the loads falls
-  // at the end of the k step, behind its MFMAs, not at its start.
-  typename Op<T>::vec2 ra[8], rb[8];
+  // TS rows x 32 columns per operand = 16 TS 2-element chunks: TS / 16 per
+  // thread. The prefetch only copies loaded values into registers; the row
+  // masks and the column signs are applied in store(), so the wait for the
+  // loads falls at the end of the k step, behind its MFMAs, not at its start.
+  typename Op<T>::vec2 ra[NV], rb[NV];
   T sg[2];
   const int nchunk = ldo / kTile;
-  const int mrows = min(kSyrkTile, xrows - mb * kSyrkTile);
-  const int nrows = min(kSyrkTile, xrows - nb * kSyrkTile);
+  const int mrows = min(TS, xrows - mb * TS);
+  const int nrows = min(TS, xrows - nb * TS);
   auto load = [&](int q, int i0) {
-    const T* xm = x + ((long long)q * xrows + mb * kSyrkTile) * ldo + i0;
-    const T* xn = x + ((long long)q * xrows + nb * kSyrkTile) * ldo + i0;
+    const T* xm = x + ((long long)q * xrows + mb * TS) * ldo + i0;
+    const T* xn = x + ((long long)q * xrows + nb * TS) * ldo + i0;
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
+    for (int v = 0; v < NV; ++v) {
       const int chunk = t + kThreads * v;
       const int row = chunk >> 4;
       const int c2 = (chunk & 15) * 2;
@@ -686,7 +689,7 @@ __device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int q
   };
   auto store = [&]() {
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
+    for (int v = 0; v < NV; ++v) {
       const int chunk = t + kThreads * v;
       const int row = chunk >> 4;
       const int c2 = (chunk & 15) * 2;
@@ -713,9 +716,9 @@ __device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int q
       const int k = kk + (lane >> 4);
       T af[MI], bf[NI];
 #pragma unroll
-      for (int mi = 0; mi < MI; ++mi) af[mi] = As[wm * 64 + mi * 16 + (lane & 15)][k];
+      for (int mi = 0; mi < MI; ++mi) af[mi] = As[wm * (TS / 2) + mi * 16 + (lane & 15)][k];
 #pragma unroll
-      for (int ni = 0; ni < NI; ++ni) bf[ni] = Bs[wn * 64 + ni * 16 + (lane & 15)][k];
+      for (int ni = 0; ni < NI; ++ni) bf[ni] = Bs[wn * (TS / 2) + ni * 16 + (lane & 15)][k];
 #pragma unroll
       for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
@@ -731,11 +734,11 @@ __device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int q
   for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
     for (int ni = 0; ni < NI; ++ni) {
-      const int col = wn * 64 + ni * 16 + (lane & 15);
+      const int col = wn * (TS / 2) + ni * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = wm * 64 + mi * 16 + Op<T>::row(lane, r);
-        dst[row * kSyrkTile + col] = acc[mi][ni][r];
+        const int row = wm * (TS / 2) + mi * 16 + Op<T>::row(lane, r);
+        dst[row * TS + col] = acc[mi][ni][r];
       }
     }
 }
@@ -1576,8 +1579,11 @@ __global__ __launch_bounds__(kThreads) void batch_syrk_kernel(BatchView v, const
     tile_of(pair, mb, nb);
     const int qb = (int)((long long)s * in.naux / in.nsplit);
     const int qe = (int)((long long)(s + 1) * in.naux / in.nsplit);
-    syrk_tile<T>(xbase + in.x_off, qb, qe, in.xrows, in.ldo, ws64 + in.sg_off, mb, nb,
-                 kbase + in.kp_off + ((long long)s * in.npairs + pair) * (kSyrkTile * kSyrkTile), sm);
+    T* dst = kbase + in.kp_off + ((long long)s * in.npairs + pair) * ((long long)in.st * in.st);
+    if (in.st == 64)
+      syrk_tile<T, 64>(xbase + in.x_off, qb, qe, in.xrows, in.ldo, ws64 + in.sg_off, mb, nb, dst, sm);
+    else
+      syrk_tile<T, kSyrkTile>(xbase + in.x_off, qb, qe, in.xrows, in.ldo, ws64 + in.sg_off, mb, nb, dst, sm);
   }
 }
 
@@ -1619,20 +1625,21 @@ __global__ __launch_bounds__(kThreads) void batch_syrk_reduce_kernel(BatchView v
     if (idx < (long long)in.nao * in.nao) {
       const int row = (int)(idx / in.nao);
       const int col = (int)(idx % in.nao);
-      const int bm = row / kSyrkTile, bn = col / kSyrkTile;
+      const int st = in.st;
+      const int bm = row / st, bn = col / st;
       long long pair;
       int local;
       if (bm >= bn) {
         pair = tidx(bm, bn);
-        local = (row % kSyrkTile) * kSyrkTile + col % kSyrkTile;
+        local = (row % st) * st + col % st;
       } else {
         pair = tidx(bn, bm);
-        local = (col % kSyrkTile) * kSyrkTile + row % kSyrkTile;
+        local = (col % st) * st + row % st;
       }
       const T* part = kbase + in.kp_off;
       double sum = 0.0;
       for (int s = 0; s < in.nsplit; ++s)
-        sum += (double)part[((long long)s * in.npairs + pair) * (kSyrkTile * kSyrkTile) + local];
+        sum += (double)part[((long long)s * in.npairs + pair) * ((long long)st * st) + local];
       kout[in.d_off + idx] = sum;
     }
   }

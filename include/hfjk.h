@@ -199,7 +199,10 @@ HFJK_API int hfjk_batch_sizes(const hfjk_batch *batch, long long *tensor_elems,
 HFJK_API int hfjk_batch_bind(hfjk_batch *batch, void *workspace, size_t bytes, void *stream);
 /* Enqueue one batched J/K build; no host synchronisation and no allocation,
  * so a run can be captured in a hipGraph. Runs on one bound workspace must not
- * overlap. */
+ * overlap. `density` may be NULL: then every instance's density is
+ * D = C C^T of its orbitals and the charges gamma_P = <B_P, D> come from the
+ * half-transformed X_P = B_P C (sum of C o X_P, one pass over X instead of
+ * one over B), as hfjk_jk_occ does for one instance. */
 HFJK_API int hfjk_batch_run(hfjk_batch *batch, const void *tensor, const double *orbitals,
                             const double *density, double *coulomb, double *exchange, void *stream);
 /* J_i = sum_P charges_i[P] B_i[P] for given charges (device float64; instance

@@ -48,6 +48,34 @@ def test_every_instance_matches_the_oracle(torch, precision, tile):
         assert jk_ref.max_relative_error(bt.matrix(k, i).cpu().numpy(), jk_ref.exchange(b, ds[i])) < TOL[precision]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+def test_charges_from_x_match_the_density_build(torch, precision):
+    """Without densities (D = C C^T) the charges come from the half-transformed X: J matches the
+    oracle and the density-form batched J, K is the same build."""
+    seeds = list(range(1500, 1500 + len(SHAPES)))
+    bt, cs, ds = build(torch, seeds)
+    c = bt.pack_orbitals(cs)
+    j_d, k_d = bt.run(c, bt.pack_matrices(ds), precision=precision)
+    j_d, k_d = j_d.clone(), k_d.clone()
+    j_x, k_x = bt.run(c, precision=precision)
+    assert torch.equal(k_x, k_d)
+    for i, ((n, q, o), seed) in enumerate(zip(SHAPES, seeds)):
+        b = synthetic.tensor_full(seed, q, n)
+        got = bt.matrix(j_x, i).cpu().numpy()
+        assert jk_ref.max_relative_error(got, jk_ref.coulomb(b, ds[i])) < TOL[precision]
+        if precision == "fp64":
+            assert jk_ref.max_relative_error(got, bt.matrix(j_d, i).cpu().numpy()) < 1e-12
+    graph, (j, k) = bt.capture(c, precision=precision)
+    for _ in range(2):
+        j.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(j, j_x) and torch.equal(k, k_x)
+        bt.run(c, precision=precision)
+        torch.cuda.synchronize()
+
+
 def test_graph_replays_match_eager_runs(torch):
     seeds = list(range(2000, 2000 + len(SHAPES)))
     bt, cs, ds = build(torch, seeds)
