@@ -179,10 +179,12 @@ __global__ __launch_bounds__(kThreads) void b_charges_x_kernel(BatchView v, cons
   BATCH_LOOP(kPhCharges)
   const double* ci = c + in.c_off;
   const long long xq = in.x_off + local * in.xrows * in.ldo;
-  const long long total = (long long)in.nao * in.nocc;
   double s = 0.0;
-  for (long long e = threadIdx.x; e < total; e += kThreads) {
-    const int mu = (int)(e / in.nocc), i = (int)(e - (long long)mu * in.nocc);
+  // Element e = mu * nocc + i of C, stepped by the block size; (mu, i) advance without a division.
+  const int total = in.nao * in.nocc;
+  const int dm = kThreads / in.nocc, di = kThreads % in.nocc;
+  int mu = (int)threadIdx.x / in.nocc, i = (int)threadIdx.x % in.nocc;
+  for (int e = threadIdx.x; e < total; e += kThreads) {
     const long long at = xq + (long long)mu * in.ldo + i;
     double x;
     if (PREC == HFJK_FP64) {
@@ -194,6 +196,12 @@ __global__ __launch_bounds__(kThreads) void b_charges_x_kernel(BatchView v, cons
       x = (double)(__uint_as_float(((uint32_t)xh[at]) << 16) + __uint_as_float(((uint32_t)xh[at + x_plane]) << 16));
     }
     s += ci[e] * x;
+    mu += dm;
+    i += di;
+    if (i >= in.nocc) {
+      i -= in.nocc;
+      ++mu;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -403,7 +411,8 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
     in.nt = hfjk::n_tiles(src.nao);
     in.xrows = hfjk::x_rows(src.nao);
     // SYRK tile: 64-row blocks where they cut the computed lower-triangle area below 80 % of the
-    // 128-row tiling (small instances); the BF16X3 SYRK has 128-row tiles only.
+    // 128-row tiling (small instances, whose few 128-row items also leave a long tail in the
+    // persistent launch); the BF16X3 SYRK has 128-row tiles only.
     {
       const long long b64 = (in.xrows + 63) / 64, b128 = (in.xrows + 127) / 128;
       const long long area64 = b64 * (b64 + 1) / 2 * 64 * 64, area128 = b128 * (b128 + 1) / 2 * 128 * 128;
