@@ -288,7 +288,7 @@ def bench_clusters(args, dev):
         torch.cuda.empty_cache()
     labels = [r[0] for r in runs] if rows else []
     totals = {label: sum(r[label]["seconds"] for r in rows) for label in labels}
-    out = {"basis": "cc-pvdz", "aux": "def2-universal-jkfit", "guess": "atomic densities (DF)",
+    out = {"basis": "cc-pvdz", "aux": "def2-universal-jkfit", "guess": "spherical atoms, exact integrals",
            "energy_tolerance": 1e-8, "mixed_precision": args.cluster_mixed, "clusters": rows,
            "total_seconds": totals,
            "speedup_total": {label: totals["fp64"] / totals[label] for label in labels if totals[label]},
