@@ -306,11 +306,29 @@ class MI355XCoulombExchange:
         return out
 
     def _exchange_real(self, d, prec: int):
+        """K of one real density: the signed factor of a symmetric D, then the occupied-form
+        exchange with a workspace sized for the retained rank; the FP64 density form otherwise."""
         torch = _torch()
-        out = torch.empty((self._nao, self._nao), dtype=torch.float64, device=self._device)
-        ws, nbytes = self._workspace_for(self._nao, prec)
-        _lib.check(self._ctx.lib.hfjk_exchange_density(self._ctx.handle, _ptr(d), _ptr(out), prec, ws, nbytes,
-                                                       self._stream()))
+        n = self._nao
+        lib = self._ctx.lib
+        factor = torch.empty(n * n, dtype=torch.float64, device=self._device)
+        signs = torch.empty(n, dtype=torch.float64, device=self._device)
+        rank = ctypes.c_int()
+        ws, nbytes = self._workspace_for(1, _lib.FP64)
+        _lib.check(lib.hfjk_density_factor(self._ctx.handle, _ptr(d), _ptr(factor), _ptr(signs), ctypes.byref(rank),
+                                           ws, nbytes, self._stream()))
+        r = int(rank.value)
+        out = torch.empty((n, n), dtype=torch.float64, device=self._device)
+        if r < 0:
+            if prec != _lib.FP64:
+                raise BackendError("a non-symmetric density is contracted in FP64 only")
+            _lib.check(lib.hfjk_exchange_general(self._ctx.handle, _ptr(d), _ptr(out), ws, nbytes, self._stream()))
+            return out
+        if r == 0:
+            return out.zero_()
+        ws, nbytes = self._workspace_for(r, prec)
+        _lib.check(lib.hfjk_exchange_occ(self._ctx.handle, _ptr(factor), r, _ptr(signs), _ptr(out), prec, ws, nbytes,
+                                         self._stream()))
         return out
 
     def exchange(self, density: Any, *, precision: str | None = None):

@@ -342,8 +342,8 @@ __global__ void sum_slices_kernel(const double* __restrict__ part, int nslices, 
 // strided-batched DGEMM (a lone n x n output with k = naux npad would fill
 // few CUs) and summed into K. Row-major arrays are read as the column-major
 // transposes rocBLAS sees. 4 naux npad^3 flops.
-void exchange_general(hfjk_ctx* ctx, const double* density, double* out, void* ws, const Layout& L,
-                      hipStream_t s) {
+void exchange_general(hfjk_ctx* ctx, const double* density, double* out, void* ws, size_t ws_bytes,
+                      size_t dens_end, hipStream_t s) {
   if (ctx->b64 == nullptr) fail(HFJK_ERR_BACKEND, "a non-symmetric density needs the bound FP64 tensor");
   const int n = ctx->nao;
   const int npad = hfjk::n_tiles(n) * hfjk::kTile;
@@ -354,7 +354,7 @@ void exchange_general(hfjk_ctx* ctx, const double* density, double* out, void* w
   const int max_slices = (int)std::min<size_t>({(size_t)64, std::max<size_t>(1, ((size_t)1 << 26) / kk),
                                                 (size_t)ctx->naux});
   const size_t parts_bytes = align_up((size_t)max_slices * kk * 8);
-  const size_t avail = L.total - L.dens_end;
+  const size_t avail = ws_bytes > dens_end ? ws_bytes - dens_end : 0;
   const bool have_full = ctx->b64_full != nullptr && ctx->x_full == 0;
   // D padded, the slice partials, then the G and F chunks (F only without a bound full copy).
   const size_t per_q = (have_full ? 1 : 2) * nn * 8;
@@ -362,10 +362,10 @@ void exchange_general(hfjk_ctx* ctx, const double* density, double* out, void* w
     fail(HFJK_ERR_WORKSPACE, "workspace too small for the general exchange");
   int chunk = (int)std::min<size_t>((size_t)ctx->naux, (avail - mat - parts_bytes - 512) / per_q);
   if (ctx->general_chunk > 0) chunk = std::min(chunk, ctx->general_chunk);
-  double* dpad = at<double>(ws, L.dens_end);
-  double* parts = at<double>(ws, L.dens_end + mat);
-  double* g = at<double>(ws, L.dens_end + mat + parts_bytes);
-  double* f = at<double>(ws, L.dens_end + mat + parts_bytes + align_up((size_t)chunk * nn * 8));
+  double* dpad = at<double>(ws, dens_end);
+  double* parts = at<double>(ws, dens_end + mat);
+  double* g = at<double>(ws, dens_end + mat + parts_bytes);
+  double* f = at<double>(ws, dens_end + mat + parts_bytes + align_up((size_t)chunk * nn * 8));
   rocblas_handle h = blas_handle(ctx);
   check_blas(rocblas_set_stream(h, s), "rocblas_set_stream");
   check_blas(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host), "rocblas_set_pointer_mode");
@@ -376,7 +376,8 @@ void exchange_general(hfjk_ctx* ctx, const double* density, double* out, void* w
     const double* fq = have_full ? ctx->b64_full + (size_t)q0 * nn : f;
     if (!have_full) {
       Stage st(ctx, s, "general_expand");
-      check_hip(hfjk::launch_expand_full(ctx->b64 + (size_t)q0 * L.p, L.p, pc, n, HFJK_FP64, f, s),
+      const long long p = hfjk::packed_elems(n);
+      check_hip(hfjk::launch_expand_full(ctx->b64 + (size_t)q0 * p, p, pc, n, HFJK_FP64, f, s),
                 "expand launch");
     }
     {
@@ -726,6 +727,73 @@ int hfjk_jk_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, double* coulomb
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// For a symmetric density (max |D - D^T| <= 1e-12 max |D|) the signed factor
+// D = C diag(signs) C^T from rocSOLVER dsyevd, eigenpairs with
+// |l| <= 1e-12 max |l| dropped: C = V |l|^1/2 into cf [nao][r] (row-major,
+// the orbital layout of the occupied-form exchange) and the signs into
+// sg [r]. Returns r, or -1 for a non-symmetric density (nothing written).
+// Uses the density region of the workspace layout; synchronises s.
+int density_factor(hfjk_ctx* ctx, const double* density, void* workspace, const Layout& L, double* cf, double* sg,
+                   hipStream_t s, double* asym_out) {
+  const int n = ctx->nao;
+  double* a = at<double>(workspace, L.dens_a);
+  double* w = at<double>(workspace, L.dens_w);
+  double* e = at<double>(workspace, L.dens_e);
+  rocblas_int* info = at<rocblas_int>(workspace, L.dens_info);
+  unsigned long long* stats = at<unsigned long long>(workspace, L.dens_stats);
+  const long long nn = (long long)n * n;
+  unsigned long long host_stats[2];
+  {
+    Stage st(ctx, s, "density_stats");
+    check_hip(hipMemsetAsync(stats, 0, 2 * sizeof(unsigned long long), s), "hipMemsetAsync");
+    hipLaunchKernelGGL(density_stats_kernel, dim3(hfjk::grid_for(nn)), dim3(256), 0, s, density, n, stats);
+    check_hip(hipGetLastError(), "density stats launch");
+  }
+  check_hip(hipMemcpyAsync(host_stats, stats, sizeof(host_stats), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+  double asym, mag;
+  std::memcpy(&asym, &host_stats[0], sizeof(double));
+  std::memcpy(&mag, &host_stats[1], sizeof(double));
+  if (asym_out != nullptr) *asym_out = asym;
+  if (!(asym <= 1e-12 * (mag > 0 ? mag : 1.0))) return -1;
+  {
+    Stage st(ctx, s, "density_eigh");
+    check_hip(hipMemcpyAsync(a, density, sizeof(double) * nn, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+    rocblas_handle h = blas_handle(ctx);
+    if (rocblas_set_stream(h, s) != rocblas_status_success) fail(HFJK_ERR_SOLVER, "rocblas_set_stream failed");
+    // The density is symmetric, so its row-major storage is its
+    // column-major storage; eigenvectors come back column-major.
+    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, e, info) !=
+        rocblas_status_success)
+      fail(HFJK_ERR_SOLVER, "rocsolver_dsyevd failed to launch");
+  }
+  std::vector<double> vals(n);
+  rocblas_int host_info = 0;
+  check_hip(hipMemcpyAsync(vals.data(), w, sizeof(double) * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  check_hip(hipMemcpyAsync(&host_info, info, sizeof(host_info), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (host_info != 0) fail(HFJK_ERR_SOLVER, "rocsolver_dsyevd did not converge (info " + std::to_string(host_info) + ")");
+  double top = 0.0;
+  for (double v : vals) top = std::max(top, std::fabs(v));
+  const double cut = 1e-12 * top;
+  int nneg = 0, npos = 0;
+  for (int j = 0; j < n && vals[j] < -cut; ++j) ++nneg;
+  for (int j = n - 1; j >= nneg && vals[j] > cut; --j) ++npos;
+  if (nneg + npos > 0) {
+    Stage st(ctx, s, "density_factor");
+    check_hip(hfjk::launch_split_factor(a, w, n, nneg, npos, cf, sg, s), "split factor launch");
+  }
+  return nneg + npos;
+}
+
+}  // namespace
+
+extern "C" {
+
 int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange, int precision, void* workspace,
                           size_t workspace_bytes, void* stream) {
   return guarded([&] {
@@ -739,67 +807,50 @@ int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange
     check_workspace(L, workspace, workspace_bytes);
     begin_call(ctx);
     hipStream_t s = (hipStream_t)stream;
-    double* a = at<double>(workspace, L.dens_a);
-    double* w = at<double>(workspace, L.dens_w);
-    double* e = at<double>(workspace, L.dens_e);
-    rocblas_int* info = at<rocblas_int>(workspace, L.dens_info);
-    unsigned long long* stats = at<unsigned long long>(workspace, L.dens_stats);
     double* cf = at<double>(workspace, L.dens_c);
     double* sg = at<double>(workspace, L.dens_signs);
-    const long long nn = (long long)n * n;
-    unsigned long long host_stats[2];
-    {
-      Stage st(ctx, s, "density_stats");
-      check_hip(hipMemsetAsync(stats, 0, 2 * sizeof(unsigned long long), s), "hipMemsetAsync");
-      hipLaunchKernelGGL(density_stats_kernel, dim3(hfjk::grid_for(nn)), dim3(256), 0, s, density, n, stats);
-      check_hip(hipGetLastError(), "density stats launch");
-    }
-    check_hip(hipMemcpyAsync(host_stats, stats, sizeof(host_stats), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
-    double asym, mag;
-    std::memcpy(&asym, &host_stats[0], sizeof(double));
-    std::memcpy(&mag, &host_stats[1], sizeof(double));
-    if (!(asym <= 1e-12 * (mag > 0 ? mag : 1.0))) {
+    double asym = 0.0;
+    const int r = density_factor(ctx, density, workspace, L, cf, sg, s, &asym);
+    if (r < 0) {
       // Not symmetric: the two-DGEMM density form, FP64 only.
       if (precision != HFJK_FP64)
         fail(HFJK_ERR_BACKEND, "a non-symmetric density (max |D - D^T| = " + std::to_string(asym) +
                                    ") is contracted in FP64 only");
-      exchange_general(ctx, density, exchange, workspace, L, s);
+      exchange_general(ctx, density, exchange, workspace, workspace_bytes, L.dens_end, s);
       return;
     }
-    {
-      Stage st(ctx, s, "density_eigh");
-      check_hip(hipMemcpyAsync(a, density, sizeof(double) * nn, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
-      rocblas_handle h = blas_handle(ctx);
-      if (rocblas_set_stream(h, s) != rocblas_status_success) fail(HFJK_ERR_SOLVER, "rocblas_set_stream failed");
-      // The density is symmetric, so its row-major storage is its
-      // column-major storage; eigenvectors come back column-major.
-      if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, e, info) !=
-          rocblas_status_success)
-        fail(HFJK_ERR_SOLVER, "rocsolver_dsyevd failed to launch");
-    }
-    std::vector<double> vals(n);
-    rocblas_int host_info = 0;
-    check_hip(hipMemcpyAsync(vals.data(), w, sizeof(double) * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-    check_hip(hipMemcpyAsync(&host_info, info, sizeof(host_info), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
-    if (host_info != 0) fail(HFJK_ERR_SOLVER, "rocsolver_dsyevd did not converge (info " + std::to_string(host_info) + ")");
-    double top = 0.0;
-    for (double v : vals) top = std::max(top, std::fabs(v));
-    const double cut = 1e-12 * top;
-    int nneg = 0, npos = 0;
-    for (int j = 0; j < n && vals[j] < -cut; ++j) ++nneg;
-    for (int j = n - 1; j >= nneg && vals[j] > cut; --j) ++npos;
-    const int r = nneg + npos;
     if (r == 0) {
       check_hip(hipMemsetAsync(exchange, 0, sizeof(double) * n * n, s), "hipMemsetAsync");
       return;
     }
-    {
-      Stage st(ctx, s, "density_factor");
-      check_hip(hfjk::launch_split_factor(a, w, n, nneg, npos, cf, sg, s), "split factor launch");
-    }
     run_exchange(ctx, cf, r, sg, exchange, precision, workspace, make_layout(ctx, r, kprec), s);
+  });
+}
+
+int hfjk_density_factor(hfjk_ctx* ctx, const double* density, double* factor, double* signs, int* rank,
+                        void* workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (ctx->naux <= 0) fail(HFJK_ERR_INPUT, "no fitted tensor is bound (call hfjk_bind)");
+    if (density == nullptr || factor == nullptr || signs == nullptr || rank == nullptr)
+      fail(HFJK_ERR_INPUT, "null pointer");
+    const Layout L = make_layout(ctx, 1, HFJK_FP64);
+    check_workspace(L, workspace, workspace_bytes);
+    begin_call(ctx);
+    *rank = density_factor(ctx, density, workspace, L, factor, signs, (hipStream_t)stream, nullptr);
+  });
+}
+
+int hfjk_exchange_general(hfjk_ctx* ctx, const double* density, double* exchange, void* workspace,
+                          size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (ctx->naux <= 0) fail(HFJK_ERR_INPUT, "no fitted tensor is bound (call hfjk_bind)");
+    if (density == nullptr || exchange == nullptr) fail(HFJK_ERR_INPUT, "null matrix pointer");
+    const Layout L = make_layout(ctx, 1, HFJK_FP64);
+    check_workspace(L, workspace, workspace_bytes);
+    begin_call(ctx);
+    exchange_general(ctx, density, exchange, workspace, workspace_bytes, L.dens_end, (hipStream_t)stream);
   });
 }
 

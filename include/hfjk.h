@@ -170,6 +170,26 @@ HFJK_API int hfjk_jk_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, double
 HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double *exchange, int precision,
                           void *workspace, size_t workspace_bytes, void *stream);
 
+/* The two halves of hfjk_exchange_density with workspaces sized for what
+ * they do (hfjk_exchange_density sizes its workspace for full rank, nocc =
+ * nao, the bound for any density). hfjk_density_factor: for a symmetric
+ * density (1e-12 relative) the signed factor D = C diag(signs) C^T into
+ * factor [nao][rank] (row-major; a buffer of nao * nao doubles) and
+ * signs [rank], *rank = the retained rank (0 for D = 0); *rank = -1 for a
+ * non-symmetric density, nothing written. Synchronises `stream`. K then is
+ * hfjk_exchange_occ(factor, rank, signs) with the workspace of
+ * hfjk_workspace_bytes(ctx, rank, precision). hfjk_exchange_general: K of
+ * any real density in FP64 by the density form (the non-symmetric branch
+ * of hfjk_exchange_density). Both take a workspace of at least
+ * hfjk_workspace_bytes(ctx, 1, HFJK_FP64) bytes; hfjk_exchange_general
+ * processes more aux functions per DGEMM when given more. Replace the same
+ * reference call, DensityFittedCoulombExchange.exchange
+ * (coulomb_exchange.py:184-190). */
+HFJK_API int hfjk_density_factor(hfjk_ctx *ctx, const double *density, double *factor, double *signs, int *rank,
+                                 void *workspace, size_t workspace_bytes, void *stream);
+HFJK_API int hfjk_exchange_general(hfjk_ctx *ctx, const double *density, double *exchange, void *workspace,
+                                   size_t workspace_bytes, void *stream);
+
 /* ---- Batched J/K: many instances of uneven size, one persistent launch per phase ----
  * No counterpart in the reference (it builds one molecule's J/K at a time:
  * coulomb_exchange.py:176-190 per call). For instance i:

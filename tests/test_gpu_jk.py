@@ -148,6 +148,45 @@ def test_indefinite_and_nonsymmetric_densities(torch, precision):
             jk.exchange(a)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_c_abi_density_exchange_matches_the_two_step_path(torch, precision):
+    """hfjk_exchange_density (one call, full-rank workspace) against the factor + occupied-form
+    path of MI355XCoulombExchange.exchange (rank-sized workspace) and the oracle."""
+    import ctypes
+
+    from hafnium_amd import _lib
+
+    rng = np.random.default_rng(9)
+    b = synthetic.tensor_full(12, 14, 70)
+    jk = builder(b, precision=precision)
+    a = rng.standard_normal((70, 70))
+    c = rng.standard_normal((70, 9))
+    prec = _lib.precision_code(precision)
+    jk._ensure_copy(prec)  # the FP32 copy the direct call needs
+    for d in (a + a.T, c @ c.T, np.zeros((70, 70))):
+        dt = torch.from_numpy(d).cuda()
+        out = torch.empty((70, 70), dtype=torch.float64, device="cuda")
+        ws, nbytes = jk._workspace_for(70, prec)
+        _lib.check(jk._ctx.lib.hfjk_exchange_density(jk._ctx.handle, ctypes.c_void_p(dt.data_ptr()),
+                                                     ctypes.c_void_p(out.data_ptr()), prec, ws, nbytes,
+                                                     jk._stream()))
+        two_step = jk.exchange(d)
+        ref = jk_ref.exchange(b, d)
+        if np.any(d):
+            assert rel(out.cpu().numpy(), ref) < TOL[precision]
+            assert rel(two_step, ref) < TOL[precision]
+        else:
+            assert not np.any(out.cpu().numpy()) and not np.any(two_step)
+    if precision == "fp64":
+        dt = torch.from_numpy(a).cuda()
+        out = torch.empty((70, 70), dtype=torch.float64, device="cuda")
+        ws, nbytes = jk._workspace_for(70, prec)
+        _lib.check(jk._ctx.lib.hfjk_exchange_density(jk._ctx.handle, ctypes.c_void_p(dt.data_ptr()),
+                                                     ctypes.c_void_p(out.data_ptr()), prec, ws, nbytes,
+                                                     jk._stream()))
+        assert rel(out.cpu().numpy(), jk_ref.exchange(b, a)) < FP64_REL_TOL
+
+
 @pytest.mark.parametrize("naux,nao,chunk,full", [(5, 1, 0, False), (7, 33, 3, False), (9, 70, 0, True),
                                                  (6, 70, 4, True), (4, 130, 1, False)])
 def test_nonsymmetric_and_complex_densities_take_the_density_form(torch, naux, nao, chunk, full):
