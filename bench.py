@@ -446,8 +446,8 @@ def main(argv=None):
         return 0
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the aux-index shard "
-              "design for several GPUs is described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); sharding the build over "
+              "the aux index across GPUs is designed but not built", file=sys.stderr)
         return 2
 
     import numpy as np

@@ -29,7 +29,8 @@ def water_problem(water):
 
 
 def test_nuclear_repulsion_matches_the_stored_geometry(water):
-    # The value recorded beside the DF fixture in the reference survey.
+    # E_nuc of the stored geometry (Z = 8, 1, 1 at the stored Bohr positions), the value with which
+    # the stored integrals reproduce the stored DF-RHF energy.
     _, _, e_nuc = water_problem(water)
     assert e_nuc == pytest.approx(9.189533762639684, abs=1e-12)
 
