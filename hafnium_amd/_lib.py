@@ -59,7 +59,7 @@ SIGNATURES = [
     ("hfjk_coulomb_charges", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
-    ("hfjk_density_factor", _c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    ("hfjk_density_factor", _c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
     ("hfjk_exchange_general", _c_int, [_vp, _vp, _vp, _vp, _c_size, _vp]),
     ("hfjk_jk_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_expand_full", _c_int, [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp]),

@@ -313,10 +313,10 @@ class MI355XCoulombExchange:
         lib = self._ctx.lib
         factor = torch.empty(n * n, dtype=torch.float64, device=self._device)
         signs = torch.empty(n, dtype=torch.float64, device=self._device)
-        rank = ctypes.c_int()
+        rank, negative = ctypes.c_int(), ctypes.c_int()
         ws, nbytes = self._workspace_for(1, _lib.FP64)
         _lib.check(lib.hfjk_density_factor(self._ctx.handle, _ptr(d), _ptr(factor), _ptr(signs), ctypes.byref(rank),
-                                           ws, nbytes, self._stream()))
+                                           ctypes.byref(negative), ws, nbytes, self._stream()))
         r = int(rank.value)
         out = torch.empty((n, n), dtype=torch.float64, device=self._device)
         if r < 0:
@@ -327,8 +327,9 @@ class MI355XCoulombExchange:
         if r == 0:
             return out.zero_()
         ws, nbytes = self._workspace_for(r, prec)
-        _lib.check(lib.hfjk_exchange_occ(self._ctx.handle, _ptr(factor), r, _ptr(signs), _ptr(out), prec, ws, nbytes,
-                                         self._stream()))
+        # All signs +1 (a positive semidefinite density): the unsigned kernels.
+        _lib.check(lib.hfjk_exchange_occ(self._ctx.handle, _ptr(factor), r, _ptr(signs) if negative.value else None,
+                                         _ptr(out), prec, ws, nbytes, self._stream()))
         return out
 
     def exchange(self, density: Any, *, precision: str | None = None):

@@ -732,13 +732,15 @@ int hfjk_jk_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, double* coulomb
 namespace {
 
 // For a symmetric density (max |D - D^T| <= 1e-12 max |D|) the signed factor
-// D = C diag(signs) C^T from rocSOLVER dsyevd, eigenpairs with
-// |l| <= 1e-12 max |l| dropped: C = V |l|^1/2 into cf [nao][r] (row-major,
-// the orbital layout of the occupied-form exchange) and the signs into
-// sg [r]. Returns r, or -1 for a non-symmetric density (nothing written).
-// Uses the density region of the workspace layout; synchronises s.
+// D = C diag(signs) C^T into cf [nao][r] (row-major, the orbital layout of
+// the occupied-form exchange) and the signs into sg [r]: by pivoted Cholesky
+// (all signs +1) when D is positive semidefinite, else from rocSOLVER dsyevd
+// with eigenpairs |l| <= 1e-12 max |l| dropped, C = V |l|^1/2. Returns r, or
+// -1 for a non-symmetric density (nothing written). Uses the density region
+// of the workspace layout; synchronises s.
 int density_factor(hfjk_ctx* ctx, const double* density, void* workspace, const Layout& L, double* cf, double* sg,
-                   hipStream_t s, double* asym_out) {
+                   hipStream_t s, double* asym_out, int* nneg_out) {
+  *nneg_out = 0;
   const int n = ctx->nao;
   double* a = at<double>(workspace, L.dens_a);
   double* w = at<double>(workspace, L.dens_w);
@@ -760,6 +762,29 @@ int density_factor(hfjk_ctx* ctx, const double* density, void* workspace, const 
   std::memcpy(&mag, &host_stats[1], sizeof(double));
   if (asym_out != nullptr) *asym_out = asym;
   if (!(asym <= 1e-12 * (mag > 0 ? mag : 1.0))) return -1;
+  // A positive semidefinite density (every aufbau or averaged-occupation
+  // density) factorises by pivoted Cholesky, D = L L^T, in milliseconds; its
+  // residual max |D - L L^T| <= 1e-11 max |D| decides, else dsyevd below.
+  int* rank_dev = (int*)info;
+  {
+    Stage st(ctx, s, "density_cholesky");
+    check_hip(hipMemsetAsync(stats, 0, sizeof(unsigned long long), s), "hipMemsetAsync");
+    check_hip(hfjk::launch_pivoted_cholesky(density, n, 1e-12, a, e, rank_dev, stats, s), "pivoted Cholesky launch");
+  }
+  int chol_rank = 0;
+  unsigned long long chol_bits = 0;
+  check_hip(hipMemcpyAsync(&chol_rank, rank_dev, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  check_hip(hipMemcpyAsync(&chol_bits, stats, sizeof(chol_bits), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+  double residual;
+  std::memcpy(&residual, &chol_bits, sizeof(double));
+  if (residual <= 1e-11 * mag && chol_rank >= 0 && chol_rank <= n) {
+    if (chol_rank > 0) {
+      Stage st(ctx, s, "density_factor");
+      check_hip(hfjk::launch_compact_factor(a, n, chol_rank, cf, sg, s), "compact factor launch");
+    }
+    return chol_rank;
+  }
   {
     Stage st(ctx, s, "density_eigh");
     check_hip(hipMemcpyAsync(a, density, sizeof(double) * nn, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
@@ -783,6 +808,7 @@ int density_factor(hfjk_ctx* ctx, const double* density, void* workspace, const 
   int nneg = 0, npos = 0;
   for (int j = 0; j < n && vals[j] < -cut; ++j) ++nneg;
   for (int j = n - 1; j >= nneg && vals[j] > cut; --j) ++npos;
+  *nneg_out = nneg;
   if (nneg + npos > 0) {
     Stage st(ctx, s, "density_factor");
     check_hip(hfjk::launch_split_factor(a, w, n, nneg, npos, cf, sg, s), "split factor launch");
@@ -810,7 +836,8 @@ int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange
     double* cf = at<double>(workspace, L.dens_c);
     double* sg = at<double>(workspace, L.dens_signs);
     double asym = 0.0;
-    const int r = density_factor(ctx, density, workspace, L, cf, sg, s, &asym);
+    int nneg = 0;
+    const int r = density_factor(ctx, density, workspace, L, cf, sg, s, &asym, &nneg);
     if (r < 0) {
       // Not symmetric: the two-DGEMM density form, FP64 only.
       if (precision != HFJK_FP64)
@@ -823,21 +850,22 @@ int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange
       check_hip(hipMemsetAsync(exchange, 0, sizeof(double) * n * n, s), "hipMemsetAsync");
       return;
     }
-    run_exchange(ctx, cf, r, sg, exchange, precision, workspace, make_layout(ctx, r, kprec), s);
+    // All signs +1 (a positive semidefinite density): the unsigned SYRK.
+    run_exchange(ctx, cf, r, nneg > 0 ? sg : nullptr, exchange, precision, workspace, make_layout(ctx, r, kprec), s);
   });
 }
 
 int hfjk_density_factor(hfjk_ctx* ctx, const double* density, double* factor, double* signs, int* rank,
-                        void* workspace, size_t workspace_bytes, void* stream) {
+                        int* negative, void* workspace, size_t workspace_bytes, void* stream) {
   return guarded([&] {
     check_ctx(ctx);
     if (ctx->naux <= 0) fail(HFJK_ERR_INPUT, "no fitted tensor is bound (call hfjk_bind)");
-    if (density == nullptr || factor == nullptr || signs == nullptr || rank == nullptr)
+    if (density == nullptr || factor == nullptr || signs == nullptr || rank == nullptr || negative == nullptr)
       fail(HFJK_ERR_INPUT, "null pointer");
     const Layout L = make_layout(ctx, 1, HFJK_FP64);
     check_workspace(L, workspace, workspace_bytes);
     begin_call(ctx);
-    *rank = density_factor(ctx, density, workspace, L, factor, signs, (hipStream_t)stream, nullptr);
+    *rank = density_factor(ctx, density, workspace, L, factor, signs, (hipStream_t)stream, nullptr, negative);
   });
 }
 

@@ -147,6 +147,13 @@ hipError_t launch_coulomb_unpack(const double* partial, int nsplit, long long p,
 
 // Exchange (jk.hip).
 hipError_t launch_orbitals(const double* c, int nao, int nocc, int precision, void* out, hipStream_t s);
+// Pivoted Cholesky of a symmetric PSD density into l (nao * nao doubles:
+// L by columns, the first *rank used; diag: nao doubles of scratch), then
+// max |D - L L^T| into *residual (bits of a double, atomic max: clear it
+// first).
+hipError_t launch_pivoted_cholesky(const double* density, int nao, double tol, double* l, double* diag, int* rank,
+                                   unsigned long long* residual, hipStream_t s);
+hipError_t launch_compact_factor(const double* l, int nao, int r, double* c, double* signs, hipStream_t s);
 // bn = column tile (32, 64, 128, 256) or 0 for the default for ldo; bm = row
 // tile (64 or 128; 128 applies to column tiles up to 128).
 hipError_t launch_half_transform(const void* b, int precision, long long p, int naux, int nao,

@@ -242,9 +242,158 @@ __global__ __launch_bounds__(kThreads) void split_factor_kernel(const double* __
   }
 }
 
+// Pivoted Cholesky of a symmetric positive semidefinite D [n][n]: D = L L^T
+// with L stored by columns (lt[k][i] = L[i][k], column stride n), each column
+// pivoted on the largest remaining diagonal; stops once that diagonal is
+// <= tol * (largest initial diagonal), after at most n columns. One
+// workgroup of kPcholThreads; the column update reads lt[j][i] across the
+// threads (coalesced) and lt[j][p] once per wave.
+// L[i][k] = (D[p][i] - sum_{j<k} L[i][j] L[p][j]) / sqrt(r_p), r -= L[:,k]^2.
+constexpr int kPcholThreads = 1024;
+
+__global__ __launch_bounds__(kPcholThreads) void pivoted_cholesky_kernel(const double* __restrict__ d, int n,
+                                                                        double tol, double* __restrict__ l,
+                                                                        double* __restrict__ diag,
+                                                                        int* __restrict__ rank_out) {
+  __shared__ double wval[kPcholThreads / 64];
+  __shared__ int widx[kPcholThreads / 64];
+  __shared__ double s_piv, s_top;
+  __shared__ int s_p;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  // Largest remaining diagonal and its index over the workgroup (ties: lowest index).
+  auto argmax = [&](double& best, int& at) {
+    best = -1.0;
+    at = -1;
+    for (int i = t; i < n; i += kPcholThreads) {
+      const double v = diag[i];
+      if (v > best) {
+        best = v;
+        at = i;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(best, off);
+      const int oi = __shfl_xor(at, off);
+      if (ov > best || (ov == best && oi >= 0 && (at < 0 || oi < at))) {
+        best = ov;
+        at = oi;
+      }
+    }
+    if (lane == 0) {
+      wval[wave] = best;
+      widx[wave] = at;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double b = wval[0];
+      int a = widx[0];
+      for (int w = 1; w < kPcholThreads / 64; ++w)
+        if (wval[w] > b || (wval[w] == b && widx[w] >= 0 && (a < 0 || widx[w] < a))) {
+          b = wval[w];
+          a = widx[w];
+        }
+      s_piv = b;
+      s_p = a;
+    }
+    __syncthreads();
+    best = s_piv;
+    at = s_p;
+  };
+  for (int i = t; i < n; i += kPcholThreads) diag[i] = d[(long long)i * n + i];
+  __syncthreads();
+  double top;
+  int at;
+  argmax(top, at);
+  if (t == 0) s_top = top;
+  __syncthreads();
+  const double cut = tol * (s_top > 0.0 ? s_top : 0.0);
+  int k = 0;
+  for (; k < n; ++k) {
+    double piv;
+    int p;
+    if (k > 0) {
+      argmax(piv, p);
+    } else {
+      piv = s_top;
+      p = at;
+    }
+    if (!(piv > cut) || p < 0) break;  // uniform: every thread holds the reduced values
+    const double inv = 1.0 / sqrt(piv);
+    for (int i = t; i < n; i += kPcholThreads) {
+      double v0 = d[(long long)p * n + i], v1 = 0.0, v2 = 0.0, v3 = 0.0;
+      int j = 0;
+      for (; j + 4 <= k; j += 4) {
+        v0 -= l[(long long)j * n + i] * l[(long long)j * n + p];
+        v1 -= l[(long long)(j + 1) * n + i] * l[(long long)(j + 1) * n + p];
+        v2 -= l[(long long)(j + 2) * n + i] * l[(long long)(j + 2) * n + p];
+        v3 -= l[(long long)(j + 3) * n + i] * l[(long long)(j + 3) * n + p];
+      }
+      for (; j < k; ++j) v0 -= l[(long long)j * n + i] * l[(long long)j * n + p];
+      const double v = ((v0 + v1) + (v2 + v3)) * inv;
+      l[(long long)k * n + i] = v;
+      diag[i] -= v * v;
+    }
+    __syncthreads();
+  }
+  if (t == 0) *rank_out = k;
+}
+
+// max |D[i][j] - sum_{k < rank} L[i][k] L[j][k]| into *out (bits of a
+// non-negative double, atomic max; L by columns as above); rank read from
+// the device.
+__global__ __launch_bounds__(kThreads) void factor_residual_kernel(const double* __restrict__ d, int n,
+                                                                   const double* __restrict__ l,
+                                                                   const int* __restrict__ rank,
+                                                                   unsigned long long* __restrict__ out) {
+  const int r = *rank;
+  double worst = 0.0;
+  const long long total = (long long)n * n;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    double v = d[idx];
+    for (int k = 0; k < r; ++k) v -= l[(long long)k * n + i] * l[(long long)k * n + j];
+    worst = fmax(worst, fabs(v));
+  }
+  for (int off = 32; off > 0; off >>= 1) worst = fmax(worst, __shfl_xor(worst, off));
+  if ((threadIdx.x & 63) == 0 && worst > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(worst));
+}
+
+// c [nao][r] = the first r columns of L (stored by columns, stride nao);
+// signs = +1.
+__global__ __launch_bounds__(kThreads) void compact_factor_kernel(const double* __restrict__ l, int nao, int r,
+                                                                  double* __restrict__ c,
+                                                                  double* __restrict__ signs) {
+  const long long total = (long long)nao * r;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int i = (int)(idx / r), k = (int)(idx % r);
+    c[idx] = l[(long long)k * nao + i];
+    if (i == 0) signs[k] = 1.0;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
+
+hipError_t launch_pivoted_cholesky(const double* density, int nao, double tol, double* l, double* diag, int* rank,
+                                   unsigned long long* residual, hipStream_t s) {
+  hipLaunchKernelGGL(pivoted_cholesky_kernel, dim3(1), dim3(kPcholThreads), 0, s, density, nao, tol, l, diag, rank);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long nn = (long long)nao * nao;
+  hipLaunchKernelGGL(factor_residual_kernel, dim3(grid_for(nn)), dim3(kThreads), 0, s, density, nao, l, rank,
+                     residual);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact_factor(const double* l, int nao, int r, double* c, double* signs, hipStream_t s) {
+  const long long n = (long long)nao * r;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(compact_factor_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, l, nao, r, c, signs);
+  return hipGetLastError();
+}
 
 hipError_t launch_weights(const double* density, int nao, double* w, hipStream_t s) {
   hipLaunchKernelGGL(weights_kernel, dim3((unsigned)n_tile_pairs(nao)), dim3(kThreads), 0, s, density, nao, w);

@@ -157,9 +157,10 @@ HFJK_API int hfjk_jk_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, double
 
 /* K = sum_P B[P] D B[P] for any real density [nao][nao]. Drop-in for
  * DensityFittedCoulombExchange.exchange (coulomb_exchange.py:184-190).
- * Symmetric to 1e-12 relative (possibly indefinite): D = V diag(l) V^T by
- * rocSOLVER dsyevd, eigenpairs with |l| <= 1e-12 max|l| dropped,
- * C = V |l|^1/2, S = sign(l), then hfjk_exchange_occ in `precision`.
+ * Symmetric to 1e-12 relative: D = C S C^T by pivoted Cholesky when D is
+ * positive semidefinite (S = 1), otherwise D = V diag(l) V^T by rocSOLVER
+ * dsyevd, eigenpairs with |l| <= 1e-12 max|l| dropped, C = V |l|^1/2,
+ * S = sign(l); then hfjk_exchange_occ in `precision`.
  * Otherwise the density form itself in FP64 (precision must be HFJK_FP64,
  * else HFJK_ERR_BACKEND): per chunk of aux functions the full rows F_P of
  * B (from the bound full FP64 copy, or expanded into the workspace),
@@ -175,10 +176,14 @@ HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double 
  * nao, the bound for any density). hfjk_density_factor: for a symmetric
  * density (1e-12 relative) the signed factor D = C diag(signs) C^T into
  * factor [nao][rank] (row-major; a buffer of nao * nao doubles) and
- * signs [rank], *rank = the retained rank (0 for D = 0); *rank = -1 for a
- * non-symmetric density, nothing written. Synchronises `stream`. K then is
- * hfjk_exchange_occ(factor, rank, signs) with the workspace of
- * hfjk_workspace_bytes(ctx, rank, precision). hfjk_exchange_general: K of
+ * signs [rank], *rank = the retained rank (0 for D = 0), *negative = the
+ * number of -1 signs (the first columns); *rank = -1 for a non-symmetric
+ * density, nothing written. A positive semidefinite density is factorised
+ * by pivoted Cholesky (accepted when max |D - C C^T| <= 1e-11 max |D|, all
+ * signs +1), any other symmetric one by dsyevd (eigenvalues above 1e-12 of
+ * the largest kept). Synchronises `stream`. K then is
+ * hfjk_exchange_occ(factor, rank, negative ? signs : NULL) with the
+ * workspace of hfjk_workspace_bytes(ctx, rank, precision). hfjk_exchange_general: K of
  * any real density in FP64 by the density form (the non-symmetric branch
  * of hfjk_exchange_density). Both take a workspace of at least
  * hfjk_workspace_bytes(ctx, 1, HFJK_FP64) bytes; hfjk_exchange_general
@@ -186,7 +191,7 @@ HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double 
  * reference call, DensityFittedCoulombExchange.exchange
  * (coulomb_exchange.py:184-190). */
 HFJK_API int hfjk_density_factor(hfjk_ctx *ctx, const double *density, double *factor, double *signs, int *rank,
-                                 void *workspace, size_t workspace_bytes, void *stream);
+                                 int *negative, void *workspace, size_t workspace_bytes, void *stream);
 HFJK_API int hfjk_exchange_general(hfjk_ctx *ctx, const double *density, double *exchange, void *workspace,
                                    size_t workspace_bytes, void *stream);
 
