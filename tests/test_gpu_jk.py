@@ -163,7 +163,11 @@ def test_c_abi_density_exchange_matches_the_two_step_path(torch, precision):
     c = rng.standard_normal((70, 9))
     prec = _lib.precision_code(precision)
     jk._ensure_copy(prec)  # the FP32 copy the direct call needs
-    for d in (a + a.T, c @ c.T, np.zeros((70, 70))):
+    w = np.r_[np.ones(4), np.full(3, 2 / 3)]  # fractional (averaged) occupations
+    q, _ = np.linalg.qr(rng.standard_normal((70, 7)))
+    # Indefinite (dsyevd), low-rank and full-rank positive semidefinite and fractionally occupied
+    # (pivoted Cholesky), zero.
+    for d in (a + a.T, c @ c.T, a @ a.T + np.eye(70), (q * w) @ q.T, np.zeros((70, 70))):
         dt = torch.from_numpy(d).cuda()
         out = torch.empty((70, 70), dtype=torch.float64, device="cuda")
         ws, nbytes = jk._workspace_for(70, prec)
