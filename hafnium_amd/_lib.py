@@ -27,6 +27,7 @@ TUNE_X_BM = 4
 TUNE_SYRK_KIND = 5
 TUNE_X_FULL = 6
 TUNE_GENERAL_CHUNK = 7
+TUNE_SIDE_GROUPS = 8
 
 OK = 0
 ERR_INPUT = 1

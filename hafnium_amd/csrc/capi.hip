@@ -233,7 +233,7 @@ void run_coulomb(hfjk_ctx* ctx, const double* density, double* out, int precisio
   }
   {
     Stage st(ctx, s, "coulomb_accumulate");
-    check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, g, jp, s),
+    check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, g, jp, 0, s),
               "coulomb accumulate launch");
   }
   {
@@ -307,6 +307,18 @@ void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double*
                   void* ws, const Layout& L, hipStream_t s) {
   exchange_front(ctx, orbitals, nocc, signs, precision, ws, L, s);
   exchange_back(ctx, signs != nullptr, out, precision, ws, L, s);
+}
+
+// Grid cap of the Coulomb kernels that hfjk_jk_occ runs beside the SYRK. In
+// FP64 one workgroup per CU by default, so the SYRK keeps its two
+// workgroups per CU (large configuration: 134.8 ms per build against 138.7
+// ms uncapped; half a workgroup per CU makes the Coulomb half outlast the
+// SYRK). The FP32 and BF16x3 SYRKs are too short to hide a capped Coulomb
+// half (it reads B in FP64 or its FP32 copy either way), so they run it
+// uncapped. 0 = no cap.
+int side_groups(const hfjk_ctx* ctx, int precision) {
+  if (ctx->side_groups > 0) return ctx->side_groups;
+  return precision == HFJK_FP64 ? ctx->cus : 0;
 }
 
 rocblas_handle blas_handle(hfjk_ctx* ctx) {
@@ -460,6 +472,7 @@ int hfjk_create(hfjk_ctx** out, int device) {
       fail(HFJK_ERR_BACKEND, std::string("the DF-J/K kernels are built for gfx950 (MI355X), not ") + prop.gcnArchName);
     hfjk_ctx* ctx = new hfjk_ctx();
     ctx->device = device;
+    ctx->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     *out = ctx;
   });
 }
@@ -608,7 +621,7 @@ int hfjk_coulomb_charges(hfjk_ctx* ctx, const double* charges, double* coulomb, 
     double* jp = at<double>(workspace, L.cjp);
     {
       Stage st(ctx, s, "coulomb_accumulate");
-      check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, charges, jp,
+      check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, charges, jp, 0,
                                                 s),
                 "coulomb accumulate launch");
     }
@@ -707,14 +720,14 @@ int hfjk_jk_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, double* coulomb
         cpad = at<void>(workspace, L.qc);
       }
       check_hip(hfjk::launch_charges_from_x(at<void>(workspace, L.xx), precision, ctx->naux, ctx->nao, L.ldo, cpad,
-                                            parts, gamma, side),
+                                            parts, gamma, side_groups(ctx, precision), side),
                 "charges launch");
     }
     const void* b = bound_tensor(ctx, precision);
     {
       Stage st(ctx, side, "coulomb_accumulate");
       check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, gamma, jp,
-                                                side),
+                                                side_groups(ctx, precision), side),
                 "coulomb accumulate launch");
     }
     {
@@ -913,6 +926,9 @@ int hfjk_set_tuning(hfjk_ctx* ctx, int param, int value) {
       case HFJK_TUNE_GENERAL_CHUNK:
         ctx->general_chunk = value;
         break;
+      case HFJK_TUNE_SIDE_GROUPS:
+        ctx->side_groups = value;
+        break;
       default:
         fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
@@ -931,6 +947,7 @@ int hfjk_get_tuning(hfjk_ctx* ctx, int param, int* value) {
       case HFJK_TUNE_SYRK_KIND: *value = ctx->syrk_kind; break;
       case HFJK_TUNE_X_FULL: *value = ctx->x_full; break;
       case HFJK_TUNE_GENERAL_CHUNK: *value = ctx->general_chunk; break;
+      case HFJK_TUNE_SIDE_GROUPS: *value = ctx->side_groups; break;
       default: fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
   });

@@ -86,6 +86,8 @@ struct hfjk_ctx {
   const float* b32_full = nullptr;
   int x_full = 0;
   int general_chunk = 0;  // aux functions per chunk of the non-symmetric exchange (0 = fit)
+  int side_groups = 0;    // grid cap of the Coulomb kernels beside the SYRK in hfjk_jk_occ (0 = default)
+  int cus = 256;          // compute units of the device
   // Second stream of hfjk_jk_occ (default priority, created on first use):
   // the Coulomb half runs there beside the SYRK, fenced by fork/join events.
   hipStream_t side = nullptr;
@@ -140,8 +142,11 @@ hipError_t launch_weights(const double* density, int nao, double* w, hipStream_t
 hipError_t launch_gamma(const void* b, int precision, long long p, int naux, int ntiles_total,
                         const double* w, double* partial, hipStream_t s);
 hipError_t launch_gamma_reduce(const double* partial, int nchunks, int naux, double* gamma, hipStream_t s);
+// max_groups > 0 caps the grid of the two Coulomb kernels below (each
+// workgroup then loops over several items): beside the SYRK in
+// hfjk_jk_occ, a capped grid keeps CU slots free for the SYRK.
 hipError_t launch_coulomb_accumulate(const void* b, int precision, long long p, int naux, int nchunks,
-                                     int nsplit, const double* gamma, double* partial, hipStream_t s);
+                                     int nsplit, const double* gamma, double* partial, int max_groups, hipStream_t s);
 hipError_t launch_coulomb_unpack(const double* partial, int nsplit, long long p, int nao,
                                  double* out, hipStream_t s);
 
@@ -181,7 +186,7 @@ hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux
 // FP32 and BF16x3 X); parts holds naux * charge_groups(nao) doubles.
 int charge_groups(int nao);
 hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao, int ldo, const void* cpad,
-                                 double* parts, double* gamma, hipStream_t s);
+                                 double* parts, double* gamma, int max_groups, hipStream_t s);
 // FP64 / FP32 half-transform from a full row-major copy of B (xk.hip):
 // launch_expand_full writes bfull[q][npad][npad] in the precision's type
 // (npad = 32 ceil(nao / 32)) from the tiled packed FP64 tensor;

@@ -154,28 +154,32 @@ __global__ __launch_bounds__(kThreads) void gamma_reduce_kernel(const double* __
   gamma[q] = s;
 }
 
-// partial[s][e] = sum over the s-th aux slice of gamma[q] B[q][e].
+// partial[s][e] = sum over the s-th aux slice of gamma[q] B[q][e], one
+// (tile, slice) item per workgroup iteration (the grid may be capped).
 template <int PREC>
 __global__ __launch_bounds__(kThreads) void coulomb_accumulate_kernel(const void* __restrict__ b, long long p,
-                                                                       int naux, int nsplit,
+                                                                       int naux, int ntiles, int nsplit,
                                                                        const double* __restrict__ gamma,
                                                                        double* __restrict__ partial) {
-  const int tile = blockIdx.x;
-  const int s = blockIdx.y;
-  const int qb = (int)((long long)s * naux / nsplit);
-  const int qe = (int)((long long)(s + 1) * naux / nsplit);
-  const long long e = (long long)tile * kTileElems + threadIdx.x * 4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int q = qb; q < qe; ++q) {
-    double bv[4];
-    load4<PREC>(b, p, naux, q, e, bv);
-    const double g = gamma[q];
+  const long long nitems = (long long)ntiles * nsplit;
+  for (long long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int tile = (int)(item % ntiles);
+    const int s = (int)(item / ntiles);
+    const int qb = (int)((long long)s * naux / nsplit);
+    const int qe = (int)((long long)(s + 1) * naux / nsplit);
+    const long long e = (long long)tile * kTileElems + threadIdx.x * 4;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = qb; q < qe; ++q) {
+      double bv[4];
+      load4<PREC>(b, p, naux, q, e, bv);
+      const double g = gamma[q];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] += g * bv[v];
+      for (int v = 0; v < 4; ++v) acc[v] += g * bv[v];
+    }
+    double2* dst = (double2*)(partial + (long long)s * p + e);
+    dst[0] = make_double2(acc[0], acc[1]);
+    dst[1] = make_double2(acc[2], acc[3]);
   }
-  double2* dst = (double2*)(partial + (long long)s * p + e);
-  dst[0] = make_double2(acc[0], acc[1]);
-  dst[1] = make_double2(acc[2], acc[3]);
 }
 
 __global__ __launch_bounds__(kThreads) void coulomb_unpack_kernel(const double* __restrict__ partial, int nsplit,
@@ -419,17 +423,19 @@ hipError_t launch_gamma_reduce(const double* partial, int ntiles, int naux, doub
 }
 
 hipError_t launch_coulomb_accumulate(const void* b, int precision, long long p, int naux, int ntiles, int nsplit,
-                                     const double* gamma, double* partial, hipStream_t s) {
-  const dim3 grid((unsigned)ntiles, (unsigned)nsplit);
+                                     const double* gamma, double* partial, int max_groups, hipStream_t s) {
+  long long groups = (long long)ntiles * nsplit;
+  if (max_groups > 0) groups = std::min<long long>(groups, max_groups);
+  const dim3 grid((unsigned)groups);
   if (precision == HFJK_FP64)
-    hipLaunchKernelGGL(coulomb_accumulate_kernel<HFJK_FP64>, grid, dim3(kThreads), 0, s, b, p, naux, nsplit, gamma,
-                       partial);
-  else if (precision == HFJK_FP32)
-    hipLaunchKernelGGL(coulomb_accumulate_kernel<HFJK_FP32>, grid, dim3(kThreads), 0, s, b, p, naux, nsplit, gamma,
-                       partial);
-  else
-    hipLaunchKernelGGL(coulomb_accumulate_kernel<HFJK_BF16X3>, grid, dim3(kThreads), 0, s, b, p, naux, nsplit,
+    hipLaunchKernelGGL(coulomb_accumulate_kernel<HFJK_FP64>, grid, dim3(kThreads), 0, s, b, p, naux, ntiles, nsplit,
                        gamma, partial);
+  else if (precision == HFJK_FP32)
+    hipLaunchKernelGGL(coulomb_accumulate_kernel<HFJK_FP32>, grid, dim3(kThreads), 0, s, b, p, naux, ntiles, nsplit,
+                       gamma, partial);
+  else
+    hipLaunchKernelGGL(coulomb_accumulate_kernel<HFJK_BF16X3>, grid, dim3(kThreads), 0, s, b, p, naux, ntiles,
+                       nsplit, gamma, partial);
   return hipGetLastError();
 }
 

@@ -539,57 +539,64 @@ __device__ __forceinline__ void load_orbitals(const TC* __restrict__ c, long lon
   }
 }
 
+// One (aux function, row group) item per workgroup iteration; the grid may
+// be smaller than the item count (a capped grid leaves room on every CU for
+// the SYRK it runs beside).
 template <int F>
 __global__ __launch_bounds__(kThreads) void charges_from_x_kernel(const void* __restrict__ x, long long plane, int nao,
                                                                    int xrows, int ldo, const void* __restrict__ cpad,
-                                                                   int ngroups, double* __restrict__ parts) {
+                                                                   int ngroups, long long nitems,
+                                                                   double* __restrict__ parts) {
   typedef typename std::conditional<F == 0, double, float>::type TC;
   const TC* __restrict__ c = (const TC*)cpad;
-  const int q = blockIdx.x / ngroups;
-  const int g = blockIdx.x - q * ngroups;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const long long xq = (long long)q * xrows * ldo;
-  const int m_end = min(nao, (g + 1) * kChargeRows);
-  constexpr int V = XVec<F>::V;
-  double part = 0.0;
-  // Rows [m_begin, m_end) of X_q are one contiguous range of
-  // (m_end - m_begin) * ldo elements: the workgroup streams it in 16-byte
-  // groups (ldo is a multiple of 32, so no group straddles two rows), group
-  // f of thread t being f = t + kThreads j. The padding is zero, so the loop
-  // body has no branch and the loads of successive groups overlap.
-  const int m_begin = g * kChargeRows;
-  const int per_row = ldo / V;
-  const int ngroups_blk = (m_end - m_begin) * per_row;
-  const int step_m = kThreads / per_row;
-  const int step_g = kThreads - step_m * per_row;
-  int m = m_begin + (int)threadIdx.x / per_row;
-  int gi = (int)threadIdx.x % per_row;  // group within the row
-#pragma unroll 4
-  for (int f = threadIdx.x; f < ngroups_blk; f += kThreads) {
-    const int i0 = gi * V;
-    double xv[V], cv[V];
-    XVec<F>::load(x, plane, xq + (long long)m * ldo + i0, xv);
-    load_orbitals<TC, V>(c, (long long)m * ldo + i0, cv);
-#pragma unroll
-    for (int e = 0; e < V; ++e) part += xv[e] * cv[e];
-    m += step_m;
-    gi += step_g;
-    if (gi >= per_row) {
-      gi -= per_row;
-      ++m;
-    }
-  }
   __shared__ double red[kThreads / 64];
+  for (long long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int q = (int)(item / ngroups);
+    const int g = (int)(item - (long long)q * ngroups);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const long long xq = (long long)q * xrows * ldo;
+    const int m_end = min(nao, (g + 1) * kChargeRows);
+    constexpr int V = XVec<F>::V;
+    double part = 0.0;
+    // Rows [m_begin, m_end) of X_q are one contiguous range of
+    // (m_end - m_begin) * ldo elements: the workgroup streams it in 16-byte
+    // groups (ldo is a multiple of 32, so no group straddles two rows), group
+    // f of thread t being f = t + kThreads j. The padding is zero, so the loop
+    // body has no branch and the loads of successive groups overlap.
+    const int m_begin = g * kChargeRows;
+    const int per_row = ldo / V;
+    const int ngroups_blk = (m_end - m_begin) * per_row;
+    const int step_m = kThreads / per_row;
+    const int step_g = kThreads - step_m * per_row;
+    int m = m_begin + (int)threadIdx.x / per_row;
+    int gi = (int)threadIdx.x % per_row;  // group within the row
+#pragma unroll 4
+    for (int f = threadIdx.x; f < ngroups_blk; f += kThreads) {
+      const int i0 = gi * V;
+      double xv[V], cv[V];
+      XVec<F>::load(x, plane, xq + (long long)m * ldo + i0, xv);
+      load_orbitals<TC, V>(c, (long long)m * ldo + i0, cv);
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
-  if (lane == 0) red[wave] = part;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double sum = 0.0;
+      for (int e = 0; e < V; ++e) part += xv[e] * cv[e];
+      m += step_m;
+      gi += step_g;
+      if (gi >= per_row) {
+        gi -= per_row;
+        ++m;
+      }
+    }
 #pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) sum += red[w];
-    parts[blockIdx.x] = sum;
+    for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
+    if (lane == 0) red[wave] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+#pragma unroll
+      for (int w = 0; w < kThreads / 64; ++w) sum += red[w];
+      parts[item] = sum;
+    }
+    __syncthreads();
   }
 }
 
@@ -1374,21 +1381,24 @@ hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out,
 int charge_groups(int nao) { return (nao + kChargeRows - 1) / kChargeRows; }
 
 hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao, int ldo, const void* cpad,
-                                 double* parts, double* gamma, hipStream_t s) {
+                                 double* parts, double* gamma, int max_groups, hipStream_t s) {
   const int xrows = x_rows(nao);
   const int ng = charge_groups(nao);
   const long long plane = (long long)naux * xrows * ldo;
-  // Grids stay below 2^31 workgroups for any bound tensor (naux * ng).
-  const dim3 grid((unsigned)((long long)naux * ng));
+  const long long nitems = (long long)naux * ng;
+  // At most 2^30 workgroups (each loops over its items), or max_groups when set.
+  long long groups = std::min<long long>(nitems, 1LL << 30);
+  if (max_groups > 0) groups = std::min<long long>(groups, max_groups);
+  const dim3 grid((unsigned)groups);
   if (precision == HFJK_FP64)
     hipLaunchKernelGGL(charges_from_x_kernel<0>, grid, dim3(kThreads), 0, s, x, plane, nao, xrows, ldo, cpad, ng,
-                       parts);
+                       nitems, parts);
   else if (precision == HFJK_FP32)
     hipLaunchKernelGGL(charges_from_x_kernel<1>, grid, dim3(kThreads), 0, s, x, plane, nao, xrows, ldo, cpad, ng,
-                       parts);
+                       nitems, parts);
   else
     hipLaunchKernelGGL(charges_from_x_kernel<2>, grid, dim3(kThreads), 0, s, x, plane, nao, xrows, ldo, cpad, ng,
-                       parts);
+                       nitems, parts);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL(charge_reduce_kernel, dim3((unsigned)((naux + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,

@@ -316,6 +316,10 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * of its precision when one is bound (hfjk_bind_full), 1 = it ignores it.
  * HFJK_TUNE_GENERAL_CHUNK: most aux functions per chunk of the non-symmetric
  * exchange of hfjk_exchange_density (0 = as many as the workspace holds).
+ * HFJK_TUNE_SIDE_GROUPS: most workgroups of each Coulomb kernel that
+ * hfjk_jk_occ runs beside the SYRK (0 = the library's choice: one per
+ * compute unit in FP64, no cap in FP32 and BF16X3, whose shorter SYRK would
+ * not hide the capped kernels).
  * Set by the project's autotuner (hafnium_amd/autotune.py); the workspace size
  * depends on HFJK_TUNE_SYRK_SPLITS, so query it after tuning. No counterpart
  * in the reference. */
@@ -326,6 +330,7 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
 #define HFJK_TUNE_SYRK_KIND 5
 #define HFJK_TUNE_X_FULL 6
 #define HFJK_TUNE_GENERAL_CHUNK 7
+#define HFJK_TUNE_SIDE_GROUPS 8
 HFJK_API int hfjk_set_tuning(hfjk_ctx *ctx, int param, int value);
 HFJK_API int hfjk_get_tuning(hfjk_ctx *ctx, int param, int *value);
 

@@ -116,3 +116,23 @@ def test_wide_occupied_block_uses_the_register_kernel():
     jk = MI355XCoulombExchange(b, precision="fp64")
     k = jk.exchange_occupied(c)
     assert jk_ref.max_relative_error(k, expect) < FP64_REL_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_capped_coulomb_grids_give_the_same_build(precision):
+    """The grid cap of the Coulomb kernels beside the SYRK changes the schedule only: J and K of
+    the combined build are bitwise equal for every cap, and J matches the oracle."""
+    from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
+
+    naux, nao, nocc = 37, 150, 40
+    b, c, d = synthetic.jk_inputs(23, naux, nao, nocc)
+    jk = MI355XCoulombExchange(b, precision=precision)
+    j0, k0 = jk.jk_occupied(c)
+    assert jk_ref.max_relative_error(j0, jk_ref.coulomb(b, d)) < TOL[precision]
+    for cap in (1, 7, 64, 1 << 20):
+        jk.set_tuning(_lib.TUNE_SIDE_GROUPS, cap)
+        j, k = jk.jk_occupied(c)
+        np.testing.assert_array_equal(j, j0)
+        np.testing.assert_array_equal(k, k0)
+    jk.set_tuning(_lib.TUNE_SIDE_GROUPS, 0)
+    assert jk.get_tuning(_lib.TUNE_SIDE_GROUPS) == 0
