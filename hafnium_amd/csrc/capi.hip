@@ -778,15 +778,26 @@ int density_factor(hfjk_ctx* ctx, const double* density, void* workspace, const 
   // A positive semidefinite density (every aufbau or averaged-occupation
   // density) factorises by pivoted Cholesky, D = L L^T, in milliseconds; its
   // residual max |D - L L^T| <= 1e-11 max |D| decides, else dsyevd below.
-  int* rank_dev = (int*)info;
+  hfjk::PcholState* pst = (hfjk::PcholState*)info;  // the 256-byte dens_info region
+  static_assert(sizeof(hfjk::PcholState) <= 256, "the Cholesky state fits the density info region");
+  hfjk::PcholState host_st{};
   {
     Stage st(ctx, s, "density_cholesky");
+    check_hip(hfjk::launch_pchol_init(density, n, 1e-12, e, pst, s), "pivoted Cholesky launch");
+    // Columns in batches of 32 steps; the state is read back after each batch.
+    constexpr int kBatch = 32;
+    for (int k0 = 0; k0 <= n; k0 += kBatch) {
+      for (int k = k0; k < std::min(k0 + kBatch, n + 1); ++k)
+        check_hip(hfjk::launch_pchol_step(density, n, k, a, e, pst, s), "pivoted Cholesky launch");
+      check_hip(hipMemcpyAsync(&host_st, pst, sizeof(host_st), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+      check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+      if (host_st.stop) break;
+    }
     check_hip(hipMemsetAsync(stats, 0, sizeof(unsigned long long), s), "hipMemsetAsync");
-    check_hip(hfjk::launch_pivoted_cholesky(density, n, 1e-12, a, e, rank_dev, stats, s), "pivoted Cholesky launch");
+    check_hip(hfjk::launch_factor_residual(density, n, a, pst, stats, s), "factor residual launch");
   }
-  int chol_rank = 0;
+  const int chol_rank = host_st.stop ? host_st.rank : -1;
   unsigned long long chol_bits = 0;
-  check_hip(hipMemcpyAsync(&chol_rank, rank_dev, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
   check_hip(hipMemcpyAsync(&chol_bits, stats, sizeof(chol_bits), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
   check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
   double residual;

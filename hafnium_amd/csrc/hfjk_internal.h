@@ -153,11 +153,21 @@ hipError_t launch_coulomb_unpack(const double* partial, int nsplit, long long p,
 // Exchange (jk.hip).
 hipError_t launch_orbitals(const double* c, int nao, int nocc, int precision, void* out, hipStream_t s);
 // Pivoted Cholesky of a symmetric PSD density into l (nao * nao doubles:
-// L by columns, the first *rank used; diag: nao doubles of scratch), then
-// max |D - L L^T| into *residual (bits of a double, atomic max: clear it
-// first).
-hipError_t launch_pivoted_cholesky(const double* density, int nao, double tol, double* l, double* diag, int* rank,
-                                   unsigned long long* residual, hipStream_t s);
+// L by columns; diag: nao doubles of scratch), one column per step with its
+// state on the device: init, then steps k = 0..nao (each a no-op once the
+// state says stop; step nao only stops), then max |D - L L^T| over the
+// first st->rank columns into *residual (bits of a double, atomic max:
+// clear it first).
+struct PcholState {
+  int p, stop, rank, pad;
+  double piv, cut;
+};
+hipError_t launch_pchol_init(const double* density, int nao, double tol, double* diag, PcholState* st,
+                             hipStream_t s);
+hipError_t launch_pchol_step(const double* density, int nao, int k, double* l, double* diag, PcholState* st,
+                             hipStream_t s);
+hipError_t launch_factor_residual(const double* density, int nao, const double* l, const PcholState* st,
+                                  unsigned long long* residual, hipStream_t s);
 hipError_t launch_compact_factor(const double* l, int nao, int r, double* c, double* signs, hipStream_t s);
 // bn = column tile (32, 64, 128, 256) or 0 for the default for ldo; bm = row
 // tile (64 or 128; 128 applies to column tiles up to 128).

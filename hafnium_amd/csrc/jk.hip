@@ -246,100 +246,105 @@ __global__ __launch_bounds__(kThreads) void split_factor_kernel(const double* __
   }
 }
 
-// Pivoted Cholesky of a symmetric positive semidefinite D [n][n]: D = L L^T
-// with L stored by columns (lt[k][i] = L[i][k], column stride n), each column
-// pivoted on the largest remaining diagonal; stops once that diagonal is
-// <= tol * (largest initial diagonal), after at most n columns. One
-// workgroup of kPcholThreads; the column update reads lt[j][i] across the
-// threads (coalesced) and lt[j][p] once per wave.
-// L[i][k] = (D[p][i] - sum_{j<k} L[i][j] L[p][j]) / sqrt(r_p), r -= L[:,k]^2.
+// Pivoted Cholesky of a symmetric positive semidefinite D [n][n], one column
+// per step: D = L L^T with L stored by columns (l[k][i] = L[i][k], column
+// stride n), column k pivoted on the largest remaining diagonal r_p,
+// L[i][k] = (D[p][i] - sum_{j<k} L[i][j] L[p][j]) / sqrt(r_p), r -= L[:,k]^2,
+// stopping once that diagonal is <= tol * (largest initial diagonal). Each
+// step is two launches (the pivot in one workgroup, the column across the
+// GPU), and the state lives on the device so the host checks it only
+// every few steps.
 constexpr int kPcholThreads = 1024;
 
-__global__ __launch_bounds__(kPcholThreads) void pivoted_cholesky_kernel(const double* __restrict__ d, int n,
-                                                                        double tol, double* __restrict__ l,
-                                                                        double* __restrict__ diag,
-                                                                        int* __restrict__ rank_out) {
+// diag = diag(D); st: cut = tol * max diag, no stop, rank 0. One workgroup.
+__global__ __launch_bounds__(kPcholThreads) void pchol_init_kernel(const double* __restrict__ d, int n, double tol,
+                                                                  double* __restrict__ diag,
+                                                                  PcholState* __restrict__ st) {
+  __shared__ double wmax[kPcholThreads / 64];
+  double top = 0.0;
+  for (int i = threadIdx.x; i < n; i += kPcholThreads) {
+    const double v = d[(long long)i * n + i];
+    diag[i] = v;
+    top = fmax(top, v);
+  }
+  for (int off = 32; off > 0; off >>= 1) top = fmax(top, __shfl_xor(top, off));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = top;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kPcholThreads / 64; ++w) top = fmax(top, wmax[w]);
+    st->cut = tol * top;
+    st->stop = 0;
+    st->rank = 0;
+    st->p = -1;
+    st->piv = 0.0;
+  }
+}
+
+// Pivot of column k: the largest remaining diagonal (ties: lowest index);
+// stop with rank k when it is not above the cut, or at k = n. One workgroup.
+__global__ __launch_bounds__(kPcholThreads) void pchol_pivot_kernel(const double* __restrict__ diag, int n, int k,
+                                                                   PcholState* __restrict__ st) {
   __shared__ double wval[kPcholThreads / 64];
   __shared__ int widx[kPcholThreads / 64];
-  __shared__ double s_piv, s_top;
-  __shared__ int s_p;
+  if (st->stop) return;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  // Largest remaining diagonal and its index over the workgroup (ties: lowest index).
-  auto argmax = [&](double& best, int& at) {
-    best = -1.0;
-    at = -1;
-    for (int i = t; i < n; i += kPcholThreads) {
-      const double v = diag[i];
-      if (v > best) {
-        best = v;
-        at = i;
-      }
+  double best = -1.0;
+  int at = -1;
+  for (int i = t; i < n; i += kPcholThreads) {
+    const double v = diag[i];
+    if (v > best) {
+      best = v;
+      at = i;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(best, off);
-      const int oi = __shfl_xor(at, off);
-      if (ov > best || (ov == best && oi >= 0 && (at < 0 || oi < at))) {
-        best = ov;
-        at = oi;
-      }
-    }
-    if (lane == 0) {
-      wval[wave] = best;
-      widx[wave] = at;
-    }
-    __syncthreads();
-    if (t == 0) {
-      double b = wval[0];
-      int a = widx[0];
-      for (int w = 1; w < kPcholThreads / 64; ++w)
-        if (wval[w] > b || (wval[w] == b && widx[w] >= 0 && (a < 0 || widx[w] < a))) {
-          b = wval[w];
-          a = widx[w];
-        }
-      s_piv = b;
-      s_p = a;
-    }
-    __syncthreads();
-    best = s_piv;
-    at = s_p;
-  };
-  for (int i = t; i < n; i += kPcholThreads) diag[i] = d[(long long)i * n + i];
-  __syncthreads();
-  double top;
-  int at;
-  argmax(top, at);
-  if (t == 0) s_top = top;
-  __syncthreads();
-  const double cut = tol * (s_top > 0.0 ? s_top : 0.0);
-  int k = 0;
-  for (; k < n; ++k) {
-    double piv;
-    int p;
-    if (k > 0) {
-      argmax(piv, p);
-    } else {
-      piv = s_top;
-      p = at;
-    }
-    if (!(piv > cut) || p < 0) break;  // uniform: every thread holds the reduced values
-    const double inv = 1.0 / sqrt(piv);
-    for (int i = t; i < n; i += kPcholThreads) {
-      double v0 = d[(long long)p * n + i], v1 = 0.0, v2 = 0.0, v3 = 0.0;
-      int j = 0;
-      for (; j + 4 <= k; j += 4) {
-        v0 -= l[(long long)j * n + i] * l[(long long)j * n + p];
-        v1 -= l[(long long)(j + 1) * n + i] * l[(long long)(j + 1) * n + p];
-        v2 -= l[(long long)(j + 2) * n + i] * l[(long long)(j + 2) * n + p];
-        v3 -= l[(long long)(j + 3) * n + i] * l[(long long)(j + 3) * n + p];
-      }
-      for (; j < k; ++j) v0 -= l[(long long)j * n + i] * l[(long long)j * n + p];
-      const double v = ((v0 + v1) + (v2 + v3)) * inv;
-      l[(long long)k * n + i] = v;
-      diag[i] -= v * v;
-    }
-    __syncthreads();
   }
-  if (t == 0) *rank_out = k;
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(best, off);
+    const int oi = __shfl_xor(at, off);
+    if (ov > best || (ov == best && oi >= 0 && (at < 0 || oi < at))) {
+      best = ov;
+      at = oi;
+    }
+  }
+  if (lane == 0) {
+    wval[wave] = best;
+    widx[wave] = at;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < kPcholThreads / 64; ++w)
+      if (wval[w] > best || (wval[w] == best && widx[w] >= 0 && (at < 0 || widx[w] < at))) {
+        best = wval[w];
+        at = widx[w];
+      }
+    if (k >= n || at < 0 || !(best > st->cut)) {
+      st->stop = 1;
+      st->rank = k;
+    } else {
+      st->p = at;
+      st->piv = best;
+    }
+  }
+}
+
+// Column k of L: four lanes per row split the sum over the earlier columns.
+__global__ __launch_bounds__(kThreads) void pchol_column_kernel(const double* __restrict__ d, int n, int k,
+                                                                double* __restrict__ l, double* __restrict__ diag,
+                                                                const PcholState* __restrict__ st) {
+  if (st->stop) return;
+  const int p = st->p;
+  const double inv = 1.0 / sqrt(st->piv);
+  const long long g = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const int row = (int)(g >> 2), part = (int)(g & 3);
+  double s = 0.0;
+  if (row < n)
+    for (int j = part; j < k; j += 4) s += l[(long long)j * n + row] * l[(long long)j * n + p];
+  s += __shfl_xor(s, 1);
+  s += __shfl_xor(s, 2);
+  if (row < n && part == 0) {
+    const double v = (d[(long long)p * n + row] - s) * inv;
+    l[(long long)k * n + row] = v;
+    diag[row] -= v * v;
+  }
 }
 
 // max |D[i][j] - sum_{k < rank} L[i][k] L[j][k]| into *out (bits of a
@@ -381,13 +386,27 @@ __global__ __launch_bounds__(kThreads) void compact_factor_kernel(const double* 
 
 // ---------------------------------------------------------------- launchers
 
-hipError_t launch_pivoted_cholesky(const double* density, int nao, double tol, double* l, double* diag, int* rank,
-                                   unsigned long long* residual, hipStream_t s) {
-  hipLaunchKernelGGL(pivoted_cholesky_kernel, dim3(1), dim3(kPcholThreads), 0, s, density, nao, tol, l, diag, rank);
+hipError_t launch_pchol_init(const double* density, int nao, double tol, double* diag, PcholState* st,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(pchol_init_kernel, dim3(1), dim3(kPcholThreads), 0, s, density, nao, tol, diag, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_pchol_step(const double* density, int nao, int k, double* l, double* diag, PcholState* st,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(pchol_pivot_kernel, dim3(1), dim3(kPcholThreads), 0, s, diag, nao, k, st);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess || k >= nao) return e;
+  const long long threads = 4LL * nao;
+  hipLaunchKernelGGL(pchol_column_kernel, dim3((unsigned)((threads + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     s, density, nao, k, l, diag, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_factor_residual(const double* density, int nao, const double* l, const PcholState* st,
+                                  unsigned long long* residual, hipStream_t s) {
   const long long nn = (long long)nao * nao;
-  hipLaunchKernelGGL(factor_residual_kernel, dim3(grid_for(nn)), dim3(kThreads), 0, s, density, nao, l, rank,
+  hipLaunchKernelGGL(factor_residual_kernel, dim3(grid_for(nn)), dim3(kThreads), 0, s, density, nao, l, &st->rank,
                      residual);
   return hipGetLastError();
 }
