@@ -26,6 +26,9 @@ fails; see tests/test_oracle.py):
   solve, eigh and Cholesky variants agreeing, and the stored DF-RHF energy.
 * SCF, DIIS, orthogonaliser, aufbau: the stored exact RHF energy and orbital
   energies of water/STO-3G from the stored four-centre integrals, the DIIS
-  linear-model solution, idempotency and electron count.
+  linear-model solution, idempotency and electron count. The averaged
+  occupation (spherical atoms): the oxygen atom's (1, 1, 2/3, 2/3, 2/3)
+  occupations, electron count and isotropic p blocks, which whole-level
+  filling breaks.
 * Tiled packed layout: round trip, and dot products against the full form.
 """

@@ -185,3 +185,34 @@ def test_tiled_layout_round_trip_and_weights():
         d = rng.standard_normal((n, n))  # need not be symmetric
         w = layout_ref.weights(d)
         np.testing.assert_allclose(packed @ w, jk_ref.charges(b, d), atol=1e-11)
+
+
+def test_averaged_occupation_gives_a_spherical_oxygen_atom():
+    """The oracle's spherical atom (the reference's _spherical_atom, atoms.py:55-73): exact
+    integrals of O in cc-pVDZ, four alpha electrons averaged over the degenerate 2p level. Its
+    occupations are (1, 1, 2/3, 2/3, 2/3), tr(D S) = 4, and every block of D between two p shells
+    is a multiple of the identity (a spherical density); whole-level filling breaks that."""
+    from hafnium_amd import integrals, molecule
+
+    atom = molecule.Molecule((8,), np.zeros((1, 3)))
+    orb = molecule.BasisOnAtoms("cc-pvdz", atom)
+    s, t, v = integrals.one_electron(orb, atom)
+    j_of, k_of = scf_ref.exact_operators(integrals.four_center(orb))
+    res = scf_ref.rhf(s, t + v, j_of, k_of, 4.0, 0.0, occupation="averaged")
+    assert res.converged
+    np.testing.assert_allclose(np.sort(res.occupations)[::-1][:6], [1, 1, 2 / 3, 2 / 3, 2 / 3, 0], atol=1e-12)
+    assert float(np.trace(res.density @ s)) == pytest.approx(4.0, abs=1e-10)
+    p_shells, at = [], 0
+    for l, count in zip(orb._l, orb._ncontr):
+        for _ in range(count):
+            if l == 1:
+                p_shells.append(list(range(at, at + 3)))
+            at += 2 * l + 1
+
+    def spherical(d):
+        return all(np.allclose(d[np.ix_(a, b)], d[a[0], b[0]] * np.eye(3), atol=1e-8)
+                   for a in p_shells for b in p_shells)
+
+    assert spherical(res.density)
+    whole = scf_ref.rhf(s, t + v, j_of, k_of, 4, 0.0)
+    assert not spherical(whole.density)
