@@ -49,7 +49,7 @@ def test_mixed_phase_stall_detection():
     assert not scf.mixed_phase_stalled([1e-1, 3e-2, 1e-2, 3e-3, 1e-3], 2, 0.7)
     # A floor near 1e-4 (the reduced-precision J/K noise) does.
     assert scf.mixed_phase_stalled([1e-1, 1e-2, 1e-3, 1.2e-4, 1.1e-4, 1.3e-4], 2, 0.7)
-    # Too few iterations to judge.
+    # Too few iterations to tell.
     assert not scf.mixed_phase_stalled([1e-1, 2e-1, 3e-1], 2, 0.7)
     # One bad iteration followed by progress is not a stall.
     assert not scf.mixed_phase_stalled([1e-1, 1e-2, 2e-2, 5e-3], 2, 0.7)
