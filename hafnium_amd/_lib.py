@@ -113,6 +113,11 @@ SIGNATURES = [
     ("hfjk_mlp_adam", _c_int, [_vp, _c_ll, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp]),
     ("hfjk_mlp_to_bf16", _c_int, [_vp, _c_ll, _vp, _vp, _vp]),
+    ("hfjk_kpoint_workspace_bytes", _c_int, [_vp, _c_int, _c_int, _c_int, ctypes.POINTER(_c_size)]),
+    ("hfjk_kpoint_coulomb", _c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_size,
+                                     _vp]),
+    ("hfjk_kpoint_exchange", _c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp,
+                                      _vp, _c_size, _vp]),
 ]
 
 _lock = threading.Lock()

@@ -241,15 +241,20 @@ class BatchedCoulombExchange:
         """A hipGraph (``torch.cuda.CUDAGraph``) of one batched build and its output buffers.
 
         The plan and tensor copy are prepared before capture; the captured work
-        is the counter reset and the persistent phase kernels only.
+        is the counter reset and the persistent phase kernels only. The graph
+        addresses the plan's workspace, the tensor copy and the input buffers
+        by pointer, so it holds references to all of them: a later ``plan`` with
+        another tile or ``release`` drops the batch's own references, and the
+        buffers live on until the graph is freed.
         """
         torch = _torch()
-        self.plan(precision, tile)
-        self._tensor(_lib.precision_code(precision))
+        plan = self.plan(precision, tile)
+        copy = self._tensor(_lib.precision_code(precision))
         out = (torch.empty(self.matrix_elems, dtype=torch.float64, device=self.device),
                torch.empty(self.matrix_elems, dtype=torch.float64, device=self.device))
         torch.cuda.synchronize(self.device)
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             self.run(orbitals, densities, precision=precision, tile=tile, out=out)
+        graph.hfjk_buffers = (plan[1], copy, self._b64, orbitals, densities, out)
         return graph, out

@@ -39,7 +39,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from hafnium_amd import _lib
-from hafnium_amd.errors import ConvergenceError, InputError
+from hafnium_amd.errors import BackendError, ConvergenceError, InputError
 from hafnium_amd.molecule import BasisOnAtoms, Molecule
 from hafnium_amd.scf import ScfOptions, mixed_phase_stalled, solve_diis
 
@@ -171,7 +171,12 @@ class BatchedRhf:
         # Orthogonaliser from the batched eigendecomposition of the padded overlaps.
         vecs = s_pad.clone()
         vals = zeros(cnt, nm)
+        # Failed batched diagonalisations (any molecule), accumulated on the device and read
+        # with the per-iteration transfer.
+        self._eig_failures = zeros(1)
         self._eig(vecs, vals)
+        if int(self._eig_failures.item()):
+            raise BackendError("the batched eigensolver failed on an overlap matrix")
         self.x = zeros(cnt, nm, nm)
         self.kept = torch.zeros((cnt, nm), dtype=torch.int32, device=self.device)
         _lib.check(self.lib.hfjk_orthogonalizer_batched(self._ctx.handle, nm, cnt, _p(vecs), _p(vals),
@@ -199,7 +204,7 @@ class BatchedRhf:
         info = torch.zeros(cnt, dtype=torch.int32, device=self.device)
         _lib.check(self.lib.hfjk_dsyev_batched(self._ctx.handle, nm, cnt, _p(a), _p(w), _p(work), _p(info),
                                                self.method, self._stream()))
-        self._eig_info = info
+        self._eig_failures += (info != 0).any().to(torch.float64)
 
     def _gemm(self, ta, tb, m, n, k, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha=1.0, beta=0.0):
         _lib.check(self.lib.hfjk_dgemm_batched(self._ctx.handle, int(ta), int(tb), m, n, k, alpha, _p(a), lda, sa,
@@ -351,9 +356,9 @@ class BatchedRhf:
         cnt, nm, nn, m = self.count, self.nm, self._nn, options.diis_size
         fh = torch.zeros((m, cnt, nm, nm), dtype=torch.float64, device=self.device)
         gh = torch.zeros_like(fh)
-        scal = torch.empty(3 * cnt + cnt * m, dtype=torch.float64, device=self.device)
+        scal = torch.empty(3 * cnt + cnt * m + 1, dtype=torch.float64, device=self.device)
         terms_dev = scal[:3 * cnt].view(3, cnt)
-        row_dev = scal[3 * cnt:].view(cnt, m)
+        row_dev = scal[3 * cnt:3 * cnt + cnt * m].view(cnt, m)
         ov = np.zeros((cnt, m, m))  # 2 <G_a, G_b> per molecule, one row/column refreshed per iteration
         precision = options.precision
         density = start.density
@@ -376,11 +381,14 @@ class BatchedRhf:
             # The newest error's overlaps with every slot, for every molecule, in one launch.
             _lib.check(self.lib.hfjk_history_dots_batched(self._ctx.handle, cnt, nn, m, _p(gh), slot, _p(row_dev),
                                                           self._stream()))
+            scal[-1:] = self._eig_failures
             yield index
             host = scal.cpu().numpy()
             mark("overlaps_transfer")
+            if host[-1] != 0.0:
+                raise BackendError("the batched eigensolver failed on a Fock matrix")
             terms = start.terms if index == 0 else host[:3 * cnt].reshape(3, cnt)
-            row = 2.0 * host[3 * cnt:].reshape(cnt, m)
+            row = 2.0 * host[3 * cnt:3 * cnt + cnt * m].reshape(cnt, m)
             ov[:, slot, :] = row
             ov[:, :, slot] = row
             energy = self.energies(terms)
@@ -423,8 +431,6 @@ class BatchedRhf:
             density, _, orbitals = self.solve(extrapolated, mark)
             slot = (slot + 1) % m
         total = time.perf_counter() - t0
-        if int((self._eig_info != 0).sum().item()):
-            raise ConvergenceError("the batched eigensolver did not converge for every molecule", None)
         out = []
         for i in range(cnt):
             if results[i] is None:

@@ -15,9 +15,15 @@ that the package's own SCF driver calls with the occupied orbitals, and
 ``precision`` ("fp64", "fp32" or "bf16x3") selecting the arithmetic of the
 contractions.
 
-The RHF assembly of the reference's ``coulomb_matrix`` / ``exchange_matrix``
-(coulomb_exchange.py:407-462: J of the spin-traced density ``2 D_a``, K of
-the stored block ``D_a``) is :func:`rhf_coulomb_exchange`.
+The assembly of the reference's ``coulomb_matrix`` / ``exchange_matrix``
+(coulomb_exchange.py:397-518) for a molecular density in any of its spin
+structures is :func:`coulomb_matrix`, :func:`exchange_matrix` and
+:func:`coulomb_exchange_matrices` on a :class:`SpinDensity` (restricted:
+J of the spin-traced ``2 D_a``, K of ``D_a``; unrestricted: J of
+``D_a + D_b`` on both blocks, K per block; generalized: J of
+``D_aa + D_bb`` on both diagonal spin blocks, K of each of the four spin
+sub-blocks); :func:`rhf_coulomb_exchange` is the restricted case on a bare
+block.
 """
 
 from __future__ import annotations
@@ -30,7 +36,8 @@ import numpy as np
 from hafnium_amd import _lib
 from hafnium_amd.errors import BackendError, InputError
 
-__all__ = ["MI355XCoulombExchange", "rhf_coulomb_exchange", "SYMMETRY_TOLERANCE"]
+__all__ = ["MI355XCoulombExchange", "rhf_coulomb_exchange", "SpinDensity", "coulomb_matrix", "exchange_matrix",
+           "coulomb_exchange_matrices", "SYMMETRY_TOLERANCE"]
 
 SYMMETRY_TOLERANCE = 1e-12
 """Largest ``max|B - B^T| / max|B|`` accepted for a fitted tensor."""
@@ -456,3 +463,112 @@ def rhf_coulomb_exchange(two_body, density_alpha):
     (K of the stored block, one spin sub-block: :444-462, :504-509).
     """
     return two_body.coulomb(2.0 * density_alpha), two_body.exchange(density_alpha)
+
+
+RESTRICTED = "restricted"
+UNRESTRICTED = "unrestricted"
+GENERALIZED = "generalized"
+_BLOCKS = {RESTRICTED: 1, UNRESTRICTED: 2, GENERALIZED: 1}
+
+
+class SpinDensity:
+    """A Hermitian molecular density in one of the three collinear/spinor structures.
+
+    The molecular (no k axis, no irrep sectors) cases of the reference's
+    ``BlockMatrix`` densities (structured/matrix.py): ``kind`` "restricted"
+    holds ``(D_a,)`` (real, each spatial orbital doubly occupied, so the spin
+    trace is ``2 D_a``); "unrestricted" holds ``(D_a, D_b)``; "generalized"
+    holds one ``[2n, 2n]`` spinor block ordered (alpha, beta). Blocks are NumPy
+    arrays or torch tensors, real or complex.
+    """
+
+    def __init__(self, kind: str, blocks) -> None:
+        if kind not in _BLOCKS:
+            raise InputError("the spin structure is restricted, unrestricted or generalized")
+        blocks = tuple(blocks)
+        if len(blocks) != _BLOCKS[kind]:
+            raise InputError(f"a {kind} density holds {_BLOCKS[kind]} block(s), got {len(blocks)}")
+        shapes = {tuple(b.shape) for b in blocks}
+        if len(shapes) != 1 or len(next(iter(shapes))) != 2 or blocks[0].shape[0] != blocks[0].shape[1]:
+            raise InputError("density blocks are square and of one shape")
+        if kind == GENERALIZED and blocks[0].shape[0] % 2:
+            raise InputError("a spinor block has an even dimension (alpha and beta)")
+        self.kind = kind
+        self.blocks = blocks
+
+    @property
+    def n_spatial(self) -> int:
+        n = int(self.blocks[0].shape[0])
+        return n // 2 if self.kind == GENERALIZED else n
+
+    def spin_trace(self):
+        """The total spatial density (structured/matrix.py:535-567 for the molecular classes)."""
+        n = self.n_spatial
+        if self.kind == RESTRICTED:
+            return 2.0 * self.blocks[0]
+        if self.kind == UNRESTRICTED:
+            return self.blocks[0] + self.blocks[1]
+        d = self.blocks[0]
+        return d[:n, :n] + d[n:, n:]
+
+
+def _real(x):
+    if isinstance(x, np.ndarray):
+        return np.ascontiguousarray(np.real(x), dtype=np.float64)
+    torch = _torch()
+    return (x.real if x.is_complex() else x).to(torch.float64).contiguous()
+
+
+def _check_basis(two_body, density: SpinDensity) -> None:
+    if density.n_spatial != two_body.n_ao:
+        raise InputError(f"the density has {density.n_spatial} atomic orbitals, the integrals {two_body.n_ao}")
+
+
+def _spinor_diagonal(j):
+    """``[[J, 0], [0, J]]`` in the array type of ``j``."""
+    if isinstance(j, np.ndarray):
+        return np.kron(np.eye(2), j)
+    torch = _torch()
+    return torch.kron(torch.eye(2, dtype=j.dtype, device=j.device), j)
+
+
+def coulomb_matrix(two_body, density: SpinDensity):
+    """The Coulomb matrix in the structure of ``density`` (coulomb_exchange.py:407-427).
+
+    J of the real part of the spin trace (the imaginary part of a Hermitian
+    density is antisymmetric and drops out), placed on every stored block:
+    the one restricted block, both collinear blocks, or both diagonal spin
+    blocks of the spinor block.
+    """
+    _check_basis(two_body, density)
+    j = two_body.coulomb(_real(density.spin_trace()))
+    if density.kind == RESTRICTED:
+        return SpinDensity(RESTRICTED, (j,))
+    if density.kind == UNRESTRICTED:
+        return SpinDensity(UNRESTRICTED, (j, j))
+    return SpinDensity(GENERALIZED, (_spinor_diagonal(j),))
+
+
+def _exchange_block(two_body, block, n: int):
+    """K of one stored block, spin sub-block by spin sub-block (coulomb_exchange.py:504-518)."""
+    m = int(block.shape[-1]) // n
+    if m == 1:
+        return two_body.exchange(block)
+    rows = [[two_body.exchange(block[s * n:(s + 1) * n, t * n:(t + 1) * n]) for t in range(m)] for s in range(m)]
+    if isinstance(block, np.ndarray):
+        return np.block(rows)
+    torch = _torch()
+    return torch.cat([torch.cat(r, dim=1) for r in rows], dim=0)
+
+
+def exchange_matrix(two_body, density: SpinDensity):
+    """The exchange matrix in the structure of ``density`` (coulomb_exchange.py:444-462): K of
+    each stored block, each spin sub-block of a spinor block contracted on its own."""
+    _check_basis(two_body, density)
+    n = density.n_spatial
+    return SpinDensity(density.kind, tuple(_exchange_block(two_body, b, n) for b in density.blocks))
+
+
+def coulomb_exchange_matrices(two_body, density: SpinDensity):
+    """``(J, K)`` in the structure of ``density`` (coulomb_exchange.py:397-404)."""
+    return coulomb_matrix(two_body, density), exchange_matrix(two_body, density)

@@ -118,7 +118,7 @@ struct Layout {
   // only the bf16 planes of C^T), and the J partials
   size_t qt, qg, qc, qj, q_end;
   size_t total;
-  int nsplit_j, nsplit_k, ldo, xrows, npairs;
+  int nsplit_j, nsplit_k, nslot_k, ldo, xrows, npairs;
   long long p, ntiles;
 };
 
@@ -135,6 +135,7 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   L.npairs = hfjk::syrk_pairs(n);
   L.nsplit_j = coulomb_splits(L.ntiles, naux);
   L.nsplit_k = ctx->syrk_splits > 0 ? ((ctx->syrk_splits + 7) / 8) * 8 : syrk_splits(L.npairs, naux);
+  L.nslot_k = hfjk::syrk_slots(precision, naux, L.ldo, L.nsplit_k);
   size_t off = 0;
   const size_t nn = (size_t)n * n;
   L.dens_a = off; off = align_up(off + nn * 8);
@@ -160,7 +161,7 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   L.xs = off; off = align_up(off + (size_t)L.ldo * 8);
   L.xx = off; off = align_up(off + (size_t)naux * L.xrows * L.ldo * es);
   const size_t tile2 = (size_t)hfjk::syrk_tile() * hfjk::syrk_tile();
-  L.xp = off; off = align_up(off + (size_t)L.nsplit_k * L.npairs * tile2 * es);
+  L.xp = off; off = align_up(off + (size_t)L.nslot_k * L.npairs * tile2 * es);
   L.xct = off; off = align_up(off + npad * L.ldo * 8);
   L.x_end = off;
   off = std::max(L.c_end, L.x_end);
@@ -173,13 +174,15 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   return L;
 }
 
-void check_workspace(const Layout& L, void* ws, size_t bytes) {
-  if (ws == nullptr && L.total > 0) fail(HFJK_ERR_WORKSPACE, "null workspace");
-  if (bytes < L.total)
+void check_workspace_size(size_t need, void* ws, size_t bytes) {
+  if (ws == nullptr && need > 0) fail(HFJK_ERR_WORKSPACE, "null workspace");
+  if (bytes < need)
     fail(HFJK_ERR_WORKSPACE, "workspace of " + std::to_string(bytes) + " bytes is smaller than the " +
-                                 std::to_string(L.total) + " bytes hfjk_workspace_bytes reports");
+                                 std::to_string(need) + " bytes the workspace query reports");
   if (((uintptr_t)ws & 255) != 0) fail(HFJK_ERR_WORKSPACE, "workspace must be 256-byte aligned");
 }
+
+void check_workspace(const Layout& L, void* ws, size_t bytes) { check_workspace_size(L.total, ws, bytes); }
 
 template <typename T>
 T* at(void* ws, size_t off) {
@@ -299,7 +302,7 @@ void exchange_back(hfjk_ctx* ctx, bool is_signed, double* out, int precision, vo
   {
     Stage st(ctx, s, "exchange_reduce");
     const int rprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;  // BF16X3 partials are FP32
-    check_hip(hfjk::launch_syrk_reduce(part, rprec, L.nsplit_k, ctx->nao, out, s), "syrk reduce launch");
+    check_hip(hfjk::launch_syrk_reduce(part, rprec, L.nslot_k, ctx->nao, out, s), "syrk reduce launch");
   }
 }
 
@@ -990,6 +993,209 @@ int hfjk_profile_read(hfjk_ctx* ctx, const char** names, float* ms, int max) {
     ctx->n_stages = 0;
   });
   return status == HFJK_OK ? count : -status;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ k-point DF-J/K
+namespace {
+
+struct KpLayout {
+  // Coulomb: weighted inner products [nk][naux] and charges [naux] (complex).
+  size_t inner, charges;
+  // Exchange: the density copy and its eigenpairs, the realified factor and
+  // its column signs, the realified block, X, the SYRK partials, the [2n][2n]
+  // real result, a complex [n][n] product for the Madelung term.
+  size_t dz, w, e, info, stats, ct, sg, bh, x, part, kh, tz;
+  size_t total;
+  int npad2, ldo, xrows, npairs, nsplit;
+};
+
+KpLayout kp_layout(const hfjk_ctx* ctx, int nk, int naux, int nao) {
+  KpLayout L{};
+  const size_t n = (size_t)nao;
+  L.npad2 = hfjk::n_tiles(2 * nao) * hfjk::kTile;
+  L.ldo = hfjk::occ_ld(nao);
+  L.xrows = hfjk::x_rows(2 * nao);
+  L.npairs = hfjk::syrk_pairs(2 * nao);
+  L.nsplit = ctx->syrk_splits > 0 ? ((ctx->syrk_splits + 7) / 8) * 8 : syrk_splits(L.npairs, naux);
+  const size_t tile2 = (size_t)hfjk::syrk_tile() * hfjk::syrk_tile();
+  size_t off = 0;
+  L.inner = off; off = align_up(off + (size_t)nk * naux * 16);
+  L.charges = off; off = align_up(off + (size_t)naux * 16);
+  L.dz = off; off = align_up(off + n * n * 16);
+  L.w = off; off = align_up(off + n * 8);
+  L.e = off; off = align_up(off + n * 8);
+  L.info = off; off = align_up(off + 8);
+  L.stats = off; off = align_up(off + 16);
+  L.ct = off; off = align_up(off + (size_t)L.ldo * L.npad2 * 8);
+  L.sg = off; off = align_up(off + (size_t)L.ldo * 8);
+  L.bh = off; off = align_up(off + (size_t)naux * L.npad2 * L.npad2 * 8);
+  L.x = off; off = align_up(off + (size_t)naux * L.xrows * L.ldo * 8);
+  L.part = off; off = align_up(off + (size_t)L.nsplit * L.npairs * tile2 * 8);
+  L.kh = off; off = align_up(off + 4 * n * n * 8);
+  L.tz = off; off = align_up(off + n * n * 16);
+  L.total = off;
+  return L;
+}
+
+void kp_check(int nk, int naux, int nao) {
+  if (nk < 1 || naux < 1 || nao < 1) fail(HFJK_ERR_INPUT, "nk, naux and nao are positive");
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfjk_kpoint_workspace_bytes(hfjk_ctx* ctx, int nk, int naux, int nao, size_t* bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    kp_check(nk, naux, nao);
+    if (bytes == nullptr) fail(HFJK_ERR_INPUT, "null output pointer");
+    *bytes = kp_layout(ctx, nk, naux, nao).total;
+  });
+}
+
+int hfjk_kpoint_coulomb(hfjk_ctx* ctx, int nk, int naux, int nao, int zero_transfer, const void* tensor,
+                        const double* signature, const double* weights, const void* density, void* coulomb,
+                        void* workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    kp_check(nk, naux, nao);
+    if (tensor == nullptr || signature == nullptr || weights == nullptr || density == nullptr || coulomb == nullptr)
+      fail(HFJK_ERR_INPUT, "null pointer");
+    const KpLayout L = kp_layout(ctx, nk, naux, nao);
+    check_workspace_size(L.total, workspace, workspace_bytes);
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    const long long nn = (long long)nao * nao;
+    // Zero-transfer blocks: B[k][k] of [nk][nk][naux] blocks with signs [nk][nk][naux], or the
+    // [nk][naux] blocks of a Coulomb-only fitting with one sign per aux function.
+    const long long kstride = zero_transfer ? (long long)naux * nn : (long long)(nk + 1) * naux * nn;
+    const long long sig_kstride = zero_transfer ? 0 : (long long)(nk + 1) * naux;
+    {
+      Stage st(ctx, s, "kpoint_charges");
+      check_hip(hfjk::launch_kp_charges(tensor, kstride, nk, naux, nao, density, weights, at<void>(workspace, L.inner),
+                                        at<void>(workspace, L.charges), s),
+                "k-point charges launch");
+    }
+    {
+      Stage st(ctx, s, "kpoint_coulomb");
+      check_hip(hfjk::launch_kp_coulomb(tensor, kstride, nk, naux, nao, signature, sig_kstride,
+                                        at<void>(workspace, L.charges), coulomb, s),
+                "k-point coulomb launch");
+    }
+  });
+}
+
+int hfjk_kpoint_exchange(hfjk_ctx* ctx, int nk, int naux, int nao, const void* tensor, const double* signature,
+                         const double* weights, const void* overlap, double madelung, const void* density,
+                         void* exchange, void* workspace, size_t workspace_bytes, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    kp_check(nk, naux, nao);
+    if (tensor == nullptr || signature == nullptr || weights == nullptr || density == nullptr || exchange == nullptr)
+      fail(HFJK_ERR_INPUT, "null pointer");
+    if (madelung != 0.0 && overlap == nullptr) fail(HFJK_ERR_INPUT, "the Madelung term needs the overlaps");
+    const KpLayout L = kp_layout(ctx, nk, naux, nao);
+    check_workspace_size(L.total, workspace, workspace_bytes);
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    const int n = nao;
+    const long long nn = (long long)n * n;
+    std::vector<double> w(nk);
+    check_hip(hipMemcpyAsync(w.data(), weights, sizeof(double) * nk, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    for (int k = 0; k < nk; ++k)
+      if (!(w[k] >= 0.0)) fail(HFJK_ERR_INPUT, "k-point weights are non-negative");
+    check_hip(hipMemsetAsync(exchange, 0, sizeof(double) * 2 * nk * nn, s), "hipMemsetAsync");
+    auto* dz = at<rocblas_double_complex>(workspace, L.dz);
+    double* ev = at<double>(workspace, L.w);
+    double* e = at<double>(workspace, L.e);
+    rocblas_int* info = at<rocblas_int>(workspace, L.info);
+    unsigned long long* stats = at<unsigned long long>(workspace, L.stats);
+    double* ct = at<double>(workspace, L.ct);
+    double* sg = at<double>(workspace, L.sg);
+    double* bh = at<double>(workspace, L.bh);
+    double* x = at<double>(workspace, L.x);
+    void* part = at<void>(workspace, L.part);
+    double* kh = at<double>(workspace, L.kh);
+    std::vector<double> vals(n);
+    for (int k2 = 0; k2 < nk; ++k2) {
+      const char* d2 = (const char*)density + 16 * k2 * nn;
+      unsigned long long host_stats[2];
+      {
+        Stage st(ctx, s, "kpoint_density_eigh");
+        check_hip(hipMemsetAsync(stats, 0, 2 * sizeof(unsigned long long), s), "hipMemsetAsync");
+        check_hip(hfjk::launch_kp_hermitian_stats(d2, n, stats, s), "hermitian stats launch");
+        check_hip(hipMemcpyAsync(dz, d2, 16 * nn, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+        rocblas_handle h = blas_handle(ctx);
+        if (rocblas_set_stream(h, s) != rocblas_status_success) fail(HFJK_ERR_SOLVER, "rocblas_set_stream failed");
+        // The column-major view of the row-major Hermitian D(k) is conj(D(k)); its
+        // eigenvectors are conjugated when the factor is formed.
+        if (rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n, dz, n, ev, e, info) !=
+            rocblas_status_success)
+          fail(HFJK_ERR_SOLVER, "rocsolver_zheevd failed to launch");
+      }
+      rocblas_int host_info = 0;
+      check_hip(hipMemcpyAsync(host_stats, stats, sizeof(host_stats), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+      check_hip(hipMemcpyAsync(vals.data(), ev, sizeof(double) * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+      check_hip(hipMemcpyAsync(&host_info, info, sizeof(host_info), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+      check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+      double asym, mag;
+      std::memcpy(&asym, &host_stats[0], sizeof(double));
+      std::memcpy(&mag, &host_stats[1], sizeof(double));
+      if (!(asym <= 1e-12 * (mag > 0 ? mag : 1.0)))
+        fail(HFJK_ERR_INPUT, "the density of k-point " + std::to_string(k2) + " is not Hermitian (max |D - D^H| = " +
+                                 std::to_string(asym) + ")");
+      if (host_info != 0)
+        fail(HFJK_ERR_SOLVER, "rocsolver_zheevd did not converge (info " + std::to_string(host_info) + ")");
+      double top = 0.0;
+      for (double v : vals) top = std::max(top, std::fabs(v));
+      const double cut = 1e-12 * top;
+      int nneg = 0, npos = 0;
+      for (int j = 0; j < n && vals[j] < -cut; ++j) ++nneg;
+      for (int j = n - 1; j >= nneg && vals[j] > cut; --j) ++npos;
+      const int r = nneg + npos;
+      if (r == 0 || w[k2] == 0.0) continue;
+      const int ldo = hfjk::occ_ld(r);
+      {
+        Stage st(ctx, s, "kpoint_factor");
+        check_hip(hfjk::launch_kp_factor(dz, ev, n, nneg, npos, w[k2], ldo, ct, sg, s), "k-point factor launch");
+      }
+      for (int k1 = 0; k1 < nk; ++k1) {
+        const char* block = (const char*)tensor + 16 * ((long long)k1 * nk + k2) * naux * nn;
+        const double* sig = signature + ((long long)k1 * nk + k2) * naux;
+        {
+          Stage st(ctx, s, "kpoint_realify");
+          check_hip(hfjk::launch_kp_realify(block, naux, n, bh, s), "k-point realify launch");
+        }
+        {
+          Stage st(ctx, s, "kpoint_half_transform");
+          check_hip(hfjk::launch_half_transform_full(bh, HFJK_FP64, naux, 2 * n, ct, ldo, x, s),
+                    "k-point half transform launch");
+        }
+        {
+          Stage st(ctx, s, "kpoint_syrk");
+          check_hip(hfjk::launch_syrk(x, HFJK_FP64, naux, 2 * n, ldo, sg, true, L.nsplit, part, 0, s, sig),
+                    "k-point syrk launch");
+          check_hip(hfjk::launch_syrk_reduce(part, HFJK_FP64, L.nsplit, 2 * n, kh, s), "k-point reduce launch");
+          check_hip(hfjk::launch_kp_fold(kh, n, true, (char*)exchange + 16 * k1 * nn, s), "k-point fold launch");
+        }
+      }
+    }
+    if (madelung != 0.0) {
+      Stage st(ctx, s, "kpoint_madelung");
+      void* tz = at<void>(workspace, L.tz);
+      for (int k = 0; k < nk; ++k) {
+        const char* sk = (const char*)overlap + 16 * k * nn;
+        const char* dk = (const char*)density + 16 * k * nn;
+        check_hip(hfjk::launch_kp_zgemm(sk, dk, n, 1.0, false, tz, s), "k-point zgemm launch");
+        check_hip(hfjk::launch_kp_zgemm(tz, sk, n, madelung, true, (char*)exchange + 16 * k * nn, s),
+                  "k-point zgemm launch");
+      }
+    }
+  });
 }
 
 }  // extern "C"

@@ -179,9 +179,21 @@ hipError_t launch_half_transform(const void* b, int precision, long long p, int 
 // one workgroup per CU.
 // is_signed = false promises signs[i] = 1 for every occupied column, which
 // lets the direct-LDS kernel skip them.
+// Partials: FP64 keeps one 128 x 128 tile per split and lower tile pair;
+// the FP32-accumulating modes (FP32, BF16x3) one per run of
+// syrk_run_aux(precision, ldo) aux functions (at most kSyrkRunProducts
+// products per FP32 accumulator, whatever the split count).
+// syrk_slots = the number of partial tiles per pair; launch_syrk_reduce sums
+// them in FP64 in a fixed order.
+constexpr int kSyrkRunProducts = 4096;
+int syrk_run_aux(int precision, int ldo);
+int syrk_slots(int precision, int naux, int ldo, int nsplit);
+// auxsign (optional, [naux]): one more factor per aux function on the signed
+// direct-LDS kernels (hipErrorInvalidValue for a launch that would not run them).
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
-                       bool is_signed, int nsplit, void* partial, int kind, hipStream_t s);
-hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, int nao, double* out,
+                       bool is_signed, int nsplit, void* partial, int kind, hipStream_t s,
+                       const double* auxsign = nullptr);
+hipError_t launch_syrk_reduce(const void* partial, int precision, int nslot, int nao, double* out,
                               hipStream_t s);
 hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* signs_out, hipStream_t s);
 
@@ -207,6 +219,31 @@ hipError_t launch_expand_full(const double* packed, long long p, int naux, int n
 hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, void* ct, hipStream_t s);
 hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
                                       void* x, hipStream_t s);
+// k-point DF-J/K (kpoint.hip): complex values as interleaved double pairs.
+// launch_kp_charges: inner [nk][naux] = w_k <B_k[P], D_k> (conjugated B) and
+// charges [naux] = sum_k inner, for blocks B_k at b + k kstride (elements);
+// launch_kp_coulomb: J_k = sum_P sig_k[P] charges[P] B_k[P] (sig_k at sig + k sig_kstride);
+// launch_kp_realify: the [naux][npad2][npad2] real form [[Re, -Im], [Im, Re]] of a
+// complex [naux][nao][nao] block (npad2 = 32 ceil(2 nao / 32));
+// launch_kp_factor: C^T [ldo][npad2] of [Re C; Im C], C = conj(U) sqrt(weight |l|)
+// over the nneg lowest and npos highest eigenpairs (U column-major from
+// rocSOLVER zheevd of the column-major view of a row-major Hermitian D), and
+// the column signs; launch_kp_fold: K (+)= (Kh_TL + Kh_BR) + i (Kh_BL - Kh_TR)
+// of a [2 nao][2 nao] real Kh; launch_kp_zgemm: c (+)= alpha a b ([nao][nao]);
+// launch_kp_hermitian_stats: max |D - D^H| and max |D| (atomic max of double
+// bits into stats[0..1]; clear first).
+hipError_t launch_kp_charges(const void* b, long long kstride, int nk, int naux, int nao, const void* density,
+                             const double* weights, void* inner, void* charges, hipStream_t s);
+hipError_t launch_kp_coulomb(const void* b, long long kstride, int nk, int naux, int nao, const double* sig,
+                             long long sig_kstride, const void* charges, void* out, hipStream_t s);
+hipError_t launch_kp_realify(const void* b, int naux, int nao, double* out, hipStream_t s);
+hipError_t launch_kp_factor(const void* eigvecs, const double* eigvals, int nao, int nneg, int npos, double weight,
+                            int ldo, double* ct, double* signs, hipStream_t s);
+hipError_t launch_kp_fold(const double* kh, int nao, bool accumulate, void* k, hipStream_t s);
+hipError_t launch_kp_zgemm(const void* a, const void* b, int nao, double alpha, bool accumulate, void* c,
+                           hipStream_t s);
+hipError_t launch_kp_hermitian_stats(const void* d, int nao, unsigned long long* stats, hipStream_t s);
+
 // Lower-triangle 128 x 128 output tiles of the SYRK and the tile edge.
 int syrk_pairs(int nao);
 int syrk_tile();

@@ -639,6 +639,33 @@ __device__ __forceinline__ void syrk_coords(int b, int npairs, int& s, int& pair
   pair = slot % npairs;
 }
 
+// Partial slots of split s and the aux functions of slot c. run = 0 (FP64):
+// one slot per split, [s naux / nsplit, (s + 1) naux / nsplit). run > 0 (the
+// FP32-accumulating modes): slot c holds aux functions [c run, (c + 1) run),
+// and split s takes slots [s nslot / nsplit, (s + 1) nslot / nsplit), so an
+// FP32 accumulator never sums more than run aux functions whatever the
+// split count.
+__device__ __forceinline__ void slot_range(int s, int nsplit, int naux, int run, int& cb, int& ce) {
+  if (run == 0) {
+    cb = s;
+    ce = s + 1;
+    return;
+  }
+  const int nslot = (naux + run - 1) / run;
+  cb = (int)((long long)s * nslot / nsplit);
+  ce = (int)((long long)(s + 1) * nslot / nsplit);
+}
+
+__device__ __forceinline__ void slot_aux(int c, int nsplit, int naux, int run, int& qb, int& qe) {
+  if (run == 0) {
+    qb = (int)((long long)c * naux / nsplit);
+    qe = (int)((long long)(c + 1) * naux / nsplit);
+    return;
+  }
+  qb = c * run;
+  qe = min(naux, qb + run);
+}
+
 template <typename T>
 struct SyrkSmem {
   T As[kSyrkTile][kTile + 1];
@@ -753,16 +780,20 @@ __device__ __forceinline__ void syrk_tile(const T* __restrict__ x, int qb, int q
 template <typename T>
 __global__ __launch_bounds__(kThreads) void syrk_kernel(const T* __restrict__ x, int naux, int xrows, int ldo,
                                                          const double* __restrict__ signs, int nsplit, int npairs,
-                                                         T* __restrict__ partial) {
+                                                         int run, T* __restrict__ partial) {
   __shared__ SyrkSmem<T> sm;
   int s, pair;
   syrk_coords(blockIdx.x, npairs, s, pair);
   int mb, nb;
   tile_of(pair, mb, nb);
-  const int qb = (int)((long long)s * naux / nsplit);
-  const int qe = (int)((long long)(s + 1) * naux / nsplit);
-  syrk_tile<T>(x, qb, qe, xrows, ldo, signs, mb, nb,
-               partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile), sm);
+  int cb, ce;
+  slot_range(s, nsplit, naux, run, cb, ce);
+  for (int c = cb; c < ce; ++c) {
+    int qb, qe;
+    slot_aux(c, nsplit, naux, run, qb, qe);
+    syrk_tile<T>(x, qb, qe, xrows, ldo, signs, mb, nb,
+                 partial + ((long long)c * npairs + pair) * (kSyrkTile * kSyrkTile), sm);
+  }
 }
 
 // ---------------------------------------------- SYRK over direct LDS loads
@@ -818,10 +849,11 @@ __device__ __forceinline__ void barrier_nofence() { asm volatile("s_barrier" :::
 // The ring: acc += sum over `steps` k steps of (A rows) x (B rows)^T, the
 // 128 rows of each operand read at step k from src(k, operand) plus this
 // lane's byte offsets off[] (operand A = chunks of waves 0-1, B = waves 2-3).
-// With SIGNED, the B operand's k values are scaled by sgn[sign_at(k) + k'].
-template <typename T, bool SIGNED, int CH, typename Src, typename SignAt>
+// With SIGNED, the B operand's k values are scaled by sgn[sign_at(k) + k']
+// times scale_at(k) (one factor per step).
+template <typename T, bool SIGNED, int CH, typename Src, typename SignAt, typename ScaleAt>
 __device__ __forceinline__ void dma_ring(const Src& src, const int (&off)[4 * CH], int steps,
-                                         const T* __restrict__ sgn, const SignAt& sign_at,
+                                         const T* __restrict__ sgn, const SignAt& sign_at, const ScaleAt& scale_at,
                                          typename Op<T>::acc_t (&acc)[4][4]) {
   typedef SyrkDma<T, CH> D;
   typedef typename D::unit unit;
@@ -854,7 +886,7 @@ __device__ __forceinline__ void dma_ring(const Src& src, const int (&off)[4 * CH
     for (int ni = 0; ni < 4; ++ni)
       b[ni] = *(const unit*)(stage + D::STAGE / 2 + ((((wn * 4 + ni) * CH + c) * 64 + lane) * E));
     if constexpr (SIGNED) {
-      const unit sg = *(const unit*)(sgn + sign_at(step) + c * D::KC + (lane >> 4) * E);
+      const unit sg = *(const unit*)(sgn + sign_at(step) + c * D::KC + (lane >> 4) * E) * scale_at(step);
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni) b[ni] *= sg;
     }
@@ -930,8 +962,8 @@ __device__ __forceinline__ void dma_ring(const Src& src, const int (&off)[4 * CH
 // columns of X are zero), so the sign table is neither built nor applied.
 template <typename T, bool SIGNED, int CH>
 __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
-    const T* __restrict__ x, int naux, int xrows, int ldo, const double* __restrict__ signs, int nsplit, int npairs,
-    T* __restrict__ partial) {
+    const T* __restrict__ x, int naux, int xrows, int ldo, const double* __restrict__ signs,
+    const double* __restrict__ auxsign, int nsplit, int npairs, int run, T* __restrict__ partial) {
   typedef SyrkDma<T, CH> D;
   constexpr int E = D::E;
   constexpr int LOADS = D::LOADS;
@@ -941,8 +973,8 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
   syrk_coords(blockIdx.x, npairs, s, pair);
   int mb, nb;
   tile_of(pair, mb, nb);
-  const int qb = (int)((long long)s * naux / nsplit);
-  const int qe = (int)((long long)(s + 1) * naux / nsplit);
+  int cb, ce;
+  slot_range(s, nsplit, naux, run, cb, ce);
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -969,37 +1001,47 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
     off[v] = (row * ldo + (ch % CH) * D::KC + (lane >> 4) * E) * (int)sizeof(T);
   }
   const int nch = ldo / D::KS;
-  const int steps = (qe - qb) * nch;
-  // Step k covers k values i0 = (k % nch) KS of aux function qb + k / nch.
-  auto src = [&](int step, int) -> const T* {
-    const int qq = step / nch;
-    return x + (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * D::KS;
-  };
   auto sign_at = [&](int step) { return (step % nch) * D::KS; };
 
-  typename Op<T>::acc_t acc[4][4];
-#pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
+  for (int c = cb; c < ce; ++c) {
+    int qb, qe;
+    slot_aux(c, nsplit, naux, run, qb, qe);
+    const int steps = (qe - qb) * nch;
+    // Step k covers k values i0 = (k % nch) KS of aux function qb + k / nch.
+    auto src = [&](int step, int) -> const T* {
+      const int qq = step / nch;
+      return x + (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * D::KS;
+    };
+    // Per-aux signs (the signature of an indefinite metric), when given.
+    auto scale_at = [&](int step) -> T { return auxsign != nullptr ? (T)auxsign[qb + step / nch] : (T)1; };
 
-  __syncthreads();  // sign table written
-  dma_ring<T, SIGNED, CH>(src, off, steps, sgn, sign_at, acc);
+    typename Op<T>::acc_t acc[4][4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
 
-  T* dst = partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile);
+    // The sign table is written, and every wave's LDS reads of the previous
+    // slot's ring have returned (each ring ends waiting for them), so the
+    // stage buffers may take this slot's loads.
+    __syncthreads();
+    dma_ring<T, SIGNED, CH>(src, off, steps, sgn, sign_at, scale_at, acc);
+
+    T* dst = partial + ((long long)c * npairs + pair) * (kSyrkTile * kSyrkTile);
 #pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) {
-      const int col = wn * 64 + ni * 16 + (lane & 15);
+      for (int ni = 0; ni < 4; ++ni) {
+        const int col = wn * 64 + ni * 16 + (lane & 15);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = wm * 64 + mi * 16 + Op<T>::row(lane, r);
-        dst[row * kSyrkTile + col] = acc[mi][ni][r];
+        for (int r = 0; r < 4; ++r) {
+          const int row = wm * 64 + mi * 16 + Op<T>::row(lane, r);
+          dst[row * kSyrkTile + col] = acc[mi][ni][r];
+        }
       }
-    }
+  }
 }
 
 // FP64 / FP32 half-transform on the ring, from a full row-major copy of B
@@ -1038,6 +1080,7 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void xform_dma_kernel(
   const T* bq = bfull + (long long)q * npad * npad;
   auto src = [&](int step, int operand) -> const T* { return (operand ? ct : bq) + step * D::KS; };
   auto no_sign = [](int) { return 0; };
+  auto no_scale = [](int) { return (T)1; };
 
   typename Op<T>::acc_t acc[4][4];
 #pragma unroll
@@ -1047,7 +1090,7 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void xform_dma_kernel(
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
 
-  dma_ring<T, false, CH>(src, off, npad / D::KS, (const T*)nullptr, no_sign, acc);
+  dma_ring<T, false, CH>(src, off, npad / D::KS, (const T*)nullptr, no_sign, no_scale, acc);
 
   T* xq = x + (long long)q * xrows * ldo;
 #pragma unroll
@@ -1239,20 +1282,25 @@ __device__ __forceinline__ void syrk16_tile(const uint16_t* __restrict__ xhi, co
 __global__ __launch_bounds__(kThreads) void syrk_bf16x3_kernel(const uint16_t* __restrict__ xhi,
                                                                 const uint16_t* __restrict__ xlo, int naux,
                                                                 int xrows, int ldo, const double* __restrict__ signs,
-                                                                int nsplit, int npairs, float* __restrict__ partial) {
+                                                                int nsplit, int npairs, int run,
+                                                                float* __restrict__ partial) {
   __shared__ __attribute__((aligned(16))) Syrk16Smem sm;
   int s, pair;
   syrk_coords(blockIdx.x, npairs, s, pair);
   int mb, nb;
   tile_of(pair, mb, nb);
-  const int qb = (int)((long long)s * naux / nsplit);
-  const int qe = (int)((long long)(s + 1) * naux / nsplit);
-  syrk16_tile(xhi, xlo, qb, qe, xrows, ldo, signs, mb, nb,
-              partial + ((long long)s * npairs + pair) * (kSyrkTile * kSyrkTile), sm);
+  int cb, ce;
+  slot_range(s, nsplit, naux, run, cb, ce);
+  for (int c = cb; c < ce; ++c) {
+    int qb, qe;
+    slot_aux(c, nsplit, naux, run, qb, qe);
+    syrk16_tile(xhi, xlo, qb, qe, xrows, ldo, signs, mb, nb,
+                partial + ((long long)c * npairs + pair) * (kSyrkTile * kSyrkTile), sm);
+  }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void syrk_reduce_kernel(const T* __restrict__ partial, int nsplit,
+__global__ __launch_bounds__(kThreads) void syrk_reduce_kernel(const T* __restrict__ partial, int nslot,
                                                                 int npairs, int nao, double* __restrict__ out) {
   const long long total = (long long)nao * nao;
   for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
@@ -1270,7 +1318,7 @@ __global__ __launch_bounds__(kThreads) void syrk_reduce_kernel(const T* __restri
       local = (col % kSyrkTile) * kSyrkTile + row % kSyrkTile;
     }
     double sum = 0.0;
-    for (int k = 0; k < nsplit; ++k)
+    for (int k = 0; k < nslot; ++k)
       sum += (double)partial[((long long)k * npairs + pair) * (kSyrkTile * kSyrkTile) + local];
     out[idx] = sum;
   }
@@ -1461,10 +1509,25 @@ hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux
   return launch_x16_t<256>(bhi, blo, p, naux, nao, cthi, ctlo, ldo, xhi, xlo, s);
 }
 
+int syrk_run_aux(int precision, int ldo) {
+  if (precision == HFJK_FP64) return 0;
+  return std::max(1, kSyrkRunProducts / std::max(ldo, 1));
+}
+
+int syrk_slots(int precision, int naux, int ldo, int nsplit) {
+  const int run = syrk_run_aux(precision, ldo);
+  return run == 0 ? nsplit : (naux + run - 1) / run;
+}
+
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
-                       bool is_signed, int nsplit, void* partial, int kind, hipStream_t s) {
+                       bool is_signed, int nsplit, void* partial, int kind, hipStream_t s, const double* auxsign) {
   const int xrows = x_rows(nao);
   const int npairs = syrk_pairs(nao);
+  const int run = syrk_run_aux(precision, ldo);
+  // Per-aux signs are applied by the signed direct-LDS kernels only.
+  if (auxsign != nullptr && (!is_signed || precision == HFJK_BF16X3 || kind == 1 || ldo > kDmaMaxLdo ||
+                             (long long)xrows * ldo >= (1LL << 31)))
+    return hipErrorInvalidValue;
   const unsigned grid = (unsigned)(npairs * nsplit);
   // The direct-load kernel keeps per-lane element offsets within one aux
   // function's X in 32 bits and the sign table in LDS.
@@ -1474,46 +1537,46 @@ hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo,
   const bool two_wg = kind != 3 && !is_signed;
   if (dma && precision == HFJK_FP64 && is_signed)
     hipLaunchKernelGGL((syrk_dma_kernel<double, true, 2>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
-                       xrows, ldo, signs, nsplit, npairs, (double*)partial);
+                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (double*)partial);
   else if (dma && precision == HFJK_FP64 && two_wg)
     hipLaunchKernelGGL((syrk_dma_kernel<double, false, 1>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
-                       xrows, ldo, signs, nsplit, npairs, (double*)partial);
+                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (double*)partial);
   else if (dma && precision == HFJK_FP64)
     hipLaunchKernelGGL((syrk_dma_kernel<double, false, 2>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
-                       xrows, ldo, signs, nsplit, npairs, (double*)partial);
+                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (double*)partial);
   else if (dma && is_signed)
     hipLaunchKernelGGL((syrk_dma_kernel<float, true, 2>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux,
-                       xrows, ldo, signs, nsplit, npairs, (float*)partial);
+                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (float*)partial);
   else if (dma && two_wg)
     hipLaunchKernelGGL((syrk_dma_kernel<float, false, 1>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux,
-                       xrows, ldo, signs, nsplit, npairs, (float*)partial);
+                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (float*)partial);
   else if (dma)
     hipLaunchKernelGGL((syrk_dma_kernel<float, false, 2>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux,
-                       xrows, ldo, signs, nsplit, npairs, (float*)partial);
+                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (float*)partial);
   else if (precision == HFJK_FP64)
     hipLaunchKernelGGL(syrk_kernel<double>, dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux, xrows, ldo,
-                       signs, nsplit, npairs, (double*)partial);
+                       signs, nsplit, npairs, run, (double*)partial);
   else if (precision == HFJK_FP32)
     hipLaunchKernelGGL(syrk_kernel<float>, dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux, xrows, ldo, signs,
-                       nsplit, npairs, (float*)partial);
+                       nsplit, npairs, run, (float*)partial);
   else {
     const long long nx = (long long)naux * xrows * ldo;
     const uint16_t* xhi = (const uint16_t*)x;
     hipLaunchKernelGGL(syrk_bf16x3_kernel, dim3(grid), dim3(kThreads), 0, s, xhi, xhi + nx, naux, xrows, ldo, signs,
-                       nsplit, npairs, (float*)partial);
+                       nsplit, npairs, run, (float*)partial);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_syrk_reduce(const void* partial, int precision, int nsplit, int nao, double* out, hipStream_t s) {
+hipError_t launch_syrk_reduce(const void* partial, int precision, int nslot, int nao, double* out, hipStream_t s) {
   const int npairs = syrk_pairs(nao);
   const long long n = (long long)nao * nao;
   if (precision == HFJK_FP64)
     hipLaunchKernelGGL(syrk_reduce_kernel<double>, dim3(grid_for(n)), dim3(kThreads), 0, s, (const double*)partial,
-                       nsplit, npairs, nao, out);
+                       nslot, npairs, nao, out);
   else
     hipLaunchKernelGGL(syrk_reduce_kernel<float>, dim3(grid_for(n)), dim3(kThreads), 0, s, (const float*)partial,
-                       nsplit, npairs, nao, out);
+                       nslot, npairs, nao, out);
   return hipGetLastError();
 }
 

@@ -1,18 +1,24 @@
-"""Exception hierarchy mirroring hafnium/errors.py:9-43 of the reference."""
+"""Exception classes of the package.
+
+The class names, their bases and the ``last`` attribute of
+``ConvergenceError`` are the reference's exception contract
+(hafnium/errors.py:9-43), kept so that callers written against the
+reference catch the same types.
+"""
 
 from typing import Any
 
 
 class HafniumError(Exception):
-    """Base class of all exceptions raised by this package."""
+    """Root of every exception this package raises."""
 
 
 class InputError(HafniumError, ValueError):
-    """A user-supplied specification is invalid or inconsistent."""
+    """Arguments that are malformed, out of range or contradict each other (shapes, counts, options)."""
 
 
 class StructureError(HafniumError, ValueError):
-    """Shapes or structures are inconsistent."""
+    """Array shapes or block layouts that do not fit together."""
 
 
 class BackendError(HafniumError):
@@ -20,10 +26,11 @@ class BackendError(HafniumError):
 
 
 class ConvergenceError(HafniumError):
-    """An iterative procedure stopped before reaching its tolerances.
+    """An iteration hit its limit with its tolerances unmet.
 
-    ``last`` holds the final state (for the SCF a result with ``converged``
-    false), so the caller can inspect it or restart from it.
+    The attribute ``last`` carries the state reached (for the SCF, a result
+    whose ``converged`` is false): it can be examined, or passed back as the
+    start of another run.
     """
 
     def __init__(self, message: str, last: Any) -> None:

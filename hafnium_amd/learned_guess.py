@@ -702,7 +702,7 @@ def learned_guess(system, model: LearnedGuessModel, sad_density: np.ndarray, *, 
     t0 = time.perf_counter()
     coeffs = model.predict(system.molecule)
     v = torch.from_numpy(system.metric @ coeffs).to(h.device).view(-1, 1).contiguous()
-    charges = tb.fit_dense.lower_solve(tb.metric_factor, v).view(-1)
+    charges = tb.fit_charges(v).view(-1)
     j_pred = tb.coulomb_from_charges(charges, precision=precision)
     sad = torch.from_numpy(np.ascontiguousarray(sad_density)).to(h.device)
     j_sad = tb.coulomb(sad, precision=precision)

@@ -35,6 +35,9 @@ class Dense:
         self.device = device
         self._partial = torch.empty(int(self._lib.hfjk_dot_workspace()), dtype=torch.float64, device=device)
         self._info = torch.zeros(1, dtype=torch.int32, device=device)
+        # Count of failed diagonalisations since the last reset, kept on the device so that
+        # checking it costs no synchronisation of its own (the SCF reads it with its energies).
+        self._eig_failures = torch.zeros(1, dtype=torch.float64, device=device)
 
     def _stream(self):
         return ctypes.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)
@@ -75,11 +78,21 @@ class Dense:
         e = torch.empty(n, dtype=torch.float64, device=self.device)
         _lib.check(self._lib.hfjk_dsyevd(self._ctx.handle, n, _p(work), _p(w), _p(e), _p(self._info),
                                          self._stream()))
+        self._eig_failures += (self._info != 0).to(torch.float64)
         rows = work  # row j = eigenvector j
         return w, self.transpose(rows)
 
     def info(self) -> int:
         return int(self._info.item())
+
+    @property
+    def eig_failures(self):
+        """Device float64 ``[1]``: diagonalisations of :meth:`eigh` whose dsyevd info was nonzero
+        since the last :meth:`reset_eig_failures`."""
+        return self._eig_failures
+
+    def reset_eig_failures(self) -> None:
+        self._eig_failures.zero_()
 
     def transpose(self, a):
         """``a^T`` through a GEMM with the identity-free transposed copy."""

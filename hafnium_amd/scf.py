@@ -30,7 +30,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from hafnium_amd.errors import ConvergenceError, InputError
+from hafnium_amd.errors import BackendError, ConvergenceError, InputError
 from hafnium_amd.linalg import Dense
 
 DEFAULT_OVERLAP_THRESHOLD = 1e-10
@@ -251,6 +251,8 @@ class _Setup:
         dn = hamiltonian.dense
         w, v = dn.eigh(hamiltonian.overlap)
         w_host = w.cpu().numpy()
+        if dn.info() != 0:
+            raise BackendError(f"the overlap could not be diagonalised (dsyevd info {dn.info()})")
         kept = int(np.sum(w_host > options.overlap_threshold))
         if kept == 0:
             raise InputError("the overlap is not positive definite")
@@ -313,6 +315,7 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
     t0 = time.perf_counter()
     setup = _Setup(hamiltonian, options)
     dn = hamiltonian.dense
+    dn.reset_eig_failures()
     n_alpha = hamiltonian.n_alpha
     if n_alpha > setup.kept:
         raise InputError(f"{n_alpha} occupied orbitals do not fit {setup.kept} retained functions")
@@ -346,6 +349,9 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
     precision = options.precision
     focks: list = []
     errors: list = []
+    # 2 tr(e_i e_j) of the stored errors, kept across iterations: each iteration's transfer
+    # brings only the newest row.
+    overlaps = np.zeros((0, 0))
     history: list[ScfIteration] = []
     converged = False
     fock = None
@@ -357,17 +363,27 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
         else:
             fock, terms = hamiltonian.fock(density, orbitals, precision)
         grad = setup.gradient(fock, density)
-        # One host transfer per iteration: energy terms, gradient norm, DIIS overlaps.
+        # One host transfer per iteration: energy terms, the newest error's overlaps with the
+        # stored ones (its own norm last), and the count of failed diagonalisations.
         focks = (focks + [fock])[-options.diis_size:]
         errors = (errors + [grad])[-options.diis_size:]
         m = len(errors)
-        scalars = torch.empty(4 + m, dtype=torch.float64, device=hamiltonian.device)
+        scalars = torch.empty(5 + m, dtype=torch.float64, device=hamiltonian.device)
         scalars[0:3] = terms
         for i, e_i in enumerate(errors):
             dn.dot(grad, e_i, scalars[3 + i:4 + i])
+        scalars[3 + m:4 + m] = dn.eig_failures
         host = scalars.cpu().numpy()
+        if host[3 + m] != 0.0:
+            raise BackendError("the Fock matrix could not be diagonalised (dsyevd reported a failure)")
+        row = 2.0 * host[3:3 + m]
+        kept_ov = overlaps[-(m - 1):, -(m - 1):] if m > 1 else np.zeros((0, 0))
+        overlaps = np.zeros((m, m))
+        overlaps[:m - 1, :m - 1] = kept_ov
+        overlaps[m - 1, :] = row
+        overlaps[:, m - 1] = row
         energies = _energy_terms(hamiltonian, host[0:3])
-        gnorm = math.sqrt(max(2.0 * host[3 + m - 1], 0.0))
+        gnorm = math.sqrt(max(row[m - 1], 0.0))
         total = energies.total
         change = total - history[-1].energy if history else math.inf
         history.append(ScfIteration(index, total, change, gnorm, precision, time.perf_counter() - t_it))
@@ -381,12 +397,12 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
             precision = "fp64"
             if options.reset_diis_on_switch:
                 focks, errors = [fock], [grad]
+                overlaps = overlaps[-1:, -1:]
         # DIIS extrapolation (diis.py:55-79) from the overlaps of the stored errors.
         if len(focks) == 1:
             extrapolated = focks[0]
         else:
-            ov = _diis_overlaps(dn, errors, host, m, hamiltonian.device)
-            coeff = solve_diis(ov)
+            coeff = solve_diis(overlaps)
             extrapolated = dn.combine(float(coeff[0]), focks[0])
             for ci, f in zip(coeff[1:], focks[1:]):
                 extrapolated = dn.combine(1.0, extrapolated, float(ci), f)
@@ -419,26 +435,3 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
             result,
         )
     return result
-
-
-def _diis_overlaps(dn, errors, host, m, device):
-    """The real overlaps ``2 tr(e_i e_j)`` of the stored errors (weight 2 of a restricted block)."""
-    torch = _torch()
-    # The last row (against the newest error) came with the iteration's transfer.
-    ov = np.zeros((m, m))
-    if m > 1:
-        rest = torch.empty((m - 1) * m // 2, dtype=torch.float64, device=device)
-        slot = 0
-        for i in range(m - 1):
-            for j in range(i, m - 1):
-                dn.dot(errors[i], errors[j], rest[slot:slot + 1])
-                slot += 1
-        vals = rest.cpu().numpy()
-        slot = 0
-        for i in range(m - 1):
-            for j in range(i, m - 1):
-                ov[i, j] = ov[j, i] = 2.0 * vals[slot]
-                slot += 1
-    for i in range(m):
-        ov[i, m - 1] = ov[m - 1, i] = 2.0 * host[3 + i]
-    return ov

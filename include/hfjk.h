@@ -440,6 +440,33 @@ HFJK_API int hfjk_mlp_adam(hfjk_ctx *ctx, long long count, float *params, const 
 /* dst = bf16(src) (nearest even) over count elements. */
 HFJK_API int hfjk_mlp_to_bf16(hfjk_ctx *ctx, long long count, const float *src, uint16_t *dst, void *stream);
 
+/* ---- k-point DF-J/K (FP64; complex arrays as interleaved (re, im) double pairs, row-major) ----
+ * Replace KPointDensityFittedCoulombExchange.coulomb / .exchange
+ * (coulomb_exchange.py:279-297) and KPointDensityFittedCoulomb.coulomb
+ * (coulomb_exchange.py:375-383). tensor: B [nk][nk][naux][nao][nao] of
+ * momentum transfer k2 - k1 with signature [nk][nk][naux] (+1/-1), or with
+ * zero_transfer = 1 the zero-transfer blocks [nk][naux][nao][nao] with one
+ * sign per aux function [naux]; weights [nk] (device float64, >= 0); density,
+ * overlap and the outputs [nk][nao][nao].
+ * hfjk_kpoint_coulomb: c_P = sum_k w_k sum_xw conj(B[k,k,P,x,w]) D(k)[x,w],
+ *   J(k) = sum_P s[k,k,P] c_P B[k,k,P].
+ * hfjk_kpoint_exchange: K(k) = sum_k' w_k' sum_P s[k,k',P] B[k,k',P] D(k')
+ *   B[k,k',P]^H + madelung S(k) D(k) S(k) for Hermitian D(k') (1e-12
+ *   relative, else HFJK_ERR_INPUT): D(k') = V diag(l) V^H by rocSOLVER zheevd
+ *   (|l| <= 1e-12 max |l| dropped), and per (k, k') block the real form
+ *   [[Re B, -Im B], [Im B, Re B]] against [Re C; Im C], C = V |l|^1/2
+ *   sqrt(w_k'), through the FP64 direct-LDS half-transform and the signed
+ *   SYRK with per-aux signs. Synchronises `stream` once per k-point.
+ * Workspace: hfjk_kpoint_workspace_bytes(nk, naux, nao) for both. */
+HFJK_API int hfjk_kpoint_workspace_bytes(hfjk_ctx *ctx, int nk, int naux, int nao, size_t *bytes);
+HFJK_API int hfjk_kpoint_coulomb(hfjk_ctx *ctx, int nk, int naux, int nao, int zero_transfer, const void *tensor,
+                                 const double *signature, const double *weights, const void *density,
+                                 void *coulomb, void *workspace, size_t workspace_bytes, void *stream);
+HFJK_API int hfjk_kpoint_exchange(hfjk_ctx *ctx, int nk, int naux, int nao, const void *tensor,
+                                  const double *signature, const double *weights, const void *overlap,
+                                  double madelung, const void *density, void *exchange, void *workspace,
+                                  size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -264,3 +264,118 @@ def nuclear_repulsion(charges, positions_bohr) -> float:
         for b in range(a):
             total += charges[a] * charges[b] / float(np.linalg.norm(pos[a] - pos[b]))
     return total
+
+
+@dataclass
+class UhfResult:
+    energy: float
+    mo_energies: tuple  # (alpha, beta)
+    mo_coefficients: tuple
+    occupations: tuple
+    density: tuple  # (D_alpha, D_beta)
+    spin_square: float
+    converged: bool
+    history: list
+
+
+def spin_square(d_alpha: np.ndarray, d_beta: np.ndarray, overlap: np.ndarray, n_alpha: float, n_beta: float) -> float:
+    """``<S^2> = S_z (S_z + 1) + n_beta - tr(D_a S D_b S)`` of a collinear determinant."""
+    sz = 0.5 * (n_alpha - n_beta)
+    return sz * (sz + 1.0) + n_beta - float(np.sum((d_alpha @ overlap) * (d_beta @ overlap).T))
+
+
+def uhf(
+    overlap: np.ndarray,
+    core: np.ndarray,
+    j_of,
+    k_of,
+    n_alpha: int,
+    n_beta: int,
+    nuclear_repulsion: float,
+    *,
+    max_iterations: int = 100,
+    energy_tolerance: float = 1e-10,
+    gradient_tolerance: float = 1e-6,
+    diis_size: int = 8,
+) -> UhfResult:
+    """Converge an unrestricted (collinear) determinant (driver.py:154-213 for the collinear
+    structure).
+
+    * Start: the spin-averaged core guess, ``n_electrons / 2`` per spin with the averaged
+      filling (driver.py:216-246).
+    * Fock: ``F_s = h + J[D_a + D_b] - K[D_s]`` (coulomb_exchange.py:407-462: J of the
+      spin trace on both blocks, K per stored block).
+    * Energy: ``tr((D_a + D_b) h) + tr((D_a + D_b) J) / 2 - (tr(D_a K_a) + tr(D_b K_b)) / 2``
+      (hamiltonian.py:331-370 with weight 1 per collinear block).
+    * Gradient ``X^T (F_s D_s S - S D_s F_s) X`` per block; its norm and the DIIS overlaps sum
+      over both blocks (driver.py:248-260, :385-395; diis.py:55-79).
+    """
+    x = canonical_orthogonalizer(overlap)
+
+    def solve(f):
+        e, c = np.linalg.eigh(x.T @ f @ x)
+        return e, x @ c
+
+    e, c = solve(core)
+    half = 0.5 * (n_alpha + n_beta)
+    occ = fill_degenerate_groups(e, half)
+    d0 = (c * occ[None, :]) @ c.T
+    dens = [d0, d0.copy()]
+    counts = (n_alpha, n_beta)
+
+    def build(dens):
+        total = dens[0] + dens[1]
+        j = j_of(total)
+        ks = [k_of(d) for d in dens]
+        focks = [core + j - k for k in ks]
+        e1 = float(np.sum(total * core.T))
+        ej = 0.5 * float(np.sum(total * j.T))
+        ek = -0.5 * sum(float(np.sum(d * k.T)) for d, k in zip(dens, ks))
+        return focks, nuclear_repulsion + e1 + ej + ek
+
+    focks_hist, errors_hist, history = [], [], []
+    converged = False
+    for index in range(max_iterations):
+        focks, total = build(dens)
+        grads = [x.T @ (f @ d @ overlap - overlap @ d @ f) @ x for f, d in zip(focks, dens)]
+        gnorm = math.sqrt(sum(float(np.sum(g * g)) for g in grads))
+        change = total - history[-1][1] if history else math.inf
+        history.append((index, total, change, gnorm))
+        if abs(change) < energy_tolerance and gnorm < gradient_tolerance:
+            converged = True
+            break
+        focks_hist = (focks_hist + [focks])[-diis_size:]
+        errors_hist = (errors_hist + [grads])[-diis_size:]
+        m = len(focks_hist)
+        if m == 1:
+            extrapolated = focks_hist[0]
+        else:
+            ov = np.array([[sum(float(np.sum(ga * gb)) for ga, gb in zip(a, b)) for b in errors_hist]
+                           for a in errors_hist])
+            coef = solve_diis(ov)
+            extrapolated = [sum(ci * f[s] for ci, f in zip(coef, focks_hist)) for s in range(2)]
+        dens = []
+        for s in range(2):
+            e, c = solve(extrapolated[s])
+            occ = fill_whole_levels(e, counts[s])
+            dens.append((c * occ[None, :]) @ c.T)
+    final = focks if converged else focks_hist[-1]
+    energies, coefficients, occupations, dens = [], [], [], []
+    for s in range(2):
+        e, c = solve(final[s])
+        occ = fill_whole_levels(e, counts[s])
+        energies.append(e)
+        coefficients.append(c)
+        occupations.append(occ)
+        dens.append((c * occ[None, :]) @ c.T)
+    _, total = build(dens)
+    return UhfResult(
+        energy=total,
+        mo_energies=tuple(energies),
+        mo_coefficients=tuple(coefficients),
+        occupations=tuple(occupations),
+        density=tuple(dens),
+        spin_square=spin_square(dens[0], dens[1], overlap, n_alpha, n_beta),
+        converged=converged,
+        history=history,
+    )

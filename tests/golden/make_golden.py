@@ -5,10 +5,11 @@ mounted at /root/reference:  python tests/golden/make_golden.py
 
 * water_sto3g.npz: arrays copied from the reference's stored PySCF 2.14
   fixtures (tests/reference/data/{one_electron,density_fitting,four_center,
-  scf,mp2}_water_sto-3g.json): S, T, V [7,7], metric [113,113],
-  three_center [113,7,7], eri [7,7,7,7], positions in Bohr, the exact RHF
-  energy and orbital energies, and the DF-RHF energy (mp2 fixture's
-  reference_energy). Data only; no reference code is run.
+  scf,mp2}_water_sto-3g.json and scf_water_cation_sto-3g.json): S, T, V
+  [7,7], metric [113,113], three_center [113,7,7], eri [7,7,7,7], positions
+  in Bohr, the exact RHF energy and orbital energies, the DF-RHF energy (mp2
+  fixture's reference_energy), and the exact UHF energy, orbital energies,
+  occupations and <S^2> of the cation at the same geometry. Data only; no reference code is run.
 * synthetic_c0.npz: the seeded configuration nao=100, naux=300, nocc=20,
   seed 0 of the project's generator: a sample of B entries at fixed
   indices (pins the generator bit for bit), C_occ, D, and the oracle's J, K
@@ -41,6 +42,8 @@ def water():
     four = load("four_center_water_sto-3g.json")
     scf = load("scf_water_sto-3g.json")
     mp2 = load("mp2_water_sto-3g.json")
+    cation = load("scf_water_cation_sto-3g.json")
+    assert np.allclose(cation["provenance"]["positions_bohr"], one["provenance"]["positions_bohr"])
     np.savez_compressed(
         os.path.join(OUT, "water_sto3g.npz"),
         overlap=np.array(one["overlap"]),
@@ -55,6 +58,10 @@ def water():
         rhf_mo_energies=np.array(scf["mo_energies"]),
         rhf_mo_occupations=np.array(scf["mo_occupations"]),
         df_rhf_energy=np.array(mp2["reference_energy"]),
+        cation_uhf_energy=np.array(cation["energy"]),
+        cation_uhf_mo_energies=np.array(cation["mo_energies"]),
+        cation_uhf_mo_occupations=np.array(cation["mo_occupations"]),
+        cation_uhf_spin_square=np.array(cation["spin_square"]),
     )
 
 

@@ -216,3 +216,20 @@ def test_averaged_occupation_gives_a_spherical_oxygen_atom():
     assert spherical(res.density)
     whole = scf_ref.rhf(s, t + v, j_of, k_of, 4, 0.0)
     assert not spherical(whole.density)
+
+
+def test_exact_uhf_of_the_cation_matches_the_stored_values(water):
+    """The stored PySCF UHF of the water cation (9 electrons, doublet) from the stored ERIs."""
+    overlap, core, e_nuc = water_problem(water)
+    j_of, k_of = scf_ref.exact_operators(water["eri"])
+    result = scf_ref.uhf(overlap, core, j_of, k_of, 5, 4, e_nuc, energy_tolerance=1e-12,
+                         gradient_tolerance=1e-8)
+    assert result.converged
+    assert result.energy == pytest.approx(float(water["cation_uhf_energy"]), abs=ENERGY_TOL)
+    for s in range(2):
+        np.testing.assert_allclose(result.mo_energies[s], water["cation_uhf_mo_energies"][s], atol=MO_ENERGY_TOL)
+        np.testing.assert_array_equal(result.occupations[s], water["cation_uhf_mo_occupations"][s])
+    assert result.spin_square == pytest.approx(float(water["cation_uhf_spin_square"]), abs=1e-8)
+    # The same-spin exchange only: K of the total density on both blocks misses the stored energy.
+    wrong = scf_ref.uhf(overlap, core, j_of, lambda d: 2.0 * k_of(d), 5, 4, e_nuc)
+    assert abs(wrong.energy - float(water["cation_uhf_energy"])) > 1e-2
