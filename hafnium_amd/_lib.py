@@ -69,6 +69,8 @@ SIGNATURES = [
     ("hfjk_int1e", _c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
     ("hfjk_int2c2e", _c_int, [_vp, _vp]),
     ("hfjk_int3c2e", _c_int, [_vp, _vp, _vp]),
+    ("hfjk_int2c2e_device", _c_int, [_vp, _vp, _vp, _vp]),
+    ("hfjk_int3c2e_device", _c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("hfjk_int2e", _c_int, [_vp, _vp]),
     ("hfjk_batch_create", _c_int, [_vp, _vp, _c_int, _c_int, _c_int, ctypes.POINTER(_vp)]),
     ("hfjk_batch_workspace_bytes", _c_int, [_vp, ctypes.POINTER(_c_size)]),

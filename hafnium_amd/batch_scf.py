@@ -78,7 +78,10 @@ class MoleculeInputs:
         return self.metric.shape[0]
 
 
-def molecule_inputs(molecule: Molecule, *, basis: str = "cc-pvdz", aux: str = "def2-universal-jkfit"):
+def molecule_inputs(molecule: Molecule, *, basis: str = "cc-pvdz", aux: str = "def2-universal-jkfit",
+                    device_integrals: bool = True, device=None):
+    """S, T + V (host) and the metric and three-centre integrals (device tensors computed on the
+    GPU, or host arrays from the host engine with ``device_integrals=False``)."""
     from hafnium_amd import integrals
 
     if molecule.n_electrons % 2:
@@ -86,8 +89,16 @@ def molecule_inputs(molecule: Molecule, *, basis: str = "cc-pvdz", aux: str = "d
     orb = BasisOnAtoms(basis, molecule)
     auxb = BasisOnAtoms(aux, molecule)
     s, t, v = integrals.one_electron(orb, molecule)
-    return MoleculeInputs(molecule, s, t + v, integrals.metric(auxb), integrals.three_center(auxb, orb),
-                          molecule.n_electrons // 2, molecule.nuclear_repulsion(), aux)
+    if device_integrals:
+        m = integrals.metric_device(auxb, device=device)
+        tc = integrals.three_center_device(auxb, orb, device=device)
+    else:
+        m, tc = integrals.metric(auxb), integrals.three_center(auxb, orb)
+    return MoleculeInputs(molecule, s, t + v, m, tc, molecule.n_electrons // 2, molecule.nuclear_repulsion(), aux)
+
+
+def _host(a) -> np.ndarray:
+    return a.cpu().numpy() if hasattr(a, "cpu") else np.asarray(a)
 
 
 @dataclass
@@ -136,9 +147,10 @@ class BatchedRhf:
         self.factors = []
         info = torch.zeros(len(self.inputs), dtype=torch.int32, device=self.device)
         for i, x in enumerate(self.inputs):
-            m = torch.from_numpy(np.ascontiguousarray(x.metric)).to(self.device)
+            m = torch.as_tensor(x.metric, dtype=torch.float64).to(self.device).clone().contiguous()
             _lib.check(self.lib.hfjk_dpotrf(self._ctx.handle, x.naux, _p(m), _p(info[i:i + 1]), self._stream()))
-            b = torch.from_numpy(np.ascontiguousarray(x.three_center).reshape(x.naux, -1)).to(self.device)
+            b = torch.as_tensor(x.three_center, dtype=torch.float64).to(self.device).reshape(
+                x.naux, -1).clone().contiguous()
             _lib.check(self.lib.hfjk_dtrsm_lower(self._ctx.handle, x.naux, x.nao * x.nao, _p(m), _p(b),
                                                  self._stream()))
             # (B + B^T) / 2: removes the triangular solve's rounding asymmetry (see density_fitting.df_tensor).
@@ -543,7 +555,7 @@ def learned_start(batch: BatchedRhf, model, sad: _Start, precision: str = "fp64"
     # Charges against B: L^-1 (P|Q) d, with the metric's Cholesky factor kept from the fit.
     charges = []
     for x, f, d in zip(batch.inputs, batch.factors, coeffs):
-        v = torch.from_numpy(x.metric @ d).to(batch.device).view(-1, 1).contiguous()
+        v = torch.from_numpy(_host(x.metric) @ d).to(batch.device).view(-1, 1).contiguous()
         _lib.check(batch.lib.hfjk_dtrsm_lower(batch._ctx.handle, x.naux, 1, _p(f), _p(v), batch._stream()))
         charges.append(v.view(-1))
     j_c = batch.jk.coulomb_from_charges(torch.cat(charges), precision=precision)

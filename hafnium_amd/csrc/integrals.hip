@@ -786,3 +786,441 @@ int hfjk_int3c2e(const hfjk_basis* aux, const hfjk_basis* orbital, double* out) 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ on the GPU
+//
+// (P|uv) and (P|Q) on the device: one 64-lane workgroup per shell triple
+// (aux shell C, orbital shell pair A >= B; for the metric A is an aux shell
+// and B a unit s function of exponent 0 on A's centre). Per primitive triple
+// the wave forms, in LDS, the Hermite expansion coefficients of the A B pair
+// and of C along x, y, z and the Hermite Coulomb integrals R_tuv (Boys
+// function, then the downward recursion one order per step, the lanes
+// sharing the entries of an order); each lane then contracts the Cartesian
+// components it owns and accumulates them, weighted by the contraction
+// coefficients, into the shell triple's Cartesian block in LDS. Last, the
+// lanes apply the spherical transforms and the normalisation and write the
+// block (and its (u, v) mirror). Shell pairs whose every primitive pair is
+// below the product cut are skipped on the host, as the host engine skips
+// them primitive by primitive; their entries stay zero.
+namespace {
+
+constexpr int kGL = 5;                  // largest l of a GPU shell (h)
+constexpr int kGMaxL = 12;              // largest total angular momentum of a triple
+constexpr int kGRtab = (kGMaxL + 1) * (kGMaxL + 2) * (kGMaxL + 3) / 6;
+constexpr int kGCart = 1024;            // Cartesian block entries per triple
+constexpr int kGEA = 128;               // Hermite coefficients of a pair per dimension
+constexpr int kGEC = (kGL + 1) * (kGL + 1);  // ... of a single shell (its partner index fixed at 0)
+
+struct DShell {
+  int l, nprim, ncontr, eoff, coff, foff;
+  double c[3];
+};
+
+struct DItem {
+  int a, b, c;  // b < 0: the unit s function on a's centre
+};
+
+__device__ __forceinline__ int ncart_d(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Index of (t, u, v), t + u + v <= L, in a table ordered by the sum s, then u + v, then v.
+__device__ __forceinline__ int rindex(int t, int u, int v) {
+  const int s = t + u + v, m = u + v;
+  return s * (s + 1) * (s + 2) / 6 + m * (m + 1) / 2 + v;
+}
+
+// Cartesian component k of l (x power descending, then y): (px, py, pz).
+__device__ __forceinline__ void cart_of(int l, int k, int& px, int& py, int& pz) {
+  int i = l, kk = k;
+  while (kk > l - i) {
+    kk -= l - i + 1;
+    --i;
+  }
+  px = i;
+  py = l - i - kk;
+  pz = kk;
+}
+
+__device__ void boys_d(int nmax, double t, double* f) {
+  if (t < 1e-14) {
+    for (int n = 0; n <= nmax; ++n) f[n] = 1.0 / (2 * n + 1) - t / (2 * n + 3);
+    return;
+  }
+  const double et = exp(-t);
+  if (t < 45.0) {
+    double term = 1.0 / (2 * nmax + 1), sum = term;
+    for (int k = 1; k < 400; ++k) {
+      term *= 2.0 * t / (2 * nmax + 2 * k + 1);
+      sum += term;
+      if (term < 1e-17 * sum) break;
+    }
+    f[nmax] = et * sum;
+    for (int n = nmax - 1; n >= 0; --n) f[n] = (2.0 * t * f[n + 1] + et) / (2 * n + 1);
+  } else {
+    f[0] = 0.5 * sqrt(kPi / t) * erf(sqrt(t));
+    for (int n = 0; n < nmax; ++n) f[n + 1] = ((2 * n + 1) * f[n] - et) / (2.0 * t);
+  }
+}
+
+// One dimension of the Hermite expansion of a primitive pair: e[(i (lb + 1) + j) nt + t].
+__device__ void hermite_d(int la, int lb, double a, double b, double xab, double* e) {
+  const int nt = la + lb + 1;
+  for (int k = 0; k < (la + 1) * (lb + 1) * nt; ++k) e[k] = 0.0;
+  const double p = a + b, mu = p > 0.0 ? a * b / p : 0.0;
+  const double xpa = p > 0.0 ? -b / p * xab : 0.0, xpb = p > 0.0 ? a / p * xab : 0.0, i2p = 0.5 / p;
+  auto get = [&](int i, int j, int t) { return (t < 0 || t > i + j) ? 0.0 : e[(i * (lb + 1) + j) * nt + t]; };
+  e[0] = exp(-mu * xab * xab);
+  for (int i = 0; i <= la; ++i)
+    for (int j = 0; j <= lb; ++j) {
+      if (i == 0 && j == 0) continue;
+      for (int t = 0; t <= i + j; ++t) {
+        double v;
+        if (i > 0)
+          v = i2p * get(i - 1, j, t - 1) + xpa * get(i - 1, j, t) + (t + 1) * get(i - 1, j, t + 1);
+        else
+          v = i2p * get(i, j - 1, t - 1) + xpb * get(i, j - 1, t) + (t + 1) * get(i, j - 1, t + 1);
+        e[(i * (lb + 1) + j) * nt + t] = v;
+      }
+    }
+}
+
+struct TripleSmem {
+  double ea[3][kGEA];       // A B pair, per dimension
+  double ec[3][kGEC];       // C alone (its B index fixed at 0)
+  double r[2][kGRtab];      // Hermite Coulomb integrals, two orders
+  double f[kGMaxL + 1];     // Boys values
+  double cart[kGCart];      // contracted Cartesian block
+};
+
+__global__ __launch_bounds__(64) void triple_kernel(const DItem* __restrict__ items, const DShell* __restrict__ orb,
+                                                    const DShell* __restrict__ aux, const double* __restrict__ exps,
+                                                    const double* __restrict__ coefs,
+                                                    const double* __restrict__ norm_o, const double* __restrict__ norm_x,
+                                                    const double* __restrict__ sph, const int* __restrict__ sph_off,
+                                                    int n, int metric_mode, double* __restrict__ out) {
+  __shared__ TripleSmem sm;
+  const int lane = threadIdx.x;
+  const DItem it = items[blockIdx.x];
+  // Shells: A and B from the orbital list (three-centre) or A from the aux list and B the unit
+  // function (metric); C from the aux list.
+  const DShell A = metric_mode ? aux[it.a] : orb[it.a];
+  DShell B;
+  if (it.b >= 0) {
+    B = orb[it.b];
+  } else {
+    B.l = 0;
+    B.nprim = 1;
+    B.ncontr = 1;
+    B.eoff = -1;
+    B.coff = -1;
+    B.foff = 0;
+    B.c[0] = A.c[0];
+    B.c[1] = A.c[1];
+    B.c[2] = A.c[2];
+  }
+  const DShell C = aux[it.c];
+  const int na = ncart_d(A.l), nb = ncart_d(B.l), nc = ncart_d(C.l);
+  const int da = A.ncontr * na, db = B.ncontr * nb, dc = C.ncontr * nc;
+  const int ncomp = na * nb * nc;
+  const int L = A.l + B.l + C.l;
+  const int nta = A.l + B.l + 1, ntc = C.l + 1;
+  for (int k = lane; k < da * db * dc; k += 64) sm.cart[k] = 0.0;
+  const double abx = A.c[0] - B.c[0], aby = A.c[1] - B.c[1], abz = A.c[2] - B.c[2];
+  const double ab2 = abx * abx + aby * aby + abz * abz;
+  for (int i = 0; i < A.nprim; ++i)
+    for (int j = 0; j < B.nprim; ++j) {
+      const double a = exps[A.eoff + i];
+      const double b = B.eoff >= 0 ? exps[B.eoff + j] : 0.0;
+      const double p = a + b;
+      if (exp(-a * b / p * ab2) < kPairCut) continue;
+      const double px = (a * A.c[0] + b * B.c[0]) / p, py = (a * A.c[1] + b * B.c[1]) / p,
+                   pz = (a * A.c[2] + b * B.c[2]) / p;
+      __syncthreads();  // the previous pair's tables are no longer read
+      if (lane < 3) hermite_d(A.l, B.l, a, b, lane == 0 ? abx : lane == 1 ? aby : abz, sm.ea[lane]);
+      for (int k = 0; k < C.nprim; ++k) {
+        const double g = exps[C.eoff + k];
+        const double alpha = p * g / (p + g);
+        const double xq = px - C.c[0], yq = py - C.c[1], zq = pz - C.c[2];
+        __syncthreads();  // the previous triple's R and E_C are no longer read
+        if (lane >= 3 && lane < 6) hermite_d(C.l, 0, g, 0.0, 0.0, sm.ec[lane - 3]);
+        if (lane == 6) boys_d(L, alpha * (xq * xq + yq * yq + zq * zq), sm.f);
+        __syncthreads();
+        // R^(L)_000 .. R^(0)_tuv: order m holds the entries with t + u + v <= L - m.
+        int cur = 0;
+        if (lane == 0) sm.r[0][0] = pow(-2.0 * alpha, L) * sm.f[L];
+        for (int m = L - 1; m >= 0; --m) {
+          __syncthreads();
+          const int top = L - m;
+          const int count = (top + 1) * (top + 2) * (top + 3) / 6;
+          const double fac = pow(-2.0 * alpha, m);
+          double* dst = sm.r[cur ^ 1];
+          const double* src = sm.r[cur];
+          for (int idx = lane; idx < count; idx += 64) {
+            // Decode idx -> (t, u, v) in the rindex order.
+            int s = 0;
+            while ((s + 1) * (s + 2) * (s + 3) / 6 <= idx) ++s;
+            int rem = idx - s * (s + 1) * (s + 2) / 6;
+            int mm = 0;
+            while ((mm + 1) * (mm + 2) / 2 <= rem) ++mm;
+            const int v = rem - mm * (mm + 1) / 2, u = mm - v, t = s - mm;
+            double val;
+            if (s == 0)
+              val = fac * sm.f[m];
+            else if (t > 0)
+              val = (t > 1 ? (t - 1) * src[rindex(t - 2, u, v)] : 0.0) + xq * src[rindex(t - 1, u, v)];
+            else if (u > 0)
+              val = (u > 1 ? (u - 1) * src[rindex(t, u - 2, v)] : 0.0) + yq * src[rindex(t, u - 1, v)];
+            else
+              val = (v > 1 ? (v - 1) * src[rindex(t, u, v - 2)] : 0.0) + zq * src[rindex(t, u, v - 1)];
+            dst[idx] = val;
+          }
+          cur ^= 1;
+        }
+        __syncthreads();
+        const double* R = sm.r[cur];
+        const double pref = 2.0 * pow(kPi, 2.5) / (p * g * sqrt(p + g));
+        for (int q = lane; q < ncomp; q += 64) {
+          const int ia = q / (nb * nc), ib = (q / nc) % nb, ic = q % nc;
+          int ax, ay, az, bx, by, bz, cx, cy, cz;
+          cart_of(A.l, ia, ax, ay, az);
+          cart_of(B.l, ib, bx, by, bz);
+          cart_of(C.l, ic, cx, cy, cz);
+          const double* ex = &sm.ea[0][(ax * (B.l + 1) + bx) * nta];
+          const double* ey = &sm.ea[1][(ay * (B.l + 1) + by) * nta];
+          const double* ez = &sm.ea[2][(az * (B.l + 1) + bz) * nta];
+          const double* gx = &sm.ec[0][cx * ntc];
+          const double* gy = &sm.ec[1][cy * ntc];
+          const double* gz = &sm.ec[2][cz * ntc];
+          double sum = 0.0;
+          for (int t1 = 0; t1 <= ax + bx; ++t1) {
+            const double e1 = ex[t1];
+            if (e1 == 0.0) continue;
+            for (int u1 = 0; u1 <= ay + by; ++u1) {
+              const double e2 = e1 * ey[u1];
+              if (e2 == 0.0) continue;
+              for (int v1 = 0; v1 <= az + bz; ++v1) {
+                const double e3 = e2 * ez[v1];
+                if (e3 == 0.0) continue;
+                for (int t2 = 0; t2 <= cx; ++t2) {
+                  const double f1 = gx[t2];
+                  if (f1 == 0.0) continue;
+                  for (int u2 = 0; u2 <= cy; ++u2) {
+                    const double f2 = f1 * gy[u2];
+                    if (f2 == 0.0) continue;
+                    for (int v2 = 0; v2 <= cz; ++v2) {
+                      const double f3 = f2 * gz[v2];
+                      if (f3 == 0.0) continue;
+                      const double sgn = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
+                      sum += e3 * f3 * sgn * R[rindex(t1 + t2, u1 + u2, v1 + v2)];
+                    }
+                  }
+                }
+              }
+            }
+          }
+          const double prim = pref * sum;
+          for (int ca = 0; ca < A.ncontr; ++ca) {
+            const double wa = coefs[A.coff + ca * A.nprim + i];
+            if (wa == 0.0) continue;
+            for (int cb = 0; cb < B.ncontr; ++cb) {
+              const double wb = B.coff >= 0 ? coefs[B.coff + cb * B.nprim + j] : 1.0;
+              if (wb == 0.0) continue;
+              for (int cc = 0; cc < C.ncontr; ++cc) {
+                const double wc = coefs[C.coff + cc * C.nprim + k];
+                if (wc == 0.0) continue;
+                sm.cart[((ca * na + ia) * db + cb * nb + ib) * dc + cc * nc + ic] += wa * wb * wc * prim;
+              }
+            }
+          }
+        }
+      }
+    }
+  __syncthreads();
+  // Spherical transform of the three indices, normalisation, and the stores.
+  const int sa = 2 * A.l + 1, sbs = 2 * B.l + 1, sc = 2 * C.l + 1;
+  const int fa_n = A.ncontr * sa, fb_n = B.ncontr * sbs, fc_n = C.ncontr * sc;
+  const double* ta = sph + sph_off[A.l];
+  const double* tb = sph + sph_off[B.l];
+  const double* tc = sph + sph_off[C.l];
+  for (int o = lane; o < fa_n * fb_n * fc_n; o += 64) {
+    const int fa = o / (fb_n * fc_n), fb = (o / fc_n) % fb_n, fc = o % fc_n;
+    const int ca = fa / sa, ma = fa % sa, cb = fb / sbs, mb = fb % sbs, cc = fc / sc, mc = fc % sc;
+    double s = 0.0;
+    for (int u = 0; u < na; ++u) {
+      const double wa = ta[ma * na + u];
+      if (wa == 0.0) continue;
+      for (int v = 0; v < nb; ++v) {
+        const double wb = tb[mb * nb + v];
+        if (wb == 0.0) continue;
+        const double* row = &sm.cart[((ca * na + u) * db + cb * nb + v) * dc + cc * nc];
+        for (int w = 0; w < nc; ++w) {
+          const double wc = tc[mc * nc + w];
+          if (wc != 0.0) s += wa * wb * wc * row[w];
+        }
+      }
+    }
+    if (metric_mode) {
+      const long long i = A.foff + fa, jj = C.foff + fc;
+      const double val = s * norm_x[i] * norm_x[jj];
+      out[i * n + jj] = val;
+      out[jj * n + i] = val;
+    } else {
+      const long long ia = A.foff + fa, ib = B.foff + fb, ic = C.foff + fc;
+      const double val = s * norm_o[ia] * norm_o[ib] * norm_x[ic];
+      out[(ic * n + ia) * n + ib] = val;
+      out[(ic * n + ib) * n + ia] = val;
+    }
+  }
+}
+
+// Device copies of the shell tables of one or two bases, the spherical
+// tables, the norms and the item list; freed with the object.
+struct DeviceTables {
+  std::vector<void*> buffers;
+  ~DeviceTables() {
+    for (void* p : buffers) (void)hipFree(p);
+  }
+  template <typename T>
+  T* upload(const std::vector<T>& host, hipStream_t s) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(host.size(), 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) throw std::runtime_error("hipMalloc of the integral tables failed");
+    buffers.push_back(p);
+    if (!host.empty() && hipMemcpyAsync(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
+      throw std::runtime_error("hipMemcpyAsync of the integral tables failed");
+    return (T*)p;
+  }
+};
+
+void device_shells(const std::vector<Shell>& shells, std::vector<DShell>& out, std::vector<double>& exps,
+                   std::vector<double>& coefs) {
+  for (const Shell& s : shells) {
+    if (s.l > kGL) throw std::invalid_argument("the GPU integrals take l <= 5");
+    DShell d;
+    d.l = s.l;
+    d.nprim = (int)s.exps.size();
+    d.ncontr = s.ncontr;
+    d.eoff = (int)exps.size();
+    d.coff = (int)coefs.size();
+    d.foff = s.offset;
+    for (int k = 0; k < 3; ++k) d.c[k] = s.c[k];
+    exps.insert(exps.end(), s.exps.begin(), s.exps.end());
+    coefs.insert(coefs.end(), s.coefs.begin(), s.coefs.end());
+    out.push_back(d);
+  }
+}
+
+// Whether any primitive pair of two shells passes the product cut (the host engine's per-pair test).
+bool pair_survives(const Shell& A, const Shell& B) {
+  const double d2 = (A.c[0] - B.c[0]) * (A.c[0] - B.c[0]) + (A.c[1] - B.c[1]) * (A.c[1] - B.c[1]) +
+                    (A.c[2] - B.c[2]) * (A.c[2] - B.c[2]);
+  for (double a : A.exps)
+    for (double b : B.exps)
+      if (std::exp(-a * b / (a + b) * d2) >= kPairCut) return true;
+  return false;
+}
+
+size_t cart_entries(const Shell& A, int bl, int bcontr, const Shell& C) {
+  return (size_t)A.ncontr * ncart(A.l) * bcontr * ncart(bl) * C.ncontr * ncart(C.l);
+}
+
+void launch_triples(const std::vector<Shell>& orb, const std::vector<Shell>& aux, const std::vector<DItem>& items,
+                    const std::vector<double>& norm_o, const std::vector<double>& norm_x, int n, bool metric_mode,
+                    double* out, hipStream_t s) {
+  std::vector<DShell> dorb, daux;
+  std::vector<double> exps, coefs;
+  device_shells(orb, dorb, exps, coefs);
+  device_shells(aux, daux, exps, coefs);
+  std::vector<double> sph;
+  std::vector<int> sph_off;
+  for (int l = 0; l <= kGL; ++l) {
+    sph_off.push_back((int)sph.size());
+    const auto& t = sph_table(l);
+    sph.insert(sph.end(), t.begin(), t.end());
+  }
+  DeviceTables dev;
+  const DItem* d_items = dev.upload(items, s);
+  const DShell* d_orb = dev.upload(dorb, s);
+  const DShell* d_aux = dev.upload(daux, s);
+  const double* d_exps = dev.upload(exps, s);
+  const double* d_coefs = dev.upload(coefs, s);
+  const double* d_no = dev.upload(norm_o, s);
+  const double* d_nx = dev.upload(norm_x, s);
+  const double* d_sph = dev.upload(sph, s);
+  const int* d_off = dev.upload(sph_off, s);
+  const size_t total = metric_mode ? (size_t)n * n : (size_t)norm_x.size() * n * n;
+  if (hipMemsetAsync(out, 0, total * sizeof(double), s) != hipSuccess) throw std::runtime_error("hipMemsetAsync failed");
+  // At most 2^31 - 1 workgroups per launch.
+  const size_t per = (size_t)1 << 30;
+  for (size_t base = 0; base < items.size(); base += per) {
+    const unsigned count = (unsigned)std::min(per, items.size() - base);
+    hipLaunchKernelGGL(triple_kernel, dim3(count), dim3(64), 0, s, d_items + base, d_orb, d_aux, d_exps, d_coefs,
+                       d_no, d_nx, d_sph, d_off, n, metric_mode ? 1 : 0, out);
+    if (hipGetLastError() != hipSuccess) throw std::runtime_error("launch of the integral kernel failed");
+  }
+  // The tables are freed on return: wait for the kernels that read them.
+  if (hipStreamSynchronize(s) != hipSuccess) throw std::runtime_error("the integral kernel failed");
+}
+
+void check_item_sizes(const std::vector<Shell>& a_list, const std::vector<Shell>& c_list, bool unit_b,
+                      const std::vector<Shell>* b_list) {
+  int la = 0, lb = 0, lc = 0;
+  size_t worst = 0;
+  for (const Shell& s : a_list) la = std::max(la, s.l);
+  for (const Shell& s : c_list) lc = std::max(lc, s.l);
+  if (b_list)
+    for (const Shell& s : *b_list) lb = std::max(lb, s.l);
+  if (la + lb + lc > kGMaxL) throw std::invalid_argument("total angular momentum beyond 12 on the GPU");
+  if ((la + 1) * (lb + 1) * (la + lb + 1) > kGEA)
+    throw std::invalid_argument("a shell pair's Hermite table exceeds the GPU kernel's LDS table");
+  for (const Shell& A : a_list)
+    for (const Shell& C : c_list) {
+      if (unit_b) worst = std::max(worst, cart_entries(A, 0, 1, C));
+      else
+        for (const Shell& B : *b_list) worst = std::max(worst, cart_entries(A, B.l, B.ncontr, C));
+    }
+  if (worst > (size_t)kGCart)
+    throw std::invalid_argument("a shell triple's Cartesian block exceeds the GPU kernel's LDS block");
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfjk_int2c2e_device(hfjk_ctx* ctx, const hfjk_basis* aux, double* metric, void* stream) {
+  return iguard([&] {
+    if (ctx == nullptr) throw std::invalid_argument("null context");
+    if (metric == nullptr) throw std::invalid_argument("null output pointer");
+    int n = 0;
+    const std::vector<Shell> sh = make_shells(aux, n);
+    const std::vector<double> nm = norms(sh, n);
+    check_item_sizes(sh, sh, true, nullptr);
+    std::vector<DItem> items;
+    for (int i = 0; i < (int)sh.size(); ++i)
+      for (int j = 0; j <= i; ++j) items.push_back(DItem{i, -1, j});
+    if (hipSetDevice(ctx->device) != hipSuccess) throw std::runtime_error("hipSetDevice failed");
+    launch_triples(sh, sh, items, nm, nm, n, true, metric, (hipStream_t)stream);
+  });
+}
+
+int hfjk_int3c2e_device(hfjk_ctx* ctx, const hfjk_basis* aux, const hfjk_basis* orbital, double* out, void* stream) {
+  return iguard([&] {
+    if (ctx == nullptr) throw std::invalid_argument("null context");
+    if (out == nullptr) throw std::invalid_argument("null output pointer");
+    int nx = 0, n = 0;
+    const std::vector<Shell> sx = make_shells(aux, nx);
+    const std::vector<Shell> so = make_shells(orbital, n);
+    const std::vector<double> nmx = norms(sx, nx);
+    const std::vector<double> nmo = norms(so, n);
+    check_item_sizes(so, sx, false, &so);
+    std::vector<DItem> items;
+    for (int i = 0; i < (int)so.size(); ++i)
+      for (int j = 0; j <= i; ++j) {
+        if (!pair_survives(so[i], so[j])) continue;
+        for (int c = 0; c < (int)sx.size(); ++c) items.push_back(DItem{i, j, c});
+      }
+    if (hipSetDevice(ctx->device) != hipSuccess) throw std::runtime_error("hipSetDevice failed");
+    launch_triples(so, sx, items, nmo, nmx, n, false, out, (hipStream_t)stream);
+  });
+}
+
+}  // extern "C"

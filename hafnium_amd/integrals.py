@@ -1,4 +1,4 @@
-"""One-electron and density-fitting integrals of a molecule (host, ``libhfjk.so``).
+"""One-electron and density-fitting integrals of a molecule (``libhfjk.so``, host and GPU).
 
 The producer of the SCF inputs: S, T, V (the reference's
 ``IntegralEngine.overlap/kinetic/nuclear_attraction``,
@@ -6,7 +6,10 @@ hafnium/integrals/engine.py:108-377) and the metric ``(P|Q)`` and
 three-centre ``(P|uv)`` of ``DensityFitting.build``
 (hafnium/integrals/density_fitting.py:58-85), and the exact four-centre
 ``(uv|wx)`` of small systems (the spherical atoms of the density guess),
-computed by the library's McMurchie-Davidson engine.
+computed by the library's McMurchie-Davidson engine: on host threads, or
+for ``(P|Q)`` and ``(P|uv)`` also on the GPU (``metric_device``,
+``three_center_device``), where the density fit then runs without a host
+round trip.
 """
 
 from __future__ import annotations
@@ -50,6 +53,41 @@ def three_center(aux: BasisOnAtoms, orbital: BasisOnAtoms):
     out = np.empty((aux.n_functions, orbital.n_functions, orbital.n_functions))
     a, o = aux.c_struct(), orbital.c_struct()
     _lib.check(lib.hfjk_int3c2e(ctypes.byref(a), ctypes.byref(o), _ptr(out)))
+    return out
+
+
+def _device_context(device):
+    import torch
+
+    from hafnium_amd.coulomb_exchange import _Context
+
+    dev = torch.device(device) if device is not None else torch.device("cuda", 0)
+    return dev, _Context(dev.index or 0)
+
+
+def metric_device(aux: BasisOnAtoms, *, device=None):
+    """``(P|Q)`` ``[naux, naux]`` computed on the GPU, as a device tensor."""
+    import torch
+
+    dev, ctx = _device_context(device)
+    out = torch.empty((aux.n_functions, aux.n_functions), dtype=torch.float64, device=dev)
+    b = aux.c_struct()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(ctx.lib.hfjk_int2c2e_device(ctx.handle, ctypes.byref(b), ctypes.c_void_p(out.data_ptr()), stream))
+    return out
+
+
+def three_center_device(aux: BasisOnAtoms, orbital: BasisOnAtoms, *, device=None):
+    """``(P|uv)`` ``[naux, n, n]`` computed on the GPU, as a device tensor."""
+    import torch
+
+    dev, ctx = _device_context(device)
+    n = orbital.n_functions
+    out = torch.empty((aux.n_functions, n, n), dtype=torch.float64, device=dev)
+    a, o = aux.c_struct(), orbital.c_struct()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(ctx.lib.hfjk_int3c2e_device(ctx.handle, ctypes.byref(a), ctypes.byref(o),
+                                           ctypes.c_void_p(out.data_ptr()), stream))
     return out
 
 

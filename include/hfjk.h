@@ -263,6 +263,16 @@ HFJK_API int hfjk_int1e(const hfjk_basis *basis, int natom, const double *charge
 HFJK_API int hfjk_int2c2e(const hfjk_basis *aux, double *metric);
 /* (P|uv) [naux][n][n]. */
 HFJK_API int hfjk_int3c2e(const hfjk_basis *aux, const hfjk_basis *orbital, double *out);
+/* The same (P|Q) and (P|uv) computed on the GPU of `ctx` into device buffers
+ * (row-major, [naux][naux] and [naux][n][n]); the basis structs are host
+ * memory. The parallel three-centre assembly of the reference's engine
+ * (crates/hafnium-core/src/integrals/two_electron.rs:212-310, :944; used by
+ * hafnium/integrals/density_fitting.py:58-85) as one 64-lane workgroup per
+ * shell triple. l <= 5, total angular momentum of a triple <= 12.
+ * Synchronises `stream` (the shell tables are freed on return). */
+HFJK_API int hfjk_int2c2e_device(hfjk_ctx *ctx, const hfjk_basis *aux, double *metric, void *stream);
+HFJK_API int hfjk_int3c2e_device(hfjk_ctx *ctx, const hfjk_basis *aux, const hfjk_basis *orbital, double *out,
+                                 void *stream);
 /* (uv|wx) [n][n][n][n], chemists' notation: the four-centre integrals of the
  * reference's two_electron.eri / IntegralEngine.coulomb_four_center
  * (hafnium/integrals/two_electron.py:26-41, engine.py:280) that

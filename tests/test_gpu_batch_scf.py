@@ -90,8 +90,9 @@ def test_coulomb_from_charges_matches_the_density_build(setup):
         c = rng.standard_normal((x.nao, x.n_alpha)) * 0.1
         d = c @ c.T
         dens.append((d, c))
-        coef = lg.fitted_coefficients(x.metric, x.three_center, d)  # fits 2 D
-        v = torch.from_numpy(x.metric @ coef).to(batch.device).view(-1, 1).contiguous()
+        metric, three_center = x.metric.cpu().numpy(), x.three_center.cpu().numpy()
+        coef = lg.fitted_coefficients(metric, three_center, d)  # fits 2 D
+        v = torch.from_numpy(metric @ coef).to(batch.device).view(-1, 1).contiguous()
         _lib.check(batch.lib.hfjk_dtrsm_lower(batch._ctx.handle, x.naux, 1, ctypes_ptr(f), ctypes_ptr(v),
                                               batch._stream()))
         charges.append(v.view(-1))
