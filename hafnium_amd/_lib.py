@@ -59,6 +59,7 @@ SIGNATURES = [
     ("hfjk_coulomb", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_coulomb_charges", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_exchange_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_exchange_occ_noalloc", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_density_factor", _c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
     ("hfjk_exchange_general", _c_int, [_vp, _vp, _vp, _vp, _c_size, _vp]),

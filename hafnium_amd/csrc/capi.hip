@@ -9,9 +9,17 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <new>
 
 #include "hfjk_internal.h"
+
+// Scratch of the allocation-free exchange (hfjk_exchange_occ_noalloc): one
+// copy per GPU the library runs on, held until the process ends. Host code
+// never passes this name to a kernel; it uses the per-device address that
+// hipGetSymbolAddress returns at context creation.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {
 
@@ -122,7 +130,7 @@ struct Layout {
   long long p, ntiles;
 };
 
-Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
+Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision, int nsplit_k = 0) {
   Layout L{};
   const int n = ctx->nao, naux = ctx->naux;
   const size_t es = elem_size(precision);
@@ -134,7 +142,9 @@ Layout make_layout(const hfjk_ctx* ctx, int nocc, int precision) {
   (void)nb;
   L.npairs = hfjk::syrk_pairs(n);
   L.nsplit_j = coulomb_splits(L.ntiles, naux);
-  L.nsplit_k = ctx->syrk_splits > 0 ? ((ctx->syrk_splits + 7) / 8) * 8 : syrk_splits(L.npairs, naux);
+  L.nsplit_k = nsplit_k > 0 ? nsplit_k
+               : ctx->syrk_splits > 0 ? ((ctx->syrk_splits + 7) / 8) * 8
+                                       : syrk_splits(L.npairs, naux);
   L.nslot_k = hfjk::syrk_slots(precision, naux, L.ldo, L.nsplit_k);
   size_t off = 0;
   const size_t nn = (size_t)n * n;
@@ -192,12 +202,16 @@ T* at(void* ws, size_t off) {
 constexpr size_t kMaxStages = 4096;
 
 // RAII stage timer: records an event pair around the launches in its scope.
+// Set for the duration of an allocation-free call: stage timers (which
+// create events on first use) stay off.
+thread_local bool g_no_alloc_call = false;
+
 struct Stage {
   hfjk_ctx* ctx;
   hipStream_t s;
   hfjk::StageTimer* timer = nullptr;
   Stage(hfjk_ctx* c, hipStream_t stream, const char* name) : ctx(c), s(stream) {
-    if (!ctx->profile || ctx->n_stages >= kMaxStages) return;
+    if (!ctx->profile || g_no_alloc_call || ctx->n_stages >= kMaxStages) return;
     if (ctx->n_stages == ctx->stages.size()) {
       hfjk::StageTimer t;
       check_hip(hipEventCreate(&t.start), "hipEventCreate");
@@ -453,6 +467,32 @@ __global__ void density_stats_kernel(const double* __restrict__ d, int n, unsign
 
 }  // namespace
 
+namespace {
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (each GPU holds its own copy of the array).
+void* noalloc_scratch(int device) {
+  static std::mutex lock;
+  static void* addresses[64] = {};
+  if (device < 0 || device >= 64) fail(HFJK_ERR_INPUT, "device index beyond 63");
+  std::lock_guard<std::mutex> guard(lock);
+  if (addresses[device] == nullptr) {
+    check_hip(hipSetDevice(device), "hipSetDevice");
+    void* ptr = nullptr;
+    check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+    if (ptr == nullptr) fail(HFJK_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+    addresses[device] = ptr;
+  }
+  return addresses[device];
+}
+
+__global__ void zero_kernel(double* __restrict__ out, long long count) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256)
+    out[i] = 0.0;
+}
+
+}  // namespace
+
 void hfjk_internal_set_error(const char* message) { g_last_error = message != nullptr ? message : ""; }
 
 extern "C" {
@@ -473,9 +513,11 @@ int hfjk_create(hfjk_ctx** out, int device) {
     check_hip(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
       fail(HFJK_ERR_BACKEND, std::string("the DF-J/K kernels are built for gfx950 (MI355X), not ") + prop.gcnArchName);
+    void* scratch_ptr = noalloc_scratch(device);
     hfjk_ctx* ctx = new hfjk_ctx();
     ctx->device = device;
     ctx->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    ctx->scratch = scratch_ptr;
     *out = ctx;
   });
 }
@@ -672,6 +714,41 @@ int hfjk_exchange_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, const dou
       return;
     }
     run_exchange(ctx, orbitals, nocc, signs, exchange, precision, workspace, L, s);
+  });
+}
+
+int hfjk_exchange_occ_noalloc(hfjk_ctx* ctx, const double* orbitals, int nocc, const double* signs,
+                              double* exchange, int precision, void* stream) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_precision(precision);
+    check_bound(ctx, precision);
+    if (nocc < 0) fail(HFJK_ERR_INPUT, "negative nocc");
+    if (exchange == nullptr) fail(HFJK_ERR_INPUT, "null output pointer");
+    if (nocc > 0 && orbitals == nullptr) fail(HFJK_ERR_INPUT, "null orbitals pointer");
+    if (ctx->scratch == nullptr) fail(HFJK_ERR_BACKEND, "the context holds no scratch address");
+    const int kprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;
+    // The workspace layout of hfjk_exchange_occ, with the fewest SYRK splits
+    // (8) when the usual count does not fit the scratch.
+    Layout L = make_layout(ctx, std::max(nocc, 1), kprec);
+    if (L.x_end > kNoAllocScratchBytes) L = make_layout(ctx, std::max(nocc, 1), kprec, 8);
+    if (L.x_end > kNoAllocScratchBytes)
+      fail(HFJK_ERR_WORKSPACE, "this exchange needs " + std::to_string(L.x_end) +
+                                   " bytes of scratch; the allocation-free entry point holds " +
+                                   std::to_string(kNoAllocScratchBytes) + " (use hfjk_exchange_occ with a workspace)");
+    begin_call(ctx);
+    hipStream_t s = (hipStream_t)stream;
+    g_no_alloc_call = true;
+    struct Reset {
+      ~Reset() { g_no_alloc_call = false; }
+    } reset;
+    if (nocc == 0) {
+      const long long count = (long long)ctx->nao * ctx->nao;
+      hipLaunchKernelGGL(zero_kernel, dim3(hfjk::grid_for(count)), dim3(256), 0, s, exchange, count);
+      check_hip(hipGetLastError(), "zero launch");
+      return;
+    }
+    run_exchange(ctx, orbitals, nocc, signs, exchange, precision, ctx->scratch, L, s);
   });
 }
 

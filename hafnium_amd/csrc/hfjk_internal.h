@@ -93,6 +93,9 @@ struct hfjk_ctx {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr;
   hipEvent_t join = nullptr;
+  // Device address of the library's fixed scratch array on this device
+  // (hfjk_exchange_occ_noalloc), looked up at context creation.
+  void* scratch = nullptr;
 };
 
 namespace hfjk {

@@ -146,6 +146,18 @@ HFJK_API int hfjk_exchange_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, 
                       double *exchange, int precision, void *workspace, size_t workspace_bytes,
                       void *stream);
 
+/* hfjk_exchange_occ without a workspace argument and without any device
+ * allocation. No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Its kernels use a fixed 256 MiB scratch array that
+ * the library holds on each GPU until the process ends (the layout of
+ * hfjk_exchange_occ, with 8 SYRK splits when the usual count does not fit);
+ * a larger problem fails with HFJK_ERR_WORKSPACE. Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other
+ * threads, other contexts) race. Nothing in the scratch must start at zero. */
+HFJK_API int hfjk_exchange_occ_noalloc(hfjk_ctx *ctx, const double *orbitals, int nocc, const double *signs,
+                                       double *exchange, int precision, void *stream);
+
 /* J and K of the density D = C C^T of C = orbitals [nao][nocc] in one call:
  * coulomb_exchange_matrices (coulomb_exchange.py:397-404) for an aufbau
  * density, i.e. hfjk_coulomb(D) and hfjk_exchange_occ(C, NULL). The charges
