@@ -470,24 +470,28 @@ def main(argv=None):
     torch.cuda.synchronize()
     log(f"generated B ({q}x{n}x{n}, packed {jk.packed_tensor().numel() * 8 / 1e9:.1f} GB fp64) in "
         f"{time.perf_counter() - t0:.2f} s")
-    # The full row-major FP64 copy of B (both triangles) for the direct-LDS half-transform, when it
-    # fits beside the packed tensor, its FP32/BF16 copies and the workspace (a 60 GB margin).
+    # A full row-major copy of B (both triangles) per mode for the direct-LDS half-transforms (FP64:
+    # 8 bytes per element, FP32: 4, BF16x3: hi + lo planes, 4), made just before the mode runs and
+    # dropped after it, when it fits beside the packed tensor, its FP32/BF16 copies and the
+    # workspace (a 60 GB margin).
     npad = -(-n // 32) * 32
-    full_bytes = 8 * q * npad * npad
-    use_full = not args.no_full_copy and torch.cuda.mem_get_info(dev)[0] > full_bytes + 60e9
-    if use_full:
-        t1 = time.perf_counter()
-        jk.expand_full()
+    full_bytes = {"fp64": 8 * q * npad * npad, "fp32": 4 * q * npad * npad, "bf16x3": 4 * q * npad * npad}
+    use_full = {}
+
+    def prepare_full(mode):
+        # BF16x3 keeps its packed half-transform: on the bf16 ring from the full copy it measured
+        # 29.1 ms against 21.9 ms at this workload (the ring's global-to-LDS traffic per flop).
+        if args.no_full_copy or mode == "bf16x3":
+            use_full[mode] = False
+            return
         torch.cuda.synchronize()
-        log(f"full FP64 copy of B ({full_bytes / 1e9:.1f} GB) in {time.perf_counter() - t1:.2f} s")
-    # The same for the FP32-MFMA mode (half the bytes), when it still fits.
-    use_full32 = (not args.no_full_copy and "fp32" in modes
-                  and torch.cuda.mem_get_info(dev)[0] > full_bytes / 2 + 60e9)
-    if use_full32:
-        t1 = time.perf_counter()
-        jk.expand_full("fp32")
-        torch.cuda.synchronize()
-        log(f"full FP32 copy of B ({full_bytes / 2e9:.1f} GB) in {time.perf_counter() - t1:.2f} s")
+        torch.cuda.empty_cache()
+        use_full[mode] = torch.cuda.mem_get_info(dev)[0] > full_bytes[mode] + 60e9
+        if use_full[mode]:
+            t1 = time.perf_counter()
+            jk.expand_full(mode)
+            torch.cuda.synchronize()
+            log(f"full {mode} copy of B ({full_bytes[mode] / 1e9:.1f} GB) in {time.perf_counter() - t1:.2f} s")
 
     f_build = flops_per_build(n, q, o)
     results = {}
@@ -496,6 +500,7 @@ def main(argv=None):
 
     for mode in modes:
         jk.precision = mode
+        prepare_full(mode)
         tuning = None
         if not args.no_tune:
             tuned = autotune(jk, c, mode, cache_path=args.tune_cache)
@@ -581,8 +586,11 @@ def main(argv=None):
         s_b = 8 if mode == "fp64" else 4
         t_min = max(f_build / (PEAK_TFLOPS[mode] * 1e12), bytes_per_build(n, q, o, s_b, s_b) / (HBM_PEAK_GBS * 1e9))
         entry["roofline_frac_build"] = t_min / (elapsed / args.steps)
+        entry["b_storage"] = "packed tiles + full row-major copy" if use_full[mode] else "packed tiles"
         results[mode] = entry
         log(f"{mode}: {entry['builds_per_s']:.3f} builds/s ({entry['ms_per_build']:.1f} ms); stages {entry['stage_ms']}")
+        if use_full[mode]:
+            jk.drop_full(mode)
 
     head = "fp64" if "fp64" in results else modes[0]
     main_entry = results[head]
@@ -603,11 +611,14 @@ def main(argv=None):
                   "bf16x3": f"xform_bf16x3_kernel<{tile}>"}[head]
         # B bytes per launch: the packed lower triangle, or with the full copy both triangles.
         b_bytes = q * n * (n + 1) / 2 * (8 if head == "fp64" else 4)
-        if head == "fp64" and use_full:
+        if head == "fp64" and use_full.get("fp64"):
             kernel = "xform_dma_kernel<double, 1>"
             b_bytes = q * n * n * 8.0
-        elif head == "fp32" and use_full32:
+        elif head == "fp32" and use_full.get("fp32"):
             kernel = "xform_dma_kernel<float, 1>"
+            b_bytes = q * n * n * 4.0
+        elif head == "bf16x3" and use_full.get("bf16x3"):
+            kernel = "xform16_dma_kernel"
             b_bytes = q * n * n * 4.0
         roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved, "peak": peak,
                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(kernel, n, q, o),
@@ -652,9 +663,7 @@ def main(argv=None):
         "config": {"workload": f"single DF-J/K build nao={n} naux={q} nocc={o} seed={wl['seed']}",
                    "nao": n, "naux": q, "nocc": o,
                    "step": "J + K of D = C C^T in one call (charges from X)",
-                   "fp64_b_storage": "packed tiles + full row-major copy" if use_full else "packed tiles",
-                   "fp32_b_storage": ("packed tiles + full row-major copy" if use_full32 else "packed tiles")
-                   if "fp32" in modes else None},
+                   "b_storage": {m: results[m]["b_storage"] for m in results}},
         "flops_per_build": f_build,
         "roofline": roofline,
         "cpu_baseline": cpu,

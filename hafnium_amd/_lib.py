@@ -62,7 +62,8 @@ SIGNATURES = [
     ("hfjk_exchange_occ_noalloc", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp]),
     ("hfjk_exchange_density", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_density_factor", _c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
-    ("hfjk_exchange_general", _c_int, [_vp, _vp, _vp, _vp, _c_size, _vp]),
+    ("hfjk_exchange_general", _c_int, [_vp, _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    ("hfjk_general_workspace_bytes", _c_int, [_vp, _c_int, _c_int, ctypes.POINTER(_c_size)]),
     ("hfjk_jk_occ", _c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp]),
     ("hfjk_expand_full", _c_int, [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp]),
     ("hfjk_bind_full", _c_int, [_vp, _c_int, _vp]),

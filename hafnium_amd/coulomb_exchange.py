@@ -172,8 +172,8 @@ class MI355XCoulombExchange:
             _lib.check(lib.hfjk_bind_full(self._ctx.handle, code, _ptr(full)))
 
     def expand_full(self, precision: str = "fp64") -> None:
-        """Keep a full row-major copy of B in ``precision`` ("fp64" or "fp32"; ``naux npad^2`` elements,
-        npad = 32 ceil(nao/32)).
+        """Keep a full row-major copy of B in ``precision`` ("fp64", "fp32" or "bf16x3"; ``naux npad^2``
+        elements, npad = 32 ceil(nao/32); for "bf16x3" the hi plane, then the lo plane).
 
         The half-transform of that precision then streams both of its operands
         straight into LDS (13 % faster in FP64 and 7 % in FP32 at the large
@@ -183,13 +183,15 @@ class MI355XCoulombExchange:
         """
         torch = _torch()
         code = _lib.precision_code(precision)
-        if code not in (_lib.FP64, _lib.FP32):
-            raise InputError("full copies are fp64 or fp32")
         if code in self._full:
             return
         npad = -(-self._nao // 32) * 32
-        dtype = torch.float64 if code == _lib.FP64 else torch.float32
-        full = torch.empty(self._naux * npad * npad, dtype=dtype, device=self._device)
+        count = self._naux * npad * npad
+        if code == _lib.BF16X3:
+            full = torch.empty(2 * count, dtype=torch.int16, device=self._device)
+        else:
+            dtype = torch.float64 if code == _lib.FP64 else torch.float32
+            full = torch.empty(count, dtype=dtype, device=self._device)
         _lib.check(self._ctx.lib.hfjk_expand_full(self._ctx.handle, _ptr(self._b64), self._naux, self._nao, code,
                                                   _ptr(full), self._stream()))
         self._full[code] = full
@@ -314,6 +316,27 @@ class MI355XCoulombExchange:
                                                       self._stream()))
         return out
 
+    GENERAL_WORKSPACE_LIMIT = 8 << 30
+    """Bytes of workspace the non-symmetric exchange may use (more aux functions per pass)."""
+
+    def _general_workspace(self, prec: int):
+        """A workspace for the non-symmetric exchange: all aux functions in one pass when that fits
+        ``GENERAL_WORKSPACE_LIMIT``, else as many as fit (at least one)."""
+        torch = _torch()
+        lib = self._ctx.lib
+        size = ctypes.c_size_t()
+        _lib.check(lib.hfjk_general_workspace_bytes(self._ctx.handle, prec, 0, ctypes.byref(size)))
+        need = int(size.value)
+        if need > self.GENERAL_WORKSPACE_LIMIT:
+            _lib.check(lib.hfjk_general_workspace_bytes(self._ctx.handle, prec, 1, ctypes.byref(size)))
+            need = max(int(size.value), self.GENERAL_WORKSPACE_LIMIT)
+        if self._workspace is None or self._workspace.numel() < need + 256:
+            self._workspace = None
+            self._workspace = torch.empty(need + 256, dtype=torch.uint8, device=self._device)
+        base = self._workspace.data_ptr()
+        aligned = (base + 255) & ~255
+        return ctypes.c_void_p(aligned), self._workspace.numel() - (aligned - base)
+
     def _exchange_real(self, d, prec: int):
         """K of one real density: the signed factor of a symmetric D, then the occupied-form
         exchange with a workspace sized for the retained rank; the FP64 density form otherwise."""
@@ -329,9 +352,9 @@ class MI355XCoulombExchange:
         r = int(rank.value)
         out = torch.empty((n, n), dtype=torch.float64, device=self._device)
         if r < 0:
-            if prec != _lib.FP64:
-                raise BackendError("a non-symmetric density is contracted in FP64 only")
-            _lib.check(lib.hfjk_exchange_general(self._ctx.handle, _ptr(d), _ptr(out), ws, nbytes, self._stream()))
+            ws, nbytes = self._general_workspace(prec)
+            _lib.check(lib.hfjk_exchange_general(self._ctx.handle, _ptr(d), _ptr(out), prec, ws, nbytes,
+                                                 self._stream()))
             return out
         if r == 0:
             return out.zero_()
@@ -347,9 +370,10 @@ class MI355XCoulombExchange:
         A symmetric density is factorised on the GPU (``D = C_+ C_+^T -
         C_- C_-^T``) and the occupied-form kernels run on the factors in the
         chosen precision, so indefinite and fractionally occupied densities
-        give the density-form result. A non-symmetric real density takes the
-        density form itself (two FP64 DGEMMs per chunk of aux functions; other
-        precisions raise ``BackendError``). A complex density gives
+        give the density-form result. A non-symmetric real density D = S + A
+        goes through the Hermitian matrix S + i A: its eigenvectors give one
+        half-transform and two SYRKs (B S B and B A B) in the chosen
+        precision. A complex density gives
         ``K[Re D] + i K[Im D]``: for a Hermitian one the real part is
         symmetric and the imaginary part antisymmetric.
         """

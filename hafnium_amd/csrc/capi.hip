@@ -274,16 +274,18 @@ void exchange_front(hfjk_ctx* ctx, const double* orbitals, int nocc, const doubl
   // FP64 / FP32 with a bound full copy of B: the half-transform on the direct-LDS ring.
   const void* bfull = precision == HFJK_FP64 ? (const void*)ctx->b64_full
                       : precision == HFJK_FP32 ? (const void*)ctx->b32_full
-                                               : nullptr;
+                                               : (const void*)ctx->b16_full;
   const bool full = bfull != nullptr && ctx->x_full == 0;
-  void* ct = at<void>(ws, L.xct);
+  // BF16X3 takes C^T's bf16 planes (cp) as the ring's second operand.
+  void* ct = split ? cp : at<void>(ws, L.xct);
   {
     Stage st(ctx, s, "exchange_prepare");
     if (split)
       check_hip(hfjk::launch_orbitals_bf16x3(orbitals, ctx->nao, nocc, cp, s), "orbitals launch");
     else
       check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, precision, cp, s), "orbitals launch");
-    if (full) check_hip(hfjk::launch_orbitals_t(orbitals, ctx->nao, nocc, precision, ct, s), "orbitals launch");
+    if (full && !split)
+      check_hip(hfjk::launch_orbitals_t(orbitals, ctx->nao, nocc, precision, ct, s), "orbitals launch");
     check_hip(hfjk::launch_occ_signs(signs, nocc, L.ldo, sg, s), "signs launch");
   }
   {
@@ -347,98 +349,192 @@ rocblas_handle blas_handle(hfjk_ctx* ctx) {
   return (rocblas_handle)ctx->blas;
 }
 
-void check_blas(rocblas_status status, const char* what) {
-  if (status != rocblas_status_success)
-    fail(HFJK_ERR_SOLVER, std::string(what) + " failed (rocblas status " + std::to_string((int)status) + ")");
+// Layout of the non-symmetric exchange from offset `base` of the workspace,
+// for aux chunks of `pc` functions and column blocks of at most kGeneralCols
+// retained eigenpairs.
+constexpr int kGeneralCols = 1024;
+
+struct GeneralLayout {
+  size_t hz, w, e, info, c, sym, cross, cp, ct, x, p1, p2, total;
+  int ld, npairs, nsplit, nslot, xrows;
+};
+
+GeneralLayout general_layout(const hfjk_ctx* ctx, int precision, int rank, int pc, size_t base) {
+  GeneralLayout G{};
+  const size_t n = (size_t)ctx->nao;
+  const size_t npad = (size_t)hfjk::n_tiles(ctx->nao) * hfjk::kTile;
+  const size_t es = precision == HFJK_FP64 ? 8 : 4;
+  const int rb = std::max(1, std::min(rank, kGeneralCols));
+  G.ld = 2 * hfjk::occ_ld(rb);
+  G.xrows = hfjk::x_rows(ctx->nao);
+  G.npairs = hfjk::syrk_pairs(ctx->nao);
+  // Split counts capped so the two partial sets stay small for small matrices.
+  G.nsplit = std::min(syrk_splits(G.npairs, pc), std::max(8, ((512 / G.npairs + 7) / 8) * 8));
+  G.nslot = hfjk::syrk_slots(precision, pc, G.ld, G.nsplit);
+  const size_t tile2 = (size_t)hfjk::syrk_tile() * hfjk::syrk_tile();
+  size_t off = base;
+  G.hz = off; off = align_up(off + n * n * 16);
+  G.w = off; off = align_up(off + n * 8);
+  G.e = off; off = align_up(off + n * 8);
+  G.info = off; off = align_up(off + 8);
+  G.c = off; off = align_up(off + n * G.ld * 8);
+  G.sym = off; off = align_up(off + (size_t)G.ld * 8);
+  G.cross = off; off = align_up(off + (size_t)G.ld * 8);
+  G.cp = off; off = align_up(off + npad * G.ld * es);
+  G.ct = off; off = align_up(off + npad * G.ld * es);
+  G.x = off; off = align_up(off + (size_t)pc * G.xrows * G.ld * es);
+  G.p1 = off; off = align_up(off + (size_t)G.nslot * G.npairs * tile2 * es);
+  G.p2 = off; off = align_up(off + (size_t)G.nslot * G.npairs * tile2 * es);
+  G.total = off;
+  return G;
 }
 
-// out = (accumulate ? out : 0) + sum over slices of part[slice][i].
-__global__ void sum_slices_kernel(const double* __restrict__ part, int nslices, long long count, int accumulate,
-                                  double* __restrict__ out) {
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256) {
-    double v = accumulate ? out[i] : 0.0;
-    for (int sl = 0; sl < nslices; ++sl) v += part[(long long)sl * count + i];
-    out[i] = v;
+__global__ void zero_fill_kernel(double* __restrict__ out, long long count) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256)
+    out[i] = 0.0;
+}
+
+// K = sum_P B_P D B_P for a real non-symmetric D (the density form of
+// coulomb_exchange.py:184-190) on the library's own kernels, in any
+// precision. With S and A the symmetric and antisymmetric parts of D,
+// H = S + i A is Hermitian, H = V diag(l) V^H (rocSOLVER zheevd, |l| <= 1e-12
+// max |l| dropped), and for a symmetric B_P and X = B_P V |l|^1/2:
+// B_P H B_P = X diag(sign l) X^H, whose real part is B_P S B_P and whose
+// imaginary part is B_P A B_P. With X' = B [Re, 0, Im, 0] of those columns
+// (the occupied-form half-transform on 2 hb columns):
+//   B S B = sum_k X'[m][k] s_k X'[n][k]                  (the signed SYRK),
+//   B A B = sum_k X'[m][k] t_k X'[n][(k + hb) mod 2 hb]  (its column-shifted form),
+// s = (sign l, sign l), t = (-sign l, sign l) over the two halves, and the
+// lower tiles give K[m][n] = S + A, their mirrors K[n][m] = S - A. Retained
+// eigenpairs in blocks of at most kGeneralCols (2 hb <= 2048, the ring's
+// limit), aux functions in chunks that fit the workspace beyond `base`;
+// every block and chunk adds into K. 16 n^2 naux r flops for rank r.
+void exchange_general(hfjk_ctx* ctx, const double* density, double* out, int precision, void* ws, size_t ws_bytes,
+                      size_t base, hipStream_t s) {
+  check_bound(ctx, precision);
+  const int n = ctx->nao, naux = ctx->naux;
+  const long long nn = (long long)n * n;
+  const long long p = hfjk::packed_elems(n);
+  const long long npad = (long long)hfjk::n_tiles(n) * hfjk::kTile;
+  // Chunk of aux functions: the most that fit (for full-rank column blocks).
+  GeneralLayout G = general_layout(ctx, precision, n, 1, base);
+  if (G.total > ws_bytes)
+    fail(HFJK_ERR_WORKSPACE, "workspace too small for the non-symmetric exchange (" + std::to_string(G.total) +
+                                 " bytes for one aux function)");
+  int pc = 1;
+  for (int lo = 1, hi = naux; lo <= hi;) {
+    const int mid = lo + (hi - lo) / 2;
+    if (general_layout(ctx, precision, n, mid, base).total <= ws_bytes) {
+      pc = mid;
+      lo = mid + 1;
+    } else {
+      hi = mid - 1;
+    }
   }
-}
-
-// K = sum_P B_P D B_P for any real D in FP64 (the density form of
-// coulomb_exchange.py:184-190), for densities the factorisation does not
-// cover. Per chunk of aux functions: F = full rows of B_P (npad x npad,
-// symmetric, zero padding), G_P = D F_P (one strided-batched DGEMM with D
-// shared by the batch), then sum_{P,w} F_P[w][u] G_P[w][v] over k = (P, w)
-// (F_P[w][u] = F_P[u][w]), split into slices of P by a second
-// strided-batched DGEMM (a lone n x n output with k = naux npad would fill
-// few CUs) and summed into K. Row-major arrays are read as the column-major
-// transposes rocBLAS sees. 4 naux npad^3 flops.
-void exchange_general(hfjk_ctx* ctx, const double* density, double* out, void* ws, size_t ws_bytes,
-                      size_t dens_end, hipStream_t s) {
-  if (ctx->b64 == nullptr) fail(HFJK_ERR_BACKEND, "a non-symmetric density needs the bound FP64 tensor");
-  const int n = ctx->nao;
-  const int npad = hfjk::n_tiles(n) * hfjk::kTile;
-  const size_t nn = (size_t)npad * npad;
-  const size_t kk = (size_t)n * n;
-  const size_t mat = align_up(nn * 8);
-  // At most 64 slices of at most 2^26 partial elements in all.
-  const int max_slices = (int)std::min<size_t>({(size_t)64, std::max<size_t>(1, ((size_t)1 << 26) / kk),
-                                                (size_t)ctx->naux});
-  const size_t parts_bytes = align_up((size_t)max_slices * kk * 8);
-  const size_t avail = ws_bytes > dens_end ? ws_bytes - dens_end : 0;
-  const bool have_full = ctx->b64_full != nullptr && ctx->x_full == 0;
-  // D padded, the slice partials, then the G and F chunks (F only without a bound full copy).
-  const size_t per_q = (have_full ? 1 : 2) * nn * 8;
-  if (avail < mat + parts_bytes + per_q + 512)
-    fail(HFJK_ERR_WORKSPACE, "workspace too small for the general exchange");
-  int chunk = (int)std::min<size_t>((size_t)ctx->naux, (avail - mat - parts_bytes - 512) / per_q);
-  if (ctx->general_chunk > 0) chunk = std::min(chunk, ctx->general_chunk);
-  double* dpad = at<double>(ws, dens_end);
-  double* parts = at<double>(ws, dens_end + mat);
-  double* g = at<double>(ws, dens_end + mat + parts_bytes);
-  double* f = at<double>(ws, dens_end + mat + parts_bytes + align_up((size_t)chunk * nn * 8));
-  rocblas_handle h = blas_handle(ctx);
-  check_blas(rocblas_set_stream(h, s), "rocblas_set_stream");
-  check_blas(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host), "rocblas_set_pointer_mode");
-  check_hip(hfjk::launch_orbitals(density, n, n, HFJK_FP64, dpad, s), "density pad launch");
-  const double one = 1.0, zero = 0.0;
-  for (int q0 = 0; q0 < ctx->naux; q0 += chunk) {
-    const int pc = std::min(chunk, ctx->naux - q0);
-    const double* fq = have_full ? ctx->b64_full + (size_t)q0 * nn : f;
-    if (!have_full) {
-      Stage st(ctx, s, "general_expand");
-      const long long p = hfjk::packed_elems(n);
-      check_hip(hfjk::launch_expand_full(ctx->b64 + (size_t)q0 * p, p, pc, n, HFJK_FP64, f, s),
-                "expand launch");
-    }
+  if (ctx->general_chunk > 0) pc = std::min(pc, ctx->general_chunk);
+  G = general_layout(ctx, precision, n, pc, base);
+  auto* hz = at<rocblas_double_complex>(ws, G.hz);
+  double* w = at<double>(ws, G.w);
+  double* e = at<double>(ws, G.e);
+  rocblas_int* info = at<rocblas_int>(ws, G.info);
+  {
+    Stage st(ctx, s, "general_eigh");
+    check_hip(hfjk::launch_hermitian_of_real(density, n, hz, s), "hermitian launch");
+    rocblas_handle h = blas_handle(ctx);
+    if (rocblas_set_stream(h, s) != rocblas_status_success) fail(HFJK_ERR_SOLVER, "rocblas_set_stream failed");
+    // The column-major view of the row-major Hermitian H is conj(H); its
+    // eigenvectors are conjugated when the columns are formed.
+    if (rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n, hz, n, w, e, info) !=
+        rocblas_status_success)
+      fail(HFJK_ERR_SOLVER, "rocsolver_zheevd failed to launch");
+  }
+  std::vector<double> vals(n);
+  rocblas_int host_info = 0;
+  check_hip(hipMemcpyAsync(vals.data(), w, sizeof(double) * n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  check_hip(hipMemcpyAsync(&host_info, info, sizeof(host_info), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (host_info != 0) fail(HFJK_ERR_SOLVER, "rocsolver_zheevd did not converge (info " + std::to_string(host_info) + ")");
+  double top = 0.0;
+  for (double v : vals) top = std::max(top, std::fabs(v));
+  const double cut = 1e-12 * top;
+  int nneg = 0, npos = 0;
+  for (int j = 0; j < n && vals[j] < -cut; ++j) ++nneg;
+  for (int j = n - 1; j >= nneg && vals[j] > cut; --j) ++npos;
+  const int r = nneg + npos;
+  if (r == 0) {
+    hipLaunchKernelGGL(zero_fill_kernel, dim3(hfjk::grid_for(nn)), dim3(256), 0, s, out, nn);
+    check_hip(hipGetLastError(), "zero launch");
+    return;
+  }
+  const bool split = precision == HFJK_BF16X3;
+  const void* bfull = precision == HFJK_FP64 ? (const void*)ctx->b64_full
+                      : precision == HFJK_FP32 ? (const void*)ctx->b32_full
+                                               : (const void*)ctx->b16_full;
+  const bool full = bfull != nullptr && ctx->x_full == 0;
+  const void* b = bound_tensor(ctx, precision);
+  const size_t es = precision == HFJK_FP64 ? 8 : 4;
+  const int rprec = precision == HFJK_FP64 ? HFJK_FP64 : HFJK_FP32;
+  double* c = at<double>(ws, G.c);
+  double* sym = at<double>(ws, G.sym);
+  double* cross = at<double>(ws, G.cross);
+  void* cp = at<void>(ws, G.cp);
+  void* ct = split ? cp : at<void>(ws, G.ct);
+  void* x = at<void>(ws, G.x);
+  void* p1 = at<void>(ws, G.p1);
+  void* p2 = at<void>(ws, G.p2);
+  bool accumulate = false;
+  for (int j0 = 0; j0 < r; j0 += kGeneralCols) {
+    const int rb = std::min(kGeneralCols, r - j0);
+    const int hb = hfjk::occ_ld(rb);
+    const int ld = 2 * hb;
     {
-      // Column-major G_P^T = F_P D^T, i.e. row-major G_P = D F_P.
-      Stage st(ctx, s, "general_dgemm_half");
-      check_blas(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, npad, npad, npad,
-                                               &one, fq, npad, (rocblas_stride)nn, dpad, npad, 0, &zero, g, npad,
-                                               (rocblas_stride)nn, pc),
-                 "rocblas_dgemm_strided_batched");
+      Stage st(ctx, s, "general_prepare");
+      check_hip(hfjk::launch_general_factor(hz, w, n, nneg, npos, j0, rb, hb, c, sym, cross, s),
+                "general factor launch");
+      if (split)
+        check_hip(hfjk::launch_orbitals_bf16x3(c, n, ld, cp, s), "orbitals launch");
+      else if (full)
+        check_hip(hfjk::launch_orbitals_t(c, n, ld, precision, ct, s), "orbitals launch");
+      else
+        check_hip(hfjk::launch_orbitals(c, n, ld, precision, cp, s), "orbitals launch");
     }
-    // Slices of ps aux functions; the last may be shorter.
-    const int ps = (pc + std::min(max_slices, pc) - 1) / std::min(max_slices, pc);
-    const int nfull = pc / ps, rem = pc - nfull * ps;
-    {
-      // Column-major partial^T[v][u] = sum_k G[v][k] F[u][k] over the slice's k = (P, w).
-      Stage st(ctx, s, "general_dgemm_sum");
-      const rocblas_stride sk = (rocblas_stride)ps * nn;
-      check_blas(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, n, n,
-                                               ps * npad, &one, g, npad, sk, fq, npad, sk, &zero, parts, n,
-                                               (rocblas_stride)kk, nfull),
-                 "rocblas_dgemm_strided_batched");
-      if (rem > 0)
-        check_blas(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, rem * npad, &one,
-                                 g + (size_t)nfull * sk, npad, fq + (size_t)nfull * sk, npad, &zero,
-                                 parts + (size_t)nfull * kk, n),
-                   "rocblas_dgemm");
-    }
-    {
-      Stage st(ctx, s, "general_sum_slices");
-      hipLaunchKernelGGL(sum_slices_kernel, dim3(hfjk::grid_for((long long)kk)), dim3(256), 0, s, parts,
-                         nfull + (rem > 0 ? 1 : 0), (long long)kk, q0 > 0 ? 1 : 0, out);
-      check_hip(hipGetLastError(), "sum slices launch");
+    for (int q0 = 0; q0 < naux; q0 += pc) {
+      const int qc = std::min(pc, naux - q0);
+      const int nsplit = std::min(syrk_splits(G.npairs, qc), G.nsplit);
+      const int nslot = hfjk::syrk_slots(precision, qc, ld, nsplit);
+      {
+        Stage st(ctx, s, "general_half_transform");
+        if (full && split) {
+          const uint16_t* bh = (const uint16_t*)bfull;
+          check_hip(hfjk::launch_half_transform_full(bh + q0 * npad * npad, precision, qc, n, ct, ld, x, s,
+                                                     bh + ((long long)naux + q0) * npad * npad),
+                    "half transform launch");
+        } else if (full) {
+          check_hip(hfjk::launch_half_transform_full((const char*)bfull + q0 * npad * npad * (long long)es, precision,
+                                                     qc, n, ct, ld, x, s),
+                    "half transform launch");
+        } else if (split) {
+          const uint16_t* bh = (const uint16_t*)b;
+          check_hip(hfjk::launch_half_transform_bf16x3(bh + q0 * p, p, qc, n, cp, ld, x, ctx->x16_bn, s,
+                                                       bh + ((long long)naux + q0) * p),
+                    "half transform launch");
+        } else {
+          check_hip(hfjk::launch_half_transform((const char*)b + q0 * p * (long long)es, precision, p, qc, n, cp, ld,
+                                                x, ctx->x_bn, ctx->x_bm == 0 ? 64 : ctx->x_bm, s),
+                    "half transform launch");
+        }
+      }
+      {
+        Stage st(ctx, s, "general_syrk");
+        check_hip(hfjk::launch_syrk(x, precision, qc, n, ld, sym, true, nsplit, p1, 0, s), "syrk launch");
+        check_hip(hfjk::launch_syrk(x, precision, qc, n, ld, cross, true, nsplit, p2, 0, s, nullptr, hb),
+                  "syrk launch");
+      }
+      {
+        Stage st(ctx, s, "general_reduce");
+        check_hip(hfjk::launch_syrk_reduce2(p1, p2, rprec, nslot, n, accumulate, out, s), "reduce launch");
+      }
+      accumulate = true;
     }
   }
 }
@@ -484,11 +580,6 @@ void* noalloc_scratch(int device) {
     addresses[device] = ptr;
   }
   return addresses[device];
-}
-
-__global__ void zero_kernel(double* __restrict__ out, long long count) {
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256)
-    out[i] = 0.0;
 }
 
 }  // namespace
@@ -602,6 +693,7 @@ int hfjk_bind(hfjk_ctx* ctx, int naux, int nao, const double* b64, const float* 
     ctx->b16 = b16;
     ctx->b64_full = nullptr;  // a full copy belongs to the tensor it was expanded from
     ctx->b32_full = nullptr;
+    ctx->b16_full = nullptr;
   });
 }
 
@@ -609,7 +701,7 @@ int hfjk_expand_full(hfjk_ctx* ctx, const double* packed, int naux, int nao, int
                      void* stream) {
   return guarded([&] {
     check_ctx(ctx);
-    if (precision != HFJK_FP64 && precision != HFJK_FP32) fail(HFJK_ERR_INPUT, "full copies are FP64 or FP32");
+    check_precision(precision);
     if (naux <= 0 || nao <= 0) fail(HFJK_ERR_INPUT, "the fitted tensor has shape [aux, ao, ao] with aux, ao > 0");
     if (packed == nullptr || full == nullptr) fail(HFJK_ERR_INPUT, "null pointer");
     const long long npad = (long long)hfjk::n_tiles(nao) * hfjk::kTile;
@@ -625,15 +717,17 @@ int hfjk_expand_full(hfjk_ctx* ctx, const double* packed, int naux, int nao, int
 int hfjk_bind_full(hfjk_ctx* ctx, int precision, const void* full) {
   return guarded([&] {
     check_ctx(ctx);
-    if (precision != HFJK_FP64 && precision != HFJK_FP32) fail(HFJK_ERR_INPUT, "full copies are FP64 or FP32");
+    check_precision(precision);
     if (full != nullptr && ctx->naux <= 0) fail(HFJK_ERR_INPUT, "bind the packed tensor first (hfjk_bind)");
     if (full != nullptr && (long long)hfjk::n_tiles(ctx->nao) * hfjk::kTile * hfjk::n_tiles(ctx->nao) * hfjk::kTile * 8 >=
                                (1LL << 31))
       fail(HFJK_ERR_INPUT, "the full copy needs nao below 16384");
     if (precision == HFJK_FP64)
       ctx->b64_full = (const double*)full;
-    else
+    else if (precision == HFJK_FP32)
       ctx->b32_full = (const float*)full;
+    else
+      ctx->b16_full = (const uint16_t*)full;
   });
 }
 
@@ -744,7 +838,7 @@ int hfjk_exchange_occ_noalloc(hfjk_ctx* ctx, const double* orbitals, int nocc, c
     } reset;
     if (nocc == 0) {
       const long long count = (long long)ctx->nao * ctx->nao;
-      hipLaunchKernelGGL(zero_kernel, dim3(hfjk::grid_for(count)), dim3(256), 0, s, exchange, count);
+      hipLaunchKernelGGL(zero_fill_kernel, dim3(hfjk::grid_for(count)), dim3(256), 0, s, exchange, count);
       check_hip(hipGetLastError(), "zero launch");
       return;
     }
@@ -943,11 +1037,8 @@ int hfjk_exchange_density(hfjk_ctx* ctx, const double* density, double* exchange
     int nneg = 0;
     const int r = density_factor(ctx, density, workspace, L, cf, sg, s, &asym, &nneg);
     if (r < 0) {
-      // Not symmetric: the two-DGEMM density form, FP64 only.
-      if (precision != HFJK_FP64)
-        fail(HFJK_ERR_BACKEND, "a non-symmetric density (max |D - D^T| = " + std::to_string(asym) +
-                                   ") is contracted in FP64 only");
-      exchange_general(ctx, density, exchange, workspace, workspace_bytes, L.dens_end, s);
+      // Not symmetric: the Hermitian factorisation of S + i A and two SYRKs.
+      exchange_general(ctx, density, exchange, precision, workspace, workspace_bytes, L.dens_end, s);
       return;
     }
     if (r == 0) {
@@ -973,16 +1064,28 @@ int hfjk_density_factor(hfjk_ctx* ctx, const double* density, double* factor, do
   });
 }
 
-int hfjk_exchange_general(hfjk_ctx* ctx, const double* density, double* exchange, void* workspace,
+int hfjk_general_workspace_bytes(hfjk_ctx* ctx, int precision, int aux_chunk, size_t* bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_precision(precision);
+    if (bytes == nullptr) fail(HFJK_ERR_INPUT, "null output pointer");
+    if (ctx->naux <= 0) fail(HFJK_ERR_INPUT, "no fitted tensor is bound (call hfjk_bind)");
+    const int pc = aux_chunk > 0 ? std::min(aux_chunk, ctx->naux) : ctx->naux;
+    *bytes = general_layout(ctx, precision, ctx->nao, pc, 0).total;
+  });
+}
+
+int hfjk_exchange_general(hfjk_ctx* ctx, const double* density, double* exchange, int precision, void* workspace,
                           size_t workspace_bytes, void* stream) {
   return guarded([&] {
     check_ctx(ctx);
+    check_precision(precision);
     if (ctx->naux <= 0) fail(HFJK_ERR_INPUT, "no fitted tensor is bound (call hfjk_bind)");
     if (density == nullptr || exchange == nullptr) fail(HFJK_ERR_INPUT, "null matrix pointer");
-    const Layout L = make_layout(ctx, 1, HFJK_FP64);
-    check_workspace(L, workspace, workspace_bytes);
+    if (workspace == nullptr || ((uintptr_t)workspace & 255) != 0)
+      fail(HFJK_ERR_WORKSPACE, "the workspace is a 256-byte aligned device buffer");
     begin_call(ctx);
-    exchange_general(ctx, density, exchange, workspace, workspace_bytes, L.dens_end, (hipStream_t)stream);
+    exchange_general(ctx, density, exchange, precision, workspace, workspace_bytes, 0, (hipStream_t)stream);
   });
 }
 

@@ -84,6 +84,7 @@ struct hfjk_ctx {
   // bound, 1 = never).
   const double* b64_full = nullptr;
   const float* b32_full = nullptr;
+  const uint16_t* b16_full = nullptr;  // hi plane [naux][npad][npad], then the lo plane
   int x_full = 0;
   int general_chunk = 0;  // aux functions per chunk of the non-symmetric exchange (0 = fit)
   int side_groups = 0;    // grid cap of the Coulomb kernels beside the SYRK in hfjk_jk_occ (0 = default)
@@ -192,10 +193,18 @@ constexpr int kSyrkRunProducts = 4096;
 int syrk_run_aux(int precision, int ldo);
 int syrk_slots(int precision, int naux, int ldo, int nsplit);
 // auxsign (optional, [naux]): one more factor per aux function on the signed
-// direct-LDS kernels (hipErrorInvalidValue for a launch that would not run them).
+// FP64/FP32 direct-LDS kernels; shift (a multiple of 32 below ldo): the
+// second operand's columns rotated, sum_k X[m][k] S_k X[n][(k + shift) mod
+// ldo], on the signed direct-LDS kernels of every precision
+// (hipErrorInvalidValue for a launch that would not run them).
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
                        bool is_signed, int nsplit, void* partial, int kind, hipStream_t s,
-                       const double* auxsign = nullptr);
+                       const double* auxsign = nullptr, int shift = 0);
+// K (+)= the symmetric tiles sym plus the antisymmetric tiles anti: lower
+// tiles give K[m][n] = sym + anti, their mirror K[n][m] = sym - anti
+// (diagonal tiles computed whole). accumulate adds to K instead of writing.
+hipError_t launch_syrk_reduce2(const void* sym, const void* anti, int precision, int nslot, int nao,
+                               bool accumulate, double* out, hipStream_t s);
 hipError_t launch_syrk_reduce(const void* partial, int precision, int nslot, int nao, double* out,
                               hipStream_t s);
 hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* signs_out, hipStream_t s);
@@ -203,8 +212,9 @@ hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* s
 // BF16x3 exchange (xk.hip): operands as bf16 hi/lo planes; launch_syrk
 // takes HFJK_BF16X3 for the split-product kernel.
 hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s);
+// b_lo: the lo planes of these aux functions (default: naux p elements after b).
 hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
-                                        void* x, int bn, hipStream_t s);
+                                        void* x, int bn, hipStream_t s, const uint16_t* b_lo = nullptr);
 // Coulomb charges of D = C C^T from the half-transformed X (xk.hip):
 // gamma[q] = sum_{m, i} X_q[m][i] cpad[m][i], cpad = the zero-padded
 // orbitals [npad][ldo] of launch_orbitals (FP64 when X is FP64, FP32 for
@@ -212,16 +222,20 @@ hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux
 int charge_groups(int nao);
 hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao, int ldo, const void* cpad,
                                  double* parts, double* gamma, int max_groups, hipStream_t s);
-// FP64 / FP32 half-transform from a full row-major copy of B (xk.hip):
+// Half-transform from a full row-major copy of B (xk.hip):
 // launch_expand_full writes bfull[q][npad][npad] in the precision's type
-// (npad = 32 ceil(nao / 32)) from the tiled packed FP64 tensor;
-// launch_orbitals_t writes C^T [ldo][npad] in that type;
-// launch_half_transform_full computes X = B C on the direct-LDS ring.
+// (npad = 32 ceil(nao / 32)) from the tiled packed FP64 tensor; for BF16X3
+// the bf16 hi plane of every element, then the lo plane (the element split
+// of launch_split);
+// launch_orbitals_t writes C^T [ldo][npad] in FP64 or FP32 (BF16X3 takes
+// the hi/lo planes of launch_orbitals_bf16x3);
+// launch_half_transform_full computes X = B C on the direct-LDS ring (the
+// bf16 ring for BF16X3, X as hi/lo planes).
 hipError_t launch_expand_full(const double* packed, long long p, int naux, int nao, int precision, void* out,
                               hipStream_t s);
 hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, void* ct, hipStream_t s);
 hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
-                                      void* x, hipStream_t s);
+                                      void* x, hipStream_t s, const void* bfull_lo = nullptr);
 // k-point DF-J/K (kpoint.hip): complex values as interleaved double pairs.
 // launch_kp_charges: inner [nk][naux] = w_k <B_k[P], D_k> (conjugated B) and
 // charges [naux] = sum_k inner, for blocks B_k at b + k kstride (elements);
@@ -246,6 +260,14 @@ hipError_t launch_kp_fold(const double* kh, int nao, bool accumulate, void* k, h
 hipError_t launch_kp_zgemm(const void* a, const void* b, int nao, double alpha, bool accumulate, void* c,
                            hipStream_t s);
 hipError_t launch_kp_hermitian_stats(const void* d, int nao, unsigned long long* stats, hipStream_t s);
+// The non-symmetric exchange (kpoint.hip): H = S + i A of a real D (complex
+// [nao][nao]); the real orbitals [nao][2 hb] = [Re V |l|^1/2, 0, Im V |l|^1/2, 0]
+// of retained eigenpairs j0 .. j0 + rb - 1 of H (the nneg lowest, then the
+// npos highest; U column-major from zheevd of conj(H)) and the column signs
+// of the symmetric and the column-shifted (cross) SYRK.
+hipError_t launch_hermitian_of_real(const double* d, int nao, void* h, hipStream_t s);
+hipError_t launch_general_factor(const void* eigvecs, const double* eigvals, int nao, int nneg, int npos, int j0,
+                                 int rb, int hb, double* c, double* sym, double* cross, hipStream_t s);
 
 // Lower-triangle 128 x 128 output tiles of the SYRK and the tile edge.
 int syrk_pairs(int nao);

@@ -840,25 +840,42 @@ __device__ __forceinline__ void cart_of(int l, int k, int& px, int& py, int& pz)
   pz = kk;
 }
 
-__device__ void boys_d(int nmax, double t, double* f) {
-  if (t < 1e-14) {
-    for (int n = 0; n <= nmax; ++n) f[n] = 1.0 / (2 * n + 1) - t / (2 * n + 3);
-    return;
-  }
+// The Boys function on the GPU: below kBoysTMax from a table of F_n on a
+// grid of step kBoysStep (n <= kGMaxL + kBoysTaylor), F_nmax by a Taylor
+// expansion about the nearest grid point (dF_n/dT = -F_{n+1}; the remainder
+// is below 1e-15 relative for |dT| <= kBoysStep / 2), then the stable
+// downward recursion; above it the erf closed form and upward recursion.
+constexpr double kBoysStep = 0.05;
+constexpr double kBoysTMax = 45.0;
+constexpr int kBoysTaylor = 7;
+constexpr int kBoysOrders = kGMaxL + kBoysTaylor + 1;
+constexpr int kBoysPoints = (int)(kBoysTMax / kBoysStep) + 2;
+
+__device__ void boys_d(int nmax, double t, const double* __restrict__ table, double* f) {
   const double et = exp(-t);
-  if (t < 45.0) {
-    double term = 1.0 / (2 * nmax + 1), sum = term;
-    for (int k = 1; k < 400; ++k) {
-      term *= 2.0 * t / (2 * nmax + 2 * k + 1);
-      sum += term;
-      if (term < 1e-17 * sum) break;
+  if (t < kBoysTMax) {
+    const int g = (int)(t / kBoysStep + 0.5);
+    const double dt = t - g * kBoysStep;
+    const double* row = table + (long long)g * kBoysOrders + nmax;
+    double term = 1.0, sum = row[0];
+    for (int k = 1; k < kBoysTaylor; ++k) {
+      term *= -dt / k;
+      sum += term * row[k];
     }
-    f[nmax] = et * sum;
+    f[nmax] = sum;
     for (int n = nmax - 1; n >= 0; --n) f[n] = (2.0 * t * f[n + 1] + et) / (2 * n + 1);
   } else {
     f[0] = 0.5 * sqrt(kPi / t) * erf(sqrt(t));
     for (int n = 0; n < nmax; ++n) f[n + 1] = ((2 * n + 1) * f[n] - et) / (2.0 * t);
   }
+}
+
+// The table of boys_d: F_n(g kBoysStep) for n < kBoysOrders, from the host
+// evaluation (series and downward recursion).
+std::vector<double> boys_table() {
+  std::vector<double> out((size_t)kBoysPoints * kBoysOrders);
+  for (int g = 0; g < kBoysPoints; ++g) boys(kBoysOrders - 1, g * kBoysStep, &out[(size_t)g * kBoysOrders]);
+  return out;
 }
 
 // One dimension of the Hermite expansion of a primitive pair: e[(i (lb + 1) + j) nt + t].
@@ -896,7 +913,8 @@ __global__ __launch_bounds__(64) void triple_kernel(const DItem* __restrict__ it
                                                     const double* __restrict__ coefs,
                                                     const double* __restrict__ norm_o, const double* __restrict__ norm_x,
                                                     const double* __restrict__ sph, const int* __restrict__ sph_off,
-                                                    int n, int metric_mode, double* __restrict__ out) {
+                                                    const double* __restrict__ boys_tab, int n, int metric_mode,
+                                                    double* __restrict__ out) {
   __shared__ TripleSmem sm;
   const int lane = threadIdx.x;
   const DItem it = items[blockIdx.x];
@@ -942,7 +960,7 @@ __global__ __launch_bounds__(64) void triple_kernel(const DItem* __restrict__ it
         const double xq = px - C.c[0], yq = py - C.c[1], zq = pz - C.c[2];
         __syncthreads();  // the previous triple's R and E_C are no longer read
         if (lane >= 3 && lane < 6) hermite_d(C.l, 0, g, 0.0, 0.0, sm.ec[lane - 3]);
-        if (lane == 6) boys_d(L, alpha * (xq * xq + yq * yq + zq * zq), sm.f);
+        if (lane == 6) boys_d(L, alpha * (xq * xq + yq * yq + zq * zq), boys_tab, sm.f);
         __syncthreads();
         // R^(L)_000 .. R^(0)_tuv: order m holds the entries with t + u + v <= L - m.
         int cur = 0;
@@ -1147,6 +1165,8 @@ void launch_triples(const std::vector<Shell>& orb, const std::vector<Shell>& aux
   const double* d_nx = dev.upload(norm_x, s);
   const double* d_sph = dev.upload(sph, s);
   const int* d_off = dev.upload(sph_off, s);
+  const std::vector<double> boys_host = boys_table();  // alive until the synchronisation below
+  const double* d_boys = dev.upload(boys_host, s);
   const size_t total = metric_mode ? (size_t)n * n : (size_t)norm_x.size() * n * n;
   if (hipMemsetAsync(out, 0, total * sizeof(double), s) != hipSuccess) throw std::runtime_error("hipMemsetAsync failed");
   // At most 2^31 - 1 workgroups per launch.
@@ -1154,7 +1174,7 @@ void launch_triples(const std::vector<Shell>& orb, const std::vector<Shell>& aux
   for (size_t base = 0; base < items.size(); base += per) {
     const unsigned count = (unsigned)std::min(per, items.size() - base);
     hipLaunchKernelGGL(triple_kernel, dim3(count), dim3(64), 0, s, d_items + base, d_orb, d_aux, d_exps, d_coefs,
-                       d_no, d_nx, d_sph, d_off, n, metric_mode ? 1 : 0, out);
+                       d_no, d_nx, d_sph, d_off, d_boys, n, metric_mode ? 1 : 0, out);
     if (hipGetLastError() != hipSuccess) throw std::runtime_error("launch of the integral kernel failed");
   }
   // The tables are freed on return: wait for the kernels that read them.

@@ -209,6 +209,56 @@ __global__ __launch_bounds__(kThreads) void kp_hermitian_stats_kernel(const doub
   }
 }
 
+// H = S + i A of a real row-major D (S, A its symmetric and antisymmetric parts), complex row-major.
+__global__ __launch_bounds__(kThreads) void hermitian_of_real_kernel(const double* __restrict__ d, int n,
+                                                                     double2* __restrict__ h) {
+  const long long nn = (long long)n * n;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < nn;
+       idx += (long long)gridDim.x * kThreads) {
+    const long long r = idx / n, c = idx % n;
+    const double a = d[idx], b = d[c * n + r];
+    h[idx] = make_double2(0.5 * (a + b), 0.5 * (a - b));
+  }
+}
+
+// Columns [j0, j0 + rb) of the retained eigenpairs of H (the nneg lowest,
+// then the npos highest; u column-major from zheevd of the view conj(H), so
+// H's eigenvectors are conj(u)) as the real orbitals of the non-symmetric
+// exchange: c [n][2 hb] = [Re V |l|^1/2 (rb columns), 0, Im V |l|^1/2, 0]
+// with hb the half width, and the column signs of its two SYRKs:
+// sym = (sign l, +1, sign l, +1), cross = (-sign l, +1, +sign l, +1).
+__global__ __launch_bounds__(kThreads) void general_factor_kernel(const double2* __restrict__ u,
+                                                                  const double* __restrict__ w, int n, int nneg,
+                                                                  int npos, int j0, int rb, int hb,
+                                                                  double* __restrict__ c, double* __restrict__ sym,
+                                                                  double* __restrict__ cross) {
+  const int ld = 2 * hb;
+  const long long total = (long long)n * ld;
+  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * kThreads) {
+    const int m = (int)(idx / ld);
+    const int col = (int)(idx % ld);
+    const int half = col >= hb ? 1 : 0;
+    const int i = col - half * hb;
+    double v = 0.0;
+    double sg = 1.0;
+    if (i < rb) {
+      const int k = j0 + i;  // index among the retained pairs
+      const int jj = k < nneg ? k : n - npos + (k - nneg);
+      const double l = w[jj];
+      const double2 z = u[(long long)jj * n + m];
+      const double scale = sqrt(fabs(l));
+      v = half ? -scale * z.y : scale * z.x;
+      sg = l < 0.0 ? -1.0 : 1.0;
+    }
+    c[idx] = v;
+    if (m == 0) {
+      sym[col] = sg;
+      cross[col] = i < rb ? (half ? sg : -sg) : 1.0;
+    }
+  }
+}
+
 }  // namespace
 
 hipError_t launch_kp_charges(const void* b, long long kstride, int nk, int naux, int nao, const void* density,
@@ -263,6 +313,19 @@ hipError_t launch_kp_zgemm(const void* a, const void* b, int nao, double alpha, 
 hipError_t launch_kp_hermitian_stats(const void* d, int nao, unsigned long long* stats, hipStream_t s) {
   hipLaunchKernelGGL(kp_hermitian_stats_kernel, dim3(grid_for((long long)nao * nao)), dim3(kThreads), 0, s,
                      (const double2*)d, nao, stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_hermitian_of_real(const double* d, int nao, void* h, hipStream_t s) {
+  hipLaunchKernelGGL(hermitian_of_real_kernel, dim3(grid_for((long long)nao * nao)), dim3(kThreads), 0, s, d, nao,
+                     (double2*)h);
+  return hipGetLastError();
+}
+
+hipError_t launch_general_factor(const void* eigvecs, const double* eigvals, int nao, int nneg, int npos, int j0,
+                                 int rb, int hb, double* c, double* sym, double* cross, hipStream_t s) {
+  hipLaunchKernelGGL(general_factor_kernel, dim3(grid_for((long long)nao * 2 * hb)), dim3(kThreads), 0, s,
+                     (const double2*)eigvecs, eigvals, nao, nneg, npos, j0, rb, hb, c, sym, cross);
   return hipGetLastError();
 }
 

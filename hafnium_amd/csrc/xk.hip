@@ -963,7 +963,7 @@ __device__ __forceinline__ void dma_ring(const Src& src, const int (&off)[4 * CH
 template <typename T, bool SIGNED, int CH>
 __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
     const T* __restrict__ x, int naux, int xrows, int ldo, const double* __restrict__ signs,
-    const double* __restrict__ auxsign, int nsplit, int npairs, int run, T* __restrict__ partial) {
+    const double* __restrict__ auxsign, int shift, int nsplit, int npairs, int run, T* __restrict__ partial) {
   typedef SyrkDma<T, CH> D;
   constexpr int E = D::E;
   constexpr int LOADS = D::LOADS;
@@ -1007,10 +1007,13 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
     int qb, qe;
     slot_aux(c, nsplit, naux, run, qb, qe);
     const int steps = (qe - qb) * nch;
-    // Step k covers k values i0 = (k % nch) KS of aux function qb + k / nch.
-    auto src = [&](int step, int) -> const T* {
+    // Step k covers k values i0 = (k % nch) KS of aux function qb + k / nch;
+    // the second operand reads columns i0 + shift (mod ldo).
+    auto src = [&](int step, int operand) -> const T* {
       const int qq = step / nch;
-      return x + (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * D::KS;
+      int i0 = (step - qq * nch) * D::KS;
+      if (operand) i0 = (i0 + shift) % ldo;
+      return x + (long long)(qb + qq) * xrows * ldo + i0;
     };
     // Per-aux signs (the signature of an indefinite metric), when given.
     auto scale_at = [&](int step) -> T { return auxsign != nullptr ? (T)auxsign[qb + step / nch] : (T)1; };
@@ -1299,28 +1302,373 @@ __global__ __launch_bounds__(kThreads) void syrk_bf16x3_kernel(const uint16_t* _
   }
 }
 
+// ------------------------------------------------------------------ BF16x3 on the direct-LDS ring
+//
+// The bf16 form of the ring: a 1 KB chunk (16 rows x 32 k of bf16, lane l
+// carrying row l % 16 and k 8 (l / 16) .. + 8) is exactly one operand of
+// v_mfma_f32_16x16x32_bf16, read back by lane l as one 16-byte LDS read. A
+// stage holds 32 chunks: for each operand (A = the 128 rows of the first
+// block, B = those of the second) its 8 row blocks in the hi and the lo
+// plane, 32 KB; four stages fill 128 KB, one workgroup per CU. Wave w loads
+// chunks 8 w .. 8 w + 7 (chunk = 16 operand + 2 block + plane), so waves
+// 0-1 load A and waves 2-3 B. Each k step of 32 runs 48 MFMAs per wave
+// (hi.hi + hi.lo + lo.hi on 4 x 4 output blocks) and reads the next stage's
+// operands into the second register set meanwhile; the barrier and wait
+// structure is the FP64/FP32 ring's (one barrier per step, the loads of
+// two steps ahead in flight).
+constexpr int kRing16Loads = 8;              // load instructions per wave per stage
+constexpr int kRing16Stage = 32 * 512;       // uint16 elements per stage (32 KB)
+
+struct Ops16 {
+  bf8 ah[4], al[4], bh[4], bl[4];
+};
+
+// src(step, operand, plane): the uint16 base of operand `operand`'s `plane`
+// at step `step`, to which this lane's byte offsets off[v] (row and k part)
+// are added. With SIGNED, the B operand's elements are xored with the
+// 16-bit masks sgn[sign_at(step) + k'] (0x8000 flips the sign).
+template <bool SIGNED, typename Src, typename SignAt>
+__device__ __forceinline__ void dma_ring16(const Src& src, const int (&off)[kRing16Loads], int steps,
+                                           const uint16_t* __restrict__ sgn, const SignAt& sign_at,
+                                           f4 (&acc)[4][4]) {
+  __shared__ __attribute__((aligned(16))) uint16_t st0[kRing16Stage];
+  __shared__ __attribute__((aligned(16))) uint16_t st1[kRing16Stage];
+  __shared__ __attribute__((aligned(16))) uint16_t st2[kRing16Stage];
+  __shared__ __attribute__((aligned(16))) uint16_t st3[kRing16Stage];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+  const int operand = wave >> 1;
+
+  auto issue = [&](int step, uint16_t* stage) {
+    const uint16_t* hi = src(step, operand, 0);
+    const uint16_t* lo = src(step, operand, 1);
+#pragma unroll
+    for (int v = 0; v < kRing16Loads; ++v) {
+      const uint16_t* base = (v & 1) ? lo : hi;
+      __builtin_amdgcn_global_load_lds((const void*)((const char*)base + (unsigned)off[v]),
+                                       (lds_void_t*)(stage + (wave * kRing16Loads + v) * 512), 16, 0, 0);
+    }
+  };
+  auto chunk = [&](const uint16_t* stage, int op, int blk, int plane) -> bf8 {
+    return lds_bf8(stage + ((op * 16 + blk * 2 + plane) * 512 + lane * 8));
+  };
+  auto read = [&](const uint16_t* stage, int step, Ops16& o) {
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      o.ah[mi] = chunk(stage, 0, wm * 4 + mi, 0);
+      o.al[mi] = chunk(stage, 0, wm * 4 + mi, 1);
+    }
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      o.bh[ni] = chunk(stage, 1, wn * 4 + ni, 0);
+      o.bl[ni] = chunk(stage, 1, wn * 4 + ni, 1);
+    }
+    if constexpr (SIGNED) {
+      const uint4 m = *(const uint4*)(sgn + sign_at(step) + (lane >> 4) * 8);
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) {
+        uint4 h = __builtin_bit_cast(uint4, o.bh[ni]);
+        uint4 l = __builtin_bit_cast(uint4, o.bl[ni]);
+        h.x ^= m.x; h.y ^= m.y; h.z ^= m.z; h.w ^= m.w;
+        l.x ^= m.x; l.y ^= m.y; l.z ^= m.z; l.w ^= m.w;
+        o.bh[ni] = __builtin_bit_cast(bf8, h);
+        o.bl[ni] = __builtin_bit_cast(bf8, l);
+      }
+    }
+  };
+  auto mma = [&](const Ops16& o) {
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = mma3(o.ah[mi], o.al[mi], o.bh[ni], o.bl[ni], acc[mi][ni]);
+  };
+
+  Ops16 r0, r1;
+  auto step_body = [&](int k, const uint16_t* next, uint16_t* free, bool odd) {
+    __builtin_amdgcn_sched_barrier(0);
+    wait_vm_lgkm0<kRing16Loads>();
+    barrier_nofence();
+    issue(min(k + 3, steps - 1), free);
+    if (k < steps) {
+      if (!odd) {
+        read(next, k + 1, r1);
+        mma(r0);
+      } else {
+        read(next, k + 1, r0);
+        mma(r1);
+      }
+    }
+  };
+  if (steps > 0) {
+    issue(0, st0);
+    issue(min(1, steps - 1), st1);
+    issue(min(2, steps - 1), st2);
+    wait_vm_lgkm0<2 * kRing16Loads>();
+    barrier_nofence();
+    read(st0, 0, r0);
+    for (int k = 0; k < steps; k += 4) {
+      step_body(k, st1, st3, false);
+      step_body(k + 1, st2, st0, true);
+      step_body(k + 2, st3, st1, false);
+      step_body(k + 3, st0, st2, true);
+    }
+    wait_vm_lgkm0<0>();
+  }
+}
+
+// BF16x3 SYRK over the ring: the tile (mb, nb) of sum_q X_q S X_q^T from the
+// hi/lo planes of X ([naux][xrows][ldo] each), partial slots as in the FP32
+// SYRK (one per run of `run` aux functions). With SIGNED the sign masks of
+// the columns sit in LDS. `shift` rotates the second operand's columns by
+// that many (a multiple of 32): sum_k X[m][k] S_k X[n][(k + shift) mod ldo].
+template <bool SIGNED>
+__global__ __launch_bounds__(kThreads, 1) void syrk16_dma_kernel(const uint16_t* __restrict__ xhi,
+                                                                 const uint16_t* __restrict__ xlo, int naux, int xrows,
+                                                                 int ldo, const double* __restrict__ signs, int shift,
+                                                                 int nsplit, int npairs, int run,
+                                                                 float* __restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) uint16_t sgn[SIGNED ? kDmaMaxLdo : 8];
+  int s, pair;
+  syrk_coords(blockIdx.x, npairs, s, pair);
+  int mb, nb;
+  tile_of(pair, mb, nb);
+  int cb, ce;
+  slot_range(s, nsplit, naux, run, cb, ce);
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+  if (SIGNED)
+    for (int i = t; i < ldo; i += kThreads) sgn[i] = signs[i] < 0.0 ? (uint16_t)0x8000 : (uint16_t)0;
+
+  // Lane byte offsets within one aux function's plane: load v of this wave is
+  // chunk 8 wave + v = 16 operand + 2 block + plane.
+  int off[kRing16Loads];
+#pragma unroll
+  for (int v = 0; v < kRing16Loads; ++v) {
+    const int ch = wave * kRing16Loads + v;
+    const int blk = (ch >> 1) & 7;
+    int row = ((ch >> 4) ? nb : mb) * kSyrkTile + blk * 16 + (lane & 15);
+    row = row < xrows ? row : 0;
+    off[v] = (row * ldo + (lane >> 4) * 8) * 2;
+  }
+  const int nch = ldo / 32;
+  auto sign_at = [&](int step) { return (step % nch) * 32; };
+  for (int c = cb; c < ce; ++c) {
+    int qb, qe;
+    slot_aux(c, nsplit, naux, run, qb, qe);
+    auto src = [&](int step, int operand, int plane) -> const uint16_t* {
+      const int qq = step / nch;
+      int i0 = (step - qq * nch) * 32;
+      if (operand) i0 = (i0 + shift) % ldo;
+      return (plane ? xlo : xhi) + (long long)(qb + qq) * xrows * ldo + i0;
+    };
+    f4 acc[4][4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
+    __syncthreads();  // sign masks written; the previous slot's LDS reads have returned
+    dma_ring16<SIGNED>(src, off, (qe - qb) * nch, sgn, sign_at, acc);
+    float* dst = partial + ((long long)c * npairs + pair) * (kSyrkTile * kSyrkTile);
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) {
+        const int col = wn * 64 + ni * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dst[(wm * 64 + mi * 16 + 4 * (lane >> 4) + r) * kSyrkTile + col] = acc[mi][ni][r];
+      }
+  }
+}
+
+// BF16x3 half-transform over the ring from the full bf16 hi/lo copy of B
+// (bhi/blo [naux][npad][npad]) and C^T's planes (cthi/ctlo [ldo][npad]):
+// the 128 x 128 tile (mb, nb) of X_q into X's hi/lo planes.
+__global__ __launch_bounds__(kThreads, 1) void xform16_dma_kernel(const uint16_t* __restrict__ bhi,
+                                                                  const uint16_t* __restrict__ blo, int npad,
+                                                                  const uint16_t* __restrict__ cthi,
+                                                                  const uint16_t* __restrict__ ctlo, int ldo,
+                                                                  uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo,
+                                                                  int xrows, int qbase, int nblocks_n, int nblocks_m,
+                                                                  int nwg) {
+  const int logical = xcd_remap(blockIdx.x, nwg);
+  const int nb = logical % nblocks_n;
+  const int mb = (logical / nblocks_n) % nblocks_m;
+  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+  int off[kRing16Loads];
+#pragma unroll
+  for (int v = 0; v < kRing16Loads; ++v) {
+    const int ch = wave * kRing16Loads + v;
+    const bool is_b = (ch >> 4) != 0;
+    const int blk = (ch >> 1) & 7;
+    int row = (is_b ? nb : mb) * kSyrkTile + blk * 16 + (lane & 15);
+    row = row < (is_b ? ldo : npad) ? row : 0;
+    off[v] = (row * npad + (lane >> 4) * 8) * 2;
+  }
+  const long long plane = (long long)q * npad * npad;
+  auto src = [&](int step, int operand, int pl) -> const uint16_t* {
+    if (operand) return (pl ? ctlo : cthi) + step * 32;
+    return (pl ? blo : bhi) + plane + step * 32;
+  };
+  auto no_sign = [](int) { return 0; };
+  f4 acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
+  dma_ring16<false>(src, off, npad / 32, (const uint16_t*)nullptr, no_sign, acc);
+  const long long xbase = (long long)q * xrows * ldo;
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const int col = nb * kSyrkTile + wn * 64 + ni * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = mb * kSyrkTile + wm * 64 + mi * 16 + 4 * (lane >> 4) + r;
+        if (row < xrows && col < ldo) {
+          const float v = row < npad ? acc[mi][ni][r] : 0.f;
+          const uint16_t h = bf16_of(v);
+          xhi[xbase + (long long)row * ldo + col] = h;
+          xlo[xbase + (long long)row * ldo + col] = bf16_of(v - f32_of(h));
+        }
+      }
+    }
+}
+
+// The full bf16 hi/lo copy of the packed FP64 tensor (hi plane [naux][npad][npad], then the lo
+// plane): the element split of launch_split (hi = bf16(fp32(b)), lo = bf16(fp32(b) - hi)), tiles
+// transposed through LDS as in expand_full_kernel.
+__global__ __launch_bounds__(kThreads) void expand_full16_kernel(const double* __restrict__ packed, long long p,
+                                                                 int naux, int ntp, int npad, uint16_t* __restrict__ hi,
+                                                                 uint16_t* __restrict__ lo) {
+  __shared__ uint32_t tile[kTile][kTile + 1];
+  const long long jobs = (long long)naux * ntp;
+  for (long long job = blockIdx.x; job < jobs; job += gridDim.x) {
+    const int q = (int)(job / ntp);
+    const int t = (int)(job - (long long)q * ntp);
+    int tr = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((tr + 1) * (tr + 2) / 2 <= t) ++tr;
+    while (tr * (tr + 1) / 2 > t) --tr;
+    const int tc = t - tr * (tr + 1) / 2;
+    const double* src = packed + q * p + (long long)t * kTileElems;
+    const long long oq = (long long)q * npad * npad;
+    for (int e = threadIdx.x; e < kTileElems; e += kThreads) {
+      const int i = e / kTile, j = e % kTile;
+      const float x = (float)src[e];
+      const uint16_t h = bf16_of(x);
+      const uint16_t l = bf16_of(x - f32_of(h));
+      tile[i][j] = (uint32_t)h | ((uint32_t)l << 16);
+      const long long o = oq + (long long)(tr * kTile + i) * npad + tc * kTile + j;
+      hi[o] = h;
+      lo[o] = l;
+    }
+    if (tr != tc) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < kTileElems; e += kThreads) {
+        const int i = e / kTile, j = e % kTile;
+        const uint32_t v = tile[j][i];
+        const long long o = oq + (long long)(tc * kTile + i) * npad + tr * kTile + j;
+        hi[o] = (uint16_t)(v & 0xFFFFu);
+        lo[o] = (uint16_t)(v >> 16);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// K from the partial tiles: one workgroup per (lower tile pair, 8-row strip
+// of the tile); each thread sums 4 elements of the strip over the slots in
+// a fixed order (reads coalesced along the tile's rows), writes them to
+// K[bm rows][bn columns], and for an off-diagonal pair the mirror
+// K[bn rows][bm columns] through an LDS transpose (writes coalesced too).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void syrk_reduce_kernel(const T* __restrict__ partial, int nslot,
                                                                 int npairs, int nao, double* __restrict__ out) {
-  const long long total = (long long)nao * nao;
-  for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < total;
-       idx += (long long)gridDim.x * kThreads) {
-    const int row = (int)(idx / nao);
-    const int col = (int)(idx % nao);
-    const int bm = row / kSyrkTile, bn = col / kSyrkTile;
-    long long pair;
-    int local;
-    if (bm >= bn) {
-      pair = tidx(bm, bn);
-      local = (row % kSyrkTile) * kSyrkTile + col % kSyrkTile;
-    } else {
-      pair = tidx(bn, bm);
-      local = (col % kSyrkTile) * kSyrkTile + row % kSyrkTile;
-    }
+  constexpr int kStrip = 8;
+  __shared__ double mirror[kStrip][kSyrkTile + 1];
+  const int pair = blockIdx.x / (kSyrkTile / kStrip);
+  const int strip = blockIdx.x % (kSyrkTile / kStrip);
+  int bm, bn;
+  tile_of(pair, bm, bn);
+  const int t = threadIdx.x;
+  const int col = t & (kSyrkTile - 1);  // 0..127
+  const int r0 = t >> 7;                 // 0..1
+#pragma unroll
+  for (int rr = 0; rr < kStrip / 2; ++rr) {
+    const int lr = r0 + 2 * rr;              // row within the strip
+    const int trow = strip * kStrip + lr;    // row within the tile
+    const T* src = partial + (long long)pair * (kSyrkTile * kSyrkTile) + trow * kSyrkTile + col;
     double sum = 0.0;
-    for (int k = 0; k < nslot; ++k)
-      sum += (double)partial[((long long)k * npairs + pair) * (kSyrkTile * kSyrkTile) + local];
-    out[idx] = sum;
+    for (int k = 0; k < nslot; ++k) sum += (double)src[(long long)k * npairs * (kSyrkTile * kSyrkTile)];
+    const int row = bm * kSyrkTile + trow, c = bn * kSyrkTile + col;
+    if (row < nao && c < nao) out[(long long)row * nao + c] = sum;
+    mirror[lr][col] = sum;
+  }
+  if (bm == bn) return;
+  __syncthreads();
+  // Mirror: K[bn * 128 + col][bm * 128 + strip rows] = strip[row][col].
+  for (int e = t; e < kStrip * kSyrkTile; e += kThreads) {
+    const int mc = e / kStrip;   // column of the tile -> row of the mirror
+    const int mr = e % kStrip;   // row of the strip -> column of the mirror
+    const int row = bn * kSyrkTile + mc, c = bm * kSyrkTile + strip * kStrip + mr;
+    if (row < nao && c < nao) out[(long long)row * nao + c] = mirror[mr][mc];
+  }
+}
+
+// The symmetric-plus-antisymmetric reduce of the non-symmetric exchange:
+// per 8-row strip of a lower tile pair, S = sum of `sym`'s slots and A = sum
+// of `anti`'s; K[m][n] (+)= S + A on the tile, K[n][m] (+)= S - A on its
+// mirror (skipped for diagonal tiles, which hold both triangles).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void syrk_reduce2_kernel(const T* __restrict__ sym, const T* __restrict__ anti,
+                                                                 int nslot, int npairs, int nao, int accumulate,
+                                                                 double* __restrict__ out) {
+  constexpr int kStrip = 8;
+  __shared__ double mirror[kStrip][kSyrkTile + 1];
+  const int pair = blockIdx.x / (kSyrkTile / kStrip);
+  const int strip = blockIdx.x % (kSyrkTile / kStrip);
+  int bm, bn;
+  tile_of(pair, bm, bn);
+  const int t = threadIdx.x;
+  const int col = t & (kSyrkTile - 1);
+  const int r0 = t >> 7;
+#pragma unroll
+  for (int rr = 0; rr < kStrip / 2; ++rr) {
+    const int lr = r0 + 2 * rr;
+    const int trow = strip * kStrip + lr;
+    const long long base = (long long)pair * (kSyrkTile * kSyrkTile) + trow * kSyrkTile + col;
+    double ss = 0.0, aa = 0.0;
+    for (int k = 0; k < nslot; ++k) {
+      const long long o = base + (long long)k * npairs * (kSyrkTile * kSyrkTile);
+      ss += (double)sym[o];
+      aa += (double)anti[o];
+    }
+    const int row = bm * kSyrkTile + trow, c = bn * kSyrkTile + col;
+    if (row < nao && c < nao) {
+      const long long o = (long long)row * nao + c;
+      out[o] = (accumulate ? out[o] : 0.0) + ss + aa;
+    }
+    mirror[lr][col] = ss - aa;
+  }
+  if (bm == bn) return;
+  __syncthreads();
+  for (int e = t; e < kStrip * kSyrkTile; e += kThreads) {
+    const int mc = e / kStrip;
+    const int mr = e % kStrip;
+    const int row = bn * kSyrkTile + mc, c = bm * kSyrkTile + strip * kStrip + mr;
+    if (row < nao && c < nao) {
+      const long long o = (long long)row * nao + c;
+      out[o] = (accumulate ? out[o] : 0.0) + mirror[mr][mc];
+    }
   }
 }
 
@@ -1377,9 +1725,12 @@ hipError_t launch_expand_full(const double* packed, long long p, int naux, int n
   if (precision == HFJK_FP64)
     hipLaunchKernelGGL(expand_full_kernel<double>, grid, dim3(kThreads), 0, s, packed, p, naux, ntp, npad,
                        (double*)out);
-  else
+  else if (precision == HFJK_FP32)
     hipLaunchKernelGGL(expand_full_kernel<float>, grid, dim3(kThreads), 0, s, packed, p, naux, ntp, npad,
                        (float*)out);
+  else
+    hipLaunchKernelGGL(expand_full16_kernel, grid, dim3(kThreads), 0, s, packed, p, naux, ntp, npad,
+                       (uint16_t*)out, (uint16_t*)out + (long long)naux * npad * npad);
   return hipGetLastError();
 }
 
@@ -1395,7 +1746,7 @@ hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, 
 }
 
 hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
-                                      void* x, hipStream_t s) {
+                                      void* x, hipStream_t s, const void* bfull_lo) {
   const int npad = n_tiles(nao) * kTile;
   const int xrows = x_rows(nao);
   const int nbn = (ldo + kSyrkTile - 1) / kSyrkTile;
@@ -1404,12 +1755,23 @@ hipError_t launch_half_transform_full(const void* bfull, int precision, int naux
   for (int base = 0; base < naux; base += per) {
     const int count = std::min(per, naux - base);
     const int nwg = nbn * nbm * count;
-    if (precision == HFJK_FP64)
+    if (precision == HFJK_FP64) {
       hipLaunchKernelGGL((xform_dma_kernel<double, 1>), dim3((unsigned)nwg), dim3(kThreads), 0, s,
                          (const double*)bfull, npad, (const double*)ct, ldo, (double*)x, xrows, base, nbn, nbm, nwg);
-    else
+    } else if (precision == HFJK_FP32) {
       hipLaunchKernelGGL((xform_dma_kernel<float, 1>), dim3((unsigned)nwg), dim3(kThreads), 0, s,
                          (const float*)bfull, npad, (const float*)ct, ldo, (float*)x, xrows, base, nbn, nbm, nwg);
+    } else {
+      // BF16x3: B's hi plane then its lo plane, C^T's likewise, X's likewise.
+      const uint16_t* bh = (const uint16_t*)bfull;
+      const uint16_t* ch = (const uint16_t*)ct;
+      uint16_t* xh = (uint16_t*)x;
+      const long long nb16 = (long long)naux * npad * npad, nc16 = (long long)ldo * npad,
+                      nx16 = (long long)naux * xrows * ldo;
+      const uint16_t* bl = bfull_lo != nullptr ? (const uint16_t*)bfull_lo : bh + nb16;
+      hipLaunchKernelGGL(xform16_dma_kernel, dim3((unsigned)nwg), dim3(kThreads), 0, s, bh, bl, npad, ch,
+                         ch + nc16, ldo, xh, xh + nx16, xrows, base, nbn, nbm, nwg);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
@@ -1493,10 +1855,10 @@ hipError_t launch_half_transform(const void* b, int precision, long long p, int 
 }
 
 hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
-                                        void* x, int bn, hipStream_t s) {
+                                        void* x, int bn, hipStream_t s, const uint16_t* b_lo) {
   bn = pick_bn(bn, ldo);
   const uint16_t* bhi = b;
-  const uint16_t* blo = b + (long long)naux * p;
+  const uint16_t* blo = b_lo != nullptr ? b_lo : b + (long long)naux * p;
   const long long nct = (long long)n_tiles(nao) * kTile * ldo;
   const uint16_t* cthi = (const uint16_t*)ct;
   const uint16_t* ctlo = cthi + nct;
@@ -1520,14 +1882,16 @@ int syrk_slots(int precision, int naux, int ldo, int nsplit) {
 }
 
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
-                       bool is_signed, int nsplit, void* partial, int kind, hipStream_t s, const double* auxsign) {
+                       bool is_signed, int nsplit, void* partial, int kind, hipStream_t s, const double* auxsign,
+                       int shift) {
   const int xrows = x_rows(nao);
   const int npairs = syrk_pairs(nao);
   const int run = syrk_run_aux(precision, ldo);
-  // Per-aux signs are applied by the signed direct-LDS kernels only.
-  if (auxsign != nullptr && (!is_signed || precision == HFJK_BF16X3 || kind == 1 || ldo > kDmaMaxLdo ||
-                             (long long)xrows * ldo >= (1LL << 31)))
-    return hipErrorInvalidValue;
+  // Per-aux signs are applied by the signed FP64/FP32 direct-LDS kernels only; a column shift by
+  // the signed direct-LDS kernels of every precision.
+  const bool ring_ok = is_signed && kind != 1 && ldo <= kDmaMaxLdo && (long long)xrows * ldo < (1LL << 30);
+  if (auxsign != nullptr && (!ring_ok || precision == HFJK_BF16X3)) return hipErrorInvalidValue;
+  if (shift != 0 && (!ring_ok || shift % 32 != 0 || shift < 0 || shift >= ldo)) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)(npairs * nsplit);
   // The direct-load kernel keeps per-lane element offsets within one aux
   // function's X in 32 bits and the sign table in LDS.
@@ -1537,22 +1901,22 @@ hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo,
   const bool two_wg = kind != 3 && !is_signed;
   if (dma && precision == HFJK_FP64 && is_signed)
     hipLaunchKernelGGL((syrk_dma_kernel<double, true, 2>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
-                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (double*)partial);
+                       xrows, ldo, signs, auxsign, shift, nsplit, npairs, run, (double*)partial);
   else if (dma && precision == HFJK_FP64 && two_wg)
     hipLaunchKernelGGL((syrk_dma_kernel<double, false, 1>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
-                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (double*)partial);
+                       xrows, ldo, signs, auxsign, shift, nsplit, npairs, run, (double*)partial);
   else if (dma && precision == HFJK_FP64)
     hipLaunchKernelGGL((syrk_dma_kernel<double, false, 2>), dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux,
-                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (double*)partial);
+                       xrows, ldo, signs, auxsign, shift, nsplit, npairs, run, (double*)partial);
   else if (dma && is_signed)
     hipLaunchKernelGGL((syrk_dma_kernel<float, true, 2>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux,
-                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (float*)partial);
+                       xrows, ldo, signs, auxsign, shift, nsplit, npairs, run, (float*)partial);
   else if (dma && two_wg)
     hipLaunchKernelGGL((syrk_dma_kernel<float, false, 1>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux,
-                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (float*)partial);
+                       xrows, ldo, signs, auxsign, shift, nsplit, npairs, run, (float*)partial);
   else if (dma)
     hipLaunchKernelGGL((syrk_dma_kernel<float, false, 2>), dim3(grid), dim3(kThreads), 0, s, (const float*)x, naux,
-                       xrows, ldo, signs, auxsign, nsplit, npairs, run, (float*)partial);
+                       xrows, ldo, signs, auxsign, shift, nsplit, npairs, run, (float*)partial);
   else if (precision == HFJK_FP64)
     hipLaunchKernelGGL(syrk_kernel<double>, dim3(grid), dim3(kThreads), 0, s, (const double*)x, naux, xrows, ldo,
                        signs, nsplit, npairs, run, (double*)partial);
@@ -1562,21 +1926,43 @@ hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo,
   else {
     const long long nx = (long long)naux * xrows * ldo;
     const uint16_t* xhi = (const uint16_t*)x;
-    hipLaunchKernelGGL(syrk_bf16x3_kernel, dim3(grid), dim3(kThreads), 0, s, xhi, xhi + nx, naux, xrows, ldo, signs,
-                       nsplit, npairs, run, (float*)partial);
+    const bool ring16 = kind != 1 && ldo <= kDmaMaxLdo && (long long)xrows * ldo * 2 < (1LL << 31);
+    if (ring16 && is_signed)
+      hipLaunchKernelGGL(syrk16_dma_kernel<true>, dim3(grid), dim3(kThreads), 0, s, xhi, xhi + nx, naux, xrows, ldo,
+                         signs, shift, nsplit, npairs, run, (float*)partial);
+    else if (ring16)
+      hipLaunchKernelGGL(syrk16_dma_kernel<false>, dim3(grid), dim3(kThreads), 0, s, xhi, xhi + nx, naux, xrows,
+                         ldo, signs, 0, nsplit, npairs, run, (float*)partial);
+    else
+      hipLaunchKernelGGL(syrk_bf16x3_kernel, dim3(grid), dim3(kThreads), 0, s, xhi, xhi + nx, naux, xrows, ldo,
+                         signs, nsplit, npairs, run, (float*)partial);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_syrk_reduce2(const void* sym, const void* anti, int precision, int nslot, int nao,
+                               bool accumulate, double* out, hipStream_t s) {
+  const int npairs = syrk_pairs(nao);
+  const unsigned grid = (unsigned)(npairs * (kSyrkTile / 8));
+  if (precision == HFJK_FP64)
+    hipLaunchKernelGGL(syrk_reduce2_kernel<double>, dim3(grid), dim3(kThreads), 0, s, (const double*)sym,
+                       (const double*)anti, nslot, npairs, nao, accumulate ? 1 : 0, out);
+  else
+    hipLaunchKernelGGL(syrk_reduce2_kernel<float>, dim3(grid), dim3(kThreads), 0, s, (const float*)sym,
+                       (const float*)anti, nslot, npairs, nao, accumulate ? 1 : 0, out);
   return hipGetLastError();
 }
 
 hipError_t launch_syrk_reduce(const void* partial, int precision, int nslot, int nao, double* out, hipStream_t s) {
   const int npairs = syrk_pairs(nao);
-  const long long n = (long long)nao * nao;
+  // One workgroup per 8-row strip of every lower tile pair.
+  const unsigned grid = (unsigned)(npairs * (kSyrkTile / 8));
   if (precision == HFJK_FP64)
-    hipLaunchKernelGGL(syrk_reduce_kernel<double>, dim3(grid_for(n)), dim3(kThreads), 0, s, (const double*)partial,
-                       nslot, npairs, nao, out);
+    hipLaunchKernelGGL(syrk_reduce_kernel<double>, dim3(grid), dim3(kThreads), 0, s, (const double*)partial, nslot,
+                       npairs, nao, out);
   else
-    hipLaunchKernelGGL(syrk_reduce_kernel<float>, dim3(grid_for(n)), dim3(kThreads), 0, s, (const float*)partial,
-                       nslot, npairs, nao, out);
+    hipLaunchKernelGGL(syrk_reduce_kernel<float>, dim3(grid), dim3(kThreads), 0, s, (const float*)partial, nslot,
+                       npairs, nao, out);
   return hipGetLastError();
 }
 

@@ -108,14 +108,16 @@ HFJK_API int hfjk_split(hfjk_ctx *ctx, const double *packed, long long count, in
 HFJK_API int hfjk_bind(hfjk_ctx *ctx, int naux, int nao, const double *b64, const float *b32,
               const uint16_t *b16);
 
-/* Optional full row-major copies of the bound tensor, for the FP64 and
- * FP32 half-transforms: hfjk_expand_full writes full[q][npad][npad] in
- * precision HFJK_FP64 (double) or HFJK_FP32 (float) (npad = 32 ceil(nao / 32),
- * both triangles, zero past nao) from the tiled packed FP64 tensor;
- * hfjk_bind_full borrows it for that precision (NULL unbinds; hfjk_bind
- * unbinds both). With it the half-transform of that precision streams both
- * operands straight into LDS (twice the memory of the packed copy for
- * 13 % (FP64) and 7 % (FP32) less time at nao 1500, naux 5000). No counterpart in the reference (its B is always full,
+/* Optional full row-major copies of the bound tensor for the
+ * half-transforms: hfjk_expand_full writes full[q][npad][npad] in precision
+ * HFJK_FP64 (double), HFJK_FP32 (float) or HFJK_BF16X3 (uint16: the bf16 hi
+ * plane of every element, then the lo plane; 2 naux npad^2 values)
+ * (npad = 32 ceil(nao / 32), both triangles, zero past nao) from the tiled
+ * packed FP64 tensor; hfjk_bind_full borrows it for that precision (NULL
+ * unbinds; hfjk_bind unbinds all three). With it the half-transform of that
+ * precision streams both operands straight into LDS (twice the memory of the
+ * packed copy for 13 % (FP64) and 7 % (FP32) less time at nao 1500,
+ * naux 5000). No counterpart in the reference (its B is always full,
  * [naux, nao, nao]: coulomb_exchange.py:124-145). */
 HFJK_API int hfjk_expand_full(hfjk_ctx *ctx, const double *packed, int naux, int nao, int precision, void *full,
                               void *stream);
@@ -173,10 +175,9 @@ HFJK_API int hfjk_jk_occ(hfjk_ctx *ctx, const double *orbitals, int nocc, double
  * positive semidefinite (S = 1), otherwise D = V diag(l) V^T by rocSOLVER
  * dsyevd, eigenpairs with |l| <= 1e-12 max|l| dropped, C = V |l|^1/2,
  * S = sign(l); then hfjk_exchange_occ in `precision`.
- * Otherwise the density form itself in FP64 (precision must be HFJK_FP64,
- * else HFJK_ERR_BACKEND): per chunk of aux functions the full rows F_P of
- * B (from the bound full FP64 copy, or expanded into the workspace),
- * G_P = D F_P by a strided-batched DGEMM and K += sum_P F_P G_P by one DGEMM.
+ * Otherwise (non-symmetric) hfjk_exchange_general's factorisation of
+ * S + i A and its two SYRKs, in `precision`, in as many aux passes as the
+ * workspace beyond the density region needs.
  * A complex density is two calls, on its real and imaginary parts.
  * Synchronises `stream` once or twice (the symmetry check, the retained
  * rank). */
@@ -195,17 +196,23 @@ HFJK_API int hfjk_exchange_density(hfjk_ctx *ctx, const double *density, double 
  * signs +1), any other symmetric one by dsyevd (eigenvalues above 1e-12 of
  * the largest kept). Synchronises `stream`. K then is
  * hfjk_exchange_occ(factor, rank, negative ? signs : NULL) with the
- * workspace of hfjk_workspace_bytes(ctx, rank, precision). hfjk_exchange_general: K of
- * any real density in FP64 by the density form (the non-symmetric branch
- * of hfjk_exchange_density). Both take a workspace of at least
- * hfjk_workspace_bytes(ctx, 1, HFJK_FP64) bytes; hfjk_exchange_general
- * processes more aux functions per DGEMM when given more. Replace the same
- * reference call, DensityFittedCoulombExchange.exchange
+ * workspace of hfjk_workspace_bytes(ctx, rank, precision); it takes a
+ * workspace of at least hfjk_workspace_bytes(ctx, 1, HFJK_FP64) bytes.
+ * hfjk_exchange_general: K of any real density, in `precision`, on the
+ * library's kernels: with S, A the symmetric and antisymmetric parts,
+ * S + i A = V diag(l) V^H (rocSOLVER zheevd; |l| <= 1e-12 max |l| dropped)
+ * and X' = B [Re V|l|^1/2, Im V|l|^1/2]: B S B is the signed SYRK of X',
+ * B A B its SYRK with the second operand's two column halves swapped and
+ * signed. The workspace holds as many aux functions per pass as fit;
+ * hfjk_general_workspace_bytes gives the bytes for passes of `aux_chunk`
+ * aux functions (<= 0: all of them at once). Synchronises `stream` once.
+ * Both replace the same reference call, DensityFittedCoulombExchange.exchange
  * (coulomb_exchange.py:184-190). */
 HFJK_API int hfjk_density_factor(hfjk_ctx *ctx, const double *density, double *factor, double *signs, int *rank,
                                  int *negative, void *workspace, size_t workspace_bytes, void *stream);
-HFJK_API int hfjk_exchange_general(hfjk_ctx *ctx, const double *density, double *exchange, void *workspace,
-                                   size_t workspace_bytes, void *stream);
+HFJK_API int hfjk_general_workspace_bytes(hfjk_ctx *ctx, int precision, int aux_chunk, size_t *bytes);
+HFJK_API int hfjk_exchange_general(hfjk_ctx *ctx, const double *density, double *exchange, int precision,
+                                   void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---- Batched J/K: many instances of uneven size, one persistent launch per phase ----
  * No counterpart in the reference (it builds one molecule's J/K at a time:
@@ -329,12 +336,13 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * HFJK_TUNE_SYRK_SPLITS: aux-slice splits of the SYRK (rounded up to a multiple of 8).
  * HFJK_TUNE_X_BM: row tile of the FP64/FP32 half-transform (64, 128; 128 is
  * used with column tiles up to 128).
- * HFJK_TUNE_SYRK_KIND: FP64/FP32 SYRK kernel: 1 = operands staged through
- * registers into one LDS buffer, 2 = operands loaded straight into a ring of
- * LDS stages (occupied widths up to 2048; two workgroups per CU unless the
- * columns carry signs), 3 = the same ring with one workgroup per CU; 0 = the
- * library's choice (2 where it applies).
- * HFJK_TUNE_X_FULL: 0 = the FP64/FP32 half-transform uses the full copy of B
+ * HFJK_TUNE_SYRK_KIND: SYRK kernel: 1 = operands staged through registers
+ * into one LDS buffer (every precision), 2 = operands loaded straight into a
+ * ring of LDS stages (occupied widths up to 2048; FP64/FP32: two workgroups
+ * per CU unless the columns carry signs; BF16X3: one), 3 = the FP64/FP32
+ * ring with one workgroup per CU; 0 = the library's choice (the ring where
+ * it applies).
+ * HFJK_TUNE_X_FULL: 0 = the half-transform uses the full copy of B
  * of its precision when one is bound (hfjk_bind_full), 1 = it ignores it.
  * HFJK_TUNE_GENERAL_CHUNK: most aux functions per chunk of the non-symmetric
  * exchange of hfjk_exchange_density (0 = as many as the workspace holds).

@@ -68,7 +68,7 @@ def test_autotune_sets_a_valid_schedule(tmp_path):
     assert jk_ref.max_relative_error(k, jk_ref.exchange_occupied(b, c)) < BF16X3_REL_TOL
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
 @pytest.mark.parametrize("naux,nao,nocc,splits", [
     (9, 150, 70, 8),     # last 128-row block holds 64 rows of X
     (5, 257, 33, 16),    # more splits than aux functions: empty splits
@@ -91,19 +91,20 @@ def test_syrk_kernels_agree_with_the_oracle(precision, naux, nao, nocc, splits):
         jk.set_tuning(_lib.TUNE_SYRK_KIND, 4)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
 @pytest.mark.parametrize("kind", [1, 2, 3])
-def test_syrk_kernels_apply_signs(kind):
-    """Signed columns (density form of an indefinite matrix) through both SYRK kernels."""
+def test_syrk_kernels_apply_signs(kind, precision):
+    """Signed columns (density form of an indefinite matrix) through every SYRK kernel."""
     from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
 
     naux, nao, nocc = 11, 130, 45
     b, c, _ = synthetic.jk_inputs(29, naux, nao, nocc)
     signs = np.where(np.arange(nocc) % 3 == 0, -1.0, 1.0)
     expect = jk_ref.exchange(b, (c * signs) @ c.T)
-    jk = MI355XCoulombExchange(b, precision="fp64")
+    jk = MI355XCoulombExchange(b, precision=precision)
     jk.set_tuning(_lib.TUNE_SYRK_KIND, kind)
     k = jk.exchange_occupied(c, signs)
-    assert jk_ref.max_relative_error(k, expect) < FP64_REL_TOL
+    assert jk_ref.max_relative_error(k, expect) < TOL[precision]
 
 
 def test_wide_occupied_block_uses_the_register_kernel():
