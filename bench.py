@@ -230,8 +230,8 @@ def bench_clusters(args, dev):
     mixed from the learned guess. The SCF wall time
     excludes the integrals and the fit, which are the same for every run; the
     learned runs include the guess itself (descriptors, network, J of the
-    predicted density, K of the atomic guess, one diagonalisation and the two
-    energies of the fallback test). Energy differences are against the FP64
+    predicted density, K of the atomic guess, one diagonalisation and the
+    Fock build of the learned start). Energy differences are against the FP64
     baseline.
     """
     import os
@@ -309,7 +309,7 @@ def bench_scf_batch(args, dev):
     Integrals and the fit are excluded from every timing (they are the same work for all runs); the
     atomic densities and the isolated water's orbitals are computed once beforehand and shared.
     Timings include the starts: the start's Fock build, and for the learned runs the descriptors,
-    the network, J of the predicted densities, one diagonalisation and the fallback test.
+    the network, J of the predicted densities, one diagonalisation and the learned start's Fock build.
     """
     import os
 

@@ -4,17 +4,19 @@ Every iteration advances all molecules at once: one batched DF-J/K build
 (``BatchedCoulombExchange``: one persistent launch per phase for every
 molecule), strided-batched GEMMs and a batched eigensolver on matrices
 zero-padded to the largest basis, and one host transfer of every
-molecule's energy terms and DIIS overlaps. Per molecule, the iteration is
-the one of ``solve_scf`` (hafnium/scf/driver.py:154-213 of the reference):
-F = h + J[2D] - K[D], the orbital gradient X^T (F D S - S D F) X, DIIS over
-the last ``diis_size`` Fock matrices (hafnium/scf/diis.py:27-106), the
+molecule's energy terms and newest DIIS overlaps (the DIIS overlap
+matrices and coefficients stay on the device). Per molecule, the iteration
+is the one of ``solve_scf`` (hafnium/scf/driver.py:154-213 of the
+reference): F = h + J[2D] - K[D], the orbital gradient X^T (F D S - S D F) X,
+DIIS over the last ``diis_size`` (at most 16) Fock matrices
+(hafnium/scf/diis.py:27-106, solved by ``hfjk_diis_batched``), the
 canonical orthogonaliser (orthogonalization.py:16-42) and aufbau filling.
 Padded and discarded directions of the orthogonal basis get a large
 diagonal shift, so they sort above every occupied level and never mix in.
 
 Precision policy: all molecules share the arithmetic of the J/K build; it
 switches to FP64 (and resets DIIS) once at least half of the unconverged
-molecules have their gradient below ``switch_threshold`` or have stalled
+molecules have their gradient below ``options.switch_at(precision)`` or have stalled
 (``mixed_phase_stalled``); switching early only moves molecules to FP64.
 Convergence of a molecule (|dE| < energy_tolerance and gradient below
 gradient_tolerance) is only declared on FP64 iterations; its energy and
@@ -41,7 +43,7 @@ import numpy as np
 from hafnium_amd import _lib
 from hafnium_amd.errors import BackendError, ConvergenceError, InputError
 from hafnium_amd.molecule import BasisOnAtoms, Molecule
-from hafnium_amd.scf import ScfOptions, mixed_phase_stalled, solve_diis
+from hafnium_amd.scf import ScfOptions, mixed_phase_stalled
 
 DISCARDED_SHIFT = 1.0e3  # Eh added to padded / discarded directions of the orthogonal basis
 
@@ -141,6 +143,7 @@ class BatchedRhf:
         self._ctx = self.jk._ctx
         self.lib = self._ctx.lib
         dense = Dense(self._ctx, self.device)
+        self.dense = dense
         t0 = time.perf_counter()
         # Fit on the GPU, one molecule at a time: M = L L^T, B = L^-1 (P|uv); L is kept
         # (the learned guess needs L^-1 (P|Q) d).
@@ -364,6 +367,8 @@ class BatchedRhf:
                 clock[0] = now
         if options.level_shift or options.damping:
             raise InputError("the batched driver has no level shift or damping")
+        if options.diis_size > 16:
+            raise InputError("the batched driver keeps at most 16 DIIS matrices")
         t0 = time.perf_counter()
         cnt, nm, nn, m = self.count, self.nm, self._nn, options.diis_size
         fh = torch.zeros((m, cnt, nm, nm), dtype=torch.float64, device=self.device)
@@ -371,7 +376,10 @@ class BatchedRhf:
         scal = torch.empty(3 * cnt + cnt * m + 1, dtype=torch.float64, device=self.device)
         terms_dev = scal[:3 * cnt].view(3, cnt)
         row_dev = scal[3 * cnt:3 * cnt + cnt * m].view(cnt, m)
-        ov = np.zeros((cnt, m, m))  # 2 <G_a, G_b> per molecule, one row/column refreshed per iteration
+        # 2 <G_a, G_b> per molecule, on the device, one row/column refreshed per iteration by the
+        # DIIS kernel, which also solves for the coefficients there.
+        ov_dev = torch.zeros((cnt, m, m), dtype=torch.float64, device=self.device)
+        coef_dev = torch.empty((cnt, 1, m), dtype=torch.float64, device=self.device)
         precision = options.precision
         density = start.density
         orbitals = None
@@ -401,10 +409,8 @@ class BatchedRhf:
                 raise BackendError("the batched eigensolver failed on a Fock matrix")
             terms = start.terms if index == 0 else host[:3 * cnt].reshape(3, cnt)
             row = 2.0 * host[3 * cnt:3 * cnt + cnt * m].reshape(cnt, m)
-            ov[:, slot, :] = row
-            ov[:, :, slot] = row
             energy = self.energies(terms)
-            gnorm = np.sqrt(np.maximum(ov[:, slot, slot], 0.0))
+            gnorm = np.sqrt(np.maximum(row[:, slot], 0.0))
             change = energy - previous
             previous = energy
             now = time.perf_counter() - t0
@@ -421,22 +427,18 @@ class BatchedRhf:
                 # stalled at the reduced-precision floor (FP64 iterations never hurt the others).
                 pending = [i for i in range(cnt) if results[i] is None]
                 mixed = [[g for (_, g, p) in history[i] if p != "fp64"] for i in pending]
-                ready = sum(1 for g in mixed if g[-1] < options.switch_threshold
+                ready = sum(1 for g in mixed if g[-1] < options.switch_at(precision)
                             or mixed_phase_stalled(g, options.stall_window, options.stall_ratio))
                 if 2 * ready >= len(mixed) or len(mixed[0]) >= options.max_iterations // 2:
                     precision = "fp64"
                     if options.reset_diis_on_switch:
                         valid = [slot]
-            # DIIS coefficients per molecule over the valid slots, then one batched combination.
-            coef = np.zeros((cnt, 1, m))
-            for i in range(cnt):
-                if len(valid) == 1:
-                    coef[i, 0, valid[0]] = 1.0
-                else:
-                    sub = ov[i][np.ix_(valid, valid)]
-                    coef[i, 0, valid] = solve_diis(sub)
             mark("host")
-            coef_dev = torch.from_numpy(coef).to(self.device)
+            # DIIS coefficients of every molecule over the valid slots on the device, then one
+            # batched combination.
+            slots = (ctypes.c_int * len(valid))(*valid)
+            _lib.check(self.lib.hfjk_diis_batched(self._ctx.handle, cnt, m, _p(ov_dev), _p(row_dev), slot, slots,
+                                                  len(valid), _p(coef_dev), self._stream()))
             extrapolated = torch.empty((cnt, nm, nm), dtype=torch.float64, device=self.device)
             self._gemm(False, False, 1, nn, m, coef_dev, m, m, fh, cnt * nn, nn, extrapolated, nn, nn)
             mark("extrapolate")
@@ -547,15 +549,16 @@ def molecular_start(batch: BatchedRhf, water_orbitals: np.ndarray, precision: st
 def learned_start(batch: BatchedRhf, model, sad: _Start, precision: str = "fp64") -> _Start:
     """F0 = h + J[d] - K[D_base] from predicted fitted coefficients d, D_base the density of the
     given start (``sad``: atomic or molecular superposition). Per molecule the start is the aufbau
-    density of F0 where its energy is not above that of the aufbau density of the base start's Fock
-    matrix (both idempotent: a variational comparison), and the latter elsewhere."""
+    density of F0, with its Fock build, unless its energy exceeds the energy of D_base itself; there
+    the given start is kept unchanged (the fallback)."""
     torch = _torch()
     t0 = time.perf_counter()
     coeffs = model.predict_many([x.molecule for x in batch.inputs])
-    # Charges against B: L^-1 (P|Q) d, with the metric's Cholesky factor kept from the fit.
+    # Charges against B: L^-1 (P|Q) d, with the metric's Cholesky factor kept from the fit; all on the device.
     charges = []
     for x, f, d in zip(batch.inputs, batch.factors, coeffs):
-        v = torch.from_numpy(_host(x.metric) @ d).to(batch.device).view(-1, 1).contiguous()
+        metric = torch.as_tensor(x.metric, dtype=torch.float64).to(batch.device).contiguous()
+        v = batch.dense.gemm(metric, torch.from_numpy(d).to(batch.device).view(-1, 1).contiguous())
         _lib.check(batch.lib.hfjk_dtrsm_lower(batch._ctx.handle, x.naux, 1, _p(f), _p(v), batch._stream()))
         charges.append(v.view(-1))
     j_c = batch.jk.coulomb_from_charges(torch.cat(charges), precision=precision)
@@ -565,12 +568,11 @@ def learned_start(batch: BatchedRhf, model, sad: _Start, precision: str = "fp64"
     guess = torch.empty_like(batch.h)
     batch._combine(1.0, sad.fock, -2.0, sad.info["coulomb"], 1.0, j_pad, guess)
     learned = batch.start_from_fock(guess, precision)
-    fallback = batch.start_from_fock(sad.fock, precision)
     e_learned = batch.energies(learned.terms)
-    e_sad = batch.energies(fallback.terms)
-    flags = e_learned <= e_sad
-    chosen = batch.select(flags, learned, fallback)
+    e_base = batch.energies(sad.terms)
+    flags = e_learned <= e_base
+    chosen = batch.select(flags, learned, sad)
     chosen.info = {"learned": int(flags.sum()), "fallback": int((~flags).sum()),
-                   "energy_learned": e_learned.tolist(), "energy_sad": e_sad.tolist(),
+                   "energy_learned": e_learned.tolist(), "energy_base": e_base.tolist(),
                    "seconds": time.perf_counter() - t0}
     return chosen

@@ -390,7 +390,6 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
   int max_ldo = 32;
   for (int i = 0; i < ninst; ++i) max_ldo = std::max(max_ldo, hfjk::occ_ld(std::max(instances[i].nocc, 1)));
   bt->tile = tile != 0 ? tile : (max_ldo <= 32 ? 32 : max_ldo <= 64 ? 64 : 128);
-  const int rows = hfjk::batch_tile_rows(precision);
   const int es = precision == HFJK_FP64 ? 8 : precision == HFJK_FP32 ? 4 : 2;  // bf16: per plane
   long long w64 = 0, cp = 0, xx = 0, kp = 0, qq = 0;
   for (int i = 0; i < ninst; ++i) {
@@ -410,20 +409,30 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
     in.ldo = hfjk::occ_ld(src.nocc);
     in.nt = hfjk::n_tiles(src.nao);
     in.xrows = hfjk::x_rows(src.nao);
-    // SYRK tile: 64-row blocks where they cut the computed lower-triangle area below 80 % of the
-    // 128-row tiling (small instances, whose few 128-row items also leave a long tail in the
-    // persistent launch); the BF16X3 SYRK has 128-row tiles only.
+    // SYRK tile: 128 rows, on the direct-LDS ring where the instance's X offsets fit it. (Small
+    // instances compute mostly padding in 128-row tiles, but the ring runs the 128-row tile near
+    // the MFMA peak, about twice the rate of the 64-row register-staged tile, so it wins on the
+    // whole batch; the 64-row tile serves only instances too large for the ring.)
     {
-      const long long b64 = (in.xrows + 63) / 64, b128 = (in.xrows + 127) / 128;
-      const long long area64 = b64 * (b64 + 1) / 2 * 64 * 64, area128 = b128 * (b128 + 1) / 2 * 128 * 128;
-      in.st = (precision != HFJK_BF16X3 && 5 * area64 < 4 * area128) ? 64 : hfjk::syrk_tile();
+      const bool fits_ring = in.ldo <= 2048 && (long long)in.xrows * in.ldo * es < (1LL << 31);
+      in.st = fits_ring || precision == HFJK_BF16X3 ? hfjk::syrk_tile() : 64;
       const long long nb = (in.xrows + in.st - 1) / in.st;
       in.npairs = (int)(nb * (nb + 1) / 2);
-      in.pad_ = 0;
+    }
+    // Half-transform column tile: the smallest of 32 / 64 / 128 that covers the instance's
+    // columns, at most the batch's tile (whose multiples then cover them); a batch tile of 256
+    // applies to every instance.
+    if (bt->tile == 256) {
+      in.bn = 256;
+    } else {
+      in.bn = in.ldo <= 32 ? 32 : in.ldo <= 64 ? 64 : 128;
+      in.bn = std::min(in.bn, bt->tile);
     }
     in.nsplit = (int)std::max(1LL, std::min(64LL, ((long long)src.naux * in.ldo + 8191) / 8192));
-    in.nbm = (in.xrows + rows - 1) / rows;
-    in.nbn = (in.ldo + bt->tile - 1) / bt->tile;
+    in.bm = bt->tile == 256 ? hfjk::batch_tile_rows(precision, 0, 256) : hfjk::batch_tile_rows(precision, in.xrows, in.bn);
+    in.pad_ = 0;
+    in.nbm = (in.xrows + in.bm - 1) / in.bm;
+    in.nbn = (in.ldo + in.bn - 1) / in.bn;
     in.p = hfjk::packed_elems(src.nao);
     in.b_off = src.b_offset;
     in.c_off = src.c_offset;
@@ -465,7 +474,11 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
     items[hfjk::kPhCoulombUnpack] = (nn + hfjk::kThreads - 1) / hfjk::kThreads;
     items[hfjk::kPhOrbitals] = ((long long)in.nt * hfjk::kTile * in.ldo + hfjk::kThreads - 1) / hfjk::kThreads;
     items[hfjk::kPhXform] = (long long)in.naux * in.nbm * in.nbn;
-    items[hfjk::kPhSyrk] = (long long)in.npairs * in.nsplit;
+    // Instances whose X offsets fit the ring's 32-bit lane offsets run on it, in a phase of
+    // their own.
+    const bool ring = in.ldo <= 2048 && (long long)in.xrows * in.ldo * es < (1LL << 31);
+    items[hfjk::kPhSyrk] = ring ? 0 : (long long)in.npairs * in.nsplit;
+    items[hfjk::kPhSyrkRing] = ring ? (long long)in.npairs * in.nsplit : 0;
     items[hfjk::kPhSyrkReduce] = (nn + hfjk::kThreads - 1) / hfjk::kThreads;
     items[hfjk::kPhCharges] = in.naux;
     for (int ph = 0; ph < hfjk::kPhCount; ++ph) {

@@ -200,6 +200,96 @@ void check_count(int count) {
   if (count < 1 || count > 65535) bfail(HFJK_ERR_INPUT, "batch count must be in 1..65535");
 }
 
+// DIIS coefficients, one thread per instance (the reference's solve_diis,
+// hafnium/scf/diis.py:82-106): the newest overlaps (dots[i][a] = <e_new,
+// e_a>) first go into row and column `newest` of ov[i] (as 2 <e_a, e_b>);
+// over the valid slots, the bordered system [[B / s, -1], [-1^T, 0]] (s =
+// the largest diagonal of B) is diagonalised by cyclic Jacobi rotations and
+// solved for the right side (0, ..., 0, -1) with eigenvalues |w| <= 1e-14
+// max |w| dropped. coef[i] holds the solution on the valid slots, zero on
+// the others.
+constexpr int kDiisMax = 16;
+
+struct DiisSlots {
+  int n;
+  int slot[kDiisMax];
+};
+
+__global__ __launch_bounds__(64) void diis_kernel(int count, int m, double* __restrict__ ov,
+                                                  const double* __restrict__ dots, int newest, DiisSlots valid,
+                                                  double* __restrict__ coef) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= count) return;
+  double* o = ov + (long long)i * m * m;
+  for (int a = 0; a < m; ++a) {
+    const double v = 2.0 * dots[(long long)i * m + a];
+    o[newest * m + a] = v;
+    o[a * m + newest] = v;
+  }
+  double* c = coef + (long long)i * m;
+  for (int a = 0; a < m; ++a) c[a] = 0.0;
+  const int nv = valid.n;
+  if (nv == 1) {
+    c[valid.slot[0]] = 1.0;
+    return;
+  }
+  const int n = nv + 1;
+  double a[kDiisMax + 1][kDiisMax + 1], v[kDiisMax + 1][kDiisMax + 1];
+  double scale = 0.0;
+  for (int p = 0; p < nv; ++p) scale = fmax(scale, o[valid.slot[p] * m + valid.slot[p]]);
+  scale = scale > 0.0 ? scale : 1.0;
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) {
+      a[p][q] = p < nv && q < nv ? o[valid.slot[p] * m + valid.slot[q]] / scale : (p == nv && q == nv ? 0.0 : -1.0);
+      v[p][q] = p == q ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int p = 0; p < n; ++p) {
+      diag += a[p][p] * a[p][p];
+      for (int q = p + 1; q < n; ++q) off += a[p][q] * a[p][q];
+    }
+    if (off <= 1e-34 * diag) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[p][q];
+        if (apq == 0.0) continue;
+        const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+        const double cs = 1.0 / sqrt(1.0 + t * t);
+        const double sn = t * cs;
+        for (int k = 0; k < n; ++k) {
+          const double kp = a[k][p], kq = a[k][q];
+          a[k][p] = cs * kp - sn * kq;
+          a[k][q] = sn * kp + cs * kq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double pk = a[p][k], qk = a[q][k];
+          a[p][k] = cs * pk - sn * qk;
+          a[q][k] = sn * pk + cs * qk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double kp = v[k][p], kq = v[k][q];
+          v[k][p] = cs * kp - sn * kq;
+          v[k][q] = sn * kp + cs * kq;
+        }
+      }
+  }
+  double wmax = 0.0;
+  for (int j = 0; j < n; ++j) wmax = fmax(wmax, fabs(a[j][j]));
+  double z[kDiisMax + 1];
+  for (int j = 0; j < n; ++j) {
+    const double w = a[j][j];
+    // y_j = (V^T r)_j with r = (0, ..., 0, -1).
+    z[j] = fabs(w) > 1e-14 * wmax ? -v[n - 1][j] / w : 0.0;
+  }
+  for (int p = 0; p < nv; ++p) {
+    double x = 0.0;
+    for (int j = 0; j < n; ++j) x += v[p][j] * z[j];
+    c[valid.slot[p]] = x;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -268,6 +358,27 @@ int hfjk_history_dots_batched(hfjk_ctx* ctx, int count, long long len, int nslot
     hipLaunchKernelGGL(history_dots_kernel, dim3(count, nslots), dim3(kT), 0, (hipStream_t)stream, len, count, hist,
                        newest, out);
     bcheck_hip(hipGetLastError(), "history_dots launch");
+  });
+}
+
+int hfjk_diis_batched(hfjk_ctx* ctx, int count, int nslots, double* ov, const double* dots, int newest,
+                      const int* valid, int nvalid, double* coef, void* stream) {
+  return bguard(ctx, [&] {
+    check_count(count);
+    if (nslots < 1 || nslots > kDiisMax || newest < 0 || newest >= nslots || nvalid < 1 || nvalid > nslots ||
+        valid == nullptr || ov == nullptr || dots == nullptr || coef == nullptr)
+      bfail(HFJK_ERR_INPUT, "hfjk_diis_batched: 1 <= nvalid <= nslots <= 16, 0 <= newest < nslots and buffers needed");
+    DiisSlots slots{};
+    slots.n = nvalid;
+    for (int a = 0; a < nvalid; ++a) {
+      if (valid[a] < 0 || valid[a] >= nslots) bfail(HFJK_ERR_INPUT, "hfjk_diis_batched: a valid slot is out of range");
+      for (int b = 0; b < a; ++b)
+        if (valid[b] == valid[a]) bfail(HFJK_ERR_INPUT, "hfjk_diis_batched: a valid slot is repeated");
+      slots.slot[a] = valid[a];
+    }
+    hipLaunchKernelGGL(diis_kernel, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, (hipStream_t)stream, count,
+                       nslots, ov, dots, newest, slots, coef);
+    bcheck_hip(hipGetLastError(), "diis launch");
   });
 }
 

@@ -279,8 +279,9 @@ int syrk_tile();
 struct BatchInst {
   int nao, naux, nocc, ldo;
   int nt, xrows, npairs, nsplit;
-  int nbm, nbn;       // half-transform row / column blocks at the batch tile
-  int st, pad_;       // SYRK tile edge (64 or 128) and padding
+  int nbm, nbn;       // half-transform row / column blocks of the instance's tile
+  int bm, pad_;       // half-transform tile rows (64 or 128; BF16x3: 128) and padding
+  int st, bn;         // SYRK tile edge (64 or 128); half-transform column tile (32, 64, 128 or 256)
   long long p;        // packed elements per aux function
   long long b_off;    // batch tensor (element units of the bound copy)
   long long c_off;    // orbitals [nao][nocc] (float64)
@@ -307,6 +308,7 @@ enum BatchPhase {
   kPhSyrk,
   kPhSyrkReduce,
   kPhCharges,  // charges from the half-transformed X (D = C C^T)
+  kPhSyrkRing,  // SYRK items of the instances with 128-row tiles (FP64 / FP32 direct-LDS ring)
   kPhCount
 };
 
@@ -337,7 +339,9 @@ hipError_t launch_batch_syrk(BatchView v, int precision, const void* xbase, long
                              void* kbase, int grid, hipStream_t s);
 hipError_t launch_batch_syrk_reduce(BatchView v, int precision, const void* kbase, double* k, int grid,
                                     hipStream_t s);
-int batch_tile_rows(int precision);  // rows of a half-transform tile: 64 (FP64/FP32) or 128 (BF16x3)
+// Rows of an instance's half-transform tile: FP64 / FP32 128 when the instance has at least 128
+// X rows and a column tile of 64, else 64; BF16x3 128.
+int batch_tile_rows(int precision, int xrows, int bn);
 
 // Density factorisation helpers (jk.hip).
 hipError_t launch_split_factor(const double* vecs, const double* vals, int nao, int first, int count,

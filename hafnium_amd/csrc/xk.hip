@@ -134,11 +134,14 @@ struct XformSmem {
 
 // One BM x BN tile (row block mb, column block nb) of X_q = B_q C for one
 // aux function: bq = its packed B, xq = its X rows (xrows of them). The
-// whole workgroup calls it; LDS is free again when it returns.
-template <typename T, int BN, int BM = 64>
+// whole workgroup calls it; LDS is free again when it returns. With SKIP the
+// MFMAs of 16-row blocks at or past nrow, of 16-column blocks at or past ncol
+// and of k steps at or past kmax are left out (B and C are zero there, so X
+// is the same); each wave decides for its own blocks.
+template <typename T, int BN, int BM = 64, bool SKIP = false>
 __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, const T* __restrict__ c, int ldo,
                                            T* __restrict__ xq, int xrows, int mb, int nb,
-                                           XformSmem<T, BN, BM>& sm) {
+                                           XformSmem<T, BN, BM>& sm, int nrow = 0, int ncol = 0, int kmax = 0) {
   typedef XShape<BN> S;
   constexpr int NTL = BM / kTile;                  // layout tiles per row block
   constexpr int MI = BM / S::WM / 16;              // 16-row blocks per wave
@@ -210,6 +213,16 @@ __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, con
     }
   };
 
+  // Wave-uniform block masks of SKIP.
+  bool rv[MI], cv[S::NI];
+  if constexpr (SKIP) {
+    const int wu = __builtin_amdgcn_readfirstlane(t >> 6);
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) rv[mi] = mb * BM + (wu % S::WM) * (BM / S::WM) + mi * 16 < nrow;
+#pragma unroll
+    for (int ni = 0; ni < S::NI; ++ni) cv[ni] = n0 + (wu / S::WM) * (BN / S::WN) + ni * 16 < ncol;
+  }
+
   load(0);
   store();
   __syncthreads();
@@ -222,6 +235,9 @@ __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, con
     for (int h = 0; h < NTL; ++h) direct[h] = (NTL * mb + h) >= kt;
 #pragma unroll
     for (int kk = 0; kk < kTile; kk += 4) {
+      if constexpr (SKIP) {
+        if (kt * kTile + kk >= kmax) break;
+      }
       const int k = kk + (lane >> 4);
       T af[MI], bf[S::NI];
 #pragma unroll
@@ -236,7 +252,12 @@ __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, con
 #pragma unroll
       for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < S::NI; ++ni) acc[mi][ni] = Op<T>::mma(af[mi], bf[ni], acc[mi][ni]);
+        for (int ni = 0; ni < S::NI; ++ni) {
+          if constexpr (SKIP) {
+            if (!(rv[mi] && cv[ni])) continue;
+          }
+          acc[mi][ni] = Op<T>::mma(af[mi], bf[ni], acc[mi][ni]);
+        }
     }
     __syncthreads();
     if (kt + 1 < nt) {
@@ -1047,6 +1068,60 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
   }
 }
 
+// One 128 x 128 tile (mb, nb) of sum_{q in [qb, qe)} X_q X_q^T on the ring
+// (all column signs +1: X's padded columns are zero), stored row-major into
+// dst. The whole workgroup calls it; its LDS reads of an earlier call must
+// have returned (a workgroup barrier between calls).
+template <typename T, int CH>
+__device__ __forceinline__ void syrk_ring_tile(const T* __restrict__ x, int xrows, int ldo, int mb, int nb, int qb,
+                                               int qe, T* __restrict__ dst) {
+  typedef SyrkDma<T, CH> D;
+  constexpr int E = D::E;
+  constexpr int LOADS = D::LOADS;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+  // Lane byte offsets within one aux function's X, as in syrk_dma_kernel;
+  // rows past xrows read row 0 (their products land in rows never read).
+  int off[LOADS];
+#pragma unroll
+  for (int v = 0; v < LOADS; ++v) {
+    const int ch = wave * LOADS + v;
+    const int blk = ch >= 8 * CH ? nb : mb;
+    int row = blk * kSyrkTile + ((ch / CH) % 8) * 16 + (lane & 15);
+    row = row < xrows ? row : 0;
+    off[v] = (row * ldo + (ch % CH) * D::KC + (lane >> 4) * E) * (int)sizeof(T);
+  }
+  const int nch = ldo / D::KS;
+  auto src = [&](int step, int operand) -> const T* {
+    (void)operand;
+    const int qq = step / nch;
+    return x + (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * D::KS;
+  };
+  auto sign_at = [&](int) { return 0; };
+  auto scale_at = [&](int) -> T { return (T)1; };
+  typename Op<T>::acc_t acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
+  dma_ring<T, false, CH>(src, off, (qe - qb) * nch, (const T*)nullptr, sign_at, scale_at, acc);
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const int col = wn * 64 + ni * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = wm * 64 + mi * 16 + Op<T>::row(lane, r);
+        dst[row * kSyrkTile + col] = acc[mi][ni][r];
+      }
+    }
+}
+
 // FP64 / FP32 half-transform on the ring, from a full row-major copy of B
 // (bfull[q][npad][npad], both triangles) and C^T (ct[ldo][npad], zero past
 // nocc and past nao): X_q[m][i] = sum_k B_q[m][k] C^T[i][k], one 128 x 128
@@ -1483,6 +1558,48 @@ __global__ __launch_bounds__(kThreads, 1) void syrk16_dma_kernel(const uint16_t*
         for (int r = 0; r < 4; ++r) dst[(wm * 64 + mi * 16 + 4 * (lane >> 4) + r) * kSyrkTile + col] = acc[mi][ni][r];
       }
   }
+}
+
+// One 128 x 128 tile (mb, nb) of sum_{q in [qb, qe)} X_q X_q^T in BF16x3 on
+// the ring (all column signs +1) from X's hi/lo planes, stored row-major
+// into dst. Called by the whole workgroup, a barrier between calls.
+__device__ __forceinline__ void syrk16_ring_tile(const uint16_t* __restrict__ xhi, const uint16_t* __restrict__ xlo,
+                                                 int xrows, int ldo, int mb, int nb, int qb, int qe,
+                                                 float* __restrict__ dst) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+  int off[kRing16Loads];
+#pragma unroll
+  for (int v = 0; v < kRing16Loads; ++v) {
+    const int ch = wave * kRing16Loads + v;
+    const int blk = (ch >> 1) & 7;
+    int row = ((ch >> 4) ? nb : mb) * kSyrkTile + blk * 16 + (lane & 15);
+    row = row < xrows ? row : 0;
+    off[v] = (row * ldo + (lane >> 4) * 8) * 2;
+  }
+  const int nch = ldo / 32;
+  auto src = [&](int step, int operand, int plane) -> const uint16_t* {
+    (void)operand;
+    const int qq = step / nch;
+    return (plane ? xlo : xhi) + (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * 32;
+  };
+  auto sign_at = [&](int) { return 0; };
+  f4 acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
+  dma_ring16<false>(src, off, (qe - qb) * nch, (const uint16_t*)nullptr, sign_at, acc);
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const int col = wn * 64 + ni * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dst[(wm * 64 + mi * 16 + 4 * (lane >> 4) + r) * kSyrkTile + col] = acc[mi][ni][r];
+    }
 }
 
 // BF16x3 half-transform over the ring from the full bf16 hi/lo copy of B
@@ -2020,6 +2137,110 @@ __global__ __launch_bounds__(kThreads) void batch_xform16_kernel(BatchView v, co
   }
 }
 
+// The half-transform with each instance's own tile (in.bm x in.bn: 64 x 32,
+// 64 x 64, 64 x 128 or 128 x 64): the tile shapes share one LDS buffer.
+template <typename T>
+union XformSmemAny {
+  XformSmem<T, 32> s32;
+  XformSmem<T, 64> s64;
+  XformSmem<T, 128> s128;
+  XformSmem<T, 64, 128> t64;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads, 2) void batch_xform_any_kernel(BatchView v, const T* __restrict__ b,
+                                                                    const T* __restrict__ cbase,
+                                                                    T* __restrict__ xbase) {
+  __shared__ XformSmemAny<T> sm;
+  const long long* pre = v.prefix + (long long)kPhXform * (v.ninst + 1);
+  for (;;) {
+    long long end;
+    const long long item = batch_next(v.prefix, v.counters, v.ninst, kPhXform, end);
+    if (item >= end) break;
+    const int i = batch_find(pre, v.ninst, item);
+    const BatchInst in = v.insts[i];
+    const long long local = item - pre[i];
+    const int nb = (int)(local % in.nbn);
+    const int mb = (int)((local / in.nbn) % in.nbm);
+    const int q = (int)(local / ((long long)in.nbn * in.nbm));
+    const T* bq = b + in.b_off + (long long)q * in.p;
+    const T* cp = cbase + in.cp_off;
+    T* xq = xbase + in.x_off + (long long)q * in.xrows * in.ldo;
+    if (in.bm == 128)
+      xform_tile<T, 64, 128, true>(bq, in.nt, cp, in.ldo, xq, in.xrows, mb, nb, sm.t64, in.nao, in.nocc, in.nao);
+    else if (in.bn == 32)
+      xform_tile<T, 32, 64, true>(bq, in.nt, cp, in.ldo, xq, in.xrows, mb, nb, sm.s32, in.nao, in.nocc, in.nao);
+    else if (in.bn == 64)
+      xform_tile<T, 64, 64, true>(bq, in.nt, cp, in.ldo, xq, in.xrows, mb, nb, sm.s64, in.nao, in.nocc, in.nao);
+    else
+      xform_tile<T, 128, 64, true>(bq, in.nt, cp, in.ldo, xq, in.xrows, mb, nb, sm.s128, in.nao, in.nocc, in.nao);
+  }
+}
+
+union Xform16SmemAny {
+  Xform16Smem<32> s32;
+  Xform16Smem<64> s64;
+  Xform16Smem<128> s128;
+};
+
+__global__ __launch_bounds__(kThreads) void batch_xform16_any_kernel(BatchView v, const uint16_t* __restrict__ b,
+                                                                      long long b_plane,
+                                                                      const uint16_t* __restrict__ cbase,
+                                                                      long long c_plane, uint16_t* __restrict__ xbase,
+                                                                      long long x_plane) {
+  __shared__ __attribute__((aligned(16))) Xform16SmemAny sm;
+  const long long* pre = v.prefix + (long long)kPhXform * (v.ninst + 1);
+  for (;;) {
+    long long end;
+    const long long item = batch_next(v.prefix, v.counters, v.ninst, kPhXform, end);
+    if (item >= end) break;
+    const int i = batch_find(pre, v.ninst, item);
+    const BatchInst in = v.insts[i];
+    const long long local = item - pre[i];
+    const int nb = (int)(local % in.nbn);
+    const int mb = (int)((local / in.nbn) % in.nbm);
+    const int q = (int)(local / ((long long)in.nbn * in.nbm));
+    const uint16_t* hq = b + in.b_off + (long long)q * in.p;
+    const uint16_t* ct = cbase + in.cp_off;
+    uint16_t* xh = xbase + in.x_off + (long long)q * in.xrows * in.ldo;
+    if (in.bn == 32)
+      xform16_tile<32>(hq, hq + b_plane, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh, xh + x_plane, in.xrows,
+                       mb, nb, sm.s32);
+    else if (in.bn == 64)
+      xform16_tile<64>(hq, hq + b_plane, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh, xh + x_plane, in.xrows,
+                       mb, nb, sm.s64);
+    else
+      xform16_tile<128>(hq, hq + b_plane, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh, xh + x_plane,
+                        in.xrows, mb, nb, sm.s128);
+  }
+}
+
+// SYRK items of the instances with 128-row tiles whose X offsets fit the
+// direct-LDS ring (32-bit byte offsets within one aux function's X): one
+// (tile pair, split) per item, the partial layout of batch_syrk_kernel.
+template <typename T>
+__global__ __launch_bounds__(kThreads, 2) void batch_syrk_ring_kernel(BatchView v, const T* __restrict__ xbase,
+                                                                       T* __restrict__ kbase) {
+  const long long* pre = v.prefix + (long long)kPhSyrkRing * (v.ninst + 1);
+  for (;;) {
+    long long end;
+    // batch_next's barriers also order this item's LDS loads after the previous item's reads.
+    const long long item = batch_next(v.prefix, v.counters, v.ninst, kPhSyrkRing, end);
+    if (item >= end) break;
+    const int i = batch_find(pre, v.ninst, item);
+    const BatchInst in = v.insts[i];
+    const long long local = item - pre[i];
+    const int pair = (int)(local % in.npairs);
+    const int s = (int)(local / in.npairs);
+    int mb, nb;
+    tile_of(pair, mb, nb);
+    const int qb = (int)((long long)s * in.naux / in.nsplit);
+    const int qe = (int)((long long)(s + 1) * in.naux / in.nsplit);
+    syrk_ring_tile<T, 1>(xbase + in.x_off, in.xrows, in.ldo, mb, nb, qb, qe,
+                         kbase + in.kp_off + ((long long)s * in.npairs + pair) * (kSyrkTile * kSyrkTile));
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void batch_syrk_kernel(BatchView v, const T* __restrict__ xbase,
                                                                const double* __restrict__ ws64, T* __restrict__ kbase) {
@@ -2070,6 +2291,31 @@ __global__ __launch_bounds__(kThreads) void batch_syrk16_kernel(BatchView v, con
   }
 }
 
+// The BF16x3 SYRK items on the direct-LDS ring (every instance whose X offsets fit it).
+__global__ __launch_bounds__(kThreads, 1) void batch_syrk16_ring_kernel(BatchView v,
+                                                                         const uint16_t* __restrict__ xbase,
+                                                                         long long x_plane,
+                                                                         float* __restrict__ kbase) {
+  const long long* pre = v.prefix + (long long)kPhSyrkRing * (v.ninst + 1);
+  for (;;) {
+    long long end;
+    const long long item = batch_next(v.prefix, v.counters, v.ninst, kPhSyrkRing, end);
+    if (item >= end) break;
+    const int i = batch_find(pre, v.ninst, item);
+    const BatchInst in = v.insts[i];
+    const long long local = item - pre[i];
+    const int pair = (int)(local % in.npairs);
+    const int s = (int)(local / in.npairs);
+    int mb, nb;
+    tile_of(pair, mb, nb);
+    const int qb = (int)((long long)s * in.naux / in.nsplit);
+    const int qe = (int)((long long)(s + 1) * in.naux / in.nsplit);
+    const uint16_t* xh = xbase + in.x_off;
+    syrk16_ring_tile(xh, xh + x_plane, in.xrows, in.ldo, mb, nb, qb, qe,
+                     kbase + in.kp_off + ((long long)s * in.npairs + pair) * (kSyrkTile * kSyrkTile));
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void batch_syrk_reduce_kernel(BatchView v, const T* __restrict__ kbase,
                                                                       double* __restrict__ kout) {
@@ -2106,35 +2352,35 @@ __global__ __launch_bounds__(kThreads) void batch_syrk_reduce_kernel(BatchView v
 
 }  // namespace
 
-int batch_tile_rows(int precision) { return precision == HFJK_BF16X3 ? 128 : kBlock; }
+int batch_tile_rows(int precision, int xrows, int bn) {
+  if (precision == HFJK_BF16X3) return 128;
+  return xrows >= 128 && bn == 64 ? 128 : kBlock;
+}
 
 hipError_t launch_batch_xform(BatchView v, int precision, int bn, const void* b, long long b_plane, const void* cbase,
                               long long c_plane, void* xbase, long long x_plane, int grid, hipStream_t s) {
   const dim3 g((unsigned)grid), blk(kThreads);
+  // bn = 256: one fixed tile for every instance; otherwise each instance's own tile.
   if (precision == HFJK_FP64) {
     const double* bb = (const double*)b;
     const double* cb = (const double*)cbase;
     double* xb = (double*)xbase;
-    if (bn == 32) hipLaunchKernelGGL((batch_xform_kernel<double, 32>), g, blk, 0, s, v, bb, cb, xb);
-    else if (bn == 64) hipLaunchKernelGGL((batch_xform_kernel<double, 64>), g, blk, 0, s, v, bb, cb, xb);
-    else if (bn == 128) hipLaunchKernelGGL((batch_xform_kernel<double, 128>), g, blk, 0, s, v, bb, cb, xb);
-    else hipLaunchKernelGGL((batch_xform_kernel<double, 256>), g, blk, 0, s, v, bb, cb, xb);
+    if (bn == 256) hipLaunchKernelGGL((batch_xform_kernel<double, 256>), g, blk, 0, s, v, bb, cb, xb);
+    else hipLaunchKernelGGL(batch_xform_any_kernel<double>, g, blk, 0, s, v, bb, cb, xb);
   } else if (precision == HFJK_FP32) {
     const float* bb = (const float*)b;
     const float* cb = (const float*)cbase;
     float* xb = (float*)xbase;
-    if (bn == 32) hipLaunchKernelGGL((batch_xform_kernel<float, 32>), g, blk, 0, s, v, bb, cb, xb);
-    else if (bn == 64) hipLaunchKernelGGL((batch_xform_kernel<float, 64>), g, blk, 0, s, v, bb, cb, xb);
-    else if (bn == 128) hipLaunchKernelGGL((batch_xform_kernel<float, 128>), g, blk, 0, s, v, bb, cb, xb);
-    else hipLaunchKernelGGL((batch_xform_kernel<float, 256>), g, blk, 0, s, v, bb, cb, xb);
+    if (bn == 256) hipLaunchKernelGGL((batch_xform_kernel<float, 256>), g, blk, 0, s, v, bb, cb, xb);
+    else hipLaunchKernelGGL(batch_xform_any_kernel<float>, g, blk, 0, s, v, bb, cb, xb);
   } else {
     const uint16_t* bb = (const uint16_t*)b;
     const uint16_t* cb = (const uint16_t*)cbase;
     uint16_t* xb = (uint16_t*)xbase;
-    if (bn == 32) hipLaunchKernelGGL((batch_xform16_kernel<32>), g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
-    else if (bn == 64) hipLaunchKernelGGL((batch_xform16_kernel<64>), g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
-    else if (bn == 128) hipLaunchKernelGGL((batch_xform16_kernel<128>), g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
-    else hipLaunchKernelGGL((batch_xform16_kernel<256>), g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
+    if (bn == 256)
+      hipLaunchKernelGGL((batch_xform16_kernel<256>), g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
+    else
+      hipLaunchKernelGGL(batch_xform16_any_kernel, g, blk, 0, s, v, bb, b_plane, cb, c_plane, xb, x_plane);
   }
   return hipGetLastError();
 }
@@ -2142,12 +2388,18 @@ hipError_t launch_batch_xform(BatchView v, int precision, int bn, const void* b,
 hipError_t launch_batch_syrk(BatchView v, int precision, const void* xbase, long long x_plane, const double* ws64,
                              void* kbase, int grid, hipStream_t s) {
   const dim3 g((unsigned)grid), blk(kThreads);
-  if (precision == HFJK_FP64)
+  // grid = two workgroups per CU, the FP64 / FP32 ring kernel's occupancy; workgroups that find
+  // no CU free start late, find their share taken and leave.
+  if (precision == HFJK_FP64) {
     hipLaunchKernelGGL(batch_syrk_kernel<double>, g, blk, 0, s, v, (const double*)xbase, ws64, (double*)kbase);
-  else if (precision == HFJK_FP32)
+    hipLaunchKernelGGL(batch_syrk_ring_kernel<double>, g, blk, 0, s, v, (const double*)xbase, (double*)kbase);
+  } else if (precision == HFJK_FP32) {
     hipLaunchKernelGGL(batch_syrk_kernel<float>, g, blk, 0, s, v, (const float*)xbase, ws64, (float*)kbase);
-  else
+    hipLaunchKernelGGL(batch_syrk_ring_kernel<float>, g, blk, 0, s, v, (const float*)xbase, (float*)kbase);
+  } else {
     hipLaunchKernelGGL(batch_syrk16_kernel, g, blk, 0, s, v, (const uint16_t*)xbase, x_plane, ws64, (float*)kbase);
+    hipLaunchKernelGGL(batch_syrk16_ring_kernel, g, blk, 0, s, v, (const uint16_t*)xbase, x_plane, (float*)kbase);
+  }
   return hipGetLastError();
 }
 

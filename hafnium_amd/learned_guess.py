@@ -23,8 +23,8 @@ squared error is the Coulomb self-energy of each atom's density error.
 Guess: ``F0 = h + J[d] - K[D_sad]`` (Coulomb from the predicted density,
 exchange from the superposition of atomic densities); the aufbau density
 of F0 starts the SCF. When its energy is above the energy of the atomic
-guess's own aufbau density (both idempotent, so the comparison is
-variational), the SCF starts from the atomic guess instead (the fallback).
+guess, the SCF starts from the atomic guess instead (the fallback), exactly
+as it would without the network.
 
 Training data: converged FP64 SCF densities of seeded water clusters
 produced by this library's own GPU SCF (``generate_dataset``).
@@ -170,44 +170,71 @@ def water_frames(molecule: Molecule) -> np.ndarray:
     nat = len(molecule.charges)
     if nat % 3 or any(tuple(molecule.charges[3 * k:3 * k + 3]) != (8, 1, 1) for k in range(nat // 3)):
         raise InputError("the learned guess is defined for water clusters in (O, H, H) order")
-    rots = np.empty((nat, 3, 3))
-    for k in range(nat // 3):
-        o, h1, h2 = pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]
-        z = (h1 - o) + (h2 - o)
-        z /= np.linalg.norm(z)
-        x = h1 - h2
-        x -= np.dot(x, z) * z
-        x /= np.linalg.norm(x)
-        rots[3 * k:3 * k + 3] = np.stack([x, np.cross(z, x), z], axis=1)
-    return rots
+    w = np.asarray(pos, dtype=np.float64).reshape(nat // 3, 3, 3)  # [water, (O, H1, H2), xyz]
+    z = (w[:, 1] - w[:, 0]) + (w[:, 2] - w[:, 0])
+    z /= np.linalg.norm(z, axis=1, keepdims=True)
+    x = w[:, 1] - w[:, 2]
+    x -= np.sum(x * z, axis=1, keepdims=True) * z
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    rots = np.stack([x, np.cross(z, x), z], axis=2)  # columns: local axes
+    return np.repeat(rots, 3, axis=0)
 
 
-def descriptors(molecule: Molecule) -> np.ndarray:
-    """``[natom, N_FEATURES]`` environment descriptors in each atom's water frame."""
-    pos = molecule.positions / ANGSTROM_IN_BOHR  # Å
-    charges = np.asarray(molecule.charges)
-    rots = water_frames(molecule)
+def _descriptor_rows(pos: np.ndarray, charges: np.ndarray, group: np.ndarray, rots: np.ndarray) -> np.ndarray:
+    """Descriptors of atoms at ``pos`` (Bohr) whose neighbours are the atoms of the same ``group``
+    (groups are contiguous runs of atoms)."""
+    pos = pos / ANGSTROM_IN_BOHR  # Å
     centres = np.linspace(R_MIN, R_MAX, N_RADIAL)
     width = centres[1] - centres[0]
     nat = len(charges)
     n_ang = (L_DESCRIPTOR + 1) ** 2
     feats = np.zeros((nat, len(ELEMENTS), N_RADIAL * n_ang))
-    # All (atom a, neighbour b) pairs within the cutoff at once.
-    d = pos[None, :, :] - pos[:, None, :]
-    r = np.linalg.norm(d, axis=2)
-    ia, ib = np.nonzero((r > 1e-8) & (r < R_MAX))
+    # The (atom a, neighbour b) pairs within the cutoff, group by group; then every pair at once.
+    pairs = []
+    bounds = np.flatnonzero(np.diff(np.concatenate([[-1], group, [-2]]))) if nat else np.zeros(1, dtype=np.int64)
+    for lo, hi in zip(bounds[:-1], bounds[1:]):
+        r = np.linalg.norm(pos[None, lo:hi, :] - pos[lo:hi, None, :], axis=2)
+        a, b = np.nonzero((r > 1e-8) & (r < R_MAX))
+        pairs.append((a + lo, b + lo))
+    ia = np.concatenate([p[0] for p in pairs]) if pairs else np.zeros(0, dtype=np.int64)
+    ib = np.concatenate([p[1] for p in pairs]) if pairs else np.zeros(0, dtype=np.int64)
     if len(ia) == 0:
         return feats.reshape(nat, -1)
-    rp = r[ia, ib]
-    local = np.einsum("pk,pkj->pj", d[ia, ib], rots[ia]) / rp[:, None]
+    d = pos[ib] - pos[ia]
+    rp = np.linalg.norm(d, axis=1)
+    local = np.einsum("pk,pkj->pj", d, rots[ia]) / rp[:, None]
     cut = 0.5 * (np.cos(np.pi * rp / R_MAX) + 1.0)
     radial = np.exp(-0.5 * ((rp[:, None] - centres[None, :]) / width) ** 2) * cut[:, None]
     ang = np.concatenate([solid_harmonics(l, local) for l in range(L_DESCRIPTOR + 1)], axis=1)
     outer = (radial[:, :, None] * ang[:, None, :]).reshape(len(ia), -1)
+    # Pairs come ordered by atom a, so each atom's neighbours of one element are one run of rows.
     for e_i, z in enumerate(ELEMENTS):
-        sel = charges[ib] == z
-        np.add.at(feats[:, e_i], ia[sel], outer[sel])
+        sel = np.flatnonzero(charges[ib] == z)
+        if len(sel) == 0:
+            continue
+        atom = ia[sel]
+        starts = np.flatnonzero(np.concatenate([[True], atom[1:] != atom[:-1]]))
+        feats[atom[starts], e_i] = np.add.reduceat(outer[sel], starts, axis=0)
     return feats.reshape(nat, -1)
+
+
+def descriptors(molecule: Molecule) -> np.ndarray:
+    """``[natom, N_FEATURES]`` environment descriptors in each atom's water frame."""
+    charges = np.asarray(molecule.charges)
+    return _descriptor_rows(np.asarray(molecule.positions, dtype=np.float64), charges,
+                            np.zeros(len(charges), dtype=np.int64), water_frames(molecule))
+
+
+def descriptors_many(molecules) -> list[np.ndarray]:
+    """:func:`descriptors` of many molecules, computed together (neighbours stay within a molecule)."""
+    if not molecules:
+        return []
+    counts = [len(m.charges) for m in molecules]
+    rows = _descriptor_rows(np.concatenate([np.asarray(m.positions, dtype=np.float64) for m in molecules]),
+                            np.concatenate([np.asarray(m.charges) for m in molecules]),
+                            np.repeat(np.arange(len(molecules)), counts),
+                            np.concatenate([water_frames(m) for m in molecules]))
+    return np.split(rows, np.cumsum(counts)[:-1])
 
 
 # ---------------------------------------------------------------- coefficient layout
@@ -297,6 +324,39 @@ def to_lab(molecule: Molecule, local: dict, aux_name: str) -> np.ndarray:
     return out
 
 
+def to_lab_many(molecules, locals_: list, aux_name: str) -> list[np.ndarray]:
+    """:func:`to_lab` of many molecules: the frames of all waters rotated in one pass per element."""
+    if not molecules:
+        return []
+    waters = [len(m.charges) // 3 for m in molecules]
+    dms = rotation_matrices_many(np.concatenate([water_frames(m)[0::3] for m in molecules]), _lmax(aux_name))
+    first_water = np.concatenate([[0], np.cumsum(waters)[:-1]])
+    outs = []
+    for m in molecules:
+        offs = atom_offsets(m, aux_name)
+        outs.append(np.zeros(offs[-1] + shell_blocks(aux_name, m.charges[-1])[1]))
+    for z in ELEMENTS:
+        n = shell_blocks(aux_name, z)[1]
+        mol_of, atom_of, coef = [], [], []
+        for k, (m, loc) in enumerate(zip(molecules, locals_)):
+            atoms = np.nonzero(np.asarray(m.charges) == z)[0]
+            if len(atoms) == 0:
+                continue
+            mol_of.append(np.full(len(atoms), k))
+            atom_of.append(atoms)
+            coef.append(np.asarray(loc[z], dtype=np.float64)[:len(atoms)])
+        if not coef:
+            continue
+        mol_of, atom_of = np.concatenate(mol_of), np.concatenate(atom_of)
+        water = first_water[mol_of] + atom_of // 3
+        lab = _rotate_atoms(np.concatenate(coef), [d[water] for d in dms], _block_columns(aux_name, z))
+        for k in np.unique(mol_of):
+            sel = np.nonzero(mol_of == k)[0]
+            offs = np.asarray(atom_offsets(molecules[k], aux_name))[atom_of[sel]]
+            outs[k][(offs[:, None] + np.arange(n)[None, :]).reshape(-1)] = lab[sel].reshape(-1)
+    return outs
+
+
 def fitted_coefficients(metric: np.ndarray, three_center: np.ndarray, density_alpha) -> np.ndarray:
     """``d = M^-1 (P|uv) (2 D_alpha)``: the Coulomb-fitted coefficients of the total density."""
     naux = metric.shape[0]
@@ -384,7 +444,7 @@ class LearnedGuessModel:
     def predict_many(self, molecules) -> list[np.ndarray]:
         """Laboratory-frame coefficients of many molecules: one network pass per element for all atoms."""
         torch = _torch()
-        feats = [descriptors(m) for m in molecules]
+        feats = descriptors_many(molecules)
         charges = [np.asarray(m.charges) for m in molecules]
         local = [dict() for _ in molecules]
         for z in ELEMENTS:
@@ -400,7 +460,7 @@ class LearnedGuessModel:
             for k, r in enumerate(rows):
                 local[k][z] = y[at:at + len(r)]
                 at += len(r)
-        return [to_lab(m, loc, self.aux_name) for m, loc in zip(molecules, local)]
+        return to_lab_many(molecules, local, self.aux_name)
 
     def save(self, path: str) -> None:
         arrays = {"aux_name": np.array(self.aux_name), "hidden": np.array(HIDDEN)}
@@ -684,14 +744,14 @@ def learned_guess(system, model: LearnedGuessModel, sad_density: np.ndarray, *, 
 
     F_sad = h + J[2 D_sad] - K[D_sad] is the Fock matrix of the atomic guess;
     F0 = h + J[d] - K[D_sad] uses the predicted coefficients d instead, with
-    J[d] = sum_P q_P B[P], q = L^-1 (P|Q) d, on the GPU. The aufbau densities
-    of F0 and of F_sad are both idempotent, so their energies compare
-    variationally: the lower one starts the SCF (the first is the learned
-    start, the second the atomic guess's first iterate, i.e. the fallback),
-    with its Fock build handed on as the SCF's first iteration.
+    J[d] = sum_P q_P B[P], q = L^-1 (P|Q) d, on the GPU. The SCF starts from
+    the aufbau density of F0 (the learned start, its Fock build handed on as
+    the SCF's first iteration) unless its energy exceeds the energy of the
+    atomic guess itself; then it starts from the atomic guess with F_sad as
+    its first iteration, exactly as without the network (the fallback).
     Returns ``(GuessStart, info)``; ``info["used"]`` is "learned" or "sad".
     """
-    from hafnium_amd.scf import _Setup, ScfOptions
+    from hafnium_amd.scf import _Setup, ScfOptions, _energy_terms
 
     torch = _torch()
     if system.metric is None:
@@ -700,22 +760,25 @@ def learned_guess(system, model: LearnedGuessModel, sad_density: np.ndarray, *, 
     tb = h.two_body
     dn = h.dense
     t0 = time.perf_counter()
-    coeffs = model.predict(system.molecule)
-    v = torch.from_numpy(system.metric @ coeffs).to(h.device).view(-1, 1).contiguous()
-    charges = tb.fit_charges(v).view(-1)
+    coeffs = torch.from_numpy(model.predict(system.molecule)).to(h.device).view(-1, 1).contiguous()
+    metric = torch.as_tensor(system.metric, dtype=torch.float64).to(h.device).contiguous()
+    charges = tb.fit_charges(dn.gemm(metric, coeffs)).view(-1)
     j_pred = tb.coulomb_from_charges(charges, precision=precision)
     sad = torch.from_numpy(np.ascontiguousarray(sad_density)).to(h.device)
     j_sad = tb.coulomb(sad, precision=precision)
     k_sad = tb.exchange(sad, precision=precision)
     f_sad = dn.combine(1.0, h.core, 2.0, j_sad, -1.0, k_sad)
+    terms_sad = torch.empty(3, dtype=torch.float64, device=h.device)
+    dn.dot(sad, h.core, terms_sad[0:1])
+    dn.dot(sad, j_sad, terms_sad[1:2])
+    dn.dot(sad, k_sad, terms_sad[2:3])
     f0 = dn.combine(1.0, h.core, 1.0, j_pred, -1.0, k_sad)
-    setup = _Setup(h, ScfOptions())
-    learned, e_learned = _aufbau_start(h, setup, f0, precision)
-    fallback, e_sad = _aufbau_start(h, setup, f_sad, precision)
+    learned, e_learned = _aufbau_start(h, _Setup(h, ScfOptions()), f0, precision)
+    e_sad = _energy_terms(h, terms_sad.cpu().numpy()).total
     use = e_learned <= e_sad
     info = {"used": "learned" if use else "sad", "energy_learned": e_learned, "energy_sad": e_sad,
             "seconds": time.perf_counter() - t0}
-    return (learned if use else fallback), info
+    return (learned if use else GuessStart(sad, None, (f_sad, terms_sad))), info
 
 
 def evaluate(model: "LearnedGuessModel", seeds, sizes, *, precision: str = "fp64", log=print) -> dict:

@@ -12,7 +12,8 @@ the last result as ``last`` (hafnium/errors.py:33-43).
 
 Added for MI355X: a precision policy. J and K run in ``options.precision``
 ("bf16x3", "fp32" or "fp64") while the orbital-gradient norm (the DIIS
-error) is at or above ``options.switch_threshold`` and still falling
+error) is at or above ``options.switch_at(precision)`` (the switch
+threshold, raised to the precision's floor) and still falling
 (``mixed_phase_stalled``), then in FP64 for every remaining iteration;
 convergence is only declared on FP64 iterations, and
 the final rebuild is FP64, so the converged energy is that of the FP64
@@ -57,6 +58,11 @@ class ScfOptions:
     overlap_threshold: float = DEFAULT_OVERLAP_THRESHOLD
     precision: str = "fp64"
     switch_threshold: float = 1e-5
+    # Gradient norms below which a reduced-precision build no longer tells the FP64 gradient:
+    # about three times the distance between the reduced-precision and the FP64 fixed points
+    # measured on water clusters of up to 12 waters (FP32 3e-5, BF16x3 3.6e-4).
+    switch_floor_fp32: float = 1e-4
+    switch_floor_bf16x3: float = 1e-3
     reset_diis_on_switch: bool = True
     stall_window: int = 2
     stall_ratio: float = 0.7
@@ -78,6 +84,14 @@ class ScfOptions:
             raise InputError("precision is fp64, fp32 or bf16x3")
         if self.stall_window < 1 or not 0.0 < self.stall_ratio <= 1.0:
             raise InputError("stall_window >= 1 and 0 < stall_ratio <= 1 are required")
+        if self.switch_threshold <= 0.0 or self.switch_floor_fp32 < 0.0 or self.switch_floor_bf16x3 < 0.0:
+            raise InputError("the switch threshold is positive and the floors non-negative")
+
+    def switch_at(self, precision: str) -> float:
+        """Gradient norm below which a ``precision`` iteration hands over to FP64: the switch
+        threshold, or that precision's floor where the floor is higher."""
+        floor = {"fp32": self.switch_floor_fp32, "bf16x3": self.switch_floor_bf16x3}.get(precision, 0.0)
+        return max(self.switch_threshold, floor)
 
 
 def mixed_phase_stalled(gradient_norms, window: int, ratio: float) -> bool:
@@ -391,7 +405,7 @@ def solve_scf(hamiltonian: RestrictedHamiltonian, *, guess=None, options: ScfOpt
             converged = True
             break
         mixed_norms = [row.gradient_norm for row in history if row.precision != "fp64"]
-        if precision != "fp64" and (gnorm < options.switch_threshold
+        if precision != "fp64" and (gnorm < options.switch_at(precision)
                                     or mixed_phase_stalled(mixed_norms, options.stall_window, options.stall_ratio)
                                     or len(mixed_norms) >= options.max_iterations // 2):
             precision = "fp64"

@@ -401,6 +401,18 @@ HFJK_API int hfjk_ddot_batched(hfjk_ctx *ctx, int count, long long len, const do
  * summation order; nslots <= 64). */
 HFJK_API int hfjk_history_dots_batched(hfjk_ctx *ctx, int count, long long len, int nslots, const double *hist,
                                        int newest, double *out, void *stream);
+/* DIIS coefficients of count instances, on the device (replaces the host
+ * solve_diis of hafnium/scf/diis.py:82-106 in the batched loop): ov is
+ * [count][nslots][nslots] (2 <e_a, e_b>, kept by the caller across calls);
+ * row and column `newest` are first set from dots ([count][nslots], the
+ * output of hfjk_history_dots_batched, doubled). Over the host list
+ * valid[0..nvalid) of slots (distinct, in the caller's order), the bordered
+ * system [[B / s, -1], [-1^T, 0]] (s the largest diagonal of B) is solved
+ * through its eigenpairs with |w| <= 1e-14 max |w| dropped; coef
+ * ([count][nslots]) gets the solution on the valid slots, zero elsewhere.
+ * nslots <= 16. */
+HFJK_API int hfjk_diis_batched(hfjk_ctx *ctx, int count, int nslots, double *ov, const double *dots, int newest,
+                               const int *valid, int nvalid, double *coef, void *stream);
 /* out_i = m_i - m_i^T for count [n][n] matrices at stride n*n. */
 HFJK_API int hfjk_antisymmetrize_batched(hfjk_ctx *ctx, int n, int count, const double *m, double *out,
                                          void *stream);

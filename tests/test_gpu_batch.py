@@ -37,7 +37,7 @@ def torch():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "bf16x3", "fp32"])
-@pytest.mark.parametrize("tile", [0, 32, 128])
+@pytest.mark.parametrize("tile", [0, 32, 64, 128, 256])
 def test_every_instance_matches_the_oracle(torch, precision, tile):
     seeds = list(range(1000, 1000 + len(SHAPES)))
     bt, cs, ds = build(torch, seeds)

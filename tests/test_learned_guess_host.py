@@ -186,3 +186,19 @@ def test_datasets_merge_and_recompute_their_descriptors():
             mol = water_cluster(n, seed)
             direct.append(lg.descriptors(mol)[np.asarray(mol.charges) == z].astype(np.float32))
         np.testing.assert_array_equal(merged[f"x_{z}"], np.concatenate(direct))
+
+
+def test_batched_descriptors_and_rotations_match_one_molecule_at_a_time():
+    mols = [water_cluster(n, 4000 + n) for n in (1, 2, 5, 3)]
+    many = lg.descriptors_many(mols)
+    for m, f in zip(mols, many):
+        np.testing.assert_array_equal(f, lg.descriptors(m))
+    rng = np.random.default_rng(5)
+    local = [{z: rng.standard_normal((int(np.sum(np.asarray(m.charges) == z)), lg.shell_blocks(AUX, z)[1]))
+              for z in lg.ELEMENTS} for m in mols]
+    for m, l, lab in zip(mols, local, lg.to_lab_many(mols, local, AUX)):
+        np.testing.assert_allclose(lab, lg.to_lab(m, l, AUX), atol=1e-14)
+        # and back: the local coefficients are recovered from the laboratory frame.
+        back = lg.to_local(m, lab, AUX)
+        for z in lg.ELEMENTS:
+            np.testing.assert_allclose(back[z], l[z], atol=1e-12)

@@ -65,3 +65,17 @@ def test_balanced_groups_deal_by_cost():
     cost = [sum([48, 288, 96, 240, 144, 192][i] ** 4 for i in g) for g in groups]
     assert max(cost) / min(cost) < 1.6
     assert balanced_groups([10], 3) == [[0]]
+
+
+def test_switch_point_is_the_threshold_raised_to_the_precision_floor():
+    from hafnium_amd.errors import InputError
+    from hafnium_amd.scf import ScfOptions
+
+    opts = ScfOptions()
+    assert opts.switch_at("fp64") == opts.switch_threshold == 1e-5
+    assert opts.switch_at("fp32") == 1e-4 and opts.switch_at("bf16x3") == 1e-3
+    low = ScfOptions(switch_floor_fp32=0.0, switch_floor_bf16x3=1e-6)
+    assert low.switch_at("fp32") == 1e-5 and low.switch_at("bf16x3") == 1e-5
+    assert ScfOptions(switch_threshold=2e-3).switch_at("bf16x3") == 2e-3
+    with pytest.raises(InputError):
+        ScfOptions(switch_floor_fp32=-1.0)
