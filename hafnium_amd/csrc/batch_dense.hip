@@ -200,14 +200,15 @@ void check_count(int count) {
   if (count < 1 || count > 65535) bfail(HFJK_ERR_INPUT, "batch count must be in 1..65535");
 }
 
-// DIIS coefficients, one thread per instance (the reference's solve_diis,
-// hafnium/scf/diis.py:82-106): the newest overlaps (dots[i][a] = <e_new,
-// e_a>) first go into row and column `newest` of ov[i] (as 2 <e_a, e_b>);
-// over the valid slots, the bordered system [[B / s, -1], [-1^T, 0]] (s =
-// the largest diagonal of B) is diagonalised by cyclic Jacobi rotations and
-// solved for the right side (0, ..., 0, -1) with eigenvalues |w| <= 1e-14
-// max |w| dropped. coef[i] holds the solution on the valid slots, zero on
-// the others.
+// DIIS coefficients, one 64-lane workgroup per instance (the reference's
+// solve_diis, hafnium/scf/diis.py:82-106): the newest overlaps (dots[i][a]
+// = <e_new, e_a>) first go into row and column `newest` of ov[i] (as 2
+// <e_a, e_b>); over the valid slots, the bordered system [[B / s, -1],
+// [-1^T, 0]] (s = the largest diagonal of B) is diagonalised by cyclic
+// Jacobi rotations in LDS (lane k updates row / column entry k of each
+// rotation) and solved for the right side (0, ..., 0, -1) with eigenvalues
+// |w| <= 1e-14 max |w| dropped. coef[i] holds the solution on the valid
+// slots, zero on the others.
 constexpr int kDiisMax = 16;
 
 struct DiisSlots {
@@ -215,78 +216,92 @@ struct DiisSlots {
   int slot[kDiisMax];
 };
 
-__global__ __launch_bounds__(64) void diis_kernel(int count, int m, double* __restrict__ ov,
-                                                  const double* __restrict__ dots, int newest, DiisSlots valid,
-                                                  double* __restrict__ coef) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= count) return;
+__global__ __launch_bounds__(64) void diis_kernel(int m, double* __restrict__ ov, const double* __restrict__ dots,
+                                                  int newest, DiisSlots valid, double* __restrict__ coef) {
+  constexpr int N = kDiisMax + 1;
+  __shared__ double a[N][N + 1], v[N][N + 1];
+  __shared__ double rot[2];
+  __shared__ int done;
+  const int i = blockIdx.x;
+  const int k = threadIdx.x;
   double* o = ov + (long long)i * m * m;
-  for (int a = 0; a < m; ++a) {
-    const double v = 2.0 * dots[(long long)i * m + a];
-    o[newest * m + a] = v;
-    o[a * m + newest] = v;
-  }
   double* c = coef + (long long)i * m;
-  for (int a = 0; a < m; ++a) c[a] = 0.0;
+  if (k < m) {
+    const double x = 2.0 * dots[(long long)i * m + k];
+    o[newest * m + k] = x;
+    o[k * m + newest] = x;
+    c[k] = 0.0;
+  }
+  __syncthreads();
   const int nv = valid.n;
   if (nv == 1) {
-    c[valid.slot[0]] = 1.0;
+    if (k == 0) c[valid.slot[0]] = 1.0;
     return;
   }
   const int n = nv + 1;
-  double a[kDiisMax + 1][kDiisMax + 1], v[kDiisMax + 1][kDiisMax + 1];
   double scale = 0.0;
   for (int p = 0; p < nv; ++p) scale = fmax(scale, o[valid.slot[p] * m + valid.slot[p]]);
   scale = scale > 0.0 ? scale : 1.0;
-  for (int p = 0; p < n; ++p)
+  if (k < n)
     for (int q = 0; q < n; ++q) {
-      a[p][q] = p < nv && q < nv ? o[valid.slot[p] * m + valid.slot[q]] / scale : (p == nv && q == nv ? 0.0 : -1.0);
-      v[p][q] = p == q ? 1.0 : 0.0;
+      a[k][q] = k < nv && q < nv ? o[valid.slot[k] * m + valid.slot[q]] / scale : (k == nv && q == nv ? 0.0 : -1.0);
+      v[k][q] = k == q ? 1.0 : 0.0;
     }
+  __syncthreads();
   for (int sweep = 0; sweep < 64; ++sweep) {
-    double off = 0.0, diag = 0.0;
-    for (int p = 0; p < n; ++p) {
-      diag += a[p][p] * a[p][p];
-      for (int q = p + 1; q < n; ++q) off += a[p][q] * a[p][q];
+    if (k == 0) {
+      double off = 0.0, diag = 0.0;
+      for (int p = 0; p < n; ++p) {
+        diag += a[p][p] * a[p][p];
+        for (int q = p + 1; q < n; ++q) off += a[p][q] * a[p][q];
+      }
+      done = off <= 1e-34 * diag;
     }
-    if (off <= 1e-34 * diag) break;
+    __syncthreads();
+    if (done) break;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
-        const double apq = a[p][q];
-        if (apq == 0.0) continue;
-        const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
-        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
-        const double cs = 1.0 / sqrt(1.0 + t * t);
-        const double sn = t * cs;
-        for (int k = 0; k < n; ++k) {
+        if (k == 0) {
+          const double apq = a[p][q];
+          double cs = 1.0, sn = 0.0;
+          if (apq != 0.0) {
+            const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+            cs = 1.0 / sqrt(1.0 + t * t);
+            sn = t * cs;
+          }
+          rot[0] = cs;
+          rot[1] = sn;
+        }
+        __syncthreads();
+        const double cs = rot[0], sn = rot[1];
+        if (k < n) {
           const double kp = a[k][p], kq = a[k][q];
           a[k][p] = cs * kp - sn * kq;
           a[k][q] = sn * kp + cs * kq;
+          const double vp = v[k][p], vq = v[k][q];
+          v[k][p] = cs * vp - sn * vq;
+          v[k][q] = sn * vp + cs * vq;
         }
-        for (int k = 0; k < n; ++k) {
+        __syncthreads();
+        if (k < n) {
           const double pk = a[p][k], qk = a[q][k];
           a[p][k] = cs * pk - sn * qk;
           a[q][k] = sn * pk + cs * qk;
         }
-        for (int k = 0; k < n; ++k) {
-          const double kp = v[k][p], kq = v[k][q];
-          v[k][p] = cs * kp - sn * kq;
-          v[k][q] = sn * kp + cs * kq;
-        }
+        __syncthreads();
       }
   }
-  double wmax = 0.0;
-  for (int j = 0; j < n; ++j) wmax = fmax(wmax, fabs(a[j][j]));
-  double z[kDiisMax + 1];
-  for (int j = 0; j < n; ++j) {
-    const double w = a[j][j];
-    // y_j = (V^T r)_j with r = (0, ..., 0, -1).
-    z[j] = fabs(w) > 1e-14 * wmax ? -v[n - 1][j] / w : 0.0;
-  }
-  for (int p = 0; p < nv; ++p) {
+  // x_p = sum_j v[p][j] z_j, z_j = (V^T r)_j / w_j = -v[n-1][j] / w_j over the kept eigenvalues.
+  if (k < nv) {
+    double wmax = 0.0;
+    for (int j = 0; j < n; ++j) wmax = fmax(wmax, fabs(a[j][j]));
     double x = 0.0;
-    for (int j = 0; j < n; ++j) x += v[p][j] * z[j];
-    c[valid.slot[p]] = x;
+    for (int j = 0; j < n; ++j) {
+      const double w = a[j][j];
+      if (fabs(w) > 1e-14 * wmax) x += v[k][j] * (-v[n - 1][j] / w);
+    }
+    c[valid.slot[k]] = x;
   }
 }
 
@@ -376,8 +391,8 @@ int hfjk_diis_batched(hfjk_ctx* ctx, int count, int nslots, double* ov, const do
         if (valid[b] == valid[a]) bfail(HFJK_ERR_INPUT, "hfjk_diis_batched: a valid slot is repeated");
       slots.slot[a] = valid[a];
     }
-    hipLaunchKernelGGL(diis_kernel, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, (hipStream_t)stream, count,
-                       nslots, ov, dots, newest, slots, coef);
+    hipLaunchKernelGGL(diis_kernel, dim3((unsigned)count), dim3(64), 0, (hipStream_t)stream, nslots, ov, dots, newest,
+                       slots, coef);
     bcheck_hip(hipGetLastError(), "diis launch");
   });
 }
