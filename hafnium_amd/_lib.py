@@ -102,6 +102,7 @@ SIGNATURES = [
     ("hfjk_ddot_batched", _c_int, [_vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_ll, _vp, _vp]),
     ("hfjk_history_dots_batched", _c_int, [_vp, _c_int, _c_ll, _c_int, _vp, _c_int, _vp, _vp]),
     ("hfjk_diis_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int, ctypes.POINTER(_c_int), _c_int, _vp, _vp]),
+    ("hfjk_diis_combine_batched", _c_int, [_vp, _c_int, _c_ll, _c_int, _vp, _vp, _vp, _vp]),
     ("hfjk_antisymmetrize_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp]),
     ("hfjk_symmetrize_batched", _c_int, [_vp, _c_int, _c_ll, _vp, _vp]),
     ("hfjk_orthogonalizer_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),

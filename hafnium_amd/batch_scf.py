@@ -440,7 +440,8 @@ class BatchedRhf:
             _lib.check(self.lib.hfjk_diis_batched(self._ctx.handle, cnt, m, _p(ov_dev), _p(row_dev), slot, slots,
                                                   len(valid), _p(coef_dev), self._stream()))
             extrapolated = torch.empty((cnt, nm, nm), dtype=torch.float64, device=self.device)
-            self._gemm(False, False, 1, nn, m, coef_dev, m, m, fh, cnt * nn, nn, extrapolated, nn, nn)
+            _lib.check(self.lib.hfjk_diis_combine_batched(self._ctx.handle, cnt, nn, m, _p(fh), _p(coef_dev),
+                                                          _p(extrapolated), self._stream()))
             mark("extrapolate")
             density, _, orbitals = self.solve(extrapolated, mark)
             slot = (slot + 1) % m

@@ -305,6 +305,24 @@ __global__ __launch_bounds__(64) void diis_kernel(int m, double* __restrict__ ov
   }
 }
 
+// The DIIS extrapolation out[i] = sum_a coef[i][a] hist[a][i] of count
+// instances (hist [nslots][count][len]); slots with a zero coefficient are
+// not read.
+__global__ __launch_bounds__(kT) void diis_combine_kernel(long long len, int count, int nslots,
+                                                          const double* __restrict__ hist,
+                                                          const double* __restrict__ coef, double* __restrict__ out) {
+  const int i = blockIdx.y;
+  const double* ci = coef + (long long)i * nslots;
+  for (long long e = (long long)blockIdx.x * kT + threadIdx.x; e < len; e += (long long)gridDim.x * kT) {
+    double acc = 0.0;
+    for (int a = 0; a < nslots; ++a) {
+      const double w = ci[a];
+      if (w != 0.0) acc += w * hist[((long long)a * count + i) * len + e];
+    }
+    out[(long long)i * len + e] = acc;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -394,6 +412,20 @@ int hfjk_diis_batched(hfjk_ctx* ctx, int count, int nslots, double* ov, const do
     hipLaunchKernelGGL(diis_kernel, dim3((unsigned)count), dim3(64), 0, (hipStream_t)stream, nslots, ov, dots, newest,
                        slots, coef);
     bcheck_hip(hipGetLastError(), "diis launch");
+  });
+}
+
+int hfjk_diis_combine_batched(hfjk_ctx* ctx, int count, long long len, int nslots, const double* hist,
+                              const double* coef, double* out, void* stream) {
+  return bguard(ctx, [&] {
+    check_count(count);
+    if (len < 0 || nslots < 1 || hist == nullptr || coef == nullptr || out == nullptr)
+      bfail(HFJK_ERR_INPUT, "hfjk_diis_combine_batched: bad input");
+    if (len == 0) return;
+    const unsigned gx = (unsigned)std::min<long long>(1024, blocks_for(len));
+    hipLaunchKernelGGL(diis_combine_kernel, dim3(gx, count), dim3(kT), 0, (hipStream_t)stream, len, count, nslots,
+                       hist, coef, out);
+    bcheck_hip(hipGetLastError(), "diis_combine launch");
   });
 }
 

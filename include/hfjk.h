@@ -413,6 +413,12 @@ HFJK_API int hfjk_history_dots_batched(hfjk_ctx *ctx, int count, long long len, 
  * nslots <= 16. */
 HFJK_API int hfjk_diis_batched(hfjk_ctx *ctx, int count, int nslots, double *ov, const double *dots, int newest,
                                const int *valid, int nvalid, double *coef, void *stream);
+/* The DIIS extrapolation of count instances (diis.py:55-79):
+ * out[i] = sum_a coef[i][a] hist[a][i] over len elements, hist
+ * [nslots][count][len], coef [count][nslots] (as hfjk_diis_batched writes
+ * them; zero-coefficient slots are not read), out [count][len]. */
+HFJK_API int hfjk_diis_combine_batched(hfjk_ctx *ctx, int count, long long len, int nslots, const double *hist,
+                                       const double *coef, double *out, void *stream);
 /* out_i = m_i - m_i^T for count [n][n] matrices at stride n*n. */
 HFJK_API int hfjk_antisymmetrize_batched(hfjk_ctx *ctx, int n, int count, const double *m, double *out,
                                          void *stream);

@@ -73,3 +73,26 @@ def test_device_diis_rejects_bad_slot_lists():
     for valid, nslots in (([1, 1], 4), ([4], 4), ([0], 17)):
         slots = (ctypes.c_int * len(valid))(*valid)
         assert ctx.lib.hfjk_diis_batched(ctx.handle, 1, nslots, p, p, 0, slots, len(valid), p, None) == _lib.ERR_INPUT
+
+
+def test_device_extrapolation_combines_the_stored_matrices():
+    import torch
+
+    from hafnium_amd import _lib
+    from hafnium_amd.coulomb_exchange import _Context
+
+    ctx = _Context(0)
+    rng = np.random.default_rng(3)
+    nslots, count, length = 5, 3, 1000
+    hist = rng.standard_normal((nslots, count, length))
+    coef = rng.standard_normal((count, nslots))
+    coef[1, 2] = 0.0
+    hist[2, 1] = np.nan  # a slot with a zero coefficient is not read
+    dev = torch.device("cuda", 0)
+    h, c = torch.from_numpy(hist).to(dev), torch.from_numpy(coef).to(dev)
+    out = torch.empty((count, length), dtype=torch.float64, device=dev)
+    _lib.check(ctx.lib.hfjk_diis_combine_batched(ctx.handle, count, length, nslots, ctypes.c_void_p(h.data_ptr()),
+                                                 ctypes.c_void_p(c.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    want = np.einsum("ia,aie->ie", coef, np.nan_to_num(hist))
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-13, atol=1e-13)
