@@ -100,20 +100,38 @@ struct hfjk_ctx {
 };
 
 namespace hfjk {
-// Persistent work fetch: the workgroup's next item of phase `ph` from its
-// XCD's share [total x / 8, total (x + 1) / 8), x = blockIdx.x % 8; returns
-// a value >= the share's end when the share is exhausted. All threads of
-// the workgroup must call it together.
+// Persistent work fetch: the workgroup's next item of phase `ph`. Items are
+// dealt in eight shares [total x / 8, total (x + 1) / 8), one per XCD (x =
+// blockIdx.x % 8, so an instance's items stay in one XCD's L2); once its own
+// share is exhausted a workgroup takes items from the other shares in turn,
+// so no XCD idles while another still has work. Returns the item and sets
+// share_end to the end of the share it came from; both are `total` once
+// every share is exhausted. All threads of the workgroup must call it
+// together.
 __device__ inline long long batch_next(const long long* prefix, unsigned long long* counters, int ninst, int ph,
                                        long long& share_end) {
-  __shared__ long long s_item;
+  __shared__ long long s_item, s_end;
   const long long total = prefix[(long long)ph * (ninst + 1) + ninst];
-  const int x = blockIdx.x & 7;
-  const long long lo = total * x / 8;
-  share_end = total * (x + 1) / 8;
-  if (threadIdx.x == 0) s_item = lo + (long long)atomicAdd(&counters[ph * 8 + x], 1ULL);
+  if (threadIdx.x == 0) {
+    const int x0 = blockIdx.x & 7;
+    long long item = total, end = total;
+    for (int d = 0; d < 8; ++d) {
+      const int x = (x0 + d) & 7;
+      const long long lo = total * x / 8, hi = total * (x + 1) / 8;
+      if (lo >= hi) continue;
+      const long long it = lo + (long long)atomicAdd(&counters[ph * 8 + x], 1ULL);
+      if (it < hi) {
+        item = it;
+        end = hi;
+        break;
+      }
+    }
+    s_item = item;
+    s_end = end;
+  }
   __syncthreads();
   const long long item = s_item;
+  share_end = s_end;
   __syncthreads();
   return item;
 }
