@@ -57,18 +57,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel, n, q, o):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
+def measured_traffic(kernel, n, q, o, live_ms=None):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary,
+    and whether it still describes the kernel timed here.
 
     The counters cannot be read inside this process; profiles/traffic_large.json
-    holds the scaled per-launch values of the last counter runs on this workload.
+    holds the scaled per-launch values of the last counter runs on this workload
+    and the kernel's average duration in the kernel trace of the same runs.
+    Returns ``(bytes or None, note)``: when that duration and ``live_ms`` differ
+    by more than 10 % the kernel has changed since the counters were taken, and
+    the bytes are withheld.
     """
     path = os.path.join(ROOT, "profiles", "traffic_large.json")
     if (n, q, o) != (1500, 5000, 250) or not os.path.exists(path):
-        return None
+        return None, "no counter profile of this workload"
     with open(path) as fh:
         data = json.load(fh)
-    return data.get("hbm_bytes_per_launch", {}).get(kernel)
+    value = data.get("hbm_bytes_per_launch", {}).get(kernel)
+    if value is None:
+        return None, f"no counter profile of {kernel}"
+    profiled_ms = data.get("avg_ms_per_launch", {}).get(kernel)
+    if live_ms is not None and profiled_ms and abs(live_ms - profiled_ms) > 0.1 * profiled_ms:
+        return None, f"stale: profiled at {profiled_ms:.2f} ms per launch, timed here at {live_ms:.2f} ms"
+    return value, f"profiles/traffic_large.json (kernel at {profiled_ms} ms per launch when profiled)"
 
 
 def cpu_baseline(jk, wl, slice_q):
@@ -620,8 +631,9 @@ def main(argv=None):
         elif head == "bf16x3" and use_full.get("bf16x3"):
             kernel = "xform16_dma_kernel"
             b_bytes = q * n * n * 4.0
+        traffic, traffic_note = measured_traffic(kernel, n, q, o, x_ms)
         roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved, "peak": peak,
-                    "unit": "TFLOP/s", "frac": achieved / peak, "traffic": measured_traffic(kernel, n, q, o),
+                    "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_note,
                     "algorithmic_flops_per_launch": x_flops,
                     "algorithmic_bytes_per_launch": b_bytes + n * q * o * (8 if head == "fp64" else 4)}
         if syrk_ms:

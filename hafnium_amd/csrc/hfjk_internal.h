@@ -298,7 +298,7 @@ struct BatchInst {
   int nao, naux, nocc, ldo;
   int nt, xrows, npairs, nsplit;
   int nbm, nbn;       // half-transform row / column blocks of the instance's tile
-  int bm, pad_;       // half-transform tile rows (64 or 128; BF16x3: 128) and padding
+  int bm, qg;         // half-transform tile rows (64 or 128; BF16x3: 128); aux functions per item (FP64 / FP32)
   int st, bn;         // SYRK tile edge (64 or 128); half-transform column tile (32, 64, 128 or 256)
   long long p;        // packed elements per aux function
   long long b_off;    // batch tensor (element units of the bound copy)

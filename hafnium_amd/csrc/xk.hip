@@ -280,6 +280,173 @@ __device__ __forceinline__ void xform_tile(const T* __restrict__ bq, int nt, con
     }
 }
 
+// The half-transform tile of xform_tile for each aux function q in
+// [q0, q1) (the batched kernels' items): b = the packed B of aux function 0 (q's at
+// stride p), x = X of aux function 0 (q's at stride xrows * ldo). The
+// (q, k tile) steps run as one pipeline: the next step's loads are issued
+// before the current step's MFMAs, across aux functions too, and each q's
+// tile is written out after its last k tile. The whole workgroup calls it;
+// LDS is free again when it returns. With SKIP the MFMAs of 16-row blocks
+// at or past nrow, of 16-column blocks at or past ncol and of k steps at or
+// past kmax are left out (B and C are zero there, so X is the same); each
+// wave decides for its own blocks.
+template <typename T, int BN, int BM = 64, bool SKIP = false>
+__device__ __forceinline__ void xform_tiles(const T* __restrict__ b, long long p, int nt, const T* __restrict__ c,
+                                            int ldo, T* __restrict__ x, int xrows, int mb, int nb, int q0, int q1,
+                                            XformSmem<T, BN, BM>& sm, int nrow = 0, int ncol = 0, int kmax = 0) {
+  typedef XShape<BN> S;
+  constexpr int NTL = BM / kTile;                  // layout tiles per row block
+  constexpr int MI = BM / S::WM / 16;              // 16-row blocks per wave
+  constexpr int CCH = kTile * BN / 2 / kThreads;  // 2-element chunks of the C tile per thread
+  auto& At = sm.At;
+  auto& Cs = sm.Cs;
+  const int n0 = nb * BN;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave % S::WM;
+  const int wn = wave / S::WM;
+
+  typename Op<T>::acc_t acc[MI][S::NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < S::NI; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
+
+  // Prefetch registers: 4 consecutive elements of each of the NTL tiles,
+  // and CCH 2-element chunks of the C tile.
+  typename Op<T>::vec2 ra[NTL][2];
+  typename Op<T>::vec2 rc[CCH];
+
+  auto load = [&](int q, int kt) {
+    const T* bq = b + (long long)q * p;
+#pragma unroll
+    for (int h = 0; h < NTL; ++h) {
+      const int r = NTL * mb + h;
+      if (r < nt) {
+        const long long off = (r >= kt ? tidx(r, kt) : tidx(kt, r)) * kTileElems + t * 4;
+        const typename Op<T>::vec2* src = (const typename Op<T>::vec2*)(bq + off);
+        ra[h][0] = src[0];
+        ra[h][1] = src[1];
+      } else {
+        ra[h][0].x = ra[h][0].y = ra[h][1].x = ra[h][1].y = (T)0;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;
+      const int kk = chunk / (BN / 2);
+      const int col = n0 + (chunk % (BN / 2)) * 2;
+      if (col < ldo) {
+        rc[v] = *(const typename Op<T>::vec2*)(c + (long long)(kt * kTile + kk) * ldo + col);
+      } else {
+        rc[v].x = rc[v].y = (T)0;
+      }
+    }
+  };
+  auto store = [&]() {
+    const int a = (t * 4) / kTile;
+    const int bc = (t * 4) % kTile;
+#pragma unroll
+    for (int h = 0; h < NTL; ++h) {
+      At[h][a][bc + 0] = ra[h][0].x;
+      At[h][a][bc + 1] = ra[h][0].y;
+      At[h][a][bc + 2] = ra[h][1].x;
+      At[h][a][bc + 3] = ra[h][1].y;
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;
+      const int kk = chunk / (BN / 2);
+      const int jj = (chunk % (BN / 2)) * 2;
+      Cs[kk][jj] = rc[v].x;
+      Cs[kk][jj + 1] = rc[v].y;
+    }
+  };
+
+  // Wave-uniform block masks of SKIP.
+  bool rv[MI], cv[S::NI];
+  if constexpr (SKIP) {
+    const int wu = __builtin_amdgcn_readfirstlane(t >> 6);
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) rv[mi] = mb * BM + (wu % S::WM) * (BM / S::WM) + mi * 16 < nrow;
+#pragma unroll
+    for (int ni = 0; ni < S::NI; ++ni) cv[ni] = n0 + (wu / S::WM) * (BN / S::WN) + ni * 16 < ncol;
+  }
+
+  const int steps = (q1 - q0) * nt;
+  if (steps <= 0) return;
+  load(q0, 0);
+  store();
+  __syncthreads();
+  int q = q0, kt = 0;
+  for (int step = 0; step < steps; ++step) {
+    const int nq = kt + 1 < nt ? q : q + 1;
+    const int nkt = kt + 1 < nt ? kt + 1 : 0;
+    if (step + 1 < steps) load(nq, nkt);
+    // Tile h is stored as (r, kt) when r >= kt (read A[i][k] = At[h][i][k]) and
+    // as (kt, r) otherwise (read A[i][k] = At[h][k][i]).
+    bool direct[NTL];
+#pragma unroll
+    for (int h = 0; h < NTL; ++h) direct[h] = (NTL * mb + h) >= kt;
+#pragma unroll
+    for (int kk = 0; kk < kTile; kk += 4) {
+      if constexpr (SKIP) {
+        if (kt * kTile + kk >= kmax) break;
+      }
+      const int k = kk + (lane >> 4);
+      T af[MI], bf[S::NI];
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi) {
+        const int row = wm * (BM / S::WM) + mi * 16 + (lane & 15);
+        const int h = row >> 5;
+        const int i = row & 31;
+        af[mi] = direct[h] ? At[h][i][k] : At[h][k][i];
+      }
+#pragma unroll
+      for (int ni = 0; ni < S::NI; ++ni) bf[ni] = Cs[k][wn * (BN / S::WN) + ni * 16 + (lane & 15)];
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < S::NI; ++ni) {
+          if constexpr (SKIP) {
+            if (!(rv[mi] && cv[ni])) continue;
+          }
+          acc[mi][ni] = Op<T>::mma(af[mi], bf[ni], acc[mi][ni]);
+        }
+    }
+    __syncthreads();
+    if (kt == nt - 1) {
+      // This aux function's tile is complete: write it and start the next one from zero.
+      const int qw = q;
+      T* xq = x + (long long)qw * xrows * ldo;
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < S::NI; ++ni) {
+          const int col = n0 + wn * (BN / S::WN) + ni * 16 + (lane & 15);
+          if (col < ldo) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = mb * BM + wm * (BM / S::WM) + mi * 16 + Op<T>::row(lane, r);
+              if (row < xrows) xq[(long long)row * ldo + col] = acc[mi][ni][r];
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
+        }
+    }
+    if (step + 1 < steps) {
+      store();
+      __syncthreads();
+    }
+    q = nq;
+    kt = nkt;
+  }
+}
+
 template <typename T, int BN, int BM>
 __global__ __launch_bounds__(kThreads) void xform_kernel(const T* __restrict__ b, long long p, int nt,
                                                           const T* __restrict__ c, int ldo, T* __restrict__ x,
@@ -2162,18 +2329,23 @@ __global__ __launch_bounds__(kThreads, 2) void batch_xform_any_kernel(BatchView 
     const long long local = item - pre[i];
     const int nb = (int)(local % in.nbn);
     const int mb = (int)((local / in.nbn) % in.nbm);
-    const int q = (int)(local / ((long long)in.nbn * in.nbm));
-    const T* bq = b + in.b_off + (long long)q * in.p;
+    const int q0 = (int)(local / ((long long)in.nbn * in.nbm)) * in.qg;
+    const int q1 = min(in.naux, q0 + in.qg);
+    const T* bb = b + in.b_off;
     const T* cp = cbase + in.cp_off;
-    T* xq = xbase + in.x_off + (long long)q * in.xrows * in.ldo;
+    T* xb = xbase + in.x_off;
     if (in.bm == 128)
-      xform_tile<T, 64, 128, true>(bq, in.nt, cp, in.ldo, xq, in.xrows, mb, nb, sm.t64, in.nao, in.nocc, in.nao);
+      xform_tiles<T, 64, 128, true>(bb, in.p, in.nt, cp, in.ldo, xb, in.xrows, mb, nb, q0, q1, sm.t64, in.nao,
+                                    in.nocc, in.nao);
     else if (in.bn == 32)
-      xform_tile<T, 32, 64, true>(bq, in.nt, cp, in.ldo, xq, in.xrows, mb, nb, sm.s32, in.nao, in.nocc, in.nao);
+      xform_tiles<T, 32, 64, true>(bb, in.p, in.nt, cp, in.ldo, xb, in.xrows, mb, nb, q0, q1, sm.s32, in.nao,
+                                   in.nocc, in.nao);
     else if (in.bn == 64)
-      xform_tile<T, 64, 64, true>(bq, in.nt, cp, in.ldo, xq, in.xrows, mb, nb, sm.s64, in.nao, in.nocc, in.nao);
+      xform_tiles<T, 64, 64, true>(bb, in.p, in.nt, cp, in.ldo, xb, in.xrows, mb, nb, q0, q1, sm.s64, in.nao,
+                                   in.nocc, in.nao);
     else
-      xform_tile<T, 128, 64, true>(bq, in.nt, cp, in.ldo, xq, in.xrows, mb, nb, sm.s128, in.nao, in.nocc, in.nao);
+      xform_tiles<T, 128, 64, true>(bb, in.p, in.nt, cp, in.ldo, xb, in.xrows, mb, nb, q0, q1, sm.s128, in.nao,
+                                    in.nocc, in.nao);
   }
 }
 
