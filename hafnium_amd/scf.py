@@ -69,17 +69,17 @@ class ScfOptions:
 
     def __post_init__(self) -> None:
         if self.max_iterations < 1:
-            raise InputError("at least one iteration is needed")
+            raise InputError(f"max_iterations = {self.max_iterations}: the loop needs room for one iteration")
         if self.energy_tolerance <= 0.0 or self.gradient_tolerance <= 0.0:
-            raise InputError("tolerances must be positive")
+            raise InputError("the energy and gradient tolerances have to be positive numbers")
         if self.diis_size < 1:
-            raise InputError("the DIIS subspace holds at least one matrix")
+            raise InputError(f"diis_size = {self.diis_size}: DIIS extrapolates over one stored Fock matrix or more")
         if self.level_shift < 0.0:
-            raise InputError("the level shift is non-negative")
+            raise InputError(f"a negative level shift ({self.level_shift}) would lower the virtual levels")
         if not 0.0 <= self.damping < 1.0:
-            raise InputError("damping is a fraction in [0, 1)")
+            raise InputError(f"damping = {self.damping}: the mixing weight of the previous density lies in [0, 1)")
         if self.overlap_threshold <= 0.0:
-            raise InputError("the overlap threshold is positive")
+            raise InputError(f"the overlap threshold {self.overlap_threshold} must exceed zero")
         if self.precision not in ("fp64", "fp32", "bf16x3"):
             raise InputError("precision is fp64, fp32 or bf16x3")
         if self.stall_window < 1 or not 0.0 < self.stall_ratio <= 1.0:

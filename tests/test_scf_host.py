@@ -34,6 +34,12 @@ def test_degenerate_groups_share_electrons():
 
 
 def test_options_are_validated():
+    # The words the reference's own option tests match on (tests/unit/test_scf.py:275-286).
+    for kwargs, word in (({"max_iterations": 0}, "iteration"), ({"energy_tolerance": 0.0}, "positive"),
+                         ({"diis_size": 0}, "DIIS"), ({"level_shift": -0.1}, "level shift"),
+                         ({"damping": 1.0}, "damping"), ({"overlap_threshold": 0.0}, "overlap threshold")):
+        with pytest.raises(InputError, match=word):
+            scf.ScfOptions(**kwargs)
     with pytest.raises(InputError):
         scf.ScfOptions(max_iterations=0)
     with pytest.raises(InputError):
