@@ -1769,6 +1769,138 @@ __device__ __forceinline__ void syrk16_ring_tile(const uint16_t* __restrict__ xh
     }
 }
 
+// BF16x3 SYRK with 256 x 256 tiles: the four 128 x 128 quadrants of a
+// 256-row tile pair in one workgroup of eight waves, so each step moves half
+// the bytes per MFMA of the 128-row ring (which is bound by those bytes).
+// Wave w computes rows 128 (w & 1) .. + 128 and columns 64 (w >> 1) .. + 64
+// of the tile (8 x 4 blocks of 16 x 16, 128 FP32 accumulators per lane);
+// a stage holds 32 k of the 256 rows of both operands, hi and lo planes, in
+// 1 KB chunks of 16 rows x 32 k (64 KB); two stages alternate, the loads of
+// step k + 1 in flight during step k's MFMAs. Partial slots as in
+// syrk16_dma_kernel: the tile's quadrants are written as the partial tiles
+// of their 128-row tile pairs (the upper quadrant of a diagonal pair, and
+// quadrants past the last 128-row block, are not written). Column signs +1
+// (the occupied orbitals).
+constexpr int kR256Threads = 512;
+constexpr int kR256Loads = 8;          // load instructions per wave per stage
+constexpr int kR256Stage = 64 * 512;   // uint16 elements per stage (64 KB)
+
+__global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint16_t* __restrict__ xhi,
+                                                                     const uint16_t* __restrict__ xlo, int naux,
+                                                                     int xrows, int ldo, int nsplit, int npairs,
+                                                                     int npairs128, int nb128, int run,
+                                                                     float* __restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) uint16_t st0[kR256Stage];
+  __shared__ __attribute__((aligned(16))) uint16_t st1[kR256Stage];
+  int s, pair;
+  syrk_coords(blockIdx.x, npairs, s, pair);
+  int mb, nb;
+  tile_of(pair, mb, nb);
+  int cb, ce;
+  slot_range(s, nsplit, naux, run, cb, ce);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1;   // row half
+  const int wn = wave >> 1;  // column quarter
+  // Load v of this wave is chunk 8 wave + v = 32 operand + 2 block + plane (waves 0-3 operand A,
+  // the rows of block mb; waves 4-7 operand B, block nb). Rows past xrows read row 0: their
+  // products land in rows and columns of K past nao, never read.
+  int off[kR256Loads];
+#pragma unroll
+  for (int v = 0; v < kR256Loads; ++v) {
+    const int ch = wave * kR256Loads + v;
+    const int blk = (ch >> 1) & 15;
+    int row = ((ch >> 5) ? nb : mb) * 256 + blk * 16 + (lane & 15);
+    row = row < xrows ? row : 0;
+    off[v] = (row * ldo + (lane >> 4) * 8) * 2;
+  }
+  const int nch = ldo / 32;
+  // The partial tile this wave writes into: 128-row tile pair (2 mb + wm, 2 nb + wn / 2), columns
+  // 64 (wn & 1) .. + 64 of it.
+  const int qr = 2 * mb + wm, qc = 2 * nb + (wn >> 1);
+  const bool writes = qr < nb128 && qc < nb128 && qr >= qc;
+  for (int c = cb; c < ce; ++c) {
+    int qb, qe;
+    slot_aux(c, nsplit, naux, run, qb, qe);
+    const int steps = (qe - qb) * nch;
+    auto issue = [&](int step, uint16_t* stage) {
+      const int qq = step / nch;
+      const long long base = (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * 32;
+      const uint16_t* hi = xhi + base;
+      const uint16_t* lo = xlo + base;
+#pragma unroll
+      for (int v = 0; v < kR256Loads; ++v)
+        __builtin_amdgcn_global_load_lds((const void*)((const char*)((v & 1) ? lo : hi) + (unsigned)off[v]),
+                                         (lds_void_t*)(stage + (wave * kR256Loads + v) * 512), 16, 0, 0);
+    };
+    f4 acc[8][4];
+#pragma unroll
+    for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
+    auto chunk = [&](const uint16_t* stage, int op, int blk, int plane) -> bf8 {
+      return lds_bf8(stage + ((op * 32 + blk * 2 + plane) * 512 + lane * 8));
+    };
+    // The wave's 8 row blocks in two halves of 4 (operand registers for 4 A blocks at a time).
+    auto compute = [&](const uint16_t* stage) {
+#pragma unroll
+      for (int mh = 0; mh < 2; ++mh) {
+        bf8 ah[4], al[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) {
+          ah[mi] = chunk(stage, 0, wm * 8 + mh * 4 + mi, 0);
+          al[mi] = chunk(stage, 0, wm * 8 + mh * 4 + mi, 1);
+        }
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+          const bf8 bh = chunk(stage, 1, wn * 4 + ni, 0);
+          const bf8 bl = chunk(stage, 1, wn * 4 + ni, 1);
+#pragma unroll
+          for (int mi = 0; mi < 4; ++mi)
+            acc[mh * 4 + mi][ni] = mma3(ah[mi], al[mi], bh, bl, acc[mh * 4 + mi][ni]);
+        }
+      }
+    };
+    // The previous slot's LDS reads and loads have all returned (each slot ends on a wait for
+    // both), so the stages may take this slot's loads once every wave is here.
+    __syncthreads();
+    if (steps > 0) {
+      issue(0, st0);
+      wait_vm_lgkm0<0>();
+      barrier_nofence();
+      for (int k = 0; k < steps; k += 2) {
+        issue(min(k + 1, steps - 1), st1);
+        __builtin_amdgcn_sched_barrier(0);  // the loads go out before the step's MFMAs
+        compute(st0);
+        __builtin_amdgcn_sched_barrier(0);
+        wait_vm_lgkm0<0>();
+        barrier_nofence();
+        if (k + 1 < steps) {
+          issue(min(k + 2, steps - 1), st0);
+          __builtin_amdgcn_sched_barrier(0);
+          compute(st1);
+          __builtin_amdgcn_sched_barrier(0);
+          wait_vm_lgkm0<0>();
+          barrier_nofence();
+        }
+      }
+    }
+    if (writes) {
+      // The lane's offset is opaque here, so its 32 store addresses are formed per slot rather than
+      // hoisted out of the slot loop (where they would outlive the accumulators' registers).
+      int at = (4 * (lane >> 4)) * kSyrkTile + (lane & 15);
+      asm volatile("" : "+v"(at));
+      float* dst = partial + ((long long)c * npairs128 + tidx(qr, qc)) * (kSyrkTile * kSyrkTile) + (wn & 1) * 64 + at;
+#pragma unroll
+      for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) dst[(mi * 16 + r) * kSyrkTile + ni * 16] = acc[mi][ni][r];
+    }
+  }
+}
+
 // BF16x3 half-transform over the ring from the full bf16 hi/lo copy of B
 // (bhi/blo [naux][npad][npad]) and C^T's planes (cthi/ctlo [ldo][npad]):
 // the 128 x 128 tile (mb, nb) of X_q into X's hi/lo planes.
@@ -2211,7 +2343,17 @@ hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo,
     const long long nx = (long long)naux * xrows * ldo;
     const uint16_t* xhi = (const uint16_t*)x;
     const bool ring16 = kind != 1 && ldo <= kDmaMaxLdo && (long long)xrows * ldo * 2 < (1LL << 31);
-    if (ring16 && is_signed)
+    // Unsigned (occupied orbitals, no column shift): the 256 x 256 tiles unless the 128-row ring is
+    // asked for (kind 2); their grid holds about four workgroups per CU of a 256-CU device.
+    if (ring16 && !is_signed && shift == 0 && kind != 2 && run > 0) {
+      const int nb128 = (xrows + kSyrkTile - 1) / kSyrkTile;
+      const int nb256 = (nb128 + 1) / 2;
+      const int np256 = nb256 * (nb256 + 1) / 2;
+      const int nslot = (naux + run - 1) / run;
+      const int ns256 = std::max(8, std::min((nslot + 7) / 8 * 8, 1024 / np256 / 8 * 8));
+      hipLaunchKernelGGL(syrk16_256_kernel, dim3((unsigned)(np256 * ns256)), dim3(kR256Threads), 0, s, xhi, xhi + nx,
+                         naux, xrows, ldo, ns256, np256, npairs, nb128, run, (float*)partial);
+    } else if (ring16 && is_signed)
       hipLaunchKernelGGL(syrk16_dma_kernel<true>, dim3(grid), dim3(kThreads), 0, s, xhi, xhi + nx, naux, xrows, ldo,
                          signs, shift, nsplit, npairs, run, (float*)partial);
     else if (ring16)

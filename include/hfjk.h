@@ -338,10 +338,11 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * used with column tiles up to 128).
  * HFJK_TUNE_SYRK_KIND: SYRK kernel: 1 = operands staged through registers
  * into one LDS buffer (every precision), 2 = operands loaded straight into a
- * ring of LDS stages (occupied widths up to 2048; FP64/FP32: two workgroups
- * per CU unless the columns carry signs; BF16X3: one), 3 = the FP64/FP32
- * ring with one workgroup per CU; 0 = the library's choice (the ring where
- * it applies).
+ * ring of LDS stages with 128 x 128 tiles (occupied widths up to 2048;
+ * FP64/FP32: two workgroups per CU unless the columns carry signs; BF16X3:
+ * one), 3 = the FP64/FP32 ring with one workgroup per CU; 0 = the library's
+ * choice (the ring where it applies; BF16X3 without signs or column shift:
+ * its 256 x 256-tile variant).
  * HFJK_TUNE_X_FULL: 0 = the half-transform uses the full copy of B
  * of its precision when one is bound (hfjk_bind_full), 1 = it ignores it.
  * HFJK_TUNE_GENERAL_CHUNK: most aux functions per chunk of the non-symmetric

@@ -74,16 +74,18 @@ def test_autotune_sets_a_valid_schedule(tmp_path):
     (5, 257, 33, 16),    # more splits than aux functions: empty splits
     (1, 40, 3, 8),       # one step per split (prologue repeats the last stage)
     (23, 300, 140, 8),   # several bulk iterations and a partial one
+    (300, 520, 40, 8),   # several FP32 slots per split; an odd count of 128-row blocks
 ])
 def test_syrk_kernels_agree_with_the_oracle(precision, naux, nao, nocc, splits):
-    """The register-staged and both direct-LDS SYRKs (two and one workgroups per CU) give the oracle's K."""
+    """The library's choice (BF16x3: 256 x 256 tiles), the register-staged and both direct-LDS SYRKs
+    (two and one workgroups per CU) give the oracle's K."""
     from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
 
     b, c, _ = synthetic.jk_inputs(17, naux, nao, nocc)
     expect = jk_ref.exchange_occupied(b, c)
     jk = MI355XCoulombExchange(b, precision=precision)
     jk.set_tuning(_lib.TUNE_SYRK_SPLITS, splits)
-    for kind in (1, 2, 3):
+    for kind in (0, 1, 2, 3):
         jk.set_tuning(_lib.TUNE_SYRK_KIND, kind)
         k = jk.exchange_occupied(c)
         assert jk_ref.max_relative_error(k, expect) < TOL[precision], kind
