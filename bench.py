@@ -490,9 +490,9 @@ def main(argv=None):
     use_full = {}
 
     def prepare_full(mode):
-        # BF16x3 keeps its packed half-transform: on the bf16 ring from the full copy it measured
-        # 29.1 ms against 21.9 ms at this workload (the ring's global-to-LDS traffic per flop).
-        if args.no_full_copy or mode == "bf16x3":
+        # BF16x3 from its full copy runs the 256 x 256-tile half-transform (17.8 ms against 21.6 ms
+        # for the packed register-staged tile at this workload).
+        if args.no_full_copy:
             use_full[mode] = False
             return
         torch.cuda.synchronize()
@@ -629,7 +629,7 @@ def main(argv=None):
             kernel = "xform_dma_kernel<float, 1>"
             b_bytes = q * n * n * 4.0
         elif head == "bf16x3" and use_full.get("bf16x3"):
-            kernel = "xform16_dma_kernel"
+            kernel = "xform16_256_kernel" if o > 128 and tile == 256 else "xform16_dma_kernel"
             b_bytes = q * n * n * 4.0
         traffic, traffic_note = measured_traffic(kernel, n, q, o, x_ms)
         roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved, "peak": peak,

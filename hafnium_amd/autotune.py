@@ -47,7 +47,11 @@ def _device_name(jk) -> str:
 
 
 def tuning_key(jk, nocc: int, precision: str) -> str:
-    return f"{_device_name(jk)}|{precision}|nao={jk.n_ao}|naux={jk.n_aux}|nocc={nocc}"
+    """Device, precision, shape and the storage of B the half-transform reads ("|full" when a full
+    copy of that precision is bound: its kernels differ from the packed tile's)."""
+    has_full = getattr(jk, "has_full", None)
+    full = "|full" if has_full is not None and has_full(precision) else ""
+    return f"{_device_name(jk)}|{precision}|nao={jk.n_ao}|naux={jk.n_aux}|nocc={nocc}{full}"
 
 
 def _split_candidates(jk) -> list[int]:

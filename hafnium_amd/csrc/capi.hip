@@ -291,7 +291,8 @@ void exchange_front(hfjk_ctx* ctx, const double* orbitals, int nocc, const doubl
   {
     Stage st(ctx, s, "exchange_half_transform");
     if (full)
-      check_hip(hfjk::launch_half_transform_full(bfull, precision, ctx->naux, ctx->nao, ct, L.ldo, x, s),
+      check_hip(hfjk::launch_half_transform_full(bfull, precision, ctx->naux, ctx->nao, ct, L.ldo, x, s, nullptr,
+                                                 ctx->x16_bn),
                 "half transform launch");
     else if (split)
       check_hip(hfjk::launch_half_transform_bf16x3((const uint16_t*)b, L.p, ctx->naux, ctx->nao, cp, L.ldo, x,
@@ -507,7 +508,7 @@ void exchange_general(hfjk_ctx* ctx, const double* density, double* out, int pre
         if (full && split) {
           const uint16_t* bh = (const uint16_t*)bfull;
           check_hip(hfjk::launch_half_transform_full(bh + q0 * npad * npad, precision, qc, n, ct, ld, x, s,
-                                                     bh + ((long long)naux + q0) * npad * npad),
+                                                     bh + ((long long)naux + q0) * npad * npad, ctx->x16_bn),
                     "half transform launch");
         } else if (full) {
           check_hip(hfjk::launch_half_transform_full((const char*)bfull + q0 * npad * npad * (long long)es, precision,

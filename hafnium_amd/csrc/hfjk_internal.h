@@ -248,12 +248,13 @@ hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao
 // launch_orbitals_t writes C^T [ldo][npad] in FP64 or FP32 (BF16X3 takes
 // the hi/lo planes of launch_orbitals_bf16x3);
 // launch_half_transform_full computes X = B C on the direct-LDS ring (the
-// bf16 ring for BF16X3, X as hi/lo planes).
+// bf16 ring for BF16X3, X as hi/lo planes; BF16X3 with ldo > 128 on 256 x 256
+// tiles unless bn16 asks for a column tile below 256).
 hipError_t launch_expand_full(const double* packed, long long p, int naux, int nao, int precision, void* out,
                               hipStream_t s);
 hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, void* ct, hipStream_t s);
 hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
-                                      void* x, hipStream_t s, const void* bfull_lo = nullptr);
+                                      void* x, hipStream_t s, const void* bfull_lo = nullptr, int bn16 = 0);
 // k-point DF-J/K (kpoint.hip): complex values as interleaved double pairs.
 // launch_kp_charges: inner [nk][naux] = w_k <B_k[P], D_k> (conjugated B) and
 // charges [naux] = sum_k inner, for blocks B_k at b + k kstride (elements);

@@ -1769,29 +1769,95 @@ __device__ __forceinline__ void syrk16_ring_tile(const uint16_t* __restrict__ xh
     }
 }
 
-// BF16x3 SYRK with 256 x 256 tiles: the four 128 x 128 quadrants of a
-// 256-row tile pair in one workgroup of eight waves, so each step moves half
-// the bytes per MFMA of the 128-row ring (which is bound by those bytes).
-// Wave w computes rows 128 (w & 1) .. + 128 and columns 64 (w >> 1) .. + 64
-// of the tile (8 x 4 blocks of 16 x 16, 128 FP32 accumulators per lane);
-// a stage holds 32 k of the 256 rows of both operands, hi and lo planes, in
-// 1 KB chunks of 16 rows x 32 k (64 KB); two stages alternate, the loads of
-// step k + 1 in flight during step k's MFMAs. Partial slots as in
-// syrk16_dma_kernel: the tile's quadrants are written as the partial tiles
-// of their 128-row tile pairs (the upper quadrant of a diagonal pair, and
-// quadrants past the last 128-row block, are not written). Column signs +1
-// (the occupied orbitals).
+// The MFMA core of the BF16x3 kernels with 256 x 256 tiles (eight waves):
+// acc += A B^T over `steps` steps of 32 k, A and B each 256 rows of bf16 hi
+// and lo planes. Wave w computes rows 128 (w & 1) .. + 128 of A against rows
+// 64 (w >> 1) .. + 64 of B (8 x 4 blocks of 16 x 16, 128 FP32 accumulators
+// per lane). A stage holds one step of both operands in 1 KB chunks of 16
+// rows x 32 k (chunk 32 operand + 2 block + plane, 64 KB); two stages
+// alternate, the loads of step k + 1 in flight during step k's MFMAs, so a
+// step moves half the bytes per MFMA of the 128 x 128 ring. Waves 0-3 load
+// A's chunks, waves 4-7 B's: addr(step, v) is this lane's 16-byte source of
+// load v of its wave (chunk 8 wave + v) at that step. The stages are two
+// separate LDS arrays, chosen statically (the loop is unrolled by two), which
+// lets the compiler see that a step's LDS reads do not touch the stage the
+// loads in flight write, so it does not wait for those loads before the
+// reads. Called by the whole workgroup; it returns with every load and LDS
+// read complete, so the stages may take new loads after the next barrier.
 constexpr int kR256Threads = 512;
 constexpr int kR256Loads = 8;          // load instructions per wave per stage
 constexpr int kR256Stage = 64 * 512;   // uint16 elements per stage (64 KB)
 
+template <class Addr>
+__device__ __forceinline__ void ring256(const Addr& addr, int steps, f4 (&acc)[8][4]) {
+  __shared__ __attribute__((aligned(16))) uint16_t st0[kR256Stage];
+  __shared__ __attribute__((aligned(16))) uint16_t st1[kR256Stage];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1;   // row half of A
+  const int wn = wave >> 1;  // row quarter of B
+  auto chunk = [&](const uint16_t* stage, int op, int blk, int plane) -> bf8 {
+    return lds_bf8(stage + ((op * 32 + blk * 2 + plane) * 512 + lane * 8));
+  };
+  // The wave's 8 row blocks of A in two halves of 4 (operand registers for 4 A blocks at a time).
+  auto compute = [&](const uint16_t* stage) {
+#pragma unroll
+    for (int mh = 0; mh < 2; ++mh) {
+      bf8 ah[4], al[4];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi) {
+        ah[mi] = chunk(stage, 0, wm * 8 + mh * 4 + mi, 0);
+        al[mi] = chunk(stage, 0, wm * 8 + mh * 4 + mi, 1);
+      }
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) {
+        const bf8 bh = chunk(stage, 1, wn * 4 + ni, 0);
+        const bf8 bl = chunk(stage, 1, wn * 4 + ni, 1);
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) acc[mh * 4 + mi][ni] = mma3(ah[mi], al[mi], bh, bl, acc[mh * 4 + mi][ni]);
+      }
+    }
+  };
+  auto issue = [&](int step, uint16_t* stage) {
+#pragma unroll
+    for (int v = 0; v < kR256Loads; ++v)
+      __builtin_amdgcn_global_load_lds((const void*)addr(step, v),
+                                       (lds_void_t*)(stage + (wave * kR256Loads + v) * 512), 16, 0, 0);
+  };
+  if (steps <= 0) return;
+  issue(0, st0);
+  wait_vm_lgkm0<0>();
+  barrier_nofence();
+  for (int k = 0; k < steps; k += 2) {
+    // Past the last step the loads repeat it into the stage no later step reads.
+    issue(min(k + 1, steps - 1), st1);
+    __builtin_amdgcn_sched_barrier(0);  // the loads go out before the step's MFMAs
+    compute(st0);
+    __builtin_amdgcn_sched_barrier(0);
+    wait_vm_lgkm0<0>();
+    barrier_nofence();
+    if (k + 1 < steps) {
+      issue(min(k + 2, steps - 1), st0);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(st1);
+      __builtin_amdgcn_sched_barrier(0);
+      wait_vm_lgkm0<0>();
+      barrier_nofence();
+    }
+  }
+}
+
+// BF16x3 SYRK with 256 x 256 tiles: the four 128 x 128 quadrants of a
+// 256-row tile pair in one workgroup (ring256). Partial slots as in
+// syrk16_dma_kernel: the tile's quadrants are written as the partial tiles
+// of their 128-row tile pairs (the upper quadrant of a diagonal pair, and
+// quadrants past the last 128-row block, are not written). Column signs +1
+// (the occupied orbitals).
 __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint16_t* __restrict__ xhi,
                                                                      const uint16_t* __restrict__ xlo, int naux,
                                                                      int xrows, int ldo, int nsplit, int npairs,
                                                                      int npairs128, int nb128, int run,
                                                                      float* __restrict__ partial) {
-  __shared__ __attribute__((aligned(16))) uint16_t st0[kR256Stage];
-  __shared__ __attribute__((aligned(16))) uint16_t st1[kR256Stage];
   int s, pair;
   syrk_coords(blockIdx.x, npairs, s, pair);
   int mb, nb;
@@ -1800,8 +1866,8 @@ __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint1
   slot_range(s, nsplit, naux, run, cb, ce);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int wm = wave & 1;   // row half
-  const int wn = wave >> 1;  // column quarter
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
   // Load v of this wave is chunk 8 wave + v = 32 operand + 2 block + plane (waves 0-3 operand A,
   // the rows of block mb; waves 4-7 operand B, block nb). Rows past xrows read row 0: their
   // products land in rows and columns of K past nao, never read.
@@ -1822,69 +1888,20 @@ __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint1
   for (int c = cb; c < ce; ++c) {
     int qb, qe;
     slot_aux(c, nsplit, naux, run, qb, qe);
-    const int steps = (qe - qb) * nch;
-    auto issue = [&](int step, uint16_t* stage) {
+    auto addr = [&](int step, int v) -> const char* {
       const int qq = step / nch;
       const long long base = (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * 32;
-      const uint16_t* hi = xhi + base;
-      const uint16_t* lo = xlo + base;
-#pragma unroll
-      for (int v = 0; v < kR256Loads; ++v)
-        __builtin_amdgcn_global_load_lds((const void*)((const char*)((v & 1) ? lo : hi) + (unsigned)off[v]),
-                                         (lds_void_t*)(stage + (wave * kR256Loads + v) * 512), 16, 0, 0);
+      return (const char*)(((v & 1) ? xlo : xhi) + base) + (unsigned)off[v];
     };
     f4 acc[8][4];
 #pragma unroll
     for (int mi = 0; mi < 8; ++mi)
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
-    auto chunk = [&](const uint16_t* stage, int op, int blk, int plane) -> bf8 {
-      return lds_bf8(stage + ((op * 32 + blk * 2 + plane) * 512 + lane * 8));
-    };
-    // The wave's 8 row blocks in two halves of 4 (operand registers for 4 A blocks at a time).
-    auto compute = [&](const uint16_t* stage) {
-#pragma unroll
-      for (int mh = 0; mh < 2; ++mh) {
-        bf8 ah[4], al[4];
-#pragma unroll
-        for (int mi = 0; mi < 4; ++mi) {
-          ah[mi] = chunk(stage, 0, wm * 8 + mh * 4 + mi, 0);
-          al[mi] = chunk(stage, 0, wm * 8 + mh * 4 + mi, 1);
-        }
-#pragma unroll
-        for (int ni = 0; ni < 4; ++ni) {
-          const bf8 bh = chunk(stage, 1, wn * 4 + ni, 0);
-          const bf8 bl = chunk(stage, 1, wn * 4 + ni, 1);
-#pragma unroll
-          for (int mi = 0; mi < 4; ++mi)
-            acc[mh * 4 + mi][ni] = mma3(ah[mi], al[mi], bh, bl, acc[mh * 4 + mi][ni]);
-        }
-      }
-    };
-    // The previous slot's LDS reads and loads have all returned (each slot ends on a wait for
+    // The previous slot's LDS reads and loads have all returned (ring256 ends on a wait for
     // both), so the stages may take this slot's loads once every wave is here.
     __syncthreads();
-    if (steps > 0) {
-      issue(0, st0);
-      wait_vm_lgkm0<0>();
-      barrier_nofence();
-      for (int k = 0; k < steps; k += 2) {
-        issue(min(k + 1, steps - 1), st1);
-        __builtin_amdgcn_sched_barrier(0);  // the loads go out before the step's MFMAs
-        compute(st0);
-        __builtin_amdgcn_sched_barrier(0);
-        wait_vm_lgkm0<0>();
-        barrier_nofence();
-        if (k + 1 < steps) {
-          issue(min(k + 2, steps - 1), st0);
-          __builtin_amdgcn_sched_barrier(0);
-          compute(st1);
-          __builtin_amdgcn_sched_barrier(0);
-          wait_vm_lgkm0<0>();
-          barrier_nofence();
-        }
-      }
-    }
+    ring256(addr, (qe - qb) * nch, acc);
     if (writes) {
       // The lane's offset is opaque here, so its 32 store addresses are formed per slot rather than
       // hoisted out of the slot loop (where they would outlive the accumulators' registers).
@@ -1898,6 +1915,82 @@ __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint1
 #pragma unroll
           for (int r = 0; r < 4; ++r) dst[(mi * 16 + r) * kSyrkTile + ni * 16] = acc[mi][ni][r];
     }
+  }
+}
+
+// BF16x3 half-transform with 256 x 256 tiles (ring256) from the full bf16
+// hi/lo copy of B (bhi/blo [naux][npad][npad]) and C^T's planes (cthi/ctlo
+// [ldo][npad]): tile (mb, nb) of X_q into X's hi/lo planes, one tile per
+// workgroup. Rows of B past npad and of C^T past ldo read row 0; X rows past
+// npad (up to xrows) are written as zeros.
+__global__ __launch_bounds__(kR256Threads, 1) void xform16_256_kernel(
+    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, int npad, const uint16_t* __restrict__ cthi,
+    const uint16_t* __restrict__ ctlo, int ldo, uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo, int xrows,
+    int qbase, int nblocks_n, int nblocks_m, int nwg) {
+  const int logical = xcd_remap(blockIdx.x, nwg);
+  const int nb = logical % nblocks_n;
+  const int mb = (logical / nblocks_n) % nblocks_m;
+  const int q = qbase + logical / (nblocks_n * nblocks_m);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+  // Waves 0-3 load B_q's rows of block mb, waves 4-7 C^T's rows of block nb.
+  const bool is_c = wave >= 4;
+  int off[kR256Loads];
+#pragma unroll
+  for (int v = 0; v < kR256Loads; ++v) {
+    const int blk = ((wave * kR256Loads + v) >> 1) & 15;
+    int row = (is_c ? nb : mb) * 256 + blk * 16 + (lane & 15);
+    row = row < (is_c ? ldo : npad) ? row : 0;
+    off[v] = (row * npad + (lane >> 4) * 8) * 2;
+  }
+  const long long plane = (long long)q * npad * npad;
+  const uint16_t* src_hi = is_c ? cthi : bhi + plane;
+  const uint16_t* src_lo = is_c ? ctlo : blo + plane;
+  auto addr = [&](int step, int v) -> const char* {
+    return (const char*)(((v & 1) ? src_lo : src_hi) + step * 32) + (unsigned)off[v];
+  };
+  f4 acc[8][4];
+#pragma unroll
+  for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
+  ring256(addr, npad / 32, acc);
+  const long long xbase = (long long)q * xrows * ldo;
+  if ((mb + 1) * 256 <= npad && (nb + 1) * 256 <= ldo) {
+    // A tile inside X (and inside B's rows): no bounds to check.
+    uint16_t* th = xhi + xbase + (long long)(mb * 256 + wm * 128 + 4 * (lane >> 4)) * ldo + nb * 256 + wn * 64 +
+                   (lane & 15);
+    uint16_t* tl = th + (xlo - xhi);
+#pragma unroll
+    for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float v = acc[mi][ni][r];
+          const uint16_t h = bf16_of(v);
+          th[(mi * 16 + r) * ldo + ni * 16] = h;
+          tl[(mi * 16 + r) * ldo + ni * 16] = bf16_of(v - f32_of(h));
+        }
+  } else {
+#pragma unroll
+    for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) {
+        const int col = nb * 256 + wn * 64 + ni * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = mb * 256 + wm * 128 + mi * 16 + 4 * (lane >> 4) + r;
+          if (row < xrows && col < ldo) {
+            const float v = row < npad ? acc[mi][ni][r] : 0.f;
+            const uint16_t h = bf16_of(v);
+            xhi[xbase + (long long)row * ldo + col] = h;
+            xlo[xbase + (long long)row * ldo + col] = bf16_of(v - f32_of(h));
+          }
+        }
+      }
   }
 }
 
@@ -2162,9 +2255,30 @@ hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, 
 }
 
 hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
-                                      void* x, hipStream_t s, const void* bfull_lo) {
+                                      void* x, hipStream_t s, const void* bfull_lo, int bn16) {
   const int npad = n_tiles(nao) * kTile;
   const int xrows = x_rows(nao);
+  if (precision == HFJK_BF16X3 && ldo > kSyrkTile && (bn16 == 0 || bn16 == 256)) {
+    // BF16x3 with more than 128 occupied columns: the 256 x 256 tiles.
+    const uint16_t* bh = (const uint16_t*)bfull;
+    const uint16_t* ch = (const uint16_t*)ct;
+    uint16_t* xh = (uint16_t*)x;
+    const long long nb16 = (long long)naux * npad * npad, nc16 = (long long)ldo * npad,
+                    nx16 = (long long)naux * xrows * ldo;
+    const uint16_t* bl = bfull_lo != nullptr ? (const uint16_t*)bfull_lo : bh + nb16;
+    const int nbn = (ldo + 255) / 256;
+    const int nbm = (xrows + 255) / 256;
+    const int per = std::max(1, (1 << 22) / (nbn * nbm));  // keep a launch below 2^32 work-items
+    for (int base = 0; base < naux; base += per) {
+      const int count = std::min(per, naux - base);
+      const int nwg = nbn * nbm * count;
+      hipLaunchKernelGGL(xform16_256_kernel, dim3((unsigned)nwg), dim3(kR256Threads), 0, s, bh, bl, npad, ch,
+                         ch + nc16, ldo, xh, xh + nx16, xrows, base, nbn, nbm, nwg);
+      hipError_t err = hipGetLastError();
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
   const int nbn = (ldo + kSyrkTile - 1) / kSyrkTile;
   const int nbm = (xrows + kSyrkTile - 1) / kSyrkTile;
   const int per = std::max(1, (1 << 24) / (nbn * nbm));  // keep a launch below 2^32 work-items

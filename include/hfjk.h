@@ -332,7 +332,9 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
 
 /* Tuning parameters of the kernels (0 = the library's default choice).
  * HFJK_TUNE_X_BN: column tile of the FP64/FP32 half-transform (32, 64, 128, 256).
- * HFJK_TUNE_X16_BN: column tile of the BF16x3 half-transform (32, 64, 128, 256).
+ * HFJK_TUNE_X16_BN: column tile of the BF16x3 half-transform (32, 64, 128, 256;
+ * from a bound full copy of B: 256 or 0 = 256 x 256 tiles past 128 occupied
+ * columns, else the 128 x 128 ring).
  * HFJK_TUNE_SYRK_SPLITS: aux-slice splits of the SYRK (rounded up to a multiple of 8).
  * HFJK_TUNE_X_BM: row tile of the FP64/FP32 half-transform (64, 128; 128 is
  * used with column tiles up to 128).

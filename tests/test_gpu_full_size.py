@@ -3,8 +3,8 @@
 * Large single build (nao 1500, naux 5000, nocc 250, seed 1): the timed path
   (full copies of B, the stored tile table, the combined J + K call) on an
   8-aux slice against the oracle; the FP32 and BF16x3 builds of all 5000 aux
-  functions against the FP64 build; the FP32 exchange error flat over the
-  SYRK split counts.
+  functions (from their full copies, as timed) against the FP64 build; the
+  FP32 exchange error flat over the SYRK split counts.
 * The 512-instance batch (nao ~ U{40..400}, naux = 3.5 nao): every instance
   in FP64 and BF16x3 against a single-instance build of the same seed, 16
   instances (the 8 largest and 8 seeded picks) against the oracle.
@@ -53,8 +53,9 @@ def apply_stored_tuning(jk, precision):
     """The tile schedule the benchmark's large-configuration entry stores for ``precision``."""
     from hafnium_amd import _lib
 
+    full = "|full" if jk.has_full(precision) else ""
     with open(TUNING) as fh:
-        entry = json.load(fh)[f"gfx950x256|{precision}|nao=1500|naux=5000|nocc=250"]
+        entry = json.load(fh)[f"gfx950x256|{precision}|nao=1500|naux=5000|nocc=250{full}"]
     jk.set_tuning(_lib.TUNE_X16_BN if precision == "bf16x3" else _lib.TUNE_X_BN, entry["x_bn"])
     if precision != "bf16x3":
         jk.set_tuning(_lib.TUNE_X_BM, entry["x_bm"])
@@ -62,8 +63,9 @@ def apply_stored_tuning(jk, precision):
 
 
 def test_timed_large_path_on_an_aux_slice_matches_the_oracle(torch):
-    """The benchmark's kernels (direct-LDS half-transform from the full FP64 / FP32 copy, tuned
-    tiles, combined call with the Coulomb half beside the SYRK) at nao 1500 / nocc 250."""
+    """The benchmark's kernels (direct-LDS half-transform from the full copy of B of each
+    precision, tuned tiles, combined call with the Coulomb half beside the SYRK) at nao 1500 /
+    nocc 250."""
     from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
 
     n, o, q0, naux = LARGE["nao"], LARGE["nocc"], 2711, 8
@@ -75,8 +77,7 @@ def test_timed_large_path_on_an_aux_slice_matches_the_oracle(torch):
     k_ref = jk_ref.exchange_occupied(b, c)
     jk = MI355XCoulombExchange.synthetic(LARGE["seed"], naux, n, q0=q0)
     for precision in ("fp64", "fp32", "bf16x3"):
-        if precision != "bf16x3":
-            jk.expand_full(precision)
+        jk.expand_full(precision)
         apply_stored_tuning(jk, precision)
         j, k = jk.jk_occupied(torch.from_numpy(c).cuda(), precision=precision)
         err_j, err_k = rel(j.cpu(), j_ref), rel(k.cpu(), k_ref)
@@ -157,8 +158,10 @@ def large(torch):
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
 def test_reduced_modes_against_fp64_at_the_large_configuration(torch, large, precision):
     jk, c, d, j64, k64 = large
+    jk.expand_full(precision)  # as the benchmark runs it
     apply_stored_tuning(jk, precision)
     j, k = jk.jk_occupied(c, precision=precision)
+    jk.drop_full(precision)
     j, k = j.cpu().numpy(), k.cpu().numpy()
     ej, ek = jk_ref.energies(d, j, k)
     ej0, ek0 = jk_ref.energies(d, j64, k64)

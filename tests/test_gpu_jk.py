@@ -323,10 +323,11 @@ def bf16_planes(x):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
 @pytest.mark.parametrize("naux,nao,nocc", [(1, 1, 1), (3, 33, 5), (7, 97, 70), (5, 150, 140), (4, 257, 33),
-                                           (2, 300, 300)])
+                                           (2, 300, 300), (3, 600, 200)])
 def test_full_copy_half_transform_matches_the_oracle(torch, precision, naux, nao, nocc):
     """Half-transform on the direct-LDS ring from the full copy of B: X rows past npad,
-    one to three column blocks, partial row blocks."""
+    one to three column blocks, partial row blocks (BF16x3 past 128 columns: 256 x 256 tiles,
+    with partial row and column tiles at nao 300 and 600)."""
     from hafnium_amd import _lib
 
     b, c, d = synthetic.jk_inputs(300 + nao, naux, nao, nocc)
@@ -351,6 +352,11 @@ def test_full_copy_half_transform_matches_the_oracle(torch, precision, naux, nao
     assert rel(k2, jk_ref.exchange(b, d)) < tol
     assert rel(j, jk_ref.coulomb(b, d)) < tol
     assert rel(k, k_packed) < {"fp64": 1e-13, "fp32": 1e-5, "bf16x3": 1e-4}[precision]
+    if precision == "bf16x3":
+        # The 128-row ring from the same copy (a column tile below 256) gives the same K.
+        jk.set_tuning(_lib.TUNE_X16_BN, 128)
+        assert rel(jk.exchange_occupied(c), k) < 1e-6
+        jk.set_tuning(_lib.TUNE_X16_BN, 0)
     jk.set_tuning(_lib.TUNE_X_FULL, 1)
     np.testing.assert_array_equal(jk.exchange_occupied(c), k_packed)
     jk.set_tuning(_lib.TUNE_X_FULL, 0)
