@@ -430,9 +430,9 @@ int hfjk_batch_create(hfjk_ctx* ctx, const hfjk_batch_instance* instances, int n
     }
     in.nsplit = (int)std::max(1LL, std::min(64LL, ((long long)src.naux * in.ldo + 8191) / 8192));
     in.bm = bt->tile == 256 ? hfjk::batch_tile_rows(precision, 0, 256) : hfjk::batch_tile_rows(precision, in.xrows, in.bn);
-    // FP64 / FP32 items cover enough aux functions for about 32 pipelined k tiles (small
-    // instances have few k tiles per aux function); BF16x3 items one aux function.
-    in.qg = precision == HFJK_BF16X3 || bt->tile == 256 ? 1 : std::max(1, std::min(src.naux, 32 / in.nt));
+    // Items cover enough aux functions for about 32 pipelined k tiles (small instances have
+    // few k tiles per aux function).
+    in.qg = bt->tile == 256 ? 1 : std::max(1, std::min(src.naux, 32 / in.nt));
     in.nbm = (in.xrows + in.bm - 1) / in.bm;
     in.nbn = (in.ldo + in.bn - 1) / in.bn;
     in.p = hfjk::packed_elems(src.nao);

@@ -644,6 +644,185 @@ __device__ __forceinline__ void xform16_tile(const uint16_t* __restrict__ hq, co
     }
 }
 
+// xform16_tile for each aux function q in [q0, q1) (the batched kernel's
+// items): hq/lq = the packed hi/lo planes of aux function 0 (q's at stride
+// p), xhi/xlo = X of aux function 0 (q's at stride xrows * ldo). The (q, k
+// tile) steps run as one pipeline, the next step's loads issued before the
+// current step's MFMAs across aux functions too, and each q's tile is
+// written after its last k tile. With SKIP the MFMAs of 16-row blocks at or
+// past nrow and of 16-column blocks at or past ncol are left out (B and C
+// are zero there, so X is the same); each wave decides for its own blocks.
+template <int BN, bool SKIP>
+__device__ __forceinline__ void xform16_tiles(const uint16_t* __restrict__ hq, const uint16_t* __restrict__ lq,
+                                              long long p, int nt, const uint16_t* __restrict__ cthi,
+                                              const uint16_t* __restrict__ ctlo, int npad, int ldo,
+                                              uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo, int xrows,
+                                              int mb, int nb, int q0, int q1, Xform16Smem<BN>& sm, int nrow,
+                                              int ncol) {
+  constexpr int MI = 4;            // 64 rows per wave
+  constexpr int NI = BN / 2 / 16;  // BN/2 columns per wave
+  constexpr int CCH = BN * 4 / kThreads > 0 ? BN * 4 / kThreads : 1;  // 16-byte chunks of C^T per plane
+  auto& Ah = sm.Ah;
+  auto& Al = sm.Al;
+  auto& Bh = sm.Bh;
+  auto& Bl = sm.Bl;
+  const int n0 = nb * BN;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave & 1;
+  const int wn = wave >> 1;
+
+  f4 acc[MI][NI];
+#pragma unroll
+  for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // Each thread owns one 4 x 4 block of one of the four layout tiles.
+  const int th = t >> 6;
+  const int a0 = ((t & 63) >> 3) * 4;
+  const int b0 = (t & 7) * 4;
+  const int r_own = 4 * mb + th;
+  uint2 rah[4], ral[4];
+  uint4 rch[CCH], rcl[CCH];
+  const bool own = r_own < nt;
+  auto load = [&](int q, int kt) {
+    const int rc = own ? r_own : 0;
+    const long long base = (long long)q * p + (rc >= kt ? tidx(rc, kt) : tidx(kt, rc)) * kTileElems + a0 * kTile + b0;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      rah[rr] = *(const uint2*)(hq + base + rr * kTile);
+      ral[rr] = *(const uint2*)(lq + base + rr * kTile);
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;  // BN rows x 4 chunks of 8 bf16
+      const int jj = chunk >> 2;
+      const int k8 = (chunk & 3) * 8;
+      const int col = n0 + jj;
+      const bool ok = chunk < BN * 4 && col < ldo;
+      const long long off = ok ? (long long)col * npad + kt * kTile + k8 : 0;
+      rch[v] = *(const uint4*)(cthi + off);
+      rcl[v] = *(const uint4*)(ctlo + off);
+    }
+  };
+  auto store = [&](int kt) {
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      rah[rr] = mask2(own, rah[rr]);
+      ral[rr] = mask2(own, ral[rr]);
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;
+      const bool ok = chunk < BN * 4 && n0 + (chunk >> 2) < ldo;
+      rch[v] = mask4(ok, rch[v]);
+      rcl[v] = mask4(ok, rcl[v]);
+    }
+    const bool direct = r_own >= kt;  // also true for r_own >= nt (zeros)
+    if (direct) {
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        *(uint2*)&Ah[th * kTile + a0 + rr][b0] = rah[rr];
+        *(uint2*)&Al[th * kTile + a0 + rr][b0] = ral[rr];
+      }
+    } else {
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        uint2 h, l;
+        const int sh = (cc & 1) * 16;
+        auto pick = [&](const uint2& v) -> uint32_t { return ((cc < 2 ? v.x : v.y) >> sh) & 0xFFFFu; };
+        h.x = pick(rah[0]) | (pick(rah[1]) << 16);
+        h.y = pick(rah[2]) | (pick(rah[3]) << 16);
+        l.x = pick(ral[0]) | (pick(ral[1]) << 16);
+        l.y = pick(ral[2]) | (pick(ral[3]) << 16);
+        *(uint2*)&Ah[th * kTile + b0 + cc][a0] = h;
+        *(uint2*)&Al[th * kTile + b0 + cc][a0] = l;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < CCH; ++v) {
+      const int chunk = t + kThreads * v;
+      if (chunk < BN * 4) {
+        const int jj = chunk >> 2;
+        const int k8 = (chunk & 3) * 8;
+        *(uint4*)&Bh[jj][k8] = rch[v];
+        *(uint4*)&Bl[jj][k8] = rcl[v];
+      }
+    }
+  };
+
+  // Wave-uniform block masks of SKIP.
+  bool rv[MI], cv[NI];
+  {
+    const int wu = __builtin_amdgcn_readfirstlane(t >> 6);
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) rv[mi] = !SKIP || mb * 128 + (wu & 1) * 64 + mi * 16 < nrow;
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) cv[ni] = !SKIP || n0 + (wu >> 1) * (BN / 2) + ni * 16 < ncol;
+  }
+
+  const int steps = (q1 - q0) * nt;
+  if (steps <= 0) return;
+  load(q0, 0);
+  store(0);
+  __syncthreads();
+  int q = q0, kt = 0;
+  for (int step = 0; step < steps; ++step) {
+    const int nq = kt + 1 < nt ? q : q + 1;
+    const int nkt = kt + 1 < nt ? kt + 1 : 0;
+    if (step + 1 < steps) load(nq, nkt);
+    const int kq = 8 * (lane >> 4);
+    bf8 ah[MI], al[MI];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) {
+      ah[mi] = lds_bf8(&Ah[wm * 64 + mi * 16 + (lane & 15)][kq]);
+      al[mi] = lds_bf8(&Al[wm * 64 + mi * 16 + (lane & 15)][kq]);
+    }
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      if (!cv[ni]) continue;
+      const bf8 bh = lds_bf8(&Bh[wn * (BN / 2) + ni * 16 + (lane & 15)][kq]);
+      const bf8 bl = lds_bf8(&Bl[wn * (BN / 2) + ni * 16 + (lane & 15)][kq]);
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi)
+        if (rv[mi]) acc[mi][ni] = mma3(ah[mi], al[mi], bh, bl, acc[mi][ni]);
+    }
+    __syncthreads();
+    if (kt == nt - 1) {
+      // This aux function's tile is complete: write it and start the next one from zero.
+      const long long xq = (long long)q * xrows * ldo;
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni) {
+          const int col = n0 + wn * (BN / 2) + ni * 16 + (lane & 15);
+          if (col < ldo) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = mb * 128 + wm * 64 + mi * 16 + 4 * (lane >> 4) + r;
+              if (row < xrows) {
+                const float v = acc[mi][ni][r];
+                const uint16_t h = bf16_of(v);
+                const long long off = xq + (long long)row * ldo + col;
+                xhi[off] = h;
+                xlo[off] = bf16_of(v - f32_of(h));
+              }
+            }
+          }
+          acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+    if (step + 1 < steps) {
+      store(nkt);
+      __syncthreads();
+    }
+    q = nq;
+    kt = nkt;
+  }
+}
+
 template <int BN>
 __global__ __launch_bounds__(kThreads) void xform_bf16x3_kernel(
     const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, long long p, int nt,
@@ -2611,7 +2790,7 @@ union Xform16SmemAny {
   Xform16Smem<128> s128;
 };
 
-__global__ __launch_bounds__(kThreads) void batch_xform16_any_kernel(BatchView v, const uint16_t* __restrict__ b,
+__global__ __launch_bounds__(kThreads, 2) void batch_xform16_any_kernel(BatchView v, const uint16_t* __restrict__ b,
                                                                       long long b_plane,
                                                                       const uint16_t* __restrict__ cbase,
                                                                       long long c_plane, uint16_t* __restrict__ xbase,
@@ -2627,19 +2806,20 @@ __global__ __launch_bounds__(kThreads) void batch_xform16_any_kernel(BatchView v
     const long long local = item - pre[i];
     const int nb = (int)(local % in.nbn);
     const int mb = (int)((local / in.nbn) % in.nbm);
-    const int q = (int)(local / ((long long)in.nbn * in.nbm));
-    const uint16_t* hq = b + in.b_off + (long long)q * in.p;
+    const int q0 = (int)(local / ((long long)in.nbn * in.nbm)) * in.qg;
+    const int q1 = min(in.naux, q0 + in.qg);
+    const uint16_t* hq = b + in.b_off;
     const uint16_t* ct = cbase + in.cp_off;
-    uint16_t* xh = xbase + in.x_off + (long long)q * in.xrows * in.ldo;
+    uint16_t* xh = xbase + in.x_off;
     if (in.bn == 32)
-      xform16_tile<32>(hq, hq + b_plane, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh, xh + x_plane, in.xrows,
-                       mb, nb, sm.s32);
+      xform16_tiles<32, true>(hq, hq + b_plane, in.p, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh,
+                              xh + x_plane, in.xrows, mb, nb, q0, q1, sm.s32, in.nao, in.nocc);
     else if (in.bn == 64)
-      xform16_tile<64>(hq, hq + b_plane, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh, xh + x_plane, in.xrows,
-                       mb, nb, sm.s64);
+      xform16_tiles<64, true>(hq, hq + b_plane, in.p, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh,
+                              xh + x_plane, in.xrows, mb, nb, q0, q1, sm.s64, in.nao, in.nocc);
     else
-      xform16_tile<128>(hq, hq + b_plane, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh, xh + x_plane,
-                        in.xrows, mb, nb, sm.s128);
+      xform16_tiles<128, true>(hq, hq + b_plane, in.p, in.nt, ct, ct + c_plane, in.nt * kTile, in.ldo, xh,
+                               xh + x_plane, in.xrows, mb, nb, q0, q1, sm.s128, in.nao, in.nocc);
   }
 }
 
