@@ -170,36 +170,64 @@ __global__ __launch_bounds__(kThreads) void b_charges_load_kernel(BatchView v, c
 
 // gamma[q] = <B[q], C C^T> = sum_{mu < nao, i < nocc} C[mu][i] X[q][mu][i] from the
 // half-transformed X of the exchange (mode layout; BF16X3: hi + lo planes)
-// and the caller's FP64 orbitals, one aux function per item.
+// and the caller's FP64 orbitals, one aux function per item. The rows
+// [0, nao) of X[q] are one contiguous range of nao * ldo elements, streamed
+// in 16-byte groups (ldo is a multiple of 32, so no group straddles two
+// rows); columns past nocc are skipped (X is zero there).
 template <int PREC>
 __global__ __launch_bounds__(kThreads) void b_charges_x_kernel(BatchView v, const double* __restrict__ c,
                                                                const void* __restrict__ xbase, long long x_plane,
                                                                double* __restrict__ ws64) {
+  constexpr int V = PREC == HFJK_FP64 ? 2 : PREC == HFJK_FP32 ? 4 : 8;  // elements per 16-byte group
   __shared__ double red[kThreads / 64];
   BATCH_LOOP(kPhCharges)
   const double* ci = c + in.c_off;
   const long long xq = in.x_off + local * in.xrows * in.ldo;
   double s = 0.0;
-  // Element e = mu * nocc + i of C, stepped by the block size; (mu, i) advance without a division.
-  const int total = in.nao * in.nocc;
-  const int dm = kThreads / in.nocc, di = kThreads % in.nocc;
-  int mu = (int)threadIdx.x / in.nocc, i = (int)threadIdx.x % in.nocc;
-  for (int e = threadIdx.x; e < total; e += kThreads) {
-    const long long at = xq + (long long)mu * in.ldo + i;
-    double x;
-    if (PREC == HFJK_FP64) {
-      x = ((const double*)xbase)[at];
-    } else if (PREC == HFJK_FP32) {
-      x = (double)((const float*)xbase)[at];
-    } else {
-      const uint16_t* xh = (const uint16_t*)xbase;
-      x = (double)(__uint_as_float(((uint32_t)xh[at]) << 16) + __uint_as_float(((uint32_t)xh[at + x_plane]) << 16));
+  const int per_row = in.ldo / V;
+  const int groups = in.nao * per_row;
+  // Group g = mu * per_row + gi, stepped by the block size without a division.
+  const int dm = kThreads / per_row, dg = kThreads % per_row;
+  int mu = (int)threadIdx.x / per_row, gi = (int)threadIdx.x % per_row;
+  for (int g = threadIdx.x; g < groups; g += kThreads) {
+    const int i0 = gi * V;
+    if (i0 < in.nocc) {
+      const long long at = xq + (long long)mu * in.ldo + i0;
+      float xf[V];
+      double xd[V];
+      if constexpr (PREC == HFJK_FP64) {
+        const double2 w = *(const double2*)((const double*)xbase + at);
+        xd[0] = w.x;
+        xd[1] = w.y;
+      } else if constexpr (PREC == HFJK_FP32) {
+        const float4 w = *(const float4*)((const float*)xbase + at);
+        xf[0] = w.x;
+        xf[1] = w.y;
+        xf[2] = w.z;
+        xf[3] = w.w;
+      } else {
+        const uint16_t* xh = (const uint16_t*)xbase;
+        const uint4 h = *(const uint4*)(xh + at);
+        const uint4 l = *(const uint4*)(xh + at + x_plane);
+        const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const uint32_t sh = (e & 1) * 16;
+          xf[e] = __uint_as_float(((hw[e >> 1] >> sh) & 0xFFFFu) << 16) +
+                  __uint_as_float(((lw[e >> 1] >> sh) & 0xFFFFu) << 16);
+        }
+      }
+      const double* crow = ci + (long long)mu * in.nocc;
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        const double x = PREC == HFJK_FP64 ? xd[e] : (double)xf[e];
+        if (i0 + e < in.nocc) s += crow[i0 + e] * x;
+      }
     }
-    s += ci[e] * x;
     mu += dm;
-    i += di;
-    if (i >= in.nocc) {
-      i -= in.nocc;
+    gi += dg;
+    if (gi >= per_row) {
+      gi -= per_row;
       ++mu;
     }
   }
