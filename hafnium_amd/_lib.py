@@ -28,6 +28,7 @@ TUNE_SYRK_KIND = 5
 TUNE_X_FULL = 6
 TUNE_GENERAL_CHUNK = 7
 TUNE_SIDE_GROUPS = 8
+TUNE_COULOMB_SERIAL = 9
 
 OK = 0
 ERR_INPUT = 1

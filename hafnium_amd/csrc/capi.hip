@@ -330,15 +330,14 @@ void run_exchange(hfjk_ctx* ctx, const double* orbitals, int nocc, const double*
 }
 
 // Grid cap of the Coulomb kernels that hfjk_jk_occ runs beside the SYRK. In
-// FP64 one workgroup per CU by default, so the SYRK keeps its two
-// workgroups per CU (large configuration: 134.8 ms per build against 138.7
-// ms uncapped; half a workgroup per CU makes the Coulomb half outlast the
-// SYRK). The FP32 and BF16x3 SYRKs are too short to hide a capped Coulomb
-// half (it reads B in FP64 or its FP32 copy either way), so they run it
-// uncapped. 0 = no cap.
+// FP64 two workgroups per CU by default (large configuration, one box:
+// 140.3 ms per build against 144.5 at one per CU, 141.0 uncapped and 141.8
+// with the Coulomb half after the SYRK). The FP32 and BF16x3 SYRKs are too
+// short to hide a capped Coulomb half (it reads B in FP64 or its FP32 copy
+// either way), so they run it uncapped. 0 = no cap.
 int side_groups(const hfjk_ctx* ctx, int precision) {
   if (ctx->side_groups > 0) return ctx->side_groups;
-  return precision == HFJK_FP64 ? ctx->cus : 0;
+  return precision == HFJK_FP64 ? 2 * ctx->cus : 0;
 }
 
 rocblas_handle blas_handle(hfjk_ctx* ctx) {
@@ -873,18 +872,24 @@ int hfjk_jk_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, double* coulomb
     double* parts = at<double>(workspace, L.qt);
     double* gamma = at<double>(workspace, L.qg);
     double* jp = at<double>(workspace, L.qj);
-    if (ctx->side == nullptr) {
+    const bool serial = ctx->coulomb_serial != 0;
+    if (ctx->side == nullptr && !serial) {
       check_hip(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
       check_hip(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming), "hipEventCreateWithFlags");
       check_hip(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming), "hipEventCreateWithFlags");
     }
-    hipStream_t side = ctx->side;
+    hipStream_t side = serial ? s : ctx->side;
     exchange_front(ctx, orbitals, nocc, nullptr, precision, workspace, L, s);
     // The Coulomb half needs only X and the padded orbitals: it runs on the
-    // side stream (HBM-bound) while the SYRK (MFMA-bound) runs on s. Its
-    // buffers (charges, J partials) lie outside the regions the SYRK uses.
-    check_hip(hipEventRecord(ctx->fork, s), "hipEventRecord");
-    check_hip(hipStreamWaitEvent(side, ctx->fork, 0), "hipStreamWaitEvent");
+    // side stream (HBM-bound) while the SYRK (MFMA-bound) runs on s, or with
+    // coulomb_serial on s after the SYRK. Its buffers (charges, J partials)
+    // lie outside the regions the SYRK uses.
+    if (serial) {
+      exchange_back(ctx, false, exchange, precision, workspace, L, s);
+    } else {
+      check_hip(hipEventRecord(ctx->fork, s), "hipEventRecord");
+      check_hip(hipStreamWaitEvent(side, ctx->fork, 0), "hipStreamWaitEvent");
+    }
     {
       // gamma_P = sum C o X_P, one pass over X.
       Stage st(ctx, side, "coulomb_gamma");
@@ -895,23 +900,25 @@ int hfjk_jk_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, double* coulomb
         cpad = at<void>(workspace, L.qc);
       }
       check_hip(hfjk::launch_charges_from_x(at<void>(workspace, L.xx), precision, ctx->naux, ctx->nao, L.ldo, cpad,
-                                            parts, gamma, side_groups(ctx, precision), side),
+                                            parts, gamma, serial ? 0 : side_groups(ctx, precision), side),
                 "charges launch");
     }
     const void* b = bound_tensor(ctx, precision);
     {
       Stage st(ctx, side, "coulomb_accumulate");
       check_hip(hfjk::launch_coulomb_accumulate(b, precision, L.p, ctx->naux, (int)L.ntiles, L.nsplit_j, gamma, jp,
-                                                side_groups(ctx, precision), side),
+                                                serial ? 0 : side_groups(ctx, precision), side),
                 "coulomb accumulate launch");
     }
     {
       Stage st(ctx, side, "coulomb_unpack");
       check_hip(hfjk::launch_coulomb_unpack(jp, L.nsplit_j, L.p, ctx->nao, coulomb, side), "coulomb unpack launch");
     }
-    check_hip(hipEventRecord(ctx->join, side), "hipEventRecord");
-    exchange_back(ctx, false, exchange, precision, workspace, L, s);
-    check_hip(hipStreamWaitEvent(s, ctx->join, 0), "hipStreamWaitEvent");
+    if (!serial) {
+      check_hip(hipEventRecord(ctx->join, side), "hipEventRecord");
+      exchange_back(ctx, false, exchange, precision, workspace, L, s);
+      check_hip(hipStreamWaitEvent(s, ctx->join, 0), "hipStreamWaitEvent");
+    }
   });
 }
 
@@ -1124,6 +1131,10 @@ int hfjk_set_tuning(hfjk_ctx* ctx, int param, int value) {
       case HFJK_TUNE_SIDE_GROUPS:
         ctx->side_groups = value;
         break;
+      case HFJK_TUNE_COULOMB_SERIAL:
+        if (value > 1) fail(HFJK_ERR_INPUT, "HFJK_TUNE_COULOMB_SERIAL is 0 (beside the SYRK) or 1 (after it)");
+        ctx->coulomb_serial = value;
+        break;
       default:
         fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
@@ -1143,6 +1154,7 @@ int hfjk_get_tuning(hfjk_ctx* ctx, int param, int* value) {
       case HFJK_TUNE_X_FULL: *value = ctx->x_full; break;
       case HFJK_TUNE_GENERAL_CHUNK: *value = ctx->general_chunk; break;
       case HFJK_TUNE_SIDE_GROUPS: *value = ctx->side_groups; break;
+      case HFJK_TUNE_COULOMB_SERIAL: *value = ctx->coulomb_serial; break;
       default: fail(HFJK_ERR_INPUT, "unknown tuning parameter " + std::to_string(param));
     }
   });

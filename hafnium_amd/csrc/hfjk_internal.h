@@ -88,6 +88,7 @@ struct hfjk_ctx {
   int x_full = 0;
   int general_chunk = 0;  // aux functions per chunk of the non-symmetric exchange (0 = fit)
   int side_groups = 0;    // grid cap of the Coulomb kernels beside the SYRK in hfjk_jk_occ (0 = default)
+  int coulomb_serial = 0; // hfjk_jk_occ: 1 = the Coulomb half after the SYRK on the launch stream
   int cus = 256;          // compute units of the device
   // Second stream of hfjk_jk_occ (default priority, created on first use):
   // the Coulomb half runs there beside the SYRK, fenced by fork/join events.

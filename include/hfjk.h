@@ -350,9 +350,11 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
  * HFJK_TUNE_GENERAL_CHUNK: most aux functions per chunk of the non-symmetric
  * exchange of hfjk_exchange_density (0 = as many as the workspace holds).
  * HFJK_TUNE_SIDE_GROUPS: most workgroups of each Coulomb kernel that
- * hfjk_jk_occ runs beside the SYRK (0 = the library's choice: one per
+ * hfjk_jk_occ runs beside the SYRK (0 = the library's choice: two per
  * compute unit in FP64, no cap in FP32 and BF16X3, whose shorter SYRK would
  * not hide the capped kernels).
+ * HFJK_TUNE_COULOMB_SERIAL: hfjk_jk_occ runs its Coulomb half 0 = on a
+ * second stream beside the SYRK, 1 = on the launch stream after the SYRK.
  * Set by the project's autotuner (hafnium_amd/autotune.py); the workspace size
  * depends on HFJK_TUNE_SYRK_SPLITS, so query it after tuning. No counterpart
  * in the reference. */
@@ -364,6 +366,7 @@ HFJK_API int hfjk_ddot(hfjk_ctx *ctx, long long n, const double *x, const double
 #define HFJK_TUNE_X_FULL 6
 #define HFJK_TUNE_GENERAL_CHUNK 7
 #define HFJK_TUNE_SIDE_GROUPS 8
+#define HFJK_TUNE_COULOMB_SERIAL 9
 HFJK_API int hfjk_set_tuning(hfjk_ctx *ctx, int param, int value);
 HFJK_API int hfjk_get_tuning(hfjk_ctx *ctx, int param, int *value);
 

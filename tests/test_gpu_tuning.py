@@ -123,8 +123,9 @@ def test_wide_occupied_block_uses_the_register_kernel():
 
 @pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
 def test_capped_coulomb_grids_give_the_same_build(precision):
-    """The grid cap of the Coulomb kernels beside the SYRK changes the schedule only: J and K of
-    the combined build are bitwise equal for every cap, and J matches the oracle."""
+    """The grid cap of the Coulomb kernels beside the SYRK, and running them after it instead,
+    change the schedule only: J and K of the combined build are bitwise equal for every choice,
+    and J matches the oracle."""
     from hafnium_amd.coulomb_exchange import MI355XCoulombExchange
 
     naux, nao, nocc = 37, 150, 40
@@ -139,3 +140,11 @@ def test_capped_coulomb_grids_give_the_same_build(precision):
         np.testing.assert_array_equal(k, k0)
     jk.set_tuning(_lib.TUNE_SIDE_GROUPS, 0)
     assert jk.get_tuning(_lib.TUNE_SIDE_GROUPS) == 0
+    # The Coulomb half after the SYRK on the launch stream: the same J and K.
+    jk.set_tuning(_lib.TUNE_COULOMB_SERIAL, 1)
+    j, k = jk.jk_occupied(c)
+    np.testing.assert_array_equal(j, j0)
+    np.testing.assert_array_equal(k, k0)
+    jk.set_tuning(_lib.TUNE_COULOMB_SERIAL, 0)
+    with pytest.raises(ValueError):
+        jk.set_tuning(_lib.TUNE_COULOMB_SERIAL, 2)
