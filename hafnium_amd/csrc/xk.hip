@@ -1217,11 +1217,14 @@ __device__ __forceinline__ void barrier_nofence() { asm volatile("s_barrier" :::
 // 128 rows of each operand read at step k from src(k, operand) plus this
 // lane's byte offsets off[] (operand A = chunks of waves 0-1, B = waves 2-3).
 // With SIGNED, the B operand's k values are scaled by sgn[sign_at(k) + k']
-// times scale_at(k) (one factor per step).
-template <typename T, bool SIGNED, int CH, typename Src, typename SignAt, typename ScaleAt>
+// times scale_at(k) (one factor per step). With MASKED, the MFMAs of this
+// wave's 16 x 16 block (mi, ni) run only where bit mi of rbits and bit ni of
+// cbits are set (wave-uniform masks; the skipped accumulators stay as they
+// were).
+template <typename T, bool SIGNED, int CH, bool MASKED = false, typename Src, typename SignAt, typename ScaleAt>
 __device__ __forceinline__ void dma_ring(const Src& src, const int (&off)[4 * CH], int steps,
                                          const T* __restrict__ sgn, const SignAt& sign_at, const ScaleAt& scale_at,
-                                         typename Op<T>::acc_t (&acc)[4][4]) {
+                                         typename Op<T>::acc_t (&acc)[4][4], int rbits = 15, int cbits = 15) {
   typedef SyrkDma<T, CH> D;
   typedef typename D::unit unit;
   constexpr int E = D::E;
@@ -1264,7 +1267,12 @@ __device__ __forceinline__ void dma_ring(const Src& src, const int (&off)[4 * CH
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = Op<T>::mma(a[mi][e], b[ni][e], acc[mi][ni]);
+        for (int ni = 0; ni < 4; ++ni) {
+          if constexpr (MASKED) {
+            if (!((rbits >> mi) & (cbits >> ni) & 1)) continue;
+          }
+          acc[mi][ni] = Op<T>::mma(a[mi][e], b[ni][e], acc[mi][ni]);
+        }
   };
 
   // Step k: wait until this wave's loads of stage k + 1 have landed (those
@@ -1416,11 +1424,13 @@ __global__ __launch_bounds__(kThreads, CH == 1 ? 2 : 1) void syrk_dma_kernel(
 
 // One 128 x 128 tile (mb, nb) of sum_{q in [qb, qe)} X_q X_q^T on the ring
 // (all column signs +1: X's padded columns are zero), stored row-major into
-// dst. The whole workgroup calls it; its LDS reads of an earlier call must
-// have returned (a workgroup barrier between calls).
+// dst; the MFMAs of 16-row blocks at or past nrow (X's rows past nao) are
+// skipped, and those rows and columns of dst hold zeros. The whole
+// workgroup calls it; its LDS reads of an earlier call must have returned (a
+// workgroup barrier between calls).
 template <typename T, int CH>
 __device__ __forceinline__ void syrk_ring_tile(const T* __restrict__ x, int xrows, int ldo, int mb, int nb, int qb,
-                                               int qe, T* __restrict__ dst) {
+                                               int qe, T* __restrict__ dst, int nrow) {
   typedef SyrkDma<T, CH> D;
   constexpr int E = D::E;
   constexpr int LOADS = D::LOADS;
@@ -1454,7 +1464,14 @@ __device__ __forceinline__ void syrk_ring_tile(const T* __restrict__ x, int xrow
     for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[mi][ni][r] = (T)0;
-  dma_ring<T, false, CH>(src, off, (qe - qb) * nch, (const T*)nullptr, sign_at, scale_at, acc);
+  // The 16-row blocks of each operand at or past nrow hold padding: their MFMAs are skipped.
+  int rbits = 0, cbits = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    rbits |= (mb * kSyrkTile + wm * 64 + b * 16 < nrow) << b;
+    cbits |= (nb * kSyrkTile + wn * 64 + b * 16 < nrow) << b;
+  }
+  dma_ring<T, false, CH, true>(src, off, (qe - qb) * nch, (const T*)nullptr, sign_at, scale_at, acc, rbits, cbits);
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
@@ -2845,7 +2862,7 @@ __global__ __launch_bounds__(kThreads, 2) void batch_syrk_ring_kernel(BatchView 
     const int qb = (int)((long long)s * in.naux / in.nsplit);
     const int qe = (int)((long long)(s + 1) * in.naux / in.nsplit);
     syrk_ring_tile<T, 1>(xbase + in.x_off, in.xrows, in.ldo, mb, nb, qb, qe,
-                         kbase + in.kp_off + ((long long)s * in.npairs + pair) * (kSyrkTile * kSyrkTile));
+                         kbase + in.kp_off + ((long long)s * in.npairs + pair) * (kSyrkTile * kSyrkTile), in.nao);
   }
 }
 
