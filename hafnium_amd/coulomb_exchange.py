@@ -173,7 +173,8 @@ class MI355XCoulombExchange:
 
     def expand_full(self, precision: str = "fp64") -> None:
         """Keep a full row-major copy of B in ``precision`` ("fp64", "fp32" or "bf16x3"; ``naux npad^2``
-        elements, npad = 32 ceil(nao/32); for "bf16x3" the hi plane, then the lo plane).
+        elements, npad = 32 ceil(nao/32); for "bf16x3" two bf16 parts per element, each 32-column
+        block of a row stored as the 32 hi parts, then the 32 lo parts).
 
         The half-transform of that precision then streams both of its operands
         straight into LDS (13 % faster in FP64, 7 % in FP32 and, on 256 x 256

@@ -276,12 +276,13 @@ void exchange_front(hfjk_ctx* ctx, const double* orbitals, int nocc, const doubl
                       : precision == HFJK_FP32 ? (const void*)ctx->b32_full
                                                : (const void*)ctx->b16_full;
   const bool full = bfull != nullptr && ctx->x_full == 0;
-  // BF16X3 takes C^T's bf16 planes (cp) as the ring's second operand.
+  // BF16X3 takes C^T's bf16 hi/lo (cp; interleaved for the ring from the
+  // full copy) as the second operand.
   void* ct = split ? cp : at<void>(ws, L.xct);
   {
     Stage st(ctx, s, "exchange_prepare");
     if (split)
-      check_hip(hfjk::launch_orbitals_bf16x3(orbitals, ctx->nao, nocc, cp, s), "orbitals launch");
+      check_hip(hfjk::launch_orbitals_bf16x3(orbitals, ctx->nao, nocc, cp, s, full), "orbitals launch");
     else
       check_hip(hfjk::launch_orbitals(orbitals, ctx->nao, nocc, precision, cp, s), "orbitals launch");
     if (full && !split)
@@ -291,7 +292,7 @@ void exchange_front(hfjk_ctx* ctx, const double* orbitals, int nocc, const doubl
   {
     Stage st(ctx, s, "exchange_half_transform");
     if (full)
-      check_hip(hfjk::launch_half_transform_full(bfull, precision, ctx->naux, ctx->nao, ct, L.ldo, x, s, nullptr,
+      check_hip(hfjk::launch_half_transform_full(bfull, precision, ctx->naux, ctx->nao, ct, L.ldo, x, s,
                                                  ctx->x16_bn),
                 "half transform launch");
     else if (split)
@@ -492,7 +493,7 @@ void exchange_general(hfjk_ctx* ctx, const double* density, double* out, int pre
       check_hip(hfjk::launch_general_factor(hz, w, n, nneg, npos, j0, rb, hb, c, sym, cross, s),
                 "general factor launch");
       if (split)
-        check_hip(hfjk::launch_orbitals_bf16x3(c, n, ld, cp, s), "orbitals launch");
+        check_hip(hfjk::launch_orbitals_bf16x3(c, n, ld, cp, s, full), "orbitals launch");
       else if (full)
         check_hip(hfjk::launch_orbitals_t(c, n, ld, precision, ct, s), "orbitals launch");
       else
@@ -506,8 +507,8 @@ void exchange_general(hfjk_ctx* ctx, const double* density, double* out, int pre
         Stage st(ctx, s, "general_half_transform");
         if (full && split) {
           const uint16_t* bh = (const uint16_t*)bfull;
-          check_hip(hfjk::launch_half_transform_full(bh + q0 * npad * npad, precision, qc, n, ct, ld, x, s,
-                                                     bh + ((long long)naux + q0) * npad * npad, ctx->x16_bn),
+          check_hip(hfjk::launch_half_transform_full(bh + 2LL * q0 * npad * npad, precision, qc, n, ct, ld, x, s,
+                                                     ctx->x16_bn),
                     "half transform launch");
         } else if (full) {
           check_hip(hfjk::launch_half_transform_full((const char*)bfull + q0 * npad * npad * (long long)es, precision,

@@ -229,8 +229,12 @@ hipError_t launch_syrk_reduce(const void* partial, int precision, int nslot, int
 hipError_t launch_occ_signs(const double* signs_in, int nocc, int ldo, double* signs_out, hipStream_t s);
 
 // BF16x3 exchange (xk.hip): operands as bf16 hi/lo planes; launch_syrk
-// takes HFJK_BF16X3 for the split-product kernel.
-hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s);
+// takes HFJK_BF16X3 for the split-product kernel. launch_orbitals_bf16x3
+// writes C^T [ldo][npad] as the hi plane then the lo plane, or interleaved
+// (per row npad / 32 blocks of 32 hi then 32 lo, the layout of the bf16
+// full copy of B) for the half-transform from that copy.
+hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s,
+                                  bool interleaved = false);
 // b_lo: the lo planes of these aux functions (default: naux p elements after b).
 hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux, int nao, const void* ct, int ldo,
                                         void* x, int bn, hipStream_t s, const uint16_t* b_lo = nullptr);
@@ -244,10 +248,11 @@ hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao
 // Half-transform from a full row-major copy of B (xk.hip):
 // launch_expand_full writes bfull[q][npad][npad] in the precision's type
 // (npad = 32 ceil(nao / 32)) from the tiled packed FP64 tensor; for BF16X3
-// the bf16 hi plane of every element, then the lo plane (the element split
-// of launch_split);
+// bfull[q][npad][npad / 32][64]: each 32-column block of a row as the bf16
+// hi parts of its elements, then their lo parts (the element split of
+// launch_split), so one 128-byte line holds both parts of 32 elements;
 // launch_orbitals_t writes C^T [ldo][npad] in FP64 or FP32 (BF16X3 takes
-// the hi/lo planes of launch_orbitals_bf16x3);
+// launch_orbitals_bf16x3's interleaved C^T);
 // launch_half_transform_full computes X = B C on the direct-LDS ring (the
 // bf16 ring for BF16X3, X as hi/lo planes; BF16X3 with ldo > 128 on 256 x 256
 // tiles unless bn16 asks for a column tile below 256).
@@ -255,7 +260,7 @@ hipError_t launch_expand_full(const double* packed, long long p, int naux, int n
                               hipStream_t s);
 hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, void* ct, hipStream_t s);
 hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
-                                      void* x, hipStream_t s, const void* bfull_lo = nullptr, int bn16 = 0);
+                                      void* x, hipStream_t s, int bn16 = 0);
 // k-point DF-J/K (kpoint.hip): complex values as interleaved double pairs.
 // launch_kp_charges: inner [nk][naux] = w_k <B_k[P], D_k> (conjugated B) and
 // charges [naux] = sum_k inner, for blocks B_k at b + k kstride (elements);

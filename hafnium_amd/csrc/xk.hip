@@ -977,8 +977,11 @@ __global__ __launch_bounds__(kThreads) void charge_reduce_kernel(const double* _
   gamma[q] = sum;
 }
 
-// Orbitals [nao][nocc] float64 -> transposed split planes Ct[ldo][npad]
-// (hi plane, then lo plane), zero-padded.
+// Orbitals [nao][nocc] float64 -> transposed split Ct[ldo][npad], zero-padded:
+// ILV = false: the hi plane, then the lo plane; ILV = true: each row as
+// npad / 32 blocks of 64 (32 hi, then the same 32 lo), the layout of the
+// bf16 full copy of B.
+template <bool ILV>
 __global__ __launch_bounds__(kThreads) void orbitals_bf16x3_kernel(const double* __restrict__ c, int nao, int nocc,
                                                                     int npad, int ldo, uint16_t* __restrict__ hi,
                                                                     uint16_t* __restrict__ lo) {
@@ -989,8 +992,14 @@ __global__ __launch_bounds__(kThreads) void orbitals_bf16x3_kernel(const double*
     const int k = (int)(idx % npad);
     const float v = (k < nao && i < nocc) ? (float)c[(long long)k * nocc + i] : 0.0f;
     const uint16_t h = bf16_of(v);
-    hi[idx] = h;
-    lo[idx] = bf16_of(v - f32_of(h));
+    if constexpr (ILV) {
+      const long long o = (long long)i * 2 * npad + (k >> 5) * 64 + (k & 31);
+      hi[o] = h;
+      hi[o + 32] = bf16_of(v - f32_of(h));
+    } else {
+      hi[idx] = h;
+      lo[idx] = bf16_of(v - f32_of(h));
+    }
   }
 }
 
@@ -1984,7 +1993,7 @@ constexpr int kR256Threads = 512;
 constexpr int kR256Loads = 8;          // load instructions per wave per stage
 constexpr int kR256Stage = 64 * 512;   // uint16 elements per stage (64 KB)
 
-template <class Addr>
+template <bool ILV = false, class Addr>
 __device__ __forceinline__ void ring256(const Addr& addr, int steps, f4 (&acc)[8][4]) {
   __shared__ __attribute__((aligned(16))) uint16_t st0[kR256Stage];
   __shared__ __attribute__((aligned(16))) uint16_t st1[kR256Stage];
@@ -1992,8 +2001,16 @@ __device__ __forceinline__ void ring256(const Addr& addr, int steps, f4 (&acc)[8
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = wave & 1;   // row half of A
   const int wn = wave >> 1;  // row quarter of B
+  // ILV: chunk 2 block + h of an operand holds rows 8 h .. + 8 of the 16-row block, each row as
+  // 8 segments of 16 bytes (logical segment 4 plane + k group, stored at segment logical ^ row),
+  // so that the MFMA operand reads of 8 consecutive rows fall in distinct banks.
+  const int r8 = lane & 7;
+  const int ilv_off = ((lane >> 3) & 1) * 512 + r8 * 64;
   auto chunk = [&](const uint16_t* stage, int op, int blk, int plane) -> bf8 {
-    return lds_bf8(stage + ((op * 32 + blk * 2 + plane) * 512 + lane * 8));
+    if constexpr (ILV)
+      return lds_bf8(stage + ((op * 32 + blk * 2) * 512 + ilv_off + ((plane * 4 + (lane >> 4)) ^ r8) * 8));
+    else
+      return lds_bf8(stage + ((op * 32 + blk * 2 + plane) * 512 + lane * 8));
   };
   // The wave's 8 row blocks of A in two halves of 4 (operand registers for 4 A blocks at a time).
   auto compute = [&](const uint16_t* stage) {
@@ -2114,15 +2131,18 @@ __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint1
   }
 }
 
-// BF16x3 half-transform with 256 x 256 tiles (ring256) from the full bf16
-// hi/lo copy of B (bhi/blo [naux][npad][npad]) and C^T's planes (cthi/ctlo
-// [ldo][npad]): tile (mb, nb) of X_q into X's hi/lo planes, one tile per
-// workgroup. Rows of B past npad and of C^T past ldo read row 0; X rows past
+// BF16x3 half-transform with 256 x 256 tiles (ring256 with its interleaved
+// chunk layout) from the bf16 full copy of B and C^T, both in the
+// interleaved layout of xform16_dma_kernel: tile (mb, nb) of X_q into X's
+// hi/lo planes, one tile per workgroup. Each load reads whole 128-byte lines
+// (8 rows x 32 k, hi and lo): 15.9 ms at nao 1500 / naux 5000 / nocc 250,
+// against 17.9 with separate planes (every load then used half of each line
+// it touched) and 17.6 with interleaved planes loaded as 16-row half-line
+// chunks. Rows of B past npad and of C^T past ldo read row 0; X rows past
 // npad (up to xrows) are written as zeros.
 __global__ __launch_bounds__(kR256Threads, 1) void xform16_256_kernel(
-    const uint16_t* __restrict__ bhi, const uint16_t* __restrict__ blo, int npad, const uint16_t* __restrict__ cthi,
-    const uint16_t* __restrict__ ctlo, int ldo, uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo, int xrows,
-    int qbase, int nblocks_n, int nblocks_m, int nwg) {
+    const uint16_t* __restrict__ b, int npad, const uint16_t* __restrict__ ct, int ldo, uint16_t* __restrict__ xhi,
+    uint16_t* __restrict__ xlo, int xrows, int qbase, int nblocks_n, int nblocks_m, int nwg) {
   const int logical = xcd_remap(blockIdx.x, nwg);
   const int nb = logical % nblocks_n;
   const int mb = (logical / nblocks_n) % nblocks_m;
@@ -2133,26 +2153,26 @@ __global__ __launch_bounds__(kR256Threads, 1) void xform16_256_kernel(
   const int wn = wave >> 1;
   // Waves 0-3 load B_q's rows of block mb, waves 4-7 C^T's rows of block nb.
   const bool is_c = wave >= 4;
+  // Load v of this wave fills chunk 8 wave + v: rows 8 (chunk & 31) .. + 8 of its operand, lane l
+  // the row's 16-byte segment l & 7 (the ring's interleaved chunk layout), i.e. logical segment
+  // (l & 7) ^ row of the row's 128-byte line of the step (32 hi, then 32 lo).
   int off[kR256Loads];
 #pragma unroll
   for (int v = 0; v < kR256Loads; ++v) {
-    const int blk = ((wave * kR256Loads + v) >> 1) & 15;
-    int row = (is_c ? nb : mb) * 256 + blk * 16 + (lane & 15);
+    const int c32 = (wave * kR256Loads + v) & 31;
+    const int r = lane >> 3;
+    int row = (is_c ? nb : mb) * 256 + c32 * 8 + r;
     row = row < (is_c ? ldo : npad) ? row : 0;
-    off[v] = (row * npad + (lane >> 4) * 8) * 2;
+    off[v] = (row * 2 * npad + ((lane & 7) ^ r) * 8) * 2;
   }
-  const long long plane = (long long)q * npad * npad;
-  const uint16_t* src_hi = is_c ? cthi : bhi + plane;
-  const uint16_t* src_lo = is_c ? ctlo : blo + plane;
-  auto addr = [&](int step, int v) -> const char* {
-    return (const char*)(((v & 1) ? src_lo : src_hi) + step * 32) + (unsigned)off[v];
-  };
+  const uint16_t* src = is_c ? ct : b + (long long)q * 2 * npad * npad;
+  auto addr = [&](int step, int v) -> const char* { return (const char*)(src + step * 64) + (unsigned)off[v]; };
   f4 acc[8][4];
 #pragma unroll
   for (int mi = 0; mi < 8; ++mi)
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
-  ring256(addr, npad / 32, acc);
+  ring256<true>(addr, npad / 32, acc);
   const long long xbase = (long long)q * xrows * ldo;
   if ((mb + 1) * 256 <= npad && (nb + 1) * 256 <= ldo) {
     // A tile inside X (and inside B's rows): no bounds to check.
@@ -2190,13 +2210,14 @@ __global__ __launch_bounds__(kR256Threads, 1) void xform16_256_kernel(
   }
 }
 
-// BF16x3 half-transform over the ring from the full bf16 hi/lo copy of B
-// (bhi/blo [naux][npad][npad]) and C^T's planes (cthi/ctlo [ldo][npad]):
-// the 128 x 128 tile (mb, nb) of X_q into X's hi/lo planes.
-__global__ __launch_bounds__(kThreads, 1) void xform16_dma_kernel(const uint16_t* __restrict__ bhi,
-                                                                  const uint16_t* __restrict__ blo, int npad,
-                                                                  const uint16_t* __restrict__ cthi,
-                                                                  const uint16_t* __restrict__ ctlo, int ldo,
+// BF16x3 half-transform over the ring from the bf16 full copy of B (b:
+// [naux][npad][npad / 32][64], each 32-column block as 32 hi then the same
+// 32 lo) and C^T in the same interleaved layout (ct: [ldo][npad / 32][64]):
+// the 128 x 128 tile (mb, nb) of X_q into X's hi/lo planes. A stage's hi and
+// lo chunks of a 16-row block come from the two halves of the same 128-byte
+// lines.
+__global__ __launch_bounds__(kThreads, 1) void xform16_dma_kernel(const uint16_t* __restrict__ b, int npad,
+                                                                  const uint16_t* __restrict__ ct, int ldo,
                                                                   uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo,
                                                                   int xrows, int qbase, int nblocks_n, int nblocks_m,
                                                                   int nwg) {
@@ -2216,12 +2237,11 @@ __global__ __launch_bounds__(kThreads, 1) void xform16_dma_kernel(const uint16_t
     const int blk = (ch >> 1) & 7;
     int row = (is_b ? nb : mb) * kSyrkTile + blk * 16 + (lane & 15);
     row = row < (is_b ? ldo : npad) ? row : 0;
-    off[v] = (row * npad + (lane >> 4) * 8) * 2;
+    off[v] = (row * 2 * npad + (lane >> 4) * 8) * 2;
   }
-  const long long plane = (long long)q * npad * npad;
+  const long long plane = (long long)q * 2 * npad * npad;
   auto src = [&](int step, int operand, int pl) -> const uint16_t* {
-    if (operand) return (pl ? ctlo : cthi) + step * 32;
-    return (pl ? blo : bhi) + plane + step * 32;
+    return (operand ? ct : b + plane) + step * 64 + pl * 32;
   };
   auto no_sign = [](int) { return 0; };
   f4 acc[4][4];
@@ -2253,8 +2273,8 @@ __global__ __launch_bounds__(kThreads, 1) void xform16_dma_kernel(const uint16_t
 // plane): the element split of launch_split (hi = bf16(fp32(b)), lo = bf16(fp32(b) - hi)), tiles
 // transposed through LDS as in expand_full_kernel.
 __global__ __launch_bounds__(kThreads) void expand_full16_kernel(const double* __restrict__ packed, long long p,
-                                                                 int naux, int ntp, int npad, uint16_t* __restrict__ hi,
-                                                                 uint16_t* __restrict__ lo) {
+                                                                 int naux, int ntp, int npad,
+                                                                 uint16_t* __restrict__ out) {
   __shared__ uint32_t tile[kTile][kTile + 1];
   const long long jobs = (long long)naux * ntp;
   for (long long job = blockIdx.x; job < jobs; job += gridDim.x) {
@@ -2265,25 +2285,26 @@ __global__ __launch_bounds__(kThreads) void expand_full16_kernel(const double* _
     while (tr * (tr + 1) / 2 > t) --tr;
     const int tc = t - tr * (tr + 1) / 2;
     const double* src = packed + q * p + (long long)t * kTileElems;
-    const long long oq = (long long)q * npad * npad;
+    const long long oq = (long long)q * 2 * npad * npad;
     for (int e = threadIdx.x; e < kTileElems; e += kThreads) {
       const int i = e / kTile, j = e % kTile;
       const float x = (float)src[e];
       const uint16_t h = bf16_of(x);
       const uint16_t l = bf16_of(x - f32_of(h));
       tile[i][j] = (uint32_t)h | ((uint32_t)l << 16);
-      const long long o = oq + (long long)(tr * kTile + i) * npad + tc * kTile + j;
-      hi[o] = h;
-      lo[o] = l;
+      // Row tr 32 + i, column block tc: hi at 64 tc + j, lo 32 further.
+      const long long o = oq + (long long)(tr * kTile + i) * 2 * npad + tc * 2 * kTile + j;
+      out[o] = h;
+      out[o + kTile] = l;
     }
     if (tr != tc) {
       __syncthreads();
       for (int e = threadIdx.x; e < kTileElems; e += kThreads) {
         const int i = e / kTile, j = e % kTile;
         const uint32_t v = tile[j][i];
-        const long long o = oq + (long long)(tc * kTile + i) * npad + tr * kTile + j;
-        hi[o] = (uint16_t)(v & 0xFFFFu);
-        lo[o] = (uint16_t)(v >> 16);
+        const long long o = oq + (long long)(tc * kTile + i) * 2 * npad + tr * 2 * kTile + j;
+        out[o] = (uint16_t)(v & 0xFFFFu);
+        out[o + kTile] = (uint16_t)(v >> 16);
       }
     }
     __syncthreads();
@@ -2435,7 +2456,7 @@ hipError_t launch_expand_full(const double* packed, long long p, int naux, int n
                        (float*)out);
   else
     hipLaunchKernelGGL(expand_full16_kernel, grid, dim3(kThreads), 0, s, packed, p, naux, ntp, npad,
-                       (uint16_t*)out, (uint16_t*)out + (long long)naux * npad * npad);
+                       (uint16_t*)out);
   return hipGetLastError();
 }
 
@@ -2451,7 +2472,7 @@ hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, 
 }
 
 hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
-                                      void* x, hipStream_t s, const void* bfull_lo, int bn16) {
+                                      void* x, hipStream_t s, int bn16) {
   const int npad = n_tiles(nao) * kTile;
   const int xrows = x_rows(nao);
   if (precision == HFJK_BF16X3 && ldo > kSyrkTile && (bn16 == 0 || bn16 == 256)) {
@@ -2459,17 +2480,15 @@ hipError_t launch_half_transform_full(const void* bfull, int precision, int naux
     const uint16_t* bh = (const uint16_t*)bfull;
     const uint16_t* ch = (const uint16_t*)ct;
     uint16_t* xh = (uint16_t*)x;
-    const long long nb16 = (long long)naux * npad * npad, nc16 = (long long)ldo * npad,
-                    nx16 = (long long)naux * xrows * ldo;
-    const uint16_t* bl = bfull_lo != nullptr ? (const uint16_t*)bfull_lo : bh + nb16;
+    const long long nx16 = (long long)naux * xrows * ldo;
     const int nbn = (ldo + 255) / 256;
     const int nbm = (xrows + 255) / 256;
     const int per = std::max(1, (1 << 22) / (nbn * nbm));  // keep a launch below 2^32 work-items
     for (int base = 0; base < naux; base += per) {
       const int count = std::min(per, naux - base);
       const int nwg = nbn * nbm * count;
-      hipLaunchKernelGGL(xform16_256_kernel, dim3((unsigned)nwg), dim3(kR256Threads), 0, s, bh, bl, npad, ch,
-                         ch + nc16, ldo, xh, xh + nx16, xrows, base, nbn, nbm, nwg);
+      hipLaunchKernelGGL(xform16_256_kernel, dim3((unsigned)nwg), dim3(kR256Threads), 0, s, bh, npad, ch, ldo, xh,
+                         xh + nx16, xrows, base, nbn, nbm, nwg);
       hipError_t err = hipGetLastError();
       if (err != hipSuccess) return err;
     }
@@ -2488,15 +2507,11 @@ hipError_t launch_half_transform_full(const void* bfull, int precision, int naux
       hipLaunchKernelGGL((xform_dma_kernel<float, 1>), dim3((unsigned)nwg), dim3(kThreads), 0, s,
                          (const float*)bfull, npad, (const float*)ct, ldo, (float*)x, xrows, base, nbn, nbm, nwg);
     } else {
-      // BF16x3: B's hi plane then its lo plane, C^T's likewise, X's likewise.
-      const uint16_t* bh = (const uint16_t*)bfull;
-      const uint16_t* ch = (const uint16_t*)ct;
+      // BF16x3: B and C^T interleaved (hi and lo per 32-column block), X as hi then lo plane.
       uint16_t* xh = (uint16_t*)x;
-      const long long nb16 = (long long)naux * npad * npad, nc16 = (long long)ldo * npad,
-                      nx16 = (long long)naux * xrows * ldo;
-      const uint16_t* bl = bfull_lo != nullptr ? (const uint16_t*)bfull_lo : bh + nb16;
-      hipLaunchKernelGGL(xform16_dma_kernel, dim3((unsigned)nwg), dim3(kThreads), 0, s, bh, bl, npad, ch,
-                         ch + nc16, ldo, xh, xh + nx16, xrows, base, nbn, nbm, nwg);
+      const long long nx16 = (long long)naux * xrows * ldo;
+      hipLaunchKernelGGL(xform16_dma_kernel, dim3((unsigned)nwg), dim3(kThreads), 0, s, (const uint16_t*)bfull, npad,
+                         (const uint16_t*)ct, ldo, xh, xh + nx16, xrows, base, nbn, nbm, nwg);
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
@@ -2504,13 +2519,17 @@ hipError_t launch_half_transform_full(const void* bfull, int precision, int naux
   return hipSuccess;
 }
 
-hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s) {
+hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out, hipStream_t s, bool interleaved) {
   const int npad = n_tiles(nao) * kTile;
   const int ldo = occ_ld(nocc);
   const long long n = (long long)npad * ldo;
   uint16_t* hi = (uint16_t*)out;
-  hipLaunchKernelGGL(orbitals_bf16x3_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, c, nao, nocc, npad, ldo, hi,
-                     hi + n);
+  if (interleaved)
+    hipLaunchKernelGGL(orbitals_bf16x3_kernel<true>, dim3(grid_for(n)), dim3(kThreads), 0, s, c, nao, nocc, npad, ldo,
+                       hi, hi + n);
+  else
+    hipLaunchKernelGGL(orbitals_bf16x3_kernel<false>, dim3(grid_for(n)), dim3(kThreads), 0, s, c, nao, nocc, npad,
+                       ldo, hi, hi + n);
   return hipGetLastError();
 }
 

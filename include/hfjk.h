@@ -110,15 +110,17 @@ HFJK_API int hfjk_bind(hfjk_ctx *ctx, int naux, int nao, const double *b64, cons
 
 /* Optional full row-major copies of the bound tensor for the
  * half-transforms: hfjk_expand_full writes full[q][npad][npad] in precision
- * HFJK_FP64 (double), HFJK_FP32 (float) or HFJK_BF16X3 (uint16: the bf16 hi
- * plane of every element, then the lo plane; 2 naux npad^2 values)
+ * HFJK_FP64 (double), HFJK_FP32 (float) or HFJK_BF16X3 (uint16
+ * [q][npad][npad / 32][64]: each 32-column block of a row as the bf16 hi
+ * parts of its elements, then their lo parts; 2 naux npad^2 values)
  * (npad = 32 ceil(nao / 32), both triangles, zero past nao) from the tiled
  * packed FP64 tensor; hfjk_bind_full borrows it for that precision (NULL
  * unbinds; hfjk_bind unbinds all three). With it the half-transform of that
  * precision streams both operands straight into LDS (twice the memory of the
- * packed copy for 13 % (FP64) and 7 % (FP32) less time at nao 1500,
- * naux 5000). No counterpart in the reference (its B is always full,
- * [naux, nao, nao]: coulomb_exchange.py:124-145). */
+ * packed copy for 13 % (FP64), 7 % (FP32) and about 25 % (BF16X3, 256 x 256
+ * tiles past 128 occupied columns) less time at nao 1500, naux 5000). No
+ * counterpart in the reference (its B is always full, [naux, nao, nao]:
+ * coulomb_exchange.py:124-145). */
 HFJK_API int hfjk_expand_full(hfjk_ctx *ctx, const double *packed, int naux, int nao, int precision, void *full,
                               void *stream);
 HFJK_API int hfjk_bind_full(hfjk_ctx *ctx, int precision, const void *full);

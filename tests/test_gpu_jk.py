@@ -338,10 +338,11 @@ def test_full_copy_half_transform_matches_the_oracle(torch, precision, naux, nao
     expect_full = np.zeros((naux, npad, npad))
     expect_full[:, :nao, :nao] = b
     if precision == "bf16x3":
-        full = jk._full[_lib.BF16X3].view(2, naux, npad, npad).cpu().numpy().view(np.uint16)
+        # Each 32-column block of a row: the 32 hi parts, then the same 32 lo parts.
+        full = jk._full[_lib.BF16X3].view(naux, npad, npad // 32, 2, 32).cpu().numpy().view(np.uint16)
         hi, lo = bf16_planes(expect_full)
-        np.testing.assert_array_equal(full[0], hi)
-        np.testing.assert_array_equal(full[1], lo)
+        np.testing.assert_array_equal(full[:, :, :, 0, :].reshape(naux, npad, npad), hi)
+        np.testing.assert_array_equal(full[:, :, :, 1, :].reshape(naux, npad, npad), lo)
     else:
         full = jk._full[_lib.precision_code(precision)].view(naux, npad, npad).cpu().numpy()
         np.testing.assert_array_equal(full, expect_full.astype(full.dtype))
