@@ -259,6 +259,15 @@ void run_coulomb(hfjk_ctx* ctx, const double* density, double* out, int precisio
   }
 }
 
+// X of the occupied-form exchange in the interleaved BF16X3 layout (hi and
+// lo per 32-column block): written by the 256 x 256-tile half-transform from
+// the full copy and read by the 256 x 256-tile SYRK and the charges, when
+// both kernels run for these arguments; the hi and lo planes otherwise.
+bool x16_interleaved(const hfjk_ctx* ctx, int precision, bool is_signed, int ldo) {
+  return precision == HFJK_BF16X3 && ctx->b16_full != nullptr && ctx->x_full == 0 &&
+         hfjk::x16_tiles256(ldo, ctx->x16_bn) && hfjk::syrk16_tiles256(ctx->nao, ldo, is_signed, ctx->syrk_kind, 0);
+}
+
 // Orbitals (and signs) into the workspace and the half-transform X = B C.
 void exchange_front(hfjk_ctx* ctx, const double* orbitals, int nocc, const double* signs, int precision, void* ws,
                     const Layout& L, hipStream_t s) {
@@ -293,7 +302,7 @@ void exchange_front(hfjk_ctx* ctx, const double* orbitals, int nocc, const doubl
     Stage st(ctx, s, "exchange_half_transform");
     if (full)
       check_hip(hfjk::launch_half_transform_full(bfull, precision, ctx->naux, ctx->nao, ct, L.ldo, x, s,
-                                                 ctx->x16_bn),
+                                                 ctx->x16_bn, x16_interleaved(ctx, precision, signs != nullptr, L.ldo)),
                 "half transform launch");
     else if (split)
       check_hip(hfjk::launch_half_transform_bf16x3((const uint16_t*)b, L.p, ctx->naux, ctx->nao, cp, L.ldo, x,
@@ -315,7 +324,8 @@ void exchange_back(hfjk_ctx* ctx, bool is_signed, double* out, int precision, vo
   {
     Stage st(ctx, s, "exchange_syrk");
     check_hip(hfjk::launch_syrk(x, precision, ctx->naux, ctx->nao, L.ldo, sg, is_signed, L.nsplit_k, part,
-                                ctx->syrk_kind, s), "syrk launch");
+                                ctx->syrk_kind, s, nullptr, 0, x16_interleaved(ctx, precision, is_signed, L.ldo)),
+              "syrk launch");
   }
   {
     Stage st(ctx, s, "exchange_reduce");
@@ -901,7 +911,8 @@ int hfjk_jk_occ(hfjk_ctx* ctx, const double* orbitals, int nocc, double* coulomb
         cpad = at<void>(workspace, L.qc);
       }
       check_hip(hfjk::launch_charges_from_x(at<void>(workspace, L.xx), precision, ctx->naux, ctx->nao, L.ldo, cpad,
-                                            parts, gamma, serial ? 0 : side_groups(ctx, precision), side),
+                                            parts, gamma, serial ? 0 : side_groups(ctx, precision), side,
+                                            x16_interleaved(ctx, precision, false, L.ldo)),
                 "charges launch");
     }
     const void* b = bound_tensor(ctx, precision);

@@ -218,7 +218,10 @@ int syrk_slots(int precision, int naux, int ldo, int nsplit);
 // (hipErrorInvalidValue for a launch that would not run them).
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
                        bool is_signed, int nsplit, void* partial, int kind, hipStream_t s,
-                       const double* auxsign = nullptr, int shift = 0);
+                       const double* auxsign = nullptr, int shift = 0, bool x_ilv = false);
+// Whether launch_syrk runs the BF16X3 256 x 256-tile SYRK for these
+// arguments (only that kernel reads X interleaved: x_ilv).
+bool syrk16_tiles256(int nao, int ldo, bool is_signed, int kind, int shift);
 // K (+)= the symmetric tiles sym plus the antisymmetric tiles anti: lower
 // tiles give K[m][n] = sym + anti, their mirror K[n][m] = sym - anti
 // (diagonal tiles computed whole). accumulate adds to K instead of writing.
@@ -244,7 +247,7 @@ hipError_t launch_half_transform_bf16x3(const uint16_t* b, long long p, int naux
 // FP32 and BF16x3 X); parts holds naux * charge_groups(nao) doubles.
 int charge_groups(int nao);
 hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao, int ldo, const void* cpad,
-                                 double* parts, double* gamma, int max_groups, hipStream_t s);
+                                 double* parts, double* gamma, int max_groups, hipStream_t s, bool x_ilv = false);
 // Half-transform from a full row-major copy of B (xk.hip):
 // launch_expand_full writes bfull[q][npad][npad] in the precision's type
 // (npad = 32 ceil(nao / 32)) from the tiled packed FP64 tensor; for BF16X3
@@ -254,13 +257,17 @@ hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao
 // launch_orbitals_t writes C^T [ldo][npad] in FP64 or FP32 (BF16X3 takes
 // launch_orbitals_bf16x3's interleaved C^T);
 // launch_half_transform_full computes X = B C on the direct-LDS ring (the
-// bf16 ring for BF16X3, X as hi/lo planes; BF16X3 with ldo > 128 on 256 x 256
-// tiles unless bn16 asks for a column tile below 256).
+// bf16 ring for BF16X3, X as hi/lo planes; BF16X3 on 256 x 256 tiles when
+// x16_tiles256(ldo, bn16): ldo > 128 and bn16 not asking for a column tile
+// below 256, and then with x_ilv X interleaved as [naux][xrows][ldo / 32][64],
+// hi then lo per 32-column block, for the 256 x 256-tile SYRK and the
+// charges).
 hipError_t launch_expand_full(const double* packed, long long p, int naux, int nao, int precision, void* out,
                               hipStream_t s);
 hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, void* ct, hipStream_t s);
 hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
-                                      void* x, hipStream_t s, int bn16 = 0);
+                                      void* x, hipStream_t s, int bn16 = 0, bool x_ilv = false);
+bool x16_tiles256(int ldo, int bn16);
 // k-point DF-J/K (kpoint.hip): complex values as interleaved double pairs.
 // launch_kp_charges: inner [nk][naux] = w_k <B_k[P], D_k> (conjugated B) and
 // charges [naux] = sum_k inner, for blocks B_k at b + k kstride (elements);

@@ -848,7 +848,8 @@ __global__ __launch_bounds__(kThreads) void xform_bf16x3_kernel(
 // rows [kChargeRows g, kChargeRows (g + 1)) of X_q into parts[q][g];
 // charge_reduce_kernel adds the parts in a fixed order.
 // F: 0 = X in FP64, 1 = FP32, 2 = BF16x3 (hi plane, then lo plane `plane`
-// elements later).
+// elements later), 3 = BF16x3 interleaved (each row as ldo / 32 blocks of 32
+// hi then 32 lo values).
 constexpr int kChargeRows = 128;
 
 // V consecutive X values from element idx (16 bytes of the plane, or of each
@@ -856,6 +857,14 @@ constexpr int kChargeRows = 128;
 template <int F>
 struct XVec {
   static constexpr int V = F == 0 ? 2 : F == 1 ? 4 : 8;
+  // Element (row m, column i) of X_q: F 0-2 row-major [xrows][ldo] per plane; F 3 (BF16x3
+  // interleaved) [xrows][ldo / 32][64], hi then lo per 32-column block.
+  static __device__ __forceinline__ long long index(long long xq, int m, int i, int ldo) {
+    if constexpr (F == 3)
+      return 2 * xq + (long long)m * 2 * ldo + (i >> 5) * 64 + (i & 31);
+    else
+      return xq + (long long)m * ldo + i;
+  }
   static __device__ __forceinline__ void load(const void* __restrict__ x, long long plane, long long idx,
                                               double (&out)[V]) {
     if constexpr (F == 0) {
@@ -871,7 +880,7 @@ struct XVec {
     } else {
       const uint16_t* h = (const uint16_t*)x;
       const uint4 hv = *(const uint4*)(h + idx);
-      const uint4 lv = *(const uint4*)(h + idx + plane);
+      const uint4 lv = *(const uint4*)(h + idx + (F == 3 ? 32 : plane));
       const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w};
       const uint32_t lw[4] = {lv.x, lv.y, lv.z, lv.w};
 #pragma unroll
@@ -927,10 +936,11 @@ __global__ __launch_bounds__(kThreads) void charges_from_x_kernel(const void* __
     constexpr int V = XVec<F>::V;
     double part = 0.0;
     // Rows [m_begin, m_end) of X_q are one contiguous range of
-    // (m_end - m_begin) * ldo elements: the workgroup streams it in 16-byte
-    // groups (ldo is a multiple of 32, so no group straddles two rows), group
-    // f of thread t being f = t + kThreads j. The padding is zero, so the loop
-    // body has no branch and the loads of successive groups overlap.
+    // (m_end - m_begin) * ldo elements (F 3: of both parts): the workgroup
+    // streams it in 16-byte groups (ldo is a multiple of 32, so no group
+    // straddles two rows), group f of thread t being f = t + kThreads j. The
+    // padding is zero, so the loop body has no branch and the loads of
+    // successive groups overlap.
     const int m_begin = g * kChargeRows;
     const int per_row = ldo / V;
     const int ngroups_blk = (m_end - m_begin) * per_row;
@@ -942,7 +952,7 @@ __global__ __launch_bounds__(kThreads) void charges_from_x_kernel(const void* __
     for (int f = threadIdx.x; f < ngroups_blk; f += kThreads) {
       const int i0 = gi * V;
       double xv[V], cv[V];
-      XVec<F>::load(x, plane, xq + (long long)m * ldo + i0, xv);
+      XVec<F>::load(x, plane, XVec<F>::index(xq, m, i0, ldo), xv);
       load_orbitals<TC, V>(c, (long long)m * ldo + i0, cv);
 #pragma unroll
       for (int e = 0; e < V; ++e) part += xv[e] * cv[e];
@@ -2061,11 +2071,15 @@ __device__ __forceinline__ void ring256(const Addr& addr, int steps, f4 (&acc)[8
 }
 
 // BF16x3 SYRK with 256 x 256 tiles: the four 128 x 128 quadrants of a
-// 256-row tile pair in one workgroup (ring256). Partial slots as in
+// 256-row tile pair in one workgroup (ring256). XILV: X interleaved (hi and
+// lo per 32-column block, as the 256 x 256 half-transform writes it), loaded
+// as whole 128-byte lines into the ring's interleaved chunks (7.5 ms against
+// 8.7 from the hi/lo planes at nao 1500 / naux 5000 / nocc 250). Partial slots as in
 // syrk16_dma_kernel: the tile's quadrants are written as the partial tiles
 // of their 128-row tile pairs (the upper quadrant of a diagonal pair, and
 // quadrants past the last 128-row block, are not written). Column signs +1
 // (the occupied orbitals).
+template <bool XILV>
 __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint16_t* __restrict__ xhi,
                                                                      const uint16_t* __restrict__ xlo, int naux,
                                                                      int xrows, int ldo, int nsplit, int npairs,
@@ -2084,14 +2098,23 @@ __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint1
   // Load v of this wave is chunk 8 wave + v = 32 operand + 2 block + plane (waves 0-3 operand A,
   // the rows of block mb; waves 4-7 operand B, block nb). Rows past xrows read row 0: their
   // products land in rows and columns of K past nao, never read.
+  // XILV (X interleaved, hi and lo per 32-column block): load v fills the ring's interleaved
+  // chunk 8 wave + v, rows 8 (chunk & 31) .. + 8, lane l the row's segment (l & 7) ^ row.
   int off[kR256Loads];
 #pragma unroll
   for (int v = 0; v < kR256Loads; ++v) {
     const int ch = wave * kR256Loads + v;
-    const int blk = (ch >> 1) & 15;
-    int row = ((ch >> 5) ? nb : mb) * 256 + blk * 16 + (lane & 15);
-    row = row < xrows ? row : 0;
-    off[v] = (row * ldo + (lane >> 4) * 8) * 2;
+    if constexpr (XILV) {
+      const int r = lane >> 3;
+      int row = ((ch >> 5) ? nb : mb) * 256 + (ch & 31) * 8 + r;
+      row = row < xrows ? row : 0;
+      off[v] = (row * 2 * ldo + ((lane & 7) ^ r) * 8) * 2;
+    } else {
+      const int blk = (ch >> 1) & 15;
+      int row = ((ch >> 5) ? nb : mb) * 256 + blk * 16 + (lane & 15);
+      row = row < xrows ? row : 0;
+      off[v] = (row * ldo + (lane >> 4) * 8) * 2;
+    }
   }
   const int nch = ldo / 32;
   // The partial tile this wave writes into: 128-row tile pair (2 mb + wm, 2 nb + wn / 2), columns
@@ -2103,8 +2126,13 @@ __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint1
     slot_aux(c, nsplit, naux, run, qb, qe);
     auto addr = [&](int step, int v) -> const char* {
       const int qq = step / nch;
-      const long long base = (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * 32;
-      return (const char*)(((v & 1) ? xlo : xhi) + base) + (unsigned)off[v];
+      if constexpr (XILV) {
+        const long long base = (long long)(qb + qq) * xrows * 2 * ldo + (step - qq * nch) * 64;
+        return (const char*)(xhi + base) + (unsigned)off[v];
+      } else {
+        const long long base = (long long)(qb + qq) * xrows * ldo + (step - qq * nch) * 32;
+        return (const char*)(((v & 1) ? xlo : xhi) + base) + (unsigned)off[v];
+      }
     };
     f4 acc[8][4];
 #pragma unroll
@@ -2114,7 +2142,7 @@ __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint1
     // The previous slot's LDS reads and loads have all returned (ring256 ends on a wait for
     // both), so the stages may take this slot's loads once every wave is here.
     __syncthreads();
-    ring256(addr, (qe - qb) * nch, acc);
+    ring256<XILV>(addr, (qe - qb) * nch, acc);
     if (writes) {
       // The lane's offset is opaque here, so its 32 store addresses are formed per slot rather than
       // hoisted out of the slot loop (where they would outlive the accumulators' registers).
@@ -2140,6 +2168,7 @@ __global__ __launch_bounds__(kR256Threads, 1) void syrk16_256_kernel(const uint1
 // it touched) and 17.6 with interleaved planes loaded as 16-row half-line
 // chunks. Rows of B past npad and of C^T past ldo read row 0; X rows past
 // npad (up to xrows) are written as zeros.
+template <bool XILV>
 __global__ __launch_bounds__(kR256Threads, 1) void xform16_256_kernel(
     const uint16_t* __restrict__ b, int npad, const uint16_t* __restrict__ ct, int ldo, uint16_t* __restrict__ xhi,
     uint16_t* __restrict__ xlo, int xrows, int qbase, int nblocks_n, int nblocks_m, int nwg) {
@@ -2173,12 +2202,17 @@ __global__ __launch_bounds__(kR256Threads, 1) void xform16_256_kernel(
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f4{0.f, 0.f, 0.f, 0.f};
   ring256<true>(addr, npad / 32, acc);
-  const long long xbase = (long long)q * xrows * ldo;
+  // XILV: X_q as [xrows][ldo / 32][64] (hi then lo per 32-column block) from xhi; else the hi
+  // plane at xhi, the lo plane at xlo.
+  const int ldx = XILV ? 2 * ldo : ldo;                                    // row stride
+  auto col_at = [&](int col) { return XILV ? (col >> 5) * 64 + (col & 31) : col; };
+  const long long lo_at = XILV ? 32 : (long long)(xlo - xhi);                // lo part from hi
+  const long long xbase = (long long)q * xrows * ldx;
   if ((mb + 1) * 256 <= npad && (nb + 1) * 256 <= ldo) {
-    // A tile inside X (and inside B's rows): no bounds to check.
-    uint16_t* th = xhi + xbase + (long long)(mb * 256 + wm * 128 + 4 * (lane >> 4)) * ldo + nb * 256 + wn * 64 +
-                   (lane & 15);
-    uint16_t* tl = th + (xlo - xhi);
+    // A tile inside X (and inside B's rows): no bounds to check. Column wn 64 + ni 16 + lane & 15
+    // stays within one 32-column block for each ni.
+    uint16_t* th = xhi + xbase + (long long)(mb * 256 + wm * 128 + 4 * (lane >> 4)) * ldx +
+                   col_at(nb * 256 + wn * 64 + (lane & 15));
 #pragma unroll
     for (int mi = 0; mi < 8; ++mi)
 #pragma unroll
@@ -2187,8 +2221,9 @@ __global__ __launch_bounds__(kR256Threads, 1) void xform16_256_kernel(
         for (int r = 0; r < 4; ++r) {
           const float v = acc[mi][ni][r];
           const uint16_t h = bf16_of(v);
-          th[(mi * 16 + r) * ldo + ni * 16] = h;
-          tl[(mi * 16 + r) * ldo + ni * 16] = bf16_of(v - f32_of(h));
+          const long long o = (mi * 16 + r) * ldx + (XILV ? (ni >> 1) * 64 + (ni & 1) * 16 : ni * 16);
+          th[o] = h;
+          th[o + lo_at] = bf16_of(v - f32_of(h));
         }
   } else {
 #pragma unroll
@@ -2202,8 +2237,9 @@ __global__ __launch_bounds__(kR256Threads, 1) void xform16_256_kernel(
           if (row < xrows && col < ldo) {
             const float v = row < npad ? acc[mi][ni][r] : 0.f;
             const uint16_t h = bf16_of(v);
-            xhi[xbase + (long long)row * ldo + col] = h;
-            xlo[xbase + (long long)row * ldo + col] = bf16_of(v - f32_of(h));
+            const long long o = xbase + (long long)row * ldx + col_at(col);
+            xhi[o] = h;
+            xhi[o + lo_at] = bf16_of(v - f32_of(h));
           }
         }
       }
@@ -2471,11 +2507,14 @@ hipError_t launch_orbitals_t(const double* c, int nao, int nocc, int precision, 
   return hipGetLastError();
 }
 
+bool x16_tiles256(int ldo, int bn16) { return ldo > kSyrkTile && (bn16 == 0 || bn16 == 256); }
+
 hipError_t launch_half_transform_full(const void* bfull, int precision, int naux, int nao, const void* ct, int ldo,
-                                      void* x, hipStream_t s, int bn16) {
+                                      void* x, hipStream_t s, int bn16, bool x_ilv) {
   const int npad = n_tiles(nao) * kTile;
   const int xrows = x_rows(nao);
-  if (precision == HFJK_BF16X3 && ldo > kSyrkTile && (bn16 == 0 || bn16 == 256)) {
+  if (x_ilv && !(precision == HFJK_BF16X3 && x16_tiles256(ldo, bn16))) return hipErrorInvalidValue;
+  if (precision == HFJK_BF16X3 && x16_tiles256(ldo, bn16)) {
     // BF16x3 with more than 128 occupied columns: the 256 x 256 tiles.
     const uint16_t* bh = (const uint16_t*)bfull;
     const uint16_t* ch = (const uint16_t*)ct;
@@ -2487,8 +2526,12 @@ hipError_t launch_half_transform_full(const void* bfull, int precision, int naux
     for (int base = 0; base < naux; base += per) {
       const int count = std::min(per, naux - base);
       const int nwg = nbn * nbm * count;
-      hipLaunchKernelGGL(xform16_256_kernel, dim3((unsigned)nwg), dim3(kR256Threads), 0, s, bh, npad, ch, ldo, xh,
-                         xh + nx16, xrows, base, nbn, nbm, nwg);
+      if (x_ilv)
+        hipLaunchKernelGGL(xform16_256_kernel<true>, dim3((unsigned)nwg), dim3(kR256Threads), 0, s, bh, npad, ch, ldo,
+                           xh, xh + nx16, xrows, base, nbn, nbm, nwg);
+      else
+        hipLaunchKernelGGL(xform16_256_kernel<false>, dim3((unsigned)nwg), dim3(kR256Threads), 0, s, bh, npad, ch,
+                           ldo, xh, xh + nx16, xrows, base, nbn, nbm, nwg);
       hipError_t err = hipGetLastError();
       if (err != hipSuccess) return err;
     }
@@ -2536,7 +2579,7 @@ hipError_t launch_orbitals_bf16x3(const double* c, int nao, int nocc, void* out,
 int charge_groups(int nao) { return (nao + kChargeRows - 1) / kChargeRows; }
 
 hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao, int ldo, const void* cpad,
-                                 double* parts, double* gamma, int max_groups, hipStream_t s) {
+                                 double* parts, double* gamma, int max_groups, hipStream_t s, bool x_ilv) {
   const int xrows = x_rows(nao);
   const int ng = charge_groups(nao);
   const long long plane = (long long)naux * xrows * ldo;
@@ -2551,8 +2594,11 @@ hipError_t launch_charges_from_x(const void* x, int precision, int naux, int nao
   else if (precision == HFJK_FP32)
     hipLaunchKernelGGL(charges_from_x_kernel<1>, grid, dim3(kThreads), 0, s, x, plane, nao, xrows, ldo, cpad, ng,
                        nitems, parts);
-  else
+  else if (!x_ilv)
     hipLaunchKernelGGL(charges_from_x_kernel<2>, grid, dim3(kThreads), 0, s, x, plane, nao, xrows, ldo, cpad, ng,
+                       nitems, parts);
+  else
+    hipLaunchKernelGGL(charges_from_x_kernel<3>, grid, dim3(kThreads), 0, s, x, plane, nao, xrows, ldo, cpad, ng,
                        nitems, parts);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
@@ -2626,9 +2672,14 @@ int syrk_slots(int precision, int naux, int ldo, int nsplit) {
   return run == 0 ? nsplit : (naux + run - 1) / run;
 }
 
+bool syrk16_tiles256(int nao, int ldo, bool is_signed, int kind, int shift) {
+  return !is_signed && shift == 0 && kind != 1 && kind != 2 && ldo <= kDmaMaxLdo &&
+         (long long)x_rows(nao) * ldo * 2 < (1LL << 31) && syrk_run_aux(HFJK_BF16X3, ldo) > 0;
+}
+
 hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo, const double* signs,
                        bool is_signed, int nsplit, void* partial, int kind, hipStream_t s, const double* auxsign,
-                       int shift) {
+                       int shift, bool x_ilv) {
   const int xrows = x_rows(nao);
   const int npairs = syrk_pairs(nao);
   const int run = syrk_run_aux(precision, ldo);
@@ -2674,14 +2725,19 @@ hipError_t launch_syrk(const void* x, int precision, int naux, int nao, int ldo,
     const bool ring16 = kind != 1 && ldo <= kDmaMaxLdo && (long long)xrows * ldo * 2 < (1LL << 31);
     // Unsigned (occupied orbitals, no column shift): the 256 x 256 tiles unless the 128-row ring is
     // asked for (kind 2); their grid holds about four workgroups per CU of a 256-CU device.
+    if (x_ilv && !syrk16_tiles256(nao, ldo, is_signed, kind, shift)) return hipErrorInvalidValue;
     if (ring16 && !is_signed && shift == 0 && kind != 2 && run > 0) {
       const int nb128 = (xrows + kSyrkTile - 1) / kSyrkTile;
       const int nb256 = (nb128 + 1) / 2;
       const int np256 = nb256 * (nb256 + 1) / 2;
       const int nslot = (naux + run - 1) / run;
       const int ns256 = std::max(8, std::min((nslot + 7) / 8 * 8, 1024 / np256 / 8 * 8));
-      hipLaunchKernelGGL(syrk16_256_kernel, dim3((unsigned)(np256 * ns256)), dim3(kR256Threads), 0, s, xhi, xhi + nx,
-                         naux, xrows, ldo, ns256, np256, npairs, nb128, run, (float*)partial);
+      if (x_ilv)
+        hipLaunchKernelGGL(syrk16_256_kernel<true>, dim3((unsigned)(np256 * ns256)), dim3(kR256Threads), 0, s, xhi,
+                           xhi + nx, naux, xrows, ldo, ns256, np256, npairs, nb128, run, (float*)partial);
+      else
+        hipLaunchKernelGGL(syrk16_256_kernel<false>, dim3((unsigned)(np256 * ns256)), dim3(kR256Threads), 0, s, xhi,
+                           xhi + nx, naux, xrows, ldo, ns256, np256, npairs, nb128, run, (float*)partial);
     } else if (ring16 && is_signed)
       hipLaunchKernelGGL(syrk16_dma_kernel<true>, dim3(grid), dim3(kThreads), 0, s, xhi, xhi + nx, naux, xrows, ldo,
                          signs, shift, nsplit, npairs, run, (float*)partial);

@@ -352,6 +352,10 @@ def test_full_copy_half_transform_matches_the_oracle(torch, precision, naux, nao
     assert rel(k, jk_ref.exchange(b, d)) < tol
     assert rel(k2, jk_ref.exchange(b, d)) < tol
     assert rel(j, jk_ref.coulomb(b, d)) < tol
+    # Signed columns from the same copy (BF16x3: X in hi/lo planes for the signed SYRK, where the
+    # unsigned build above interleaves them past 128 columns).
+    signs = np.where(np.arange(nocc) % 3 == 1, -1.0, 1.0)
+    assert rel(jk.exchange_occupied(c, signs), jk_ref.exchange(b, (c * signs) @ c.T)) < tol
     assert rel(k, k_packed) < {"fp64": 1e-13, "fp32": 1e-5, "bf16x3": 1e-4}[precision]
     if precision == "bf16x3":
         # The 128-row ring from the same copy (a column tile below 256) gives the same K.
