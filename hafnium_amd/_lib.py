@@ -100,6 +100,8 @@ SIGNATURES = [
                                     _vp, _c_int, _c_ll, ctypes.c_double, _vp, _c_int, _c_ll, _c_int, _vp]),
     ("hfjk_dsyev_batched_work", _c_ll, [_c_int, _c_int]),
     ("hfjk_dsyev_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _vp]),
+    ("hfjk_eigh_lowest_batched_work", _c_ll, [_c_int, _c_int, _c_int]),
+    ("hfjk_eigh_lowest_batched", _c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     ("hfjk_ddot_batched", _c_int, [_vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_ll, _vp, _vp]),
     ("hfjk_history_dots_batched", _c_int, [_vp, _c_int, _c_ll, _c_int, _vp, _c_int, _vp, _vp]),
     ("hfjk_diis_batched", _c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int, ctypes.POINTER(_c_int), _c_int, _vp, _vp]),

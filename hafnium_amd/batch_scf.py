@@ -3,7 +3,9 @@
 Every iteration advances all molecules at once: one batched DF-J/K build
 (``BatchedCoulombExchange``: one persistent launch per phase for every
 molecule), strided-batched GEMMs and a batched eigensolver on matrices
-zero-padded to the largest basis, and one host transfer of every
+zero-padded to the largest basis (for the Fock matrices the library's
+one-launch solver of the occupied eigenpairs, ``hfjk_eigh_lowest_batched``;
+rocSOLVER for the overlaps), and one host transfer of every
 molecule's energy terms and newest DIIS overlaps (the DIIS overlap
 matrices and coefficients stay on the device). Per molecule, the iteration
 is the one of ``solve_scf`` (hafnium/scf/driver.py:154-213 of the
@@ -46,6 +48,7 @@ from hafnium_amd.molecule import BasisOnAtoms, Molecule
 from hafnium_amd.scf import ScfOptions, mixed_phase_stalled
 
 DISCARDED_SHIFT = 1.0e3  # Eh added to padded / discarded directions of the orthogonal basis
+EIGH_LOWEST_MAX_N = 512  # largest matrix of hfjk_eigh_lowest_batched
 
 
 def _torch():
@@ -123,7 +126,7 @@ class _Start:
 class BatchedRhf:
     """Host integrals of N molecules -> fitted tensors in one batch, padded FP64 state on the GPU."""
 
-    def __init__(self, inputs, *, device=None, eigensolver: str = "divide") -> None:
+    def __init__(self, inputs, *, device=None, eigensolver: str = "lowest") -> None:
         from hafnium_amd.batch import BatchedCoulombExchange
         from hafnium_amd.coulomb_exchange import _Context
         from hafnium_amd.linalg import Dense
@@ -132,13 +135,18 @@ class BatchedRhf:
         self.inputs = list(inputs)
         if not self.inputs:
             raise InputError("a batch holds at least one molecule")
-        if eigensolver not in ("jacobi", "divide"):
-            raise InputError("eigensolver is 'jacobi' or 'divide'")
+        if eigensolver not in ("lowest", "jacobi", "divide"):
+            raise InputError("eigensolver is 'lowest', 'jacobi' or 'divide'")
+        # The overlaps need every eigenvector (rocSOLVER, divide and conquer or Jacobi); the Fock
+        # solves need the occupied ones: "lowest" takes them from the library's one-launch solver
+        # (matrices up to 512 functions; divide and conquer beyond).
         self.method = 1 if eigensolver == "jacobi" else 0
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
         self.count = len(self.inputs)
         self.nm = max(x.nao for x in self.inputs)
         self.om = max(x.n_alpha for x in self.inputs)
+        self.fock_lowest = eigensolver == "lowest" and self.nm <= EIGH_LOWEST_MAX_N
+        self._lowest = None  # (work, info) of the lowest-eigenpair solver
         self.jk = BatchedCoulombExchange([(x.nao, x.naux, x.n_alpha) for x in self.inputs], device=self.device)
         self._ctx = self.jk._ctx
         self.lib = self._ctx.lib
@@ -221,6 +229,18 @@ class BatchedRhf:
                                                self.method, self._stream()))
         self._eig_failures += (info != 0).any().to(torch.float64)
 
+    def _eig_lowest(self, a, w):
+        """The om lowest eigenpairs of every matrix of ``a`` (rows 0 .. om - 1, eigenvalues in ``w[:, :om]``)."""
+        torch = _torch()
+        cnt, nm, om = self.count, self.nm, self.om
+        if self._lowest is None:
+            work = torch.empty(int(self.lib.hfjk_eigh_lowest_batched_work(nm, cnt, om)), dtype=torch.float64,
+                               device=self.device)
+            self._lowest = (work, torch.zeros(cnt, dtype=torch.int32, device=self.device))
+        work, info = self._lowest
+        _lib.check(self.lib.hfjk_eigh_lowest_batched(self._ctx.handle, nm, cnt, om, _p(a), _p(w), _p(work), _p(info),
+                                                     self._stream()))
+
     def _gemm(self, ta, tb, m, n, k, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha=1.0, beta=0.0):
         _lib.check(self.lib.hfjk_dgemm_batched(self._ctx.handle, int(ta), int(tb), m, n, k, alpha, _p(a), lda, sa,
                                                _p(b), ldb, sb, beta, _p(c), ldc, sc, self.count, self._stream()))
@@ -275,7 +295,10 @@ class BatchedRhf:
                                                          DISCARDED_SHIFT, _p(fp), self._stream()))
         w = torch.empty((self.count, nm), dtype=torch.float64, device=self.device)
         mark("solve_transform")
-        self._eig(fp, w)
+        if self.fock_lowest:
+            self._eig_lowest(fp, w)
+        else:
+            self._eig(fp, w)
         mark("solve_eigensolver")
         c_occ = torch.empty((self.count, nm, om), dtype=torch.float64, device=self.device)
         # C_occ = X V_occ^T: rows 0..om-1 of each eigenvector matrix are the lowest levels.

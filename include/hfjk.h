@@ -401,6 +401,21 @@ HFJK_API long long hfjk_dsyev_batched_work(int n, int count);
  * (dsyevj). info[count] device (!= 0: that instance did not converge). */
 HFJK_API int hfjk_dsyev_batched(hfjk_ctx *ctx, int n, int count, double *a, double *w, double *work, int *info,
                                 int method, void *stream);
+/* The lowest eigenpairs of count symmetric [n][n] matrices at stride n*n
+ * (row-major, n <= 512), one workgroup per matrix: Householder
+ * tridiagonalisation, multisection on the Sturm count for the nvec lowest
+ * eigenvalues (w[i][0 .. nvec), ascending; w is [count][n], its other
+ * entries untouched), inverse iteration and orthonormalisation for their
+ * eigenvectors, which become rows 0 .. nvec - 1 of each matrix (its other
+ * rows are left as scratch). One launch, no allocation: work is device
+ * scratch of hfjk_eigh_lowest_batched_work(n, count, nvec) doubles; info[i]
+ * is set to 0. Replaces the rocSOLVER call of hfjk_dsyev_batched where only
+ * the occupied orbitals are needed (the batched SCF); no counterpart in the
+ * reference, whose diagonalisation is numpy.linalg.eigh
+ * (hafnium/scf/driver.py). */
+HFJK_API long long hfjk_eigh_lowest_batched_work(int n, int count, int nvec);
+HFJK_API int hfjk_eigh_lowest_batched(hfjk_ctx *ctx, int n, int count, int nvec, double *a, double *w, double *work,
+                                      int *info, void *stream);
 /* out[i] = <x_i, y_i> over len elements, fixed summation order. */
 HFJK_API int hfjk_ddot_batched(hfjk_ctx *ctx, int count, long long len, const double *x, long long stride_x,
                                const double *y, long long stride_y, double *out, void *stream);

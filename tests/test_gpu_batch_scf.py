@@ -30,14 +30,13 @@ def setup():
     return BatchedRhf(inputs), serial, cache, inputs
 
 
-@pytest.mark.parametrize("eigensolver", ["jacobi", "divide"])
+@pytest.mark.parametrize("eigensolver", ["lowest", "jacobi", "divide"])
 def test_batched_fp64_matches_serial(setup, eigensolver):
     from hafnium_amd.batch_scf import BatchedRhf, atomic_start
     from hafnium_amd.scf import ScfOptions
 
     batch, serial, cache, inputs = setup
-    if eigensolver != "jacobi":
-        batch = BatchedRhf(inputs, eigensolver=eigensolver)
+    batch = BatchedRhf(inputs, eigensolver=eigensolver)
     opts = ScfOptions(energy_tolerance=1e-8, gradient_tolerance=1e-6)
     results = batch.run(atomic_start(batch, cache), opts)
     for got, ref in zip(results, serial):
