@@ -23,6 +23,8 @@
 
 namespace {
 
+typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+
 constexpr int kEigThreads = 512;
 constexpr int kEigWaves = kEigThreads / 64;
 constexpr int kEigMaxN = 512;
@@ -89,51 +91,111 @@ __device__ double start_entry(int j, int i) {
   return (double)(h & 0xFFFFFFu) / (double)0x800000 - 1.0;
 }
 
-// One pass over the m x m trailing block (rows and columns c0 .. n - 1 of A, base at (c0, c0)):
-// applies the previous step's rank-2 update A -= v' w'^T + w' v'^T (pv, pw indexed from row
-// c0 - 1) and accumulates the column sums of the updated block times v (v[0] is v0) into
-// partw. Row c0 of the result also goes to nxt. Wave `wave` takes rows wave, wave + 8, ...,
-// R rows per batch of loads.
+constexpr int kOutOfRange = 1 << 30;  // a byte offset beyond any matrix of at most 512 x 512 doubles
+
+__device__ inline double buf_ld(__amdgpu_buffer_rsrc_t rsrc, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+}
+__device__ inline void buf_st(__amdgpu_buffer_rsrc_t rsrc, double v, int off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rsrc, off, 0, 0);
+}
+
+// R rows j0, j0 + 8, ... of one pass, CB chunks each, without branches (so that the compiler
+// keeps the per-row sums in place): rows past m load the last row and store out of range,
+// lanes off the lower triangle enter the sums with a zero factor.
+template <int NC, int CB, int R>
+__device__ __forceinline__ void tri_batch(__amdgpu_buffer_rsrc_t rsrc, int boff, int n, int m, int j0, int lane,
+                                          const double (&pvc)[NC], const double (&pwc)[NC], const double (&vc)[NC],
+                                          double (&acc)[NC], const double* pv, const double* pw, const double* vv,
+                                          double v0, double* nxt, double* prow, double* lsink, int lane_off,
+                                          int last_off) {
+  double val[R][CB];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int roff = boff + min(j0 + r * kEigWaves, m - 1) * n * 8;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) val[r][c] = buf_ld(rsrc, roff + (c == NC - 1 ? last_off : lane_off) + 512 * c);
+  }
+  double rs[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) rs[r] = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = j0 + r * kEigWaves;
+    const bool live = j < m;
+    const int jc = min(j, m - 1);
+    const int jm = live ? j : -1;  // lanes i <= jm are in the lower triangle of a live row
+    const double vj = live ? (jc == 0 ? v0 : vv[jc]) : 0.0;
+    const double pvj = pv[1 + jc], pwj = pw[1 + jc];
+    const int roff = live ? boff + j * n * 8 : kOutOfRange;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      const int i = lane + 64 * c;
+      const double a = val[r][c] - pvj * pwc[c] - pwj * pvc[c];
+      buf_st(rsrc, a, roff + (c == NC - 1 ? last_off : lane_off) + 512 * c);
+      if (c + 2 < CB) {  // below the diagonal in every row of the batch (a batch spans < 64 rows)
+        acc[c] = fma(a, vj, acc[c]);
+        rs[r] = fma(a, vc[c], rs[r]);  // the sums of rows past m are not used
+      } else {
+        acc[c] = fma(a, i <= jm ? vj : 0.0, acc[c]);
+        rs[r] = fma(a, i < jm ? vc[c] : 0.0, rs[r]);
+      }
+      if (c == 0) *(live && lane == 0 ? nxt + j : lsink + lane) = a;
+    }
+  }
+  const double tot = wave_sums8(rs, lane);
+  if ((lane & 7) == 0) {
+    const int r = lane >> 3, j = j0 + r * kEigWaves;
+    if (r < R && j < m) prow[j] = tot;
+  }
+}
+
+template <int NC, int CB, int R>
+__device__ __forceinline__ void tri_batch_cb(int cb, __amdgpu_buffer_rsrc_t rsrc, int boff, int n, int m, int j0,
+                                             int lane, const double (&pvc)[NC], const double (&pwc)[NC],
+                                             const double (&vc)[NC], double (&acc)[NC], const double* pv,
+                                             const double* pw, const double* vv, double v0, double* nxt, double* prow,
+                                             double* lsink, int lane_off, int last_off) {
+  if constexpr (CB <= NC) {
+    if (cb == CB)
+      tri_batch<NC, CB, R>(rsrc, boff, n, m, j0, lane, pvc, pwc, vc, acc, pv, pw, vv, v0, nxt, prow, lsink, lane_off,
+                           last_off);
+    else
+      tri_batch_cb<NC, CB + 1, R>(cb, rsrc, boff, n, m, j0, lane, pvc, pwc, vc, acc, pv, pw, vv, v0, nxt, prow, lsink,
+                                  lane_off, last_off);
+  }
+}
+
+// One pass over the lower triangle of the m x m trailing block (rows and columns c0 .. n - 1 of
+// A, at byte offset boff of the buffer rsrc, which covers A): applies the previous step's rank-2
+// update A -= v' w'^T + w' v'^T (pv, pw indexed from row c0 - 1) and forms the product of the
+// updated block with v (v[0] is v0): the column part sum_{j >= i} a_ji v_j into partw[i], the
+// row part sum_{i < j} a_ji v_i into prow[j]; column c0 of the result also goes to nxt. Wave
+// `wave` takes rows wave, wave + 8, ..., R rows per batch, each batch as many 64-column chunks as
+// its last row reaches. Buffer loads and stores keep the VALU work per chunk to the arithmetic:
+// the chunk's offset is an immediate, and lanes of the last chunk past the row end get an
+// offset outside the buffer (loads read zero, stores are dropped). Lanes above the diagonal load
+// and store the upper triangle of the block, which no pass reads.
 template <int NC>
-__device__ __forceinline__ void tri_pass(double* __restrict__ base, int n, int m, int wave, int lane, const double* pv,
-                         const double* pw, const double* vv, double v0, double* nxt, double* partw) {
-  constexpr int R = NC <= 2 ? 8 : NC <= 4 ? 4 : 2;
-  double pvc[NC], pwc[NC], acc[NC];
+__device__ __forceinline__ void tri_pass(__amdgpu_buffer_rsrc_t rsrc, int boff, int n, int m, int wave, int lane,
+                                         const double* pv, const double* pw, const double* vv, double v0,
+                                         double* nxt, double* partw, double* prow, double* lsink) {
+  constexpr int R = NC <= 4 ? 8 : 4;
+  double pvc[NC], pwc[NC], vc[NC], acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int i = lane + 64 * c;
     pvc[c] = i < m ? pv[1 + i] : 0.0;
     pwc[c] = i < m ? pw[1 + i] : 0.0;
+    vc[c] = i == 0 ? v0 : i < m ? vv[i] : 0.0;
     acc[c] = 0.0;
   }
+  const int lane_off = lane * 8;
+  const int last_off = lane + 64 * (NC - 1) < m ? lane * 8 : kOutOfRange;  // last chunk, less 512 (NC - 1)
   for (int j0 = wave; j0 < m; j0 += kEigWaves * R) {
-    double val[R][NC];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int j = min(j0 + r * kEigWaves, m - 1);
-      const double* row = base + (long long)j * n;
-#pragma unroll
-      for (int c = 0; c < NC; ++c) val[r][c] = row[min(lane + 64 * c, m - 1)];
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int j = j0 + r * kEigWaves;
-      if (j < m) {
-        const double vj = j == 0 ? v0 : vv[j];
-        const double pvj = pv[1 + j], pwj = pw[1 + j];
-        double* row = base + (long long)j * n;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          const int i = lane + 64 * c;
-          const double a = val[r][c] - pvj * pwc[c] - pwj * pvc[c];
-          if (i < m) {
-            row[i] = a;
-            if (j == 0) nxt[i] = a;
-          }
-          acc[c] += a * vj;
-        }
-      }
-    }
+    const int cb = (min(j0 + kEigWaves * (R - 1), m - 1) >> 6) + 1;
+    tri_batch_cb<NC, 1, R>(cb, rsrc, boff, n, m, j0, lane, pvc, pwc, vc, acc, pv, pw, vv, v0, nxt, prow, lsink,
+                           lane_off, last_off);
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -360,10 +422,11 @@ __global__ __launch_bounds__(kEigThreads) void eig_lowest_kernel(int n, int nvec
                                                                   double* __restrict__ w_all,
                                                                   double* __restrict__ work_all, int* __restrict__ info) {
   __shared__ double part[kEigWaves][kEigMaxN];
-  __shared__ double lv[2][kEigMaxN], lw[2][kEigMaxN], nxt[kEigMaxN], lbeta[kEigMaxN];
+  __shared__ double lv[2][kEigMaxN], lw[2][kEigMaxN], nxt[kEigMaxN], prow[kEigMaxN], lbeta[kEigMaxN];
   __shared__ double gm[kEigRegVecs * kGs], li[kEigRegVecs * kGs];
   __shared__ double red[kEigWaves];
   __shared__ double bounds[4];
+  __shared__ double lsink[64];
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -381,12 +444,14 @@ __global__ __launch_bounds__(kEigThreads) void eig_lowest_kernel(int n, int nvec
   double* F = Y + (long long)nvec * n;   // [5][n][nvec]: 1 / pivot, U's two superdiagonals, multiplier, swap flag
   double* Yt = F + 5LL * nvec * n;       // [n][nvec]
 
-  // 1. Householder tridiagonalisation. Step k reflects column k below the diagonal; its rank-2
-  // update of the trailing block is applied in the next step's single pass over the block, which
-  // also forms that step's product with the trailing block. nxt holds row k of the matrix as the
-  // previous pass left it; lv / lw hold v and w of the current and the previous step.
+  // 1. Householder tridiagonalisation on the lower triangle. Step k reflects column k below the
+  // diagonal; its rank-2 update of the trailing block is applied in the next step's single pass
+  // over the block, which also forms that step's product with the trailing block. nxt holds
+  // column k of the matrix as the previous pass left it; lv / lw hold v and w of the current and
+  // the previous step. The reflector vectors go to the upper triangle.
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(A, 0, n * n * 8, 0x00020000);
   for (int i = t; i < n; i += kEigThreads) {
-    nxt[i] = A[i];
+    nxt[i] = A[(long long)i * n];
     lv[1][i] = 0.0;
     lw[1][i] = 0.0;
   }
@@ -424,16 +489,16 @@ __global__ __launch_bounds__(kEigThreads) void eig_lowest_kernel(int n, int nvec
       vv[0] = v0;
     }
     if (t == k) lbeta[k] = bk;
-    double* blk = A + (long long)c0 * n + c0;
+    const int boff = (c0 * n + c0) * 8;
     switch ((m + 63) >> 6) {
-      case 1: tri_pass<1>(blk, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave]); break;
-      case 2: tri_pass<2>(blk, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave]); break;
-      case 3: tri_pass<3>(blk, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave]); break;
-      case 4: tri_pass<4>(blk, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave]); break;
-      case 5: tri_pass<5>(blk, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave]); break;
-      case 6: tri_pass<6>(blk, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave]); break;
-      case 7: tri_pass<7>(blk, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave]); break;
-      default: tri_pass<8>(blk, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave]); break;
+      case 1: tri_pass<1>(rsrc, boff, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave], prow, lsink); break;
+      case 2: tri_pass<2>(rsrc, boff, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave], prow, lsink); break;
+      case 3: tri_pass<3>(rsrc, boff, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave], prow, lsink); break;
+      case 4: tri_pass<4>(rsrc, boff, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave], prow, lsink); break;
+      case 5: tri_pass<5>(rsrc, boff, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave], prow, lsink); break;
+      case 6: tri_pass<6>(rsrc, boff, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave], prow, lsink); break;
+      case 7: tri_pass<7>(rsrc, boff, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave], prow, lsink); break;
+      default: tri_pass<8>(rsrc, boff, n, m, wave, lane, pv, pw, vv, v0, nxt, part[wave], prow, lsink); break;
     }
     __syncthreads();
     double p = 0.0, pdv = 0.0, vi = 0.0;
@@ -441,7 +506,7 @@ __global__ __launch_bounds__(kEigThreads) void eig_lowest_kernel(int n, int nvec
       double sacc = 0.0;
 #pragma unroll
       for (int g = 0; g < kEigWaves; ++g) sacc += part[g][t];
-      p = bk * sacc;
+      p = bk * (sacc + prow[t]);
       vi = t == 0 ? v0 : vv[t];
       pdv = p * vi;
       A[(long long)k * n + c0 + t] = vi;  // the reflector vector, row k above the diagonal
