@@ -629,7 +629,7 @@ def main(argv=None):
             kernel = "xform_dma_kernel<float, 1>"
             b_bytes = q * n * n * 4.0
         elif head == "bf16x3" and use_full.get("bf16x3"):
-            kernel = "xform16_256_kernel" if o > 128 and tile == 256 else "xform16_dma_kernel"
+            kernel = "xform16_256_kernel<true>" if o > 128 and tile == 256 else "xform16_dma_kernel"
             b_bytes = q * n * n * 4.0
         traffic, traffic_note = measured_traffic(kernel, n, q, o, x_ms)
         roofline = {"bound": "mfma", "kernel": kernel, "achieved": achieved, "peak": peak,
