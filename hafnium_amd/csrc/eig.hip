@@ -50,8 +50,21 @@ __device__ double wg_sum(double v, double* red) {
   return s;
 }
 
+// A lane's partner value under a DPP row permutation (ctrl: 0x140 row mirror, 0x141 half-row
+// mirror, quad permutations below 0x100); every lane of the row takes part.
+template <int CTRL>
+__device__ inline double dpp_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // Eight sums over the wave at once: on return every lane holds the total of s[lane >> 3].
 // Each halving exchange sends the half of the values the partner keeps (10 exchanges, not 48).
+// The partners differ in lane bit 5, 4, 3, 2, 1, 0 in turn: lane ^ 32 and lane ^ 16 by
+// shuffles, then within a row of 16 by DPP (row mirror 15 - l, half-row mirror 7 - l, quad
+// permutations l ^ 2 and l ^ 1), which pair lanes the same way for these sums.
 __device__ inline double wave_sums8(const double (&s)[8], int lane) {
   const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
   double t4[4], t2[2];
@@ -59,10 +72,10 @@ __device__ inline double wave_sums8(const double (&s)[8], int lane) {
   for (int x = 0; x < 4; ++x) t4[x] = (h5 ? s[4 + x] : s[x]) + __shfl_xor(h5 ? s[x] : s[4 + x], 32);
 #pragma unroll
   for (int x = 0; x < 2; ++x) t2[x] = (h4 ? t4[2 + x] : t4[x]) + __shfl_xor(h4 ? t4[x] : t4[2 + x], 16);
-  double t1 = (h3 ? t2[1] : t2[0]) + __shfl_xor(h3 ? t2[0] : t2[1], 8);
-  t1 += __shfl_xor(t1, 4);
-  t1 += __shfl_xor(t1, 2);
-  t1 += __shfl_xor(t1, 1);
+  double t1 = (h3 ? t2[1] : t2[0]) + dpp_d<0x140>(h3 ? t2[0] : t2[1]);
+  t1 += dpp_d<0x141>(t1);
+  t1 += dpp_d<0x4E>(t1);  // quad permutation [2, 3, 0, 1]
+  t1 += dpp_d<0xB1>(t1);  // quad permutation [1, 0, 3, 2]
   return t1;
 }
 
@@ -73,10 +86,10 @@ __device__ inline double readlane_d(double x, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// 1 / q from the hardware estimate and two Newton steps.
+// 1 / q from the hardware estimate and one Newton step (the Sturm count needs only the signs
+// of a recurrence that is stable under small relative perturbations).
 __device__ inline double recip(double q) {
-  double r = __builtin_amdgcn_rcp(q);
-  r = fma(r, fma(-q, r, 1.0), r);
+  const double r = __builtin_amdgcn_rcp(q);
   return fma(r, fma(-q, r, 1.0), r);
 }
 
