@@ -7,7 +7,9 @@ vectors equal to numpy's (the quantity the SCF density uses). Cases: random
 matrices of several sizes including 1 and 2, exactly degenerate spectra
 (repeated blocks), a Fock-like matrix with directions shifted by 1e3 (the
 batched SCF's discarded functions), and a diagonal matrix (every reflector
-empty).
+empty). The sizes cover one to eight 64-row chunks per vector (n up to the
+limit, 512), the register path (nvec <= 64) and the Gram-Schmidt path past
+it; nvec = 0 leaves the matrices as they were.
 """
 
 from __future__ import annotations
@@ -20,7 +22,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _solve(mats: np.ndarray, nvec: int):
+def _solve(mats: np.ndarray, nvec: int, full: bool = False):
     import torch
 
     from hafnium_amd import _lib
@@ -38,7 +40,7 @@ def _solve(mats: np.ndarray, nvec: int):
                                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
     torch.cuda.synchronize()
     assert (info.cpu().numpy() == 0).all()
-    return w.cpu().numpy(), a.cpu().numpy()[:, :nvec, :]
+    return w.cpu().numpy(), a.cpu().numpy() if full else a.cpu().numpy()[:, :nvec, :]
 
 
 def _check(mats, nvec, w, x, projector=True):
@@ -55,7 +57,8 @@ def _check(mats, nvec, w, x, projector=True):
             assert np.abs(xi.T @ xi - want).max() < 1e-10, i
 
 
-@pytest.mark.parametrize("n,nvec", [(1, 1), (2, 1), (2, 2), (3, 2), (50, 12), (130, 40), (287, 60), (300, 300)])
+@pytest.mark.parametrize("n,nvec", [(1, 1), (2, 1), (2, 2), (3, 2), (50, 12), (64, 64), (65, 64), (128, 65), (130, 40),
+                                    (287, 60), (300, 300), (512, 64)])
 def test_random_symmetric_matrices(n, nvec):
     rng = np.random.default_rng(n + nvec)
     g = rng.standard_normal((4, n, n))
@@ -85,3 +88,11 @@ def test_degenerate_and_shifted_spectra():
     mats = np.array(mats)
     w, x = _solve(mats, nvec)
     _check(mats, nvec, w, x)
+
+
+def test_no_vectors_leaves_the_matrices():
+    rng = np.random.default_rng(9)
+    g = rng.standard_normal((3, 96, 96))
+    mats = g + g.transpose(0, 2, 1)
+    _, a = _solve(mats, 0, full=True)
+    np.testing.assert_array_equal(a, mats)
