@@ -37,7 +37,7 @@ constexpr int kGs = 65;                     // row stride of the Gram matrix and
 __host__ __device__ inline long long eig_work_per(int n, int nvec) { return 3LL * n + 7LL * nvec * n; }
 
 // Sum over the workgroup (every thread passes its part; all get the total).
-__device__ double wg_sum(double v, double* red) {
+__device__ __forceinline__ double wg_sum(double v, double* red) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -364,32 +364,49 @@ __device__ __forceinline__ void tri_product(double (&y)[8][NC], const double* Y,
   }
 }
 
-// G = L L^T in place (lower triangle of gm) and li = L^{-1}, by wave 0 (lane i owns row i for
-// the factorisation, column i for the inverse). A non-positive pivot is raised to DBL_MIN.
-__device__ __forceinline__ void cholesky_inverse(double* gm, double* li, int nv, int lane) {
+// G = L L^T (G in the lower triangle of gm, nv <= 64) and li = L^{-1}, by wave 0 with every
+// row in registers: lane i holds row i of G, each column of L is broadcast through LDS (the
+// first row of li), and the update of the rows is unrolled over the columns; then L goes to gm
+// and lane c forms column c of L^{-1} by forward substitution, reading L by broadcast. A
+// non-positive pivot is raised to DBL_MIN. Not inlined: one copy serves every vector length.
+__device__ __attribute__((noinline)) void cholesky_inverse(double* gm, double* li, int nv, int lane) {
   const int i = lane;
-  for (int k = 0; k < nv; ++k) {
-    const double lkk = sqrt(fmax(gm[k * kGs + k], DBL_MIN));
-    double lik = 0.0;
-    if (i > k && i < nv) {
-      lik = gm[i * kGs + k] / lkk;
-      gm[i * kGs + k] = lik;
-    }
-    if (i == k) gm[k * kGs + k] = lkk;
-    __builtin_amdgcn_wave_barrier();
-    if (i > k && i < nv)
-      for (int jj = k + 1; jj <= i; ++jj) gm[i * kGs + jj] -= lik * gm[jj * kGs + k];
-    __builtin_amdgcn_wave_barrier();
-  }
-  const int k = lane;
-  if (k < nv) {
-    li[k * kGs + k] = 1.0 / gm[k * kGs + k];
-    for (int r = k + 1; r < nv; ++r) {
-      double s = 0.0;
-      for (int p = k; p < r; ++p) s += gm[r * kGs + p] * li[p * kGs + k];
-      li[r * kGs + k] = -s / gm[r * kGs + r];
+  double g[kEigRegVecs];
+#pragma unroll
+  for (int j = 0; j < kEigRegVecs; ++j) g[j] = (i < nv && j <= i) ? gm[i * kGs + j] : 0.0;
+#pragma unroll
+  for (int k = 0; k < kEigRegVecs; ++k) {
+    if (k < nv) {
+      const double lkk = sqrt(fmax(readlane_d(g[k], k), DBL_MIN));
+      const double lik = i == k ? lkk : g[k] / lkk;  // L[i][k] for i >= k (lanes above: unused)
+      g[k] = lik;
+      li[i] = lik;
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int j = k + 1; j < kEigRegVecs; ++j) g[j] = fma(-lik, li[j], g[j]);
+      __builtin_amdgcn_wave_barrier();
     }
   }
+#pragma unroll
+  for (int j = 0; j < kEigRegVecs; ++j) gm[i * kGs + j] = g[j];
+  __builtin_amdgcn_wave_barrier();
+  const int c = lane;
+  double x[kEigRegVecs];  // column c of L^{-1}; x[r] = 0 for r < c falls out of the recurrence
+#pragma unroll
+  for (int r = 0; r < kEigRegVecs; ++r) {
+    if (r < nv) {
+      double sacc = r == c ? 1.0 : 0.0;
+#pragma unroll
+      for (int p = 0; p < r; ++p) sacc = fma(-gm[r * kGs + p], x[p], sacc);
+      x[r] = sacc / gm[r * kGs + r];
+    } else {
+      x[r] = 0.0;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int r = 0; r < kEigRegVecs; ++r)
+    if (r < nv && c < nv) li[r * kGs + c] = x[r];
 }
 
 // Orthonormalisation (Cholesky QR twice) and back-transformation for nvec <= 64, each wave's
