@@ -79,6 +79,29 @@ __device__ inline double wave_sums8(const double (&s)[8], int lane) {
   return t1;
 }
 
+// Sum over the wave, every lane gets it (shuffles across rows, DPP within a row).
+__device__ inline double wave_sum(double v) {
+  v += __shfl_xor(v, 32);
+  v += __shfl_xor(v, 16);
+  v += dpp_d<0x140>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0xB1>(v);
+  return v;
+}
+
+// Sum over the workgroup with one barrier: red must not be written again before a later barrier
+// (callers alternate two buffers between barriers).
+__device__ __forceinline__ double wg_sum1(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < kEigWaves; ++k) s += red[k];
+  return s;
+}
+
 __device__ inline double readlane_d(double x, int l) {
   const long long b = __double_as_longlong(x);
   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
@@ -454,7 +477,7 @@ __global__ __launch_bounds__(kEigThreads) void eig_lowest_kernel(int n, int nvec
   __shared__ double part[kEigWaves][kEigMaxN];
   __shared__ double lv[2][kEigMaxN], lw[2][kEigMaxN], nxt[kEigMaxN], prow[kEigMaxN], lbeta[kEigMaxN];
   __shared__ double gm[kEigRegVecs * kGs], li[kEigRegVecs * kGs];
-  __shared__ double red[kEigWaves];
+  __shared__ double red[kEigWaves], red2[2][kEigWaves];
   __shared__ double bounds[4];
   __shared__ double lsink[64];
   const int t = threadIdx.x;
@@ -503,7 +526,7 @@ __global__ __launch_bounds__(kEigThreads) void eig_lowest_kernel(int n, int nvec
     }
     const double x0 = nxt[1] - pv[1] * pw0 - pw[1] * pv0;
     const double dk = nxt[0] - 2.0 * pv0 * pw0;
-    const double ss = wg_sum(sq, red);
+    const double ss = wg_sum1(sq, red2[0]);
     const double sigma = ss - x0 * x0;
     double bk = 0.0, ek = x0, v0 = x0;
     if (sigma > 0.0) {
@@ -541,7 +564,7 @@ __global__ __launch_bounds__(kEigThreads) void eig_lowest_kernel(int n, int nvec
       pdv = p * vi;
       A[(long long)k * n + c0 + t] = vi;  // the reflector vector, row k above the diagonal
     }
-    const double K = 0.5 * bk * wg_sum(pdv, red);
+    const double K = 0.5 * bk * wg_sum1(pdv, red2[1]);
     if (t < m) ww[t] = p - K * vi;
     __syncthreads();
   }
